@@ -1,0 +1,155 @@
+"""Benchmark of the particle-likelihood hot path on one MI355X.
+
+One step = one pass of the hot path over the whole particle population held
+in HBM: eigen-decomposition of every class rate matrix, P(t) for every
+(node, rate), level-ordered pruning with the root mixture fused, the
+per-particle lnL reduction (Particle::refreshLikelihood(true) for every
+particle, Particle.cpp:530-675), then computeNextStep (Miscellaneous.hpp:42-109)
+and systematic resampling (main.cpp:179-195) on the resulting lnLs.
+
+Default workload: 64 taxa x 2,000 sites, GTR+G4+I, 1,024 particles, plain FP64
+(synthetic alignment simulated under GTR+G4+I, particle states drawn from the
+prior).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "smc-cat_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "pruned node·pattern·category updates/s; SMC ESS/sec vs plain FP64, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_column(T, word):
+    """Persisted-partials traffic per (pattern, rate) column of one full sweep:
+    T-2 internal non-root children read 20 words, T tip children read a 1-byte
+    code, T-1 internal nodes write 20 words + an 8-byte log-scale."""
+    return (T - 2) * 20 * word + T * 1 + (T - 1) * (20 * word + 8)
+
+
+def cpu_baseline(codes, batch, R, threads, budget_s=10.0):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_binding as orc
+    T, S = codes.shape
+    ps = [batch.particle(i) for i in range(batch.n)]
+    done, t0 = 0, time.perf_counter()
+    while True:
+        chunk = [ps[(done + k) % len(ps)] for k in range(threads)]
+        orc.lnl_batch(codes, chunk, R, threads=threads)
+        done += len(chunk)
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done * (T - 1) * S * R / dt, "unit": "updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d particles (%d taxa x %d sites, K_r=%d) through the CPU FP64 oracle in "
+                      "reference loop order, one particle per OpenMP thread, %.1f s" %
+                      (done, T, S, R, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--particles", type=int, default=0)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(particles would shard across GPUs by slot; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from smccat import Engine, lg_base_matrix
+    from smccat import simulate as sim
+
+    codes, cfg = sim.simulate_config(args.config)
+    T, S, R = cfg["taxa"], cfg["sites"], cfg["rates"]
+    N = args.particles or cfg["particles"]
+    rng = np.random.default_rng(1)
+    batch = sim.prior_particles(rng, N, T, S, num_rates=R, invar=cfg["invar"], lg=cfg["lg"],
+                                lg_table=lg_base_matrix() if cfg["lg"] else None)
+    eng = Engine(T, S, R, N, 1, args.precision)
+    eng.set_alignment(codes)
+    eng.set_particles(batch)
+    target = int(0.6 * N)
+
+    def step():
+        lnl, nf = eng.refresh_likelihood()
+        w = np.full(N, -np.log(N))
+        w, norm, ess, temp = eng.compute_next_step(w, lnl, target, 0.0)
+        eng.systematic_resample(norm, 0.5)
+        return lnl, nf, ess, temp
+
+    for _ in range(args.warmup):
+        step()
+    eng.set_timing(True)
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lnl, nf, ess, temp = step()
+    dt = time.perf_counter() - t0
+    stats = eng.kernel_stats()
+    eng.set_timing(False)
+
+    updates = N * (T - 1) * S * R
+    value = updates * args.steps / dt
+    word = 8 if args.precision == "fp64" else 4
+    bytes_sweep = N * S * R * algorithmic_bytes_per_column(T, word)
+    n_prune, ms_prune = stats["prune"]
+    launches_per_sweep = n_prune / args.steps
+    prune_ms_per_sweep = ms_prune / args.steps
+    achieved = bytes_sweep / (prune_ms_per_sweep * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "%s: %d taxa x %d sites, %s+G%d%s, %d particles; step = full "
+                               "refreshLikelihood(true) of every particle + computeNextStep + "
+                               "systematic resampling" %
+                               (args.config, T, S, "LG" if cfg["lg"] else "GTR", R,
+                                "+I" if cfg["invar"] else "", N),
+                   "taxa": T, "sites": S, "rates": R, "particles": N,
+                   "precision": args.precision, "updates_per_step": updates},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_prune<%s>" % ("double" if word == 8 else "float"),
+                     "launches_per_step": launches_per_sweep,
+                     "avg_launch_ms": ms_prune / max(n_prune, 1),
+                     "algorithmic_bytes_per_step": bytes_sweep},
+        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items()},
+        "check": {"lnl_finite": bool(np.all(np.isfinite(lnl)) and not nf.any()),
+                  "lnl_mean": float(np.mean(lnl)), "ess": ess, "temp": temp},
+    }
+    eng.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        sample = sim.prior_particles(np.random.default_rng(2), 64, T, S, num_rates=R,
+                                     invar=cfg["invar"], lg=cfg["lg"],
+                                     lg_table=lg_base_matrix() if cfg["lg"] else None)
+        out["cpu_baseline"] = cpu_baseline(codes, sample, R, threads, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+/*
+ * smccat_hip.h -- C-ABI of the MI355X particle-likelihood engine (libsmccat_hip.so).
+ *
+ * The reference (Smc-CAT) is one C++ executable with no FFI; its seam is the
+ * C++ Particle API (src/analysis/Particle.hpp:51-83) plus the free SMC
+ * functions of src/core/Miscellaneous.hpp.  Each entry point below names the
+ * reference interface it replaces.  The C++ facade in
+ * smc-cat_amd/csrc/host/particle.hpp keeps the reference's call shapes on
+ * top of these functions.
+ *
+ * Conventions: every call returns an int status (SMC_OK == 0) and leaves a
+ * message in smc_engine_last_error(); nothing throws across the ABI.  Host
+ * buffers are copied in or out before the call returns; the engine never
+ * keeps a host pointer.  One host thread per engine (the reference's
+ * Particle is thread-confined in the same way, main.cpp:295-298).
+ */
+#ifndef SMCCAT_HIP_H
+#define SMCCAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SMC_OK = 0,
+    SMC_ERR_ARG = 1,    /* invalid argument or size */
+    SMC_ERR_HIP = 2,    /* HIP runtime error (message in last_error) */
+    SMC_ERR_NOMEM = 3,  /* device allocation failed */
+    SMC_ERR_STATE = 4   /* call order violated (e.g. no alignment yet) */
+};
+
+enum { SMC_PREC_FP64 = 0, SMC_PREC_FP32 = 1 };
+
+/* Kernel families reported by smc_engine_kernel_stats. */
+enum {
+    SMC_K_EIGEN = 0,   /* 20x20 eigen-decomposition per (particle, class)    */
+    SMC_K_PMAT = 1,    /* P(t) back-transform per (particle, class, node, rate) */
+    SMC_K_PRUNE = 2,   /* level-ordered pruning, root mixture fused          */
+    SMC_K_REDUCE = 3,  /* per-particle log-likelihood reduction              */
+    SMC_K_SMC = 4,     /* reweight / ESS bisection / resampling              */
+    SMC_K_COUNT = 5
+};
+
+typedef struct smc_engine smc_engine;
+
+typedef struct {
+    int32_t device;        /* HIP device ordinal                                     */
+    int32_t taxa;          /* T: Alignment::getNumTaxa (Alignment.hpp)               */
+    int32_t sites;         /* S: numChar, one pattern per column (Particle.cpp:22)   */
+    int32_t num_rates;     /* K_r: Particle::numRates (Particle.hpp:101)             */
+    int32_t max_particles; /* resident particle slots                                */
+    int32_t max_classes;   /* CAT classes per particle (1 without CAT)               */
+    int32_t precision;     /* SMC_PREC_FP64 (plain) or SMC_PREC_FP32 (fast path)     */
+    int32_t reserved;      /* must be 0                                              */
+} smc_engine_config;
+
+/* Engine lifetime.  Replaces the per-thread Particle construction of
+ * main.cpp:112-115 / Particle::Particle (Particle.cpp:21-40): all device
+ * buffers for max_particles resident particles are allocated here. */
+int smc_engine_create(const smc_engine_config *cfg, smc_engine **out);
+int smc_engine_destroy(smc_engine *e);
+const char *smc_engine_last_error(const smc_engine *e);
+
+/* Alignment upload: codes[taxon][site] in the Alignment code table
+ * (Alignment.cpp:7-31: 0..19 amino acids, 20 '-', 21 'X', 22 '?').
+ * Replaces the tip encoding and invariant detection of Particle.cpp:41-73. */
+int smc_set_alignment(smc_engine *e, const int8_t *codes);
+
+/* Particle state upload for slots [first, first+count).  Replaces
+ * Particle::copyFromSerialized (Particle.cpp:218-300) on already-parsed state:
+ *   children    [count][T-1][2]  children of internal node T+i (Tree.cpp:62-67 ids)
+ *   branch_len  [count][2T-1]    by node id; the root entry is ignored
+ *   base        [count][190]     log-exchangeabilities (RateMatrices.hpp:97-111 order)
+ *   num_classes [count]          K per particle, 1..max_classes
+ *   logits      [count][max_classes][20]  stationary logits per class
+ *   assignment  [count][S]       class index of every site
+ *   pinv        [count]          proportion of invariant sites
+ *   shape       [count]          gamma shape; rates = discretizeGamma(shape, K_r)
+ *                                (Miscellaneous.hpp:22-37), 1.0 when K_r == 1 */
+int smc_set_particles(smc_engine *e, int first, int count, const int32_t *children,
+                      const double *branch_len, const double *base, const int32_t *num_classes,
+                      const double *logits, const int32_t *assignment, const double *pinv,
+                      const double *shape);
+
+/* Full likelihood of slots [first, first+count): Particle::refreshLikelihood(true)
+ * followed by Particle::lnLikelihood() (Particle.cpp:530-675, 480-482).
+ * lnl_out[count] and nonfinite_out[count] may be NULL.  nonfinite_out[i] = 1
+ * when the particle's lnL is NaN/Inf (the reference's NaN guard, main.cpp:158-174). */
+int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
+                           uint8_t *nonfinite_out);
+
+/* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
+ * updated in place, normalized[n] and *ess / *current_temp are written. */
+int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,
+                          double *normalized, double *ess, int target_ess,
+                          double *current_temp, double last_temp);
+
+/* Systematic resampling (main.cpp:179-195) on the GPU: u in [0,1). */
+int smc_systematic_resample(smc_engine *e, int n, const double *normalized, double u,
+                            int32_t *assignments);
+
+/* discretizeGamma (Miscellaneous.hpp:22-37): out[num] mean-one gamma rates. */
+int smc_discretize_gamma(double shape, int num, double *out);
+
+/* The LG exchangeabilities as the reference's LG particle stores them
+ * (RateMatrices.hpp:7-95 read in the order of :97-111, Particle.cpp:84-93). */
+void smc_lg_base_matrix(double out[190]);
+
+/* Measurement (no reference counterpart; the reference only has TIME_PROFILE
+ * prints, Particle.cpp:533-631): when enabled, every kernel launch is bracketed
+ * by HIP events on the engine's stream and its duration accumulated per family. */
+int smc_engine_set_timing(smc_engine *e, int enabled);
+int smc_engine_kernel_stats(smc_engine *e, int family, int64_t *launches, double *total_ms);
+int smc_engine_reset_stats(smc_engine *e);
+
+/* Bytes of device memory held by the engine. */
+int64_t smc_engine_device_bytes(const smc_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

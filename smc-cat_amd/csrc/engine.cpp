@@ -1,0 +1,653 @@
+// C-ABI of the particle-likelihood engine: device memory, level scheduling of
+// the per-particle trees and the launch sequence of one likelihood sweep.
+//
+// Device layout (all resident in HBM, sized at create time for max_particles):
+//   codes    uint8  [T][S_pad]                 alignment, original column order
+//   perm     int32  [P][S_pad]                 slot -> column, class-sorted per particle
+//   tiles    Tile   [P][tile_stride]           <= 64 slots of one class each
+//   eig      f64    [P*Kmax][840]              A | lam | B | pi per class
+//   Pmat     TX     [P*Kmax][2T-1][K_r][20x21] transition matrices (+ row sums)
+//   X        TX     [P][T-1][K_r][20][S_pad]   internal-node partials, state-major
+//   E        int32  [P][T-1][K_r][S_pad]       cumulative binary exponents
+//   tile_lnl f64    [P][tile_stride]           per-tile site-lnL sums
+// TX is double on the plain path and float on the fast path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "smccat_hip.h"
+
+using namespace smccat;
+
+struct PendingTiming {
+    int family;
+    hipEvent_t a, b;
+};
+
+struct smc_engine {
+    smc_engine_config cfg{};
+    int T = 0, S = 0, S_pad = 0, R = 0, nn = 0, Pmax = 0, Kmax = 0, tile_stride = 0;
+    bool fp32 = false;
+    size_t tx_size = 8;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t device_bytes = 0;
+    bool have_alignment = false;
+
+    // device buffers
+    uint8_t *d_codes = nullptr;
+    double *d_isinv = nullptr;
+    int *d_invchar = nullptr;
+    int *d_children = nullptr;
+    double *d_blen = nullptr;
+    double *d_base = nullptr;
+    double *d_logits = nullptr;
+    int *d_nclass = nullptr;
+    double *d_pinv = nullptr;
+    double *d_rates = nullptr;
+    int *d_root = nullptr;
+    int *d_perm = nullptr;
+    Tile *d_tiles = nullptr;
+    int *d_ntiles = nullptr;
+    double *d_eig = nullptr;
+    void *d_P = nullptr;
+    void *d_X = nullptr;
+    int *d_E = nullptr;
+    double *d_tile_lnl = nullptr;
+    double *d_lnl = nullptr;
+    uint8_t *d_nonfinite = nullptr;
+    Item *d_items = nullptr;
+    // SMC scratch (grown on demand)
+    int smc_cap = 0;
+    double *d_w = nullptr, *d_l = nullptr, *d_norm = nullptr, *d_out2 = nullptr;
+    int *d_assign = nullptr;
+
+    // host mirrors
+    std::vector<int> h_children;  // [P][T-1][2]
+    std::vector<int> h_root;      // [P]
+    std::vector<int> h_height;    // [P][T-1]
+    std::vector<int> h_valid;     // [P] particle uploaded
+
+    // level schedule cache
+    int sched_first = -1, sched_count = -1;
+    uint64_t sched_version = 0, topo_version = 1;
+    std::vector<int> level_off;  // items of level L: [level_off[L], level_off[L+1])
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<PendingTiming> pending;
+    int64_t launches[SMC_K_COUNT] = {};
+    double total_ms[SMC_K_COUNT] = {};
+};
+
+namespace {
+
+int fail(smc_engine *e, int code, const std::string &msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define HIP_TRY(e, call)                                                                      \
+    do {                                                                                      \
+        hipError_t _st = (call);                                                              \
+        if (_st != hipSuccess)                                                                \
+            return fail((e), SMC_ERR_HIP,                                                     \
+                        std::string(#call) + " failed: " + hipGetErrorString(_st));          \
+    } while (0)
+
+template <typename T>
+int dev_alloc(smc_engine *e, T **ptr, size_t count)
+{
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 16;
+    hipError_t st = hipMalloc((void **)ptr, bytes);
+    if (st != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e, SMC_ERR_NOMEM,
+                    "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                        hipGetErrorString(st));
+    }
+    e->device_bytes += (int64_t)bytes;
+    return SMC_OK;
+}
+
+hipEvent_t take_event(smc_engine *e)
+{
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Bracket a launch with events when timing is on.
+struct TimedLaunch {
+    smc_engine *e;
+    int family;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(smc_engine *eng, int fam) : e(eng), family(fam)
+    {
+        if (e->timing) {
+            a = take_event(e);
+            b = take_event(e);
+            if (a) (void)hipEventRecord(a, e->stream);
+        }
+    }
+    ~TimedLaunch()
+    {
+        if (e->timing && a && b) {
+            (void)hipEventRecord(b, e->stream);
+            e->pending.push_back({family, a, b});
+        }
+    }
+};
+
+// Call after a stream synchronisation.
+void collect_timings(smc_engine *e)
+{
+    for (auto &pt : e->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pt.a, pt.b) == hipSuccess) {
+            e->launches[pt.family] += 1;
+            e->total_ms[pt.family] += ms;
+        }
+        e->ev_pool.push_back(pt.a);
+        e->ev_pool.push_back(pt.b);
+    }
+    e->pending.clear();
+}
+
+// Build the per-level item lists for slots [first, first+count).
+int build_schedule(smc_engine *e, int first, int count)
+{
+    if (e->sched_first == first && e->sched_count == count && e->sched_version == e->topo_version)
+        return SMC_OK;
+    const int Ti = e->T - 1;
+    int maxh = 0;
+    for (int p = first; p < first + count; ++p)
+        for (int i = 0; i < Ti; ++i) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
+    std::vector<int> cnt(maxh + 2, 0);
+    for (int p = first; p < first + count; ++p)
+        for (int i = 0; i < Ti; ++i) cnt[e->h_height[(size_t)p * Ti + i]]++;
+    e->level_off.assign(maxh + 2, 0);
+    for (int L = 1; L <= maxh; ++L) e->level_off[L + 1] = e->level_off[L] + cnt[L];
+    std::vector<Item> items((size_t)count * Ti);
+    std::vector<int> fill(e->level_off);
+    for (int p = first; p < first + count; ++p)
+        for (int i = 0; i < Ti; ++i) {
+            int h = e->h_height[(size_t)p * Ti + i];
+            items[fill[h]++] = Item{p, e->T + i};
+        }
+    HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
+                              hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->sched_first = first;
+    e->sched_count = count;
+    e->sched_version = e->topo_version;
+    return SMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
+{
+    if (!cfg || !out) return SMC_ERR_ARG;
+    *out = nullptr;
+    if (cfg->taxa < 3 || cfg->sites < 1 || cfg->num_rates < 1 || cfg->num_rates > 64 ||
+        cfg->max_particles < 1 || cfg->max_classes < 1 || cfg->max_classes > 4096 ||
+        (cfg->precision != SMC_PREC_FP64 && cfg->precision != SMC_PREC_FP32) || cfg->reserved != 0)
+        return SMC_ERR_ARG;
+    smc_engine *e = new smc_engine();
+    e->cfg = *cfg;
+    e->T = cfg->taxa;
+    e->S = cfg->sites;
+    e->S_pad = (cfg->sites + kTile - 1) / kTile * kTile;
+    e->R = cfg->num_rates;
+    e->nn = 2 * cfg->taxa - 1;
+    e->Pmax = cfg->max_particles;
+    e->Kmax = cfg->max_classes;
+    e->tile_stride = (cfg->sites + kTile - 1) / kTile + cfg->max_classes;
+    e->fp32 = cfg->precision == SMC_PREC_FP32;
+    e->tx_size = e->fp32 ? 4 : 8;
+    if (e->tile_stride > 65535) {
+        int rc = fail(e, SMC_ERR_ARG, "too many tiles per particle");
+        smc_engine_destroy(e);
+        return rc;
+    }
+    hipError_t st = hipSetDevice(cfg->device);
+    if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (st != hipSuccess) {
+        delete e;
+        return SMC_ERR_HIP;
+    }
+    const size_t P = e->Pmax, T = e->T, Sp = e->S_pad, R = e->R, K = e->Kmax, nn = e->nn;
+    int rc = SMC_OK;
+    auto A = [&](auto **ptr, size_t n) {
+        if (rc == SMC_OK) rc = dev_alloc(e, ptr, n);
+    };
+    A(&e->d_codes, T * Sp);
+    A(&e->d_isinv, Sp);
+    A(&e->d_invchar, Sp);
+    A(&e->d_children, P * (T - 1) * 2);
+    A(&e->d_blen, P * nn);
+    A(&e->d_base, P * 190);
+    A(&e->d_logits, P * K * 20);
+    A(&e->d_nclass, P);
+    A(&e->d_pinv, P);
+    A(&e->d_rates, P * R);
+    A(&e->d_root, P);
+    A(&e->d_perm, P * Sp);
+    A(&e->d_tiles, P * e->tile_stride);
+    A(&e->d_ntiles, P);
+    A(&e->d_eig, P * K * kEigStride);
+    A(&e->d_tile_lnl, P * e->tile_stride);
+    A(&e->d_lnl, P);
+    A(&e->d_nonfinite, P);
+    A(&e->d_items, P * (T - 1));
+    A(&e->d_E, P * (T - 1) * R * Sp);
+    if (e->fp32) {
+        A((float **)&e->d_P, P * K * nn * R * kPStride);
+        A((float **)&e->d_X, P * (T - 1) * R * 20 * Sp);
+    } else {
+        A((double **)&e->d_P, P * K * nn * R * kPStride);
+        A((double **)&e->d_X, P * (T - 1) * R * 20 * Sp);
+    }
+    if (rc != SMC_OK) {
+        std::string msg = e->err;
+        smc_engine_destroy(e);
+        fprintf(stderr, "smc_engine_create: %s\n", msg.c_str());
+        return rc;
+    }
+    (void)hipMemsetAsync(e->d_ntiles, 0, P * sizeof(int), e->stream);
+    (void)hipMemsetAsync(e->d_nclass, 0, P * sizeof(int), e->stream);
+    (void)hipStreamSynchronize(e->stream);
+    e->h_children.assign(P * (T - 1) * 2, 0);
+    e->h_root.assign(P, -1);
+    e->h_height.assign(P * (T - 1), 0);
+    e->h_valid.assign(P, 0);
+    *out = e;
+    return SMC_OK;
+}
+
+int smc_engine_destroy(smc_engine *e)
+{
+    if (!e) return SMC_ERR_ARG;
+    void *bufs[] = {e->d_codes,  e->d_isinv, e->d_invchar, e->d_children, e->d_blen,
+                    e->d_base,   e->d_logits, e->d_nclass, e->d_pinv,     e->d_rates,
+                    e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
+                    e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
+                    e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
+                    e->d_out2,   e->d_assign};
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &pt : e->pending) {
+        e->ev_pool.push_back(pt.a);
+        e->ev_pool.push_back(pt.b);
+    }
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return SMC_OK;
+}
+
+const char *smc_engine_last_error(const smc_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int64_t smc_engine_device_bytes(const smc_engine *e) { return e ? e->device_bytes : 0; }
+
+int smc_set_alignment(smc_engine *e, const int8_t *codes)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (!codes) return fail(e, SMC_ERR_ARG, "codes is null");
+    const int T = e->T, S = e->S, Sp = e->S_pad;
+    std::vector<uint8_t> hc((size_t)T * Sp, 22);
+    for (int t = 0; t < T; ++t)
+        for (int s = 0; s < S; ++s) {
+            int c = codes[(size_t)t * S + s];
+            if (c < 0 || c > 22)
+                return fail(e, SMC_ERR_ARG, "alignment code out of range 0..22 at taxon " +
+                                                std::to_string(t) + " site " + std::to_string(s));
+            hc[(size_t)t * Sp + s] = (uint8_t)c;
+        }
+    // invariant detection, Particle.cpp:44-72 (exact code equality, gaps included: Q5)
+    std::vector<double> inv(Sp, 0.0);
+    std::vector<int> ich(Sp, 0);
+    for (int s = 0; s < S; ++s) {
+        int c1 = hc[s];
+        bool same = true;
+        for (int t = 1; t < T && same; ++t) same = hc[(size_t)t * Sp + s] == c1;
+        inv[s] = same ? 1.0 : 0.0;
+        ich[s] = (same && c1 < 20) ? c1 : 0;
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->d_codes, hc.data(), hc.size(), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(e->d_isinv, inv.data(), Sp * sizeof(double), hipMemcpyHostToDevice,
+                              e->stream));
+    HIP_TRY(e, hipMemcpyAsync(e->d_invchar, ich.data(), Sp * sizeof(int), hipMemcpyHostToDevice,
+                              e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->have_alignment = true;
+    return SMC_OK;
+}
+
+int smc_set_particles(smc_engine *e, int first, int count, const int32_t *children,
+                      const double *branch_len, const double *base, const int32_t *num_classes,
+                      const double *logits, const int32_t *assignment, const double *pinv,
+                      const double *shape)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (first < 0 || count < 1 || first + count > e->Pmax)
+        return fail(e, SMC_ERR_ARG, "particle range out of bounds");
+    if (!children || !branch_len || !base || !num_classes || !logits || !assignment || !pinv ||
+        !shape)
+        return fail(e, SMC_ERR_ARG, "null particle array");
+    const int T = e->T, Ti = T - 1, nn = e->nn, S = e->S, Sp = e->S_pad, R = e->R, K = e->Kmax;
+    const int ts = e->tile_stride;
+    std::vector<double> blen((size_t)count * nn), rates((size_t)count * R);
+    std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
+    std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
+    for (int i = 0; i < count; ++i) {
+        const int p = first + i;
+        const int32_t *ch = children + (size_t)i * Ti * 2;
+        // topology checks: every non-root node has exactly one parent
+        std::vector<int> parent(nn, -1);
+        for (int k = 0; k < Ti; ++k)
+            for (int c = 0; c < 2; ++c) {
+                int d = ch[2 * k + c];
+                if (d < 0 || d >= nn || d == T + k || parent[d] != -1)
+                    return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
+                                                    ": invalid children array");
+                parent[d] = T + k;
+            }
+        int root = -1, nroots = 0;
+        for (int n = T; n < nn; ++n)
+            if (parent[n] == -1) {
+                root = n;
+                nroots++;
+            }
+        for (int t = 0; t < T; ++t)
+            if (parent[t] == -1) nroots = -1;
+        if (nroots != 1)
+            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": not a rooted binary tree");
+        // heights (tips 0); iterate in an order where children come first
+        std::vector<int> h(nn, 0), order;
+        order.reserve(Ti);
+        {
+            std::vector<int> st{root};
+            while (!st.empty()) {
+                int n = st.back();
+                st.pop_back();
+                order.push_back(n);
+                for (int c = 0; c < 2; ++c) {
+                    int d = ch[2 * (n - T) + c];
+                    if (d >= T) st.push_back(d);
+                }
+            }
+            if ((int)order.size() != Ti)
+                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": disconnected tree");
+            for (int k = Ti - 1; k >= 0; --k) {
+                int n = order[k];
+                h[n] = 1 + std::max(h[ch[2 * (n - T)]], h[ch[2 * (n - T) + 1]]);
+            }
+        }
+        for (int k = 0; k < Ti; ++k) {
+            e->h_height[(size_t)p * Ti + k] = h[T + k];
+            e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
+            e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
+        }
+        e->h_root[p] = root;
+        roots[i] = root;
+        for (int n = 0; n < nn; ++n) {
+            double bl = branch_len[(size_t)i * nn + n];
+            blen[(size_t)i * nn + n] = (n == root) ? 0.0 : bl;
+        }
+        // rates: discretizeGamma when K_r > 1 (Particle.cpp:224-225), else 1.0
+        if (R > 1) {
+            if (!(shape[i] > 0.0))
+                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": shape must be > 0");
+            discretize_gamma(shape[i], R, &rates[(size_t)i * R]);
+        } else {
+            rates[(size_t)i] = 1.0;
+        }
+        // class-sorted slot order and tiles (each tile is one class)
+        const int Kp = num_classes[i];
+        if (Kp < 1 || Kp > K)
+            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": num_classes out of range");
+        const int32_t *as = assignment + (size_t)i * S;
+        std::vector<int> cnt(Kp + 1, 0);
+        for (int s = 0; s < S; ++s) {
+            if (as[s] < 0 || as[s] >= Kp)
+                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": assignment out of range");
+            cnt[as[s] + 1]++;
+        }
+        for (int k = 0; k < Kp; ++k) cnt[k + 1] += cnt[k];
+        std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+        for (int s = 0; s < S; ++s) perm[(size_t)i * Sp + fillp[as[s]]++] = s;
+        int nt = 0;
+        for (int k = 0; k < Kp; ++k)
+            for (int st = cnt[k]; st < cnt[k + 1]; st += kTile) {
+                tiles[(size_t)i * ts + nt++] = Tile{st, std::min(kTile, cnt[k + 1] - st), k, 0};
+            }
+        ntiles[i] = nt;
+        e->h_valid[p] = 1;
+    }
+    std::vector<int> hch((size_t)count * Ti * 2);
+    std::memcpy(hch.data(), children, hch.size() * sizeof(int));
+    hipStream_t s = e->stream;
+    const size_t P0 = first;
+    HIP_TRY(e, hipMemcpyAsync(e->d_children + P0 * Ti * 2, hch.data(), hch.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_blen + P0 * nn, blen.data(), blen.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_base + P0 * 190, base, (size_t)count * 190 * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_logits + P0 * K * 20, logits,
+                              (size_t)count * K * 20 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_nclass + P0, num_classes, (size_t)count * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_pinv + P0, pinv, (size_t)count * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_rates + P0 * R, rates.data(), rates.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_root + P0, roots.data(), roots.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_perm + P0 * Sp, perm.data(), perm.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_tiles + P0 * ts, tiles.data(), tiles.size() * sizeof(Tile),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_ntiles + P0, ntiles.data(), ntiles.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    e->topo_version++;
+    return SMC_OK;
+}
+
+int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
+                           uint8_t *nonfinite_out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (first < 0 || count < 1 || first + count > e->Pmax)
+        return fail(e, SMC_ERR_ARG, "particle range out of bounds");
+    if (!e->have_alignment) return fail(e, SMC_ERR_STATE, "no alignment uploaded");
+    for (int p = first; p < first + count; ++p)
+        if (!e->h_valid[p])
+            return fail(e, SMC_ERR_STATE, "particle slot " + std::to_string(p) + " not uploaded");
+    int rc = build_schedule(e, first, count);
+    if (rc != SMC_OK) return rc;
+    hipStream_t s = e->stream;
+    {
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first, count},
+                     s);
+    }
+    {
+        TimedLaunch tl(e, SMC_K_PMAT);
+        launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_P, e->nn, e->R,
+                             e->Kmax, first, count, e->fp32},
+                    s);
+    }
+    const int nlev = (int)e->level_off.size() - 1;
+    for (int L = 1; L < nlev; ++L) {
+        const int n_items = e->level_off[L + 1] - e->level_off[L];
+        if (n_items <= 0) continue;
+        TimedLaunch tl(e, SMC_K_PRUNE);
+        PruneArgs a{};
+        a.items = e->d_items + e->level_off[L];
+        a.n_items = n_items;
+        a.tiles = e->d_tiles;
+        a.ntiles = e->d_ntiles;
+        a.tile_stride = e->tile_stride;
+        a.children = e->d_children;
+        a.root_of = e->d_root;
+        a.perm = e->d_perm;
+        a.codes = e->d_codes;
+        a.P = e->d_P;
+        a.eig = e->d_eig;
+        a.pinv = e->d_pinv;
+        a.isinv = e->d_isinv;
+        a.invchar = e->d_invchar;
+        a.X = e->d_X;
+        a.E = e->d_E;
+        a.tile_lnl = e->d_tile_lnl;
+        a.T = e->T;
+        a.R = e->R;
+        a.S_pad = e->S_pad;
+        a.max_classes = e->Kmax;
+        a.fp32 = e->fp32;
+        launch_prune(a, s);
+    }
+    {
+        TimedLaunch tl(e, SMC_K_REDUCE);
+        launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,
+                          e->d_nonfinite, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    if (lnl_out)
+        HIP_TRY(e, hipMemcpyAsync(lnl_out, e->d_lnl + first, (size_t)count * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+    if (nonfinite_out)
+        HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
+                                  hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    return SMC_OK;
+}
+
+static int ensure_smc(smc_engine *e, int n)
+{
+    if (n <= e->smc_cap) return SMC_OK;
+    void *bufs[] = {e->d_w, e->d_l, e->d_norm, e->d_out2, e->d_assign};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    e->d_w = e->d_l = e->d_norm = e->d_out2 = nullptr;
+    e->d_assign = nullptr;
+    e->smc_cap = 0;
+    int rc = dev_alloc(e, &e->d_w, n);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_l, n);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_norm, n);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_out2, 2);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_assign, n);
+    if (rc == SMC_OK) e->smc_cap = n;
+    return rc;
+}
+
+int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,
+                          double *normalized, double *ess, int target_ess, double *current_temp,
+                          double last_temp)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (n < 1 || !raw_weights || !raw_lnl || !normalized || !ess || !current_temp)
+        return fail(e, SMC_ERR_ARG, "invalid computeNextStep arguments");
+    int rc = ensure_smc(e, n);
+    if (rc != SMC_OK) return rc;
+    hipStream_t s = e->stream;
+    HIP_TRY(e, hipMemcpyAsync(e->d_w, raw_weights, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_l, raw_lnl, n * sizeof(double), hipMemcpyHostToDevice, s));
+    {
+        TimedLaunch tl(e, SMC_K_SMC);
+        launch_next_step(n, e->d_w, e->d_l, e->d_norm, e->d_out2, target_ess, last_temp, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    double out2[2];
+    HIP_TRY(e, hipMemcpyAsync(raw_weights, e->d_w, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipMemcpyAsync(normalized, e->d_norm, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipMemcpyAsync(out2, e->d_out2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    *current_temp = out2[0];
+    *ess = out2[1];
+    return SMC_OK;
+}
+
+int smc_systematic_resample(smc_engine *e, int n, const double *normalized, double u,
+                            int32_t *assignments)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (n < 1 || !normalized || !assignments || !(u >= 0.0 && u < 1.0))
+        return fail(e, SMC_ERR_ARG, "invalid resampling arguments");
+    int rc = ensure_smc(e, n);
+    if (rc != SMC_OK) return rc;
+    hipStream_t s = e->stream;
+    HIP_TRY(e, hipMemcpyAsync(e->d_norm, normalized, n * sizeof(double), hipMemcpyHostToDevice, s));
+    {
+        TimedLaunch tl(e, SMC_K_SMC);
+        launch_resample(n, e->d_norm, u, e->d_assign, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipMemcpyAsync(assignments, e->d_assign, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    return SMC_OK;
+}
+
+int smc_discretize_gamma(double shape, int num, double *out)
+{
+    if (!(shape > 0.0) || num < 1 || !out) return SMC_ERR_ARG;
+    discretize_gamma(shape, num, out);
+    return SMC_OK;
+}
+
+void smc_lg_base_matrix(double out[190])
+{
+    if (out) std::memcpy(out, kLG190, sizeof(double) * 190);
+}
+
+int smc_engine_set_timing(smc_engine *e, int enabled)
+{
+    if (!e) return SMC_ERR_ARG;
+    e->timing = enabled != 0;
+    return SMC_OK;
+}
+
+int smc_engine_kernel_stats(smc_engine *e, int family, int64_t *launches, double *total_ms)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (family < 0 || family >= SMC_K_COUNT) return fail(e, SMC_ERR_ARG, "bad kernel family");
+    if (launches) *launches = e->launches[family];
+    if (total_ms) *total_ms = e->total_ms[family];
+    return SMC_OK;
+}
+
+int smc_engine_reset_stats(smc_engine *e)
+{
+    if (!e) return SMC_ERR_ARG;
+    for (int i = 0; i < SMC_K_COUNT; ++i) {
+        e->launches[i] = 0;
+        e->total_ms[i] = 0.0;
+    }
+    return SMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smccat {
+
+constexpr int kStates = 20;        // amino-acid states
+constexpr int kTile = 64;          // sites per pruning tile = one wavefront
+constexpr int kEigStride = 840;    // A[400] | lam[20] | B[400] | pi[20]
+constexpr int kEigLam = 400;
+constexpr int kEigB = 420;
+constexpr int kEigPi = 820;
+constexpr int kPStride = 420;      // P row-major with a 21st column: P[i][20] = sum_j P[i][j]
+
+extern const double kLG190[190];
+
+// A pruning tile: `len` consecutive (class-sorted) site slots of one class.
+struct Tile {
+    int start;
+    int len;
+    int cls;
+    int pad;
+};
+
+// One (particle, internal node) pair of a pruning level.
+struct Item {
+    int particle;
+    int node;
+};
+
+// Host-side discretizeGamma (Miscellaneous.hpp:22-37).
+void discretize_gamma(double shape, int num, double *out);
+
+// ---- kernel launchers (stream-ordered, no synchronisation) ----
+struct EigenArgs {
+    const double *base;      // [P][190]
+    const double *logits;    // [P][Kmax][20]
+    const int *num_classes;  // [P]
+    double *eig;             // [P*Kmax][840]
+    int max_classes;
+    int first;               // first particle
+    int count;               // particles
+};
+void launch_eigen(const EigenArgs &a, hipStream_t s);
+
+struct PmatArgs {
+    const double *eig;
+    const double *blen;      // [P][nn]
+    const double *rates;     // [P][R]
+    const int *num_classes;
+    void *P;                 // [P*Kmax][nn][R][420] (double or float)
+    int nn, R, max_classes, first, count;
+    bool fp32;
+};
+void launch_pmat(const PmatArgs &a, hipStream_t s);
+
+struct PruneArgs {
+    const Item *items;       // this level's items
+    int n_items;
+    const Tile *tiles;       // [P][tile_stride]
+    const int *ntiles;       // [P]
+    int tile_stride;
+    const int *children;     // [P][T-1][2]
+    const int *root_of;      // [P]
+    const int *perm;         // [P][S_pad] slot -> alignment column
+    const uint8_t *codes;    // [T][S_pad]
+    const void *P;           // [P*Kmax][nn][R][420]
+    const double *eig;       // [P*Kmax][840]
+    const double *pinv;      // [P]
+    const double *isinv;     // [S_pad]
+    const int *invchar;      // [S_pad]
+    void *X;                 // partials [P][T-1][R][20][S_pad]
+    int *E;                  // cumulative binary exponents [P][T-1][R][S_pad]
+    double *tile_lnl;        // [P][tile_stride]
+    int T, R, S_pad, max_classes;
+    bool fp32;
+};
+void launch_prune(const PruneArgs &a, hipStream_t s);
+
+void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
+                       int count, double *lnl, uint8_t *nonfinite, hipStream_t s);
+
+void launch_next_step(int n, double *w, const double *l, double *norm, double *out2,
+                      int target_ess, double last_temp, hipStream_t s);
+void launch_resample(int n, const double *norm, double u, int *assign, hipStream_t s);
+
+}  // namespace smccat

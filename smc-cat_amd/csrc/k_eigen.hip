@@ -1,0 +1,254 @@
+// Substitution-model kernels: batched eigen-decomposition of the reversible
+// 20x20 rate matrix and the P(t) back-transform.
+//
+// Reference: TransitionProbabilityClass::recomputeEigens / recomputeTransitionProbs
+// (TransitionProbabilityClass.cpp:50-95).  The reference runs a general complex
+// eigensolver on Q and inverts V by LU.  Q = R diag(pi) / mu is reversible, so
+// here the symmetric matrix S = Pi^{1/2} Q Pi^{-1/2} is diagonalised with a
+// parallel cyclic Jacobi method (one wavefront per matrix, 10 disjoint rotations
+// per round, round-robin ordering), giving Q = A diag(lam) B with
+// A = Pi^{-1/2} U and B = U^T Pi^{1/2}: all real, FP64.
+#include "internal.hpp"
+
+namespace smccat {
+
+namespace {
+
+__device__ __forceinline__ int tri_index(int i, int j)
+{
+    // 190-vector index of the unordered pair (i, j), i != j (RateMatrices.hpp:97-111)
+    return i > j ? i * (i - 1) / 2 + j : j * (j - 1) / 2 + i;
+}
+
+// Round-robin (circle method) pairing: in round r (0..18), pair k (0..9).
+__device__ __forceinline__ void rr_pair(int r, int k, int &p, int &q)
+{
+    int a, b;
+    if (k == 0) {
+        a = r;
+        b = 19;
+    } else {
+        a = (r + k) % 19;
+        b = (r - k + 19) % 19;
+    }
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+}
+
+__global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ base,
+                                                   const double *__restrict__ logits,
+                                                   const int *__restrict__ num_classes,
+                                                   double *__restrict__ eig, int max_classes,
+                                                   int first)
+{
+    __shared__ double S[20][21];
+    __shared__ double U[20][21];
+    __shared__ double pi_s[20], sp_s[20], diag_s[20];
+    __shared__ double rc[10], rs[10];
+    __shared__ int rp[10], rq[10];
+    __shared__ double red[64];
+
+    const int lane = threadIdx.x;
+    const int p = first + blockIdx.x / max_classes;
+    const int k = blockIdx.x % max_classes;
+    if (k >= num_classes[p]) return;  // whole block exits together
+    const size_t g = (size_t)p * max_classes + k;
+    const double *b = base + (size_t)p * 190;
+    const double *lg = logits + g * 20;
+
+    // pi = softmax(logits), TransitionProbabilityClass.cpp:101-109
+    if (lane < 20) pi_s[lane] = exp(lg[lane]);
+    __syncthreads();
+    double tot = 0.0;
+    for (int i = 0; i < 20; i++) tot += pi_s[i];
+    __syncthreads();
+    if (lane < 20) {
+        pi_s[lane] = pi_s[lane] / tot;
+        sp_s[lane] = sqrt(pi_s[lane]);
+    }
+    __syncthreads();
+
+    // Q_ij = exp(b_c) pi_j (TransitionProbabilityClass.cpp:52-60)
+    for (int idx = lane; idx < 400; idx += 64) {
+        int i = idx / 20, j = idx % 20;
+        S[i][j] = (i == j) ? 0.0 : exp(b[tri_index(i, j)]) * pi_s[j];
+    }
+    __syncthreads();
+    if (lane < 20) {  // Q_ii = -sum_{j != i} Q_ij (:62-68)
+        double off = 0.0;
+        for (int j = 0; j < 20; j++)
+            if (j != lane) off += S[lane][j];
+        diag_s[lane] = -off;
+    }
+    __syncthreads();
+    double mu = 0.0;  // mean rate (:70-75)
+    for (int i = 0; i < 20; i++) mu += -diag_s[i] * pi_s[i];
+    // symmetric form of Q / mu
+    double sv[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        int idx = lane + 64 * m;
+        if (idx < 400) {
+            int i = idx / 20, j = idx % 20;
+            if (i == j) {
+                sv[m] = diag_s[i] / mu;
+            } else {
+                double qij = S[i][j] / mu, qji = S[j][i] / mu;
+                sv[m] = 0.5 * (sp_s[i] * qij / sp_s[j] + sp_s[j] * qji / sp_s[i]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        int idx = lane + 64 * m;
+        if (idx < 400) {
+            int i = idx / 20, j = idx % 20;
+            S[i][j] = sv[m];
+            U[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+
+    // parallel cyclic Jacobi (Golub & Van Loan, Alg. 8.4.2 rotations; round-robin order)
+    for (int sweep = 0; sweep < 40; sweep++) {
+        double off = 0.0, all = 0.0;
+        for (int idx = lane; idx < 400; idx += 64) {
+            int i = idx / 20, j = idx % 20;
+            double v = S[i][j] * S[i][j];
+            all += v;
+            if (i != j) off += v;
+        }
+        red[lane] = off;
+        __syncthreads();
+        double off_t = 0.0;
+        for (int i = 0; i < 64; i++) off_t += red[i];
+        __syncthreads();
+        red[lane] = all;
+        __syncthreads();
+        double all_t = 0.0;
+        for (int i = 0; i < 64; i++) all_t += red[i];
+        __syncthreads();
+        if (off_t <= 1e-34 * all_t) break;
+
+        for (int r = 0; r < 19; r++) {
+            if (lane < 10) {
+                int pp, qq;
+                rr_pair(r, lane, pp, qq);
+                double apq = S[pp][qq];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    double tau = (S[qq][qq] - S[pp][pp]) / (2.0 * apq);
+                    double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = t * c;
+                }
+                rc[lane] = c;
+                rs[lane] = s;
+                rp[lane] = pp;
+                rq[lane] = qq;
+            }
+            __syncthreads();
+            for (int t = lane; t < 200; t += 64) {  // S <- S J, U <- U J
+                int kk = t / 20, i = t % 20;
+                int pp = rp[kk], qq = rq[kk];
+                double c = rc[kk], s = rs[kk];
+                double a0 = S[i][pp], a1 = S[i][qq];
+                S[i][pp] = c * a0 - s * a1;
+                S[i][qq] = s * a0 + c * a1;
+                double u0 = U[i][pp], u1 = U[i][qq];
+                U[i][pp] = c * u0 - s * u1;
+                U[i][qq] = s * u0 + c * u1;
+            }
+            __syncthreads();
+            for (int t = lane; t < 200; t += 64) {  // S <- J^T S
+                int kk = t / 20, j = t % 20;
+                int pp = rp[kk], qq = rq[kk];
+                double c = rc[kk], s = rs[kk];
+                double a0 = S[pp][j], a1 = S[qq][j];
+                S[pp][j] = c * a0 - s * a1;
+                S[qq][j] = s * a0 + c * a1;
+            }
+            __syncthreads();
+        }
+    }
+
+    double *out = eig + g * kEigStride;
+    for (int idx = lane; idx < 400; idx += 64) {
+        int i = idx / 20, kk = idx % 20;
+        out[i * 20 + kk] = U[i][kk] / sp_s[i];          // A = Pi^{-1/2} U
+        out[kEigB + kk * 20 + i] = U[i][kk] * sp_s[i];   // B = U^T Pi^{1/2}
+    }
+    if (lane < 20) {
+        out[kEigLam + lane] = S[lane][lane];
+        out[kEigPi + lane] = pi_s[lane];
+    }
+}
+
+// One thread = one whole P matrix (particle, class, node, rate).  Every lane of
+// a block shares the class, so A, lam and B are wave-uniform scalar loads.
+template <typename TP>
+__global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
+                                              const double *__restrict__ blen,
+                                              const double *__restrict__ rates,
+                                              const int *__restrict__ num_classes,
+                                              TP *__restrict__ Pout, int nn, int R,
+                                              int max_classes, int first)
+{
+    const int p = first + blockIdx.x / max_classes;
+    const int k = blockIdx.x % max_classes;
+    if (k >= num_classes[p]) return;
+    const int idx = blockIdx.y * 64 + threadIdx.x;
+    if (idx >= nn * R) return;
+    const int n = idx / R, r = idx % R;
+    const size_t g = (size_t)p * max_classes + k;
+    const double *A = eig + g * kEigStride;
+    const double *lam = A + kEigLam;
+    const double *B = A + kEigB;
+    const double t = blen[(size_t)p * nn + n];
+    const double rate = rates[(size_t)p * R + r];
+
+    double e[20];
+#pragma unroll
+    for (int kk = 0; kk < 20; kk++) e[kk] = exp(lam[kk] * t * rate);
+
+    TP *out = Pout + ((g * nn + n) * R + r) * kPStride;
+    for (int i = 0; i < 20; i++) {
+        double a[20];
+#pragma unroll
+        for (int kk = 0; kk < 20; kk++) a[kk] = A[i * 20 + kk] * e[kk];
+        double rowsum = 0.0;
+#pragma unroll
+        for (int j = 0; j < 20; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 20; kk++) acc = fma(a[kk], B[kk * 20 + j], acc);
+            out[i * 21 + j] = (TP)acc;
+            rowsum += acc;
+        }
+        out[i * 21 + 20] = (TP)rowsum;
+    }
+}
+
+}  // namespace
+
+void launch_eigen(const EigenArgs &a, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL(k_eigen_f64, dim3(a.count * a.max_classes), dim3(64), 0, s, a.base,
+                       a.logits, a.num_classes, a.eig, a.max_classes, a.first);
+}
+
+void launch_pmat(const PmatArgs &a, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    dim3 grid(a.count * a.max_classes, (a.nn * a.R + 63) / 64);
+    if (a.fp32)
+        hipLaunchKernelGGL(k_pmat<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, (float *)a.P, a.nn, a.R, a.max_classes, a.first);
+    else
+        hipLaunchKernelGGL(k_pmat<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, (double *)a.P, a.nn, a.R, a.max_classes, a.first);
+}
+
+}  // namespace smccat

@@ -1,0 +1,153 @@
+// SMC reweighting kernels.
+//
+// k_next_step restates computeNextStep (Miscellaneous.hpp:42-109): bisection on
+// the next temperature T in [T_last, 1] while high - low > 1e-5, where each probe
+// is ESS(T) = 1 / sum(w_hat^2) of the max-shifted softmax of
+// w_n + (T - T_last) lnL_n; then the weights are committed and normalised.  One
+// workgroup of 1024 lanes; every reduction is a fixed-order tree, so results
+// are deterministic.
+//
+// k_resample restates the systematic resampling walk of main.cpp:179-195.  It is
+// a sequential walk with accumulated rounding (ruler += 1/N, cumulative +=
+// w_k); one lane runs it so the assignments are bit-identical to the
+// reference's for the same normalized weights and u.
+#include "internal.hpp"
+
+namespace smccat {
+
+namespace {
+
+constexpr int kSmcThreads = 1024;
+
+__device__ double block_sum(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = kSmcThreads / 2; off > 0; off >>= 1) {
+        if (t < off) sh[t] += sh[t + off];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = kSmcThreads / 2; off > 0; off >>= 1) {
+        if (t < off) sh[t] = fmax(sh[t], sh[t + off]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ESS of the weights w + (temp - last) l (Miscellaneous.hpp:49-74).
+__device__ double ess_at(int n, const double *w, const double *l, double temp, double last,
+                         double *sh)
+{
+    double mx = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) mx = fmax(mx, w[i] + (temp - last) * l[i]);
+    mx = block_max(mx, sh);
+    double tot = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) tot += exp(w[i] + (temp - last) * l[i] - mx);
+    tot = block_sum(tot, sh);
+    double sq = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) {
+        double v = exp(w[i] + (temp - last) * l[i] - mx) / tot;
+        sq += v * v;
+    }
+    sq = block_sum(sq, sh);
+    return 1.0 / sq;
+}
+
+__global__ __launch_bounds__(kSmcThreads) void k_next_step(int n, double *__restrict__ w,
+                                                           const double *__restrict__ l,
+                                                           double *__restrict__ norm,
+                                                           double *__restrict__ out2,
+                                                           int target_ess, double last_temp)
+{
+    __shared__ double sh[kSmcThreads];
+    double low = last_temp, high = 1.0;
+    while (high - low > 1e-5) {
+        double mid = 0.5 * (low + high);
+        double e = ess_at(n, w, l, mid, last_temp, sh);
+        if (e > (double)target_ess)
+            low = mid;
+        else
+            high = mid;
+    }
+    const double temp = high;
+    double mx = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) {
+        double v = w[i] + (temp - last_temp) * l[i];
+        w[i] = v;
+        mx = fmax(mx, v);
+    }
+    mx = block_max(mx, sh);
+    double tot = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) {
+        double v = exp(w[i] - mx);
+        norm[i] = v;
+        tot += v;
+    }
+    tot = block_sum(tot, sh);
+    double sq = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSmcThreads) {
+        double v = norm[i] / tot;
+        norm[i] = v;
+        sq += v * v;
+    }
+    sq = block_sum(sq, sh);
+    if (threadIdx.x == 0) {
+        out2[0] = temp;
+        out2[1] = 1.0 / sq;
+    }
+}
+
+__global__ void k_resample(int n, const double *__restrict__ norm, double u, int *__restrict__ assign)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double cumulative = norm[0];
+    int k = 0, cnt = 0;
+    double ruler = u / (double)n;
+    const double step = 1.0 / (double)n;
+    // The walk ends after at most 2n steps unless rounding pushes the ruler
+    // past 1.0 (where the reference would not terminate); bound it anyway.
+    long guard = 4L * n + 64;
+    while (cnt < n && guard-- > 0) {
+        if (ruler <= cumulative) {
+            assign[cnt++] = k;
+            ruler += step;
+        } else {
+            if (++k >= n) {
+                k = n - 1;
+                cumulative = 1.0;
+            } else {
+                cumulative += norm[k];
+            }
+        }
+    }
+    while (cnt < n) assign[cnt++] = n - 1;
+}
+
+}  // namespace
+
+void launch_next_step(int n, double *w, const double *l, double *norm, double *out2,
+                      int target_ess, double last_temp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_next_step, dim3(1), dim3(kSmcThreads), 0, s, n, w, l, norm, out2,
+                       target_ess, last_temp);
+}
+
+void launch_resample(int n, const double *norm, double u, int *assign, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_resample, dim3(1), dim3(64), 0, s, n, norm, u, assign);
+}
+
+}  // namespace smccat

@@ -1,0 +1,68 @@
+"""ctypes binding of libsmccat_hip.so (the C-ABI declared in include/smccat_hip.h).
+
+The library is loaded from this directory only; a missing library is an error
+(there is no CPU fallback for the engine).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsmccat_hip.so")
+
+SMC_OK, SMC_ERR_ARG, SMC_ERR_HIP, SMC_ERR_NOMEM, SMC_ERR_STATE = 0, 1, 2, 3, 4
+SMC_PREC_FP64, SMC_PREC_FP32 = 0, 1
+SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC = 0, 1, 2, 3, 4
+KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc")
+
+# Every exported symbol and its (restype, argtypes).
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+_BP = ctypes.POINTER(ctypes.c_uint8)
+_CP = ctypes.POINTER(ctypes.c_int8)
+
+
+class EngineConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("taxa", ctypes.c_int32), ("sites", ctypes.c_int32),
+                ("num_rates", ctypes.c_int32), ("max_particles", ctypes.c_int32),
+                ("max_classes", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+SIGNATURES = {
+    "smc_engine_create": (_I, [ctypes.POINTER(EngineConfig), ctypes.POINTER(_P)]),
+    "smc_engine_destroy": (_I, [_P]),
+    "smc_engine_last_error": (ctypes.c_char_p, [_P]),
+    "smc_set_alignment": (_I, [_P, _CP]),
+    "smc_set_particles": (_I, [_P, _I, _I, _IP, _DP, _DP, _IP, _DP, _IP, _DP, _DP]),
+    "smc_refresh_likelihood": (_I, [_P, _I, _I, _DP, _BP]),
+    "smc_compute_next_step": (_I, [_P, _I, _DP, _DP, _DP, _DP, _I, _DP, _D]),
+    "smc_systematic_resample": (_I, [_P, _I, _DP, _D, _IP]),
+    "smc_discretize_gamma": (_I, [_D, _I, _DP]),
+    "smc_lg_base_matrix": (None, [_DP]),
+    "smc_engine_set_timing": (_I, [_P, _I]),
+    "smc_engine_kernel_stats": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _DP]),
+    "smc_engine_reset_stats": (_I, [_P]),
+    "smc_engine_device_bytes": (ctypes.c_int64, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsmccat_hip.so not built: run `python smc-cat_amd/smccat/build.py` "
+                           "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

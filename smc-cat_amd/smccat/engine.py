@@ -1,0 +1,192 @@
+"""Python view of the engine's C-ABI (numpy in, numpy out).
+
+`ParticleBatch` holds particle states in the layout smc_set_particles takes;
+`Engine` owns one libsmccat_hip engine.  Method names follow the reference's
+Particle / SMC vocabulary (refreshLikelihood, computeNextStep, ...).
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+@dataclass
+class ParticleBatch:
+    """Particle states, one row per particle (SerializedParticle fields, parsed).
+
+    children   int32 [N, T-1, 2]   children of internal node T+i (node ids as Tree.cpp)
+    branch_len f64   [N, 2T-1]     by node id (root entry ignored)
+    base       f64   [N, 190]      log-exchangeabilities
+    num_classes int32 [N]
+    logits     f64   [N, Kmax, 20]
+    assignment int32 [N, S]
+    pinv       f64   [N]
+    shape      f64   [N]
+    """
+    children: np.ndarray
+    branch_len: np.ndarray
+    base: np.ndarray
+    num_classes: np.ndarray
+    logits: np.ndarray
+    assignment: np.ndarray
+    pinv: np.ndarray
+    shape: np.ndarray
+
+    @property
+    def n(self):
+        return self.children.shape[0]
+
+    def take(self, idx):
+        idx = np.asarray(idx)
+        return ParticleBatch(*(np.ascontiguousarray(getattr(self, f)[idx]) for f in
+                               ("children", "branch_len", "base", "num_classes", "logits",
+                                "assignment", "pinv", "shape")))
+
+    def particle(self, i):
+        """Single particle as a dict (for the oracle)."""
+        k = int(self.num_classes[i])
+        return dict(children=self.children[i], branch_len=self.branch_len[i], base=self.base[i],
+                    num_classes=k, logits=self.logits[i, :k], assignment=self.assignment[i],
+                    pinv=float(self.pinv[i]), shape=float(self.shape[i]))
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Engine:
+    def __init__(self, taxa, sites, num_rates=1, max_particles=1, max_classes=1,
+                 precision="fp64", device=0):
+        self.lib = _lib.load()
+        cfg = _lib.EngineConfig(device, taxa, sites, num_rates, max_particles, max_classes,
+                                _lib.SMC_PREC_FP64 if precision == "fp64" else _lib.SMC_PREC_FP32, 0)
+        h = ctypes.c_void_p()
+        rc = self.lib.smc_engine_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise EngineError("smc_engine_create failed with status %d" % rc)
+        self.h = h
+        self.taxa, self.sites, self.num_rates = taxa, sites, num_rates
+        self.max_particles, self.max_classes = max_particles, max_classes
+        self.precision = precision
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.smc_engine_last_error(self.h)
+            raise EngineError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smc_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.smc_engine_device_bytes(self.h))
+
+    def set_alignment(self, codes):
+        codes = np.ascontiguousarray(codes, dtype=np.int8)
+        assert codes.shape == (self.taxa, self.sites)
+        self._check(self.lib.smc_set_alignment(self.h, codes.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))),
+                    "smc_set_alignment")
+
+    def set_particles(self, batch, first=0):
+        n = batch.n
+        T, S, K = self.taxa, self.sites, self.max_classes
+        ch = np.ascontiguousarray(batch.children, dtype=np.int32).reshape(n, T - 1, 2)
+        bl = np.ascontiguousarray(batch.branch_len, dtype=np.float64).reshape(n, 2 * T - 1)
+        base = np.ascontiguousarray(batch.base, dtype=np.float64).reshape(n, 190)
+        nc = np.ascontiguousarray(batch.num_classes, dtype=np.int32).reshape(n)
+        lg = np.zeros((n, K, 20))
+        kk = min(K, batch.logits.shape[1])
+        lg[:, :kk] = batch.logits[:, :kk]
+        asg = np.ascontiguousarray(batch.assignment, dtype=np.int32).reshape(n, S)
+        pinv = np.ascontiguousarray(batch.pinv, dtype=np.float64).reshape(n)
+        shape = np.ascontiguousarray(batch.shape, dtype=np.float64).reshape(n)
+        self._check(self.lib.smc_set_particles(self.h, first, n, _ip(ch), _dp(bl), _dp(base), _ip(nc),
+                                               _dp(lg), _ip(asg), _dp(pinv), _dp(shape)),
+                    "smc_set_particles")
+
+    def refresh_likelihood(self, first=0, count=None):
+        """refreshLikelihood(true) + lnLikelihood() for slots [first, first+count)."""
+        if count is None:
+            count = self.max_particles - first
+        lnl = np.empty(count)
+        nf = np.empty(count, dtype=np.uint8)
+        self._check(self.lib.smc_refresh_likelihood(self.h, first, count, _dp(lnl),
+                                                    nf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+                    "smc_refresh_likelihood")
+        return lnl, nf
+
+    def compute_next_step(self, raw_weights, raw_lnl, target_ess, last_temp):
+        """computeNextStep: returns (new_raw_weights, normalized, ess, current_temp)."""
+        w = np.array(raw_weights, dtype=np.float64)
+        l = np.ascontiguousarray(raw_lnl, dtype=np.float64)
+        norm = np.empty_like(w)
+        ess = ctypes.c_double()
+        temp = ctypes.c_double()
+        self._check(self.lib.smc_compute_next_step(self.h, len(w), _dp(w), _dp(l), _dp(norm),
+                                                   ctypes.byref(ess), int(target_ess),
+                                                   ctypes.byref(temp), float(last_temp)),
+                    "smc_compute_next_step")
+        return w, norm, ess.value, temp.value
+
+    def systematic_resample(self, normalized, u):
+        w = np.ascontiguousarray(normalized, dtype=np.float64)
+        out = np.empty(len(w), dtype=np.int32)
+        self._check(self.lib.smc_systematic_resample(self.h, len(w), _dp(w), float(u), _ip(out)),
+                    "smc_systematic_resample")
+        return out
+
+    def set_timing(self, on=True):
+        self._check(self.lib.smc_engine_set_timing(self.h, 1 if on else 0), "smc_engine_set_timing")
+
+    def reset_stats(self):
+        self._check(self.lib.smc_engine_reset_stats(self.h), "smc_engine_reset_stats")
+
+    def kernel_stats(self):
+        out = {}
+        for fam, name in enumerate(_lib.KERNEL_FAMILIES):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            self._check(self.lib.smc_engine_kernel_stats(self.h, fam, ctypes.byref(n), ctypes.byref(ms)),
+                        "smc_engine_kernel_stats")
+            out[name] = (int(n.value), float(ms.value))
+        return out
+
+
+def discretize_gamma(shape, num):
+    lib = _lib.load()
+    out = np.empty(num)
+    rc = lib.smc_discretize_gamma(float(shape), int(num), _dp(out))
+    if rc != 0:
+        raise EngineError("smc_discretize_gamma failed (%d)" % rc)
+    return out
+
+
+def lg_base_matrix():
+    lib = _lib.load()
+    out = np.empty(190)
+    lib.smc_lg_base_matrix(_dp(out))
+    return out

@@ -1,0 +1,166 @@
+"""Parity of the HIP engine (through the C-ABI) against the CPU FP64 oracle.
+
+Bar: per-particle log-likelihood within 1e-10 relative on the plain FP64 path
+(the tolerance the plain drop-in must meet); resampling assignments bit-exact;
+computeNextStep temperature within the bisection tolerance and ESS within 1e-12.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+from smccat import Engine
+from smccat import simulate as sim
+
+pytestmark = pytest.mark.gpu
+
+RTOL_FP64 = 1e-10
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _engine_lnl(codes, batch, R, Kmax, precision="fp64"):
+    T, S = codes.shape
+    with Engine(T, S, R, batch.n, Kmax, precision) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        return eng.refresh_likelihood()
+
+
+def _oracle_lnl(codes, batch, R):
+    return np.array([orc.particle_lnl(codes, batch.particle(i), R) for i in range(batch.n)])
+
+
+def _with_gaps(rng, codes, frac=0.03):
+    c = codes.copy()
+    m = rng.random(c.shape) < frac
+    c[m] = rng.choice([20, 21, 22], size=int(m.sum()))
+    c[:, 0] = 20                    # an all-gap column: invariant, character 0 (Q5)
+    return c
+
+
+def test_c1_full_config(gpu):
+    _say("case c1: 16 taxa x 200 sites, LG+G4, 64 particles")
+    codes, cfg = sim.simulate_config("c1")
+    rng = np.random.default_rng(101)
+    batch = sim.prior_particles(rng, 64, 16, 200, num_rates=4, lg=True,
+                                lg_table=sim_lg())
+    lnl, nf = _engine_lnl(codes, batch, 4, 1)
+    want = _oracle_lnl(codes, batch, 4)
+    assert not nf.any()
+    np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
+
+
+def sim_lg():
+    from smccat import lg_base_matrix
+    return lg_base_matrix()
+
+
+@pytest.mark.parametrize("taxa,sites,R,invar,seed", [(64, 300, 4, True, 1), (9, 130, 1, True, 2),
+                                                     (33, 77, 8, False, 3), (3, 1, 2, True, 4)])
+def test_gtr_gamma_invariant(gpu, taxa, sites, R, invar, seed):
+    _say("case gtr: taxa=%d sites=%d R=%d invar=%s" % (taxa, sites, R, invar))
+    rng = np.random.default_rng(seed)
+    ch, bl, root = sim.random_tree(rng, taxa)
+    codes = rng.integers(0, 20, size=(taxa, sites)).astype(np.int8)
+    if sites > 4:
+        codes[:, 1] = 5                 # an invariant column
+        codes = _with_gaps(rng, codes)
+    batch = sim.prior_particles(rng, 12, taxa, sites, num_rates=R, invar=invar)
+    lnl, nf = _engine_lnl(codes, batch, R, 1)
+    want = _oracle_lnl(codes, batch, R)
+    assert not nf.any()
+    np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
+
+
+@pytest.mark.parametrize("taxa,sites,R,krange", [(20, 400, 2, (2, 12)), (12, 150, 4, (5, 40)),
+                                                 (7, 64, 1, (64, 64))])
+def test_cat_classes_fp64(gpu, taxa, sites, R, krange):
+    _say("case cat: taxa=%d sites=%d R=%d K=%s" % (taxa, sites, R, krange))
+    rng = np.random.default_rng(taxa + sites)
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(taxa, sites)).astype(np.int8))
+    batch = sim.prior_particles(rng, 10, taxa, sites, num_rates=R, invar=True, k_range=krange)
+    lnl, nf = _engine_lnl(codes, batch, R, krange[1])
+    want = _oracle_lnl(codes, batch, R)
+    assert not nf.any()
+    np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
+
+
+def test_nonfinite_flag(gpu):
+    _say("case nonfinite: pinv = 1.5")
+    rng = np.random.default_rng(9)
+    codes = rng.integers(0, 20, size=(6, 40)).astype(np.int8)
+    batch = sim.prior_particles(rng, 3, 6, 40, num_rates=2, invar=True)
+    batch.pinv[1] = 1.5          # log of a negative site likelihood, as the reference
+    lnl, nf = _engine_lnl(codes, batch, 2, 1)
+    assert nf.tolist() == [0, 1, 0]
+    assert math.isnan(lnl[1]) and math.isnan(orc.particle_lnl(codes, batch.particle(1), 2))
+
+
+def test_partial_refresh_and_reupload(gpu):
+    _say("case partial refresh: slots [3, 7) after re-upload")
+    rng = np.random.default_rng(17)
+    T, S, R = 14, 90, 4
+    codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
+    b1 = sim.prior_particles(rng, 10, T, S, num_rates=R, invar=True, k_range=(1, 3))
+    b2 = sim.prior_particles(rng, 4, T, S, num_rates=R, invar=True, k_range=(1, 3))
+    with Engine(T, S, R, 10, 3) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(b1)
+        full, _ = eng.refresh_likelihood()
+        eng.set_particles(b2, first=3)
+        part, _ = eng.refresh_likelihood(3, 4)
+        again, _ = eng.refresh_likelihood()
+    np.testing.assert_allclose(full, _oracle_lnl(codes, b1, R), rtol=RTOL_FP64)
+    np.testing.assert_allclose(part, _oracle_lnl(codes, b2, R), rtol=RTOL_FP64)
+    np.testing.assert_allclose(again[3:7], part, rtol=0, atol=0)
+    np.testing.assert_allclose(again[:3], full[:3], rtol=0, atol=0)
+
+
+def test_c2_full_size_properties(gpu):
+    _say("case c2 full size: 64 taxa x 2000 sites, GTR+G4+I, 1024 particles")
+    codes, cfg = sim.simulate_config("c2")
+    rng = np.random.default_rng(202)
+    batch = sim.prior_particles(rng, 1024, 64, 2000, num_rates=4, invar=True)
+    lnl, nf = _engine_lnl(codes, batch, 4, 1)
+    assert not nf.any() and np.all(np.isfinite(lnl)) and np.all(lnl < 0)
+    for i in [0, 511, 1023]:
+        want = orc.particle_lnl(codes, batch.particle(i), 4)
+        assert lnl[i] == pytest.approx(want, rel=RTOL_FP64)
+
+
+def test_next_step_vs_oracle(gpu):
+    _say("case computeNextStep")
+    rng = np.random.default_rng(31)
+    with Engine(4, 8, 1, 1, 1) as eng:
+        for n, last in [(64, 0.0), (1024, 0.3), (4096, 0.95), (3, 0.0)]:
+            l = -1000.0 + 30.0 * rng.standard_normal(n)
+            w = np.log(rng.random(n) + 0.1)
+            w -= np.log(np.exp(w).sum())
+            target = int(0.6 * n)
+            gw, gn, gess, gt = eng.compute_next_step(w, l, target, last)
+            ow, on, oess, ot = orc.compute_next_step(w, l, target, last)
+            assert abs(gt - ot) <= 1e-5
+            if gt == ot:
+                assert gess == pytest.approx(oess, rel=1e-12)
+                np.testing.assert_allclose(gn, on, rtol=1e-12, atol=1e-300)
+                np.testing.assert_allclose(gw, ow, rtol=1e-15)
+
+
+def test_systematic_resample_bit_exact(gpu):
+    _say("case systematic resampling")
+    rng = np.random.default_rng(41)
+    with Engine(4, 8, 1, 1, 1) as eng:
+        for n in [1, 2, 7, 64, 1000, 4096]:
+            for _ in range(5):
+                w = rng.random(n) ** 3
+                w /= w.sum()
+                u = rng.random()
+                assert eng.systematic_resample(w, u).tolist() == orc.systematic_resample(w, u).tolist()
+        w = np.array([0.1, 0.2, 0.3, 0.4])
+        assert eng.systematic_resample(w, 0.5).tolist() == [1, 2, 3, 3]
