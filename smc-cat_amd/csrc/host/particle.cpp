@@ -1,0 +1,364 @@
+// C++ facade over the engine's C-ABI; see particle.hpp for the mapping to the
+// reference's classes.
+#include "particle.hpp"
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+
+namespace smccat {
+
+namespace {
+
+void check(smc_engine *e, int rc, const char *what)
+{
+    if (rc != SMC_OK)
+        throw std::runtime_error(std::string(what) + " failed: " +
+                                 (e ? smc_engine_last_error(e) : "status " + std::to_string(rc)));
+}
+
+}  // namespace
+
+int alignment_code(char c)
+{
+    static const char *aa = "ARNDCQEGHILKMFPSTWYV";
+    const char u = (char)std::toupper((unsigned char)c);
+    for (int i = 0; i < 20; ++i)
+        if (aa[i] == u) return i;
+    if (u == '-') return 20;
+    if (u == 'X') return 21;
+    if (u == '?') return 22;
+    return -1;
+}
+
+Alignment::Alignment(const std::string &path)
+{
+    std::ifstream file(path);
+    if (!file.is_open()) throw std::runtime_error("Unable to open alignment file: " + path);
+    std::vector<std::vector<int8_t>> seqs;
+    std::string line;
+    bool in_seq = false;
+    while (std::getline(file, line)) {
+        if (!line.empty() && line[0] == '#') continue;  // comments
+        if (!line.empty() && line[0] == '>') {
+            std::string name;
+            for (size_t i = 1; i < line.size(); ++i)
+                if (std::isprint((unsigned char)line[i])) name += line[i];
+            taxaNames.push_back(name);
+            seqs.emplace_back();
+            in_seq = true;
+            continue;
+        }
+        if (!in_seq) continue;
+        for (char c : line) {
+            int code = alignment_code(c);
+            if (code >= 0)
+                seqs.back().push_back((int8_t)code);
+            else if (std::isprint((unsigned char)c) && !std::isspace((unsigned char)c))
+                std::cerr << "Warning: Detected unknown character in fasta (we are ignoring it): "
+                          << c << std::endl;  // Alignment.cpp:62-71 (Q6)
+        }
+    }
+    if (seqs.empty()) throw std::runtime_error("empty alignment: " + path);
+    numChar = (int)seqs[0].size();
+    for (size_t i = 1; i < seqs.size(); ++i)
+        if ((int)seqs[i].size() != numChar)
+            throw std::runtime_error("Sequence 1 had length " + std::to_string(numChar) +
+                                     " but sequence " + std::to_string(i + 1) + " had length " +
+                                     std::to_string(seqs[i].size()));
+    codes.reserve(seqs.size() * numChar);
+    for (auto &s : seqs) codes.insert(codes.end(), s.begin(), s.end());
+}
+
+Alignment::Alignment(std::vector<std::string> names, std::vector<int8_t> c, int num_char)
+    : taxaNames(std::move(names)), codes(std::move(c)), numChar(num_char)
+{
+    if (codes.size() != taxaNames.size() * (size_t)numChar)
+        throw std::runtime_error("alignment size mismatch");
+}
+
+// ---------------------------------------------------------------- Newick
+TreeTopology parseNewick(const std::string &newick, const std::vector<std::string> &taxaNames)
+{
+    const int T = (int)taxaNames.size();
+    // tokenise: ( ) , : ; are single tokens, everything else accumulates (Tree.cpp:377-396)
+    std::vector<std::string> toks;
+    std::string cur;
+    for (char c : newick) {
+        if (c == '(' || c == ')' || c == ',' || c == ':' || c == ';') {
+            if (!cur.empty()) toks.push_back(cur), cur.clear();
+            toks.emplace_back(1, c);
+        } else {
+            cur += c;
+        }
+    }
+    // build with creation-order node records (Tree.cpp:79-150)
+    struct N {
+        bool tip;
+        int taxon;
+        int parent;
+        double len;
+        std::vector<int> kids;
+    };
+    std::vector<N> nodes;
+    int p = -1, root = -1;
+    bool reading_len = false;
+    for (std::string tok : toks) {
+        if (tok == "(") {
+            nodes.push_back(N{false, -1, p, 1.0, {}});
+            int id = (int)nodes.size() - 1;
+            if (p < 0) {
+                root = id;
+                nodes[id].len = 0.0;
+            } else {
+                nodes[p].kids.push_back(id);
+            }
+            p = id;
+        } else if (tok == ")" || tok == ",") {
+            if (p < 0 || nodes[p].parent < 0)
+                throw std::runtime_error("Poorly formatted Newick! Non-root node has no ancestor!");
+            p = nodes[p].parent;
+        } else if (tok == ":") {
+            reading_len = true;
+        } else if (tok == ";") {
+            if (p != root) throw std::runtime_error("Poorly formatted Newick! Did not end at root.");
+        } else {
+            if (reading_len) {
+                if (p < 0) throw std::runtime_error("Poorly formatted Newick! Length without node.");
+                nodes[p].len = std::atof(tok.c_str());
+            } else {
+                size_t a = tok.find_first_not_of(' '), b = tok.find_last_not_of(' ');
+                std::string name = (a == std::string::npos) ? "" : tok.substr(a, b - a + 1);
+                if (!name.empty()) {
+                    int taxon = -1;
+                    for (int i = 0; i < T; ++i)
+                        if (taxaNames[i] == name) taxon = i;
+                    if (taxon < 0)
+                        throw std::runtime_error("Token '" + name + "' is not in taxa names");
+                    if (p < 0) throw std::runtime_error("Poorly formatted Newick! Tip outside a clade.");
+                    nodes.push_back(N{true, taxon, p, 1.0, {}});
+                    int id = (int)nodes.size() - 1;
+                    nodes[p].kids.push_back(id);
+                    p = id;
+                }
+            }
+            reading_len = false;
+        }
+    }
+    // ids: tips = taxon index, internal nodes T.. in creation order (Tree.cpp:152-157)
+    std::vector<int> id(nodes.size(), -1);
+    int next = T, ntips = 0;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if (nodes[i].tip) {
+            id[i] = nodes[i].taxon;
+            ntips++;
+        } else {
+            id[i] = next++;
+        }
+    }
+    if (ntips != T || next != 2 * T - 1)
+        throw std::runtime_error("Newick tree is not binary over all taxa");
+    TreeTopology t;
+    t.children.assign((size_t)(T - 1) * 2, -1);
+    t.branchLen.assign(2 * T - 1, 0.0);
+    t.names = taxaNames;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        t.branchLen[id[i]] = nodes[i].len;
+        if (!nodes[i].tip) {
+            if (nodes[i].kids.size() != 2) throw std::runtime_error("Newick node is not binary");
+            t.children[(size_t)(id[i] - T) * 2] = id[nodes[i].kids[0]];
+            t.children[(size_t)(id[i] - T) * 2 + 1] = id[nodes[i].kids[1]];
+        }
+    }
+    t.root = id[root];
+    t.branchLen[t.root] = 0.0;
+    return t;
+}
+
+std::string writeNewick(const TreeTopology &t)
+{
+    const int T = (int)t.names.size();
+    std::function<void(int, std::string &)> rec = [&](int n, std::string &s) {
+        if (n >= T) {
+            s += "(";
+            rec(t.children[(size_t)(n - T) * 2], s);
+            s += ",";
+            rec(t.children[(size_t)(n - T) * 2 + 1], s);
+            s += ")";
+            if (n != t.root) s += ":" + std::to_string(t.branchLen[n]);
+        } else {
+            s += t.names[n] + ":" + std::to_string(t.branchLen[n]);
+        }
+    };
+    std::string s;
+    rec(t.root, s);
+    return s + ";";
+}
+
+// ---------------------------------------------------------------- engine
+ParticleEngine::ParticleEngine(const Alignment &a, int numRates, int maxParticles, int maxClasses,
+                               bool usingLG, int precision, int device)
+    : aln(a), rates(numRates), kmax(maxClasses), lg(usingLG)
+{
+    smc_engine_config cfg{device, a.getNumTaxa(), a.getNumChar(), numRates, maxParticles,
+                          maxClasses, precision, 0};
+    check(nullptr, smc_engine_create(&cfg, &eng), "smc_engine_create");
+    check(eng, smc_set_alignment(eng, a.data().data()), "smc_set_alignment");
+    slots.resize(maxParticles);
+    lnls.assign(maxParticles, 0.0);
+}
+
+ParticleEngine::~ParticleEngine()
+{
+    if (eng) smc_engine_destroy(eng);
+}
+
+void ParticleEngine::upload(int slot, const SerializedParticle &sp)
+{
+    const int T = aln.getNumTaxa(), S = aln.getNumChar();
+    TreeTopology t = parseNewick(sp.newick, aln.getTaxaNames());
+    const int K = (int)sp.stationaries.size();
+    if (K < 1 || K > kmax) throw std::runtime_error("number of classes out of range");
+    if ((int)sp.assignments.size() != S) throw std::runtime_error("assignments size != sites");
+    std::vector<double> logits((size_t)kmax * 20, 0.0);
+    for (int k = 0; k < K; ++k)
+        for (int i = 0; i < 20; ++i) logits[(size_t)k * 20 + i] = sp.stationaries[k][i];
+    int32_t nc = K;
+    double pinv = sp.pInvar, shape = sp.shape;
+    (void)T;
+    check(eng, smc_set_particles(eng, slot, 1, t.children.data(), t.branchLen.data(),
+                                 sp.baseMatrix.data(), &nc, logits.data(), sp.assignments.data(),
+                                 &pinv, &shape),
+          "smc_set_particles");
+    slots.at(slot).sp = sp;
+    slots.at(slot).tree = std::move(t);
+    slots.at(slot).valid = true;
+}
+
+void ParticleEngine::refreshAll(int n, std::vector<double> &out)
+{
+    out.resize(n);
+    check(eng, smc_refresh_likelihood(eng, 0, n, out.data(), nullptr), "smc_refresh_likelihood");
+    for (int i = 0; i < n; ++i) lnls[i] = out[i];
+}
+
+// ---------------------------------------------------------------- Particle
+void Particle::copyFromSerialized(SerializedParticle &sp)
+{
+    e.upload(slot, sp);
+    refreshLikelihood(true);  // Particle.cpp:276
+}
+
+void Particle::writeToSerialized(SerializedParticle &sp)
+{
+    const auto &s = e.slots.at(slot);
+    sp = s.sp;
+    sp.newick = writeNewick(s.tree);
+    // post-order branch lengths without the root (Particle.cpp:310-314)
+    sp.branchLengths.clear();
+    const int T = (int)s.tree.names.size();
+    std::function<void(int)> post = [&](int n) {
+        if (n >= T) {
+            post(s.tree.children[(size_t)(n - T) * 2]);
+            post(s.tree.children[(size_t)(n - T) * 2 + 1]);
+        }
+        if (n != s.tree.root) sp.branchLengths.push_back(s.tree.branchLen[n]);
+    };
+    post(s.tree.root);
+    sp.categorySize.assign(s.sp.stationaries.size(), 0);
+    for (int a : s.sp.assignments) sp.categorySize[a]++;
+}
+
+void Particle::refreshLikelihood(bool)
+{
+    // The engine keeps no stale state between uploads: every refresh is the
+    // forced form (incremental dirty-node updates are not exposed here yet).
+    double l = 0.0;
+    uint8_t nf = 0;
+    check(e.handle(), smc_refresh_likelihood(e.handle(), slot, 1, &l, &nf), "smc_refresh_likelihood");
+    currentLnLikelihood = l;
+    e.lnls[slot] = l;
+}
+
+double Particle::lnPrior()
+{
+    const auto &s = e.slots.at(slot);
+    const int T = (int)s.tree.names.size(), nn = 2 * T - 1;
+    auto log_gamma_pdf = [](double k, double theta, double x) {
+        return (k - 1.0) * std::log(x) - x / theta - std::lgamma(k) - k * std::log(theta);
+    };
+    auto log_norm = [](double z) { return -0.5 * z * z - 0.5 * std::log(2.0 * M_PI); };
+    double len = 0.0;
+    for (int n = 0; n < nn; ++n)
+        if (n != s.tree.root) len += s.tree.branchLen[n];
+    double lnP = log_gamma_pdf((double)(nn - 1), 0.1, len);  // tree prior, rate 10
+    if (e.numRates() > 1) lnP += log_gamma_pdf(3.0, 2.0, s.sp.shape);
+    // GTR prior: LG particles carry the fixed table and add nothing (Particle.cpp:500-505)
+    if (!e.usingLG())
+        for (double b : s.sp.baseMatrix) lnP += log_norm(b);
+    const int K = (int)s.sp.stationaries.size();
+    lnP += std::log(dppAlpha) * K;
+    std::vector<int> sizes(K, 0);
+    for (int a : s.sp.assignments) sizes[a]++;
+    for (int k = 0; k < K; ++k) {
+        for (double z : s.sp.stationaries[k]) lnP += log_norm(z);
+        lnP += std::lgamma((double)sizes[k]);
+    }
+    lnP += std::lgamma(dppAlpha) - std::lgamma(dppAlpha + (double)s.sp.assignments.size());
+    return lnP;
+}
+
+int Particle::getNumCategories() const { return (int)e.slots.at(slot).sp.stationaries.size(); }
+double Particle::getPInvar() const { return e.slots.at(slot).sp.pInvar; }
+double Particle::getShape() const { return e.slots.at(slot).sp.shape; }
+std::vector<int> Particle::getAssignments() const { return e.slots.at(slot).sp.assignments; }
+std::vector<std::array<double, 20>> Particle::getCategories() const
+{
+    return e.slots.at(slot).sp.stationaries;
+}
+std::string Particle::getNewick() const { return writeNewick(e.slots.at(slot).tree); }
+std::vector<double> Particle::getRates() const
+{
+    std::vector<double> r;
+    discretizeGamma(r, getShape(), e.numRates());
+    if (e.numRates() == 1) r.assign(1, 1.0);
+    return r;
+}
+
+// ---------------------------------------------------------------- free functions
+void computeNextStep(ParticleEngine &engine, std::vector<double> &rawWeights,
+                     std::vector<double> &rawLogLikelihoods, std::vector<double> &normalizedWeights,
+                     double &ESS, const int &targetESS, double &currentTemp, const double &lastTemp)
+{
+    const int n = (int)rawWeights.size();
+    normalizedWeights.resize(n);
+    check(engine.handle(),
+          smc_compute_next_step(engine.handle(), n, rawWeights.data(), rawLogLikelihoods.data(),
+                                normalizedWeights.data(), &ESS, targetESS, &currentTemp, lastTemp),
+          "smc_compute_next_step");
+}
+
+std::vector<int> systematicResample(ParticleEngine &engine, const std::vector<double> &normalized,
+                                    double u)
+{
+    std::vector<int> out(normalized.size());
+    check(engine.handle(),
+          smc_systematic_resample(engine.handle(), (int)normalized.size(), normalized.data(), u,
+                                  out.data()),
+          "smc_systematic_resample");
+    return out;
+}
+
+void discretizeGamma(std::vector<double> &outVec, double shape, int num)
+{
+    outVec.assign(num, 1.0);
+    if (num > 1 && smc_discretize_gamma(shape, num, outVec.data()) != SMC_OK)
+        throw std::runtime_error("discretizeGamma: invalid arguments");
+}
+
+}  // namespace smccat

@@ -16,6 +16,7 @@ CSRC = os.path.join(ROOT_PKG, "csrc")
 BUILD = os.path.join(ROOT_PKG, "_build")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libsmccat_hip.so")
+FACADE_EXE = os.path.join(PKG, "facade_selftest")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -56,10 +57,10 @@ def build_library(verbose=False):
 def build_facade_test(verbose=False):
     """C++ facade (reference Particle API) + its self-test executable."""
     os.makedirs(BUILD, exist_ok=True)
-    exe = os.path.join(BUILD, "facade_selftest")
+    exe = FACADE_EXE
     cmd = ["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-I", os.path.join(CSRC, "host"),
            os.path.join(CSRC, "host", "particle.cpp"), os.path.join(CSRC, "host", "facade_selftest.cpp"),
-           "-L", PKG, "-lsmccat_hip", "-Wl,-rpath," + PKG, "-o", exe]
+           "-L", PKG, "-lsmccat_hip", "-Wl,-rpath,$ORIGIN", "-o", exe]
     _run(cmd)
     if verbose:
         print("built", exe)

