@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--variant", default="mfma", choices=["mfma", "valu"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -79,6 +80,7 @@ def main():
     batch = sim.prior_particles(rng, N, T, S, num_rates=R, invar=cfg["invar"], lg=cfg["lg"],
                                 lg_table=lg_base_matrix() if cfg["lg"] else None)
     eng = Engine(T, S, R, N, 1, args.precision)
+    eng.set_prune_variant(args.variant)
     eng.set_alignment(codes)
     eng.set_particles(batch)
     target = int(0.6 * N)
@@ -128,10 +130,12 @@ def main():
                                (args.config, T, S, "LG" if cfg["lg"] else "GTR", R,
                                 "+I" if cfg["invar"] else "", N),
                    "taxa": T, "sites": S, "rates": R, "particles": N,
-                   "precision": args.precision, "updates_per_step": updates},
+                   "precision": args.precision, "prune_kernel": args.variant,
+                   "updates_per_step": updates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_prune<%s>" % ("double" if word == 8 else "float"),
+                     "kernel": "%s<%s>" % ("k_prune_mfma" if args.variant == "mfma" else "k_prune",
+                                           "double" if word == 8 else "float"),
                      "launches_per_step": launches_per_sweep,
                      "avg_launch_ms": ms_prune / max(n_prune, 1),
                      "algorithmic_bytes_per_step": bytes_sweep},

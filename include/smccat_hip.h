@@ -115,6 +115,12 @@ int smc_engine_set_timing(smc_engine *e, int enabled);
 int smc_engine_kernel_stats(smc_engine *e, int family, int64_t *launches, double *total_ms);
 int smc_engine_reset_stats(smc_engine *e);
 
+/* Kernel variant of a family (tuning; no reference counterpart).  For
+ * SMC_K_PRUNE: SMC_PRUNE_MFMA (matrix cores, the default) or SMC_PRUNE_VALU
+ * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding. */
+enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1 };
+int smc_engine_set_variant(smc_engine *e, int family, int variant);
+
 /* Bytes of device memory held by the engine. */
 int64_t smc_engine_device_bytes(const smc_engine *e);
 

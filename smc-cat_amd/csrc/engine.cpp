@@ -39,6 +39,7 @@ struct smc_engine {
     std::string err;
     int64_t device_bytes = 0;
     bool have_alignment = false;
+    int prune_variant = SMC_PRUNE_MFMA;
 
     // device buffers
     uint8_t *d_codes = nullptr;
@@ -168,7 +169,9 @@ void collect_timings(smc_engine *e)
     e->pending.clear();
 }
 
-// Build the per-level item lists for slots [first, first+count).
+// Build the per-level item lists for slots [first, first+count): level L holds
+// the non-root internal nodes of height L (tips have height 0); all roots
+// follow in one final list, so the level kernels never carry the root mixture.
 int build_schedule(smc_engine *e, int first, int count)
 {
     if (e->sched_first == first && e->sched_count == count && e->sched_version == e->topo_version)
@@ -176,18 +179,25 @@ int build_schedule(smc_engine *e, int first, int count)
     const int Ti = e->T - 1;
     int maxh = 0;
     for (int p = first; p < first + count; ++p)
-        for (int i = 0; i < Ti; ++i) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
+        for (int i = 0; i < Ti; ++i)
+            if (e->T + i != e->h_root[p]) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
     std::vector<int> cnt(maxh + 2, 0);
     for (int p = first; p < first + count; ++p)
-        for (int i = 0; i < Ti; ++i) cnt[e->h_height[(size_t)p * Ti + i]]++;
+        for (int i = 0; i < Ti; ++i)
+            if (e->T + i != e->h_root[p]) cnt[e->h_height[(size_t)p * Ti + i]]++;
     e->level_off.assign(maxh + 2, 0);
     for (int L = 1; L <= maxh; ++L) e->level_off[L + 1] = e->level_off[L] + cnt[L];
     std::vector<Item> items((size_t)count * Ti);
     std::vector<int> fill(e->level_off);
+    const int root_off = e->level_off[maxh + 1];
     for (int p = first; p < first + count; ++p)
         for (int i = 0; i < Ti; ++i) {
-            int h = e->h_height[(size_t)p * Ti + i];
-            items[fill[h]++] = Item{p, e->T + i};
+            if (e->T + i == e->h_root[p]) {
+                items[root_off + (p - first)] = Item{p, e->T + i};
+            } else {
+                int h = e->h_height[(size_t)p * Ti + i];
+                items[fill[h]++] = Item{p, e->T + i};
+            }
         }
     HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
                               hipMemcpyHostToDevice, e->stream));
@@ -499,13 +509,17 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                     s);
     }
     const int nlev = (int)e->level_off.size() - 1;
-    for (int L = 1; L < nlev; ++L) {
-        const int n_items = e->level_off[L + 1] - e->level_off[L];
+    for (int L = 1; L <= nlev; ++L) {
+        // levels 1..nlev-1, then the roots of every particle in the range
+        const bool roots = L == nlev;
+        const int off = e->level_off[roots ? nlev : L];
+        const int n_items = roots ? count : e->level_off[L + 1] - e->level_off[L];
         if (n_items <= 0) continue;
         TimedLaunch tl(e, SMC_K_PRUNE);
         PruneArgs a{};
-        a.items = e->d_items + e->level_off[L];
+        a.items = e->d_items + off;
         a.n_items = n_items;
+        a.roots = roots;
         a.tiles = e->d_tiles;
         a.ntiles = e->d_ntiles;
         a.tile_stride = e->tile_stride;
@@ -526,7 +540,10 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
         a.S_pad = e->S_pad;
         a.max_classes = e->Kmax;
         a.fp32 = e->fp32;
-        launch_prune(a, s);
+        if (e->prune_variant == SMC_PRUNE_VALU)
+            launch_prune(a, s);
+        else
+            launch_prune_mfma(a, s);
     }
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
@@ -622,6 +639,15 @@ int smc_discretize_gamma(double shape, int num, double *out)
 void smc_lg_base_matrix(double out[190])
 {
     if (out) std::memcpy(out, kLG190, sizeof(double) * 190);
+}
+
+int smc_engine_set_variant(smc_engine *e, int family, int variant)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (family != SMC_K_PRUNE || (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU))
+        return fail(e, SMC_ERR_ARG, "unknown kernel variant");
+    e->prune_variant = variant;
+    return SMC_OK;
 }
 
 int smc_engine_set_timing(smc_engine *e, int enabled)

@@ -76,8 +76,10 @@ struct PruneArgs {
     double *tile_lnl;        // [P][tile_stride]
     int T, R, S_pad, max_classes;
     bool fp32;
+    bool roots;              // items are tree roots: fuse the root mixture, store nothing
 };
-void launch_prune(const PruneArgs &a, hipStream_t s);
+void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
+void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores
 
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s);

@@ -39,7 +39,7 @@ struct PruneTypes<float> {
     using acc = float;
 };
 
-// y = P x for one child.  Tips: x is one-hot (code < 20) or all ones, so
+// y = P x for the first child.  Tips: x is one-hot (code < 20) or all ones, so
 // P x is column `code` of P or its row sums (the stored 21st column).
 template <typename TX>
 __device__ __forceinline__ void child_vector(int child, int T, int r, int s, int orig,
@@ -71,8 +71,40 @@ __device__ __forceinline__ void child_vector(int child, int T, int r, int s, int
     }
 }
 
+// y *= P x for the second child, row by row, so only y and x are live.
 template <typename TX>
-__global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
+__device__ __forceinline__ void child_multiply(int child, int T, int r, int s, int orig,
+                                               const TX *__restrict__ Pc,
+                                               const uint8_t *__restrict__ codes, int S_pad,
+                                               const TX *__restrict__ Xp,
+                                               const int *__restrict__ Ep, int R, TX y[20],
+                                               int &e)
+{
+    if (child < T) {
+        int code = codes[(size_t)child * S_pad + orig];
+        int col = code < 20 ? code : 20;
+#pragma unroll
+        for (int i = 0; i < 20; i++) y[i] *= Pc[i * 21 + col];
+        e = 0;
+    } else {
+        const size_t cl = (size_t)(child - T);
+        const TX *x = Xp + ((cl * R + r) * 20) * (size_t)S_pad + s;
+        TX xv[20];
+#pragma unroll
+        for (int j = 0; j < 20; j++) xv[j] = x[(size_t)j * S_pad];
+        e = Ep[(cl * R + r) * (size_t)S_pad + s];
+#pragma unroll
+        for (int i = 0; i < 20; i++) {
+            TX acc = 0;
+#pragma unroll
+            for (int j = 0; j < 20; j++) acc = fma(Pc[i * 21 + j], xv[j], acc);
+            y[i] *= acc;
+        }
+    }
+}
+
+template <typename TX, bool IS_ROOT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_prune(const Item *__restrict__ items,
                                                const Tile *__restrict__ tiles,
                                                const int *__restrict__ ntiles, int tile_stride,
                                                const int *__restrict__ children,
@@ -86,14 +118,19 @@ __global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
                                                const int *__restrict__ invchar,
                                                TX *__restrict__ X, int *__restrict__ E,
                                                double *__restrict__ tile_lnl, int T, int R,
-                                               int S_pad, int max_classes)
+                                               int S_pad, int max_classes, int blocks_per_item)
 {
-    const Item it = items[blockIdx.x];
+    // 1-D grid: consecutive blocks walk the tiles of one item, 4 tiles (one per
+    // wavefront) per block, so concurrently running waves stream contiguous
+    // stretches of each partial row.
+    const int item = blockIdx.x / blocks_per_item;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile = (blockIdx.x % blocks_per_item) * 4 + wave;
+    const Item it = items[item];
     const int p = it.particle, n = it.node;
-    const int tile = blockIdx.y;
-    if (tile >= ntiles[p]) return;
+    if (tile >= ntiles[p]) return;  // whole wavefront exits; no block barrier follows
     const Tile tl = tiles[(size_t)p * tile_stride + tile];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const bool active = lane < tl.len;
     const int s = tl.start + (active ? lane : 0);
     const int orig = perm[(size_t)p * S_pad + s];
@@ -102,7 +139,8 @@ __global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
     const int nl = n - T;
     const int c0 = children[((size_t)p * (T - 1) + nl) * 2];
     const int c1 = children[((size_t)p * (T - 1) + nl) * 2 + 1];
-    const bool is_root = root_of[p] == n;
+    constexpr bool is_root = IS_ROOT;
+    (void)root_of;
 
     const size_t node_elems = (size_t)R * 20 * S_pad;
     TX *Xp = X + (size_t)p * (T - 1) * node_elems;
@@ -112,18 +150,15 @@ __global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
 
     double lse_m = -INFINITY, lse_s = 0.0;
     for (int r = 0; r < R; r++) {
-        TX y0[20], y1[20];
+        TX y0[20];
         int e0, e1;
         child_vector<TX>(c0, T, r, s, orig, Pg + ((size_t)c0 * R + r) * kPStride, codes, S_pad,
                          Xp, Ep, R, y0, e0);
-        child_vector<TX>(c1, T, r, s, orig, Pg + ((size_t)c1 * R + r) * kPStride, codes, S_pad,
-                         Xp, Ep, R, y1, e1);
+        child_multiply<TX>(c1, T, r, s, orig, Pg + ((size_t)c1 * R + r) * kPStride, codes, S_pad,
+                           Xp, Ep, R, y0, e1);
         TX mx = 0;
 #pragma unroll
-        for (int i = 0; i < 20; i++) {
-            y0[i] = y0[i] * y1[i];
-            mx = fmax(mx, y0[i]);
-        }
+        for (int i = 0; i < 20; i++) mx = fmax(mx, y0[i]);
         int ex;
         (void)frexp(mx, &ex);
         const bool ok = (mx > (TX)0) && isfinite(mx);
@@ -132,7 +167,7 @@ __global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
 #pragma unroll
         for (int i = 0; i < 20; i++) y0[i] = ok ? (TX)ldexp(y0[i], -ex) : nanv;
         const int etot = e0 + e1 + (ok ? ex : 0);
-        if (!is_root) {
+        if constexpr (!is_root) {
             if (active) {
                 TX *xo = Xp + (((size_t)nl * R + r) * 20) * S_pad + s;
 #pragma unroll
@@ -153,7 +188,7 @@ __global__ __launch_bounds__(64) void k_prune(const Item *__restrict__ items,
             }
         }
     }
-    if (is_root) {
+    if constexpr (is_root) {
         double lf = 0.0;
         if (active) {
             double lsite = lse_m + log(lse_s);
@@ -187,17 +222,26 @@ __global__ void k_reduce_lnl(const double *__restrict__ tile_lnl, const int *__r
 void launch_prune(const PruneArgs &a, hipStream_t s)
 {
     if (a.n_items <= 0) return;
-    dim3 grid(a.n_items, a.tile_stride);
-    if (a.fp32)
-        hipLaunchKernelGGL(k_prune<float>, grid, dim3(64), 0, s, a.items, a.tiles, a.ntiles,
-                           a.tile_stride, a.children, a.root_of, a.perm, a.codes,
-                           (const float *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (float *)a.X,
-                           a.E, a.tile_lnl, a.T, a.R, a.S_pad, a.max_classes);
-    else
-        hipLaunchKernelGGL(k_prune<double>, grid, dim3(64), 0, s, a.items, a.tiles, a.ntiles,
-                           a.tile_stride, a.children, a.root_of, a.perm, a.codes,
-                           (const double *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (double *)a.X,
-                           a.E, a.tile_lnl, a.T, a.R, a.S_pad, a.max_classes);
+    const int bpi = (a.tile_stride + 3) / 4;
+    dim3 grid((unsigned)((size_t)a.n_items * bpi));
+#define SMC_PRUNE_ARGS                                                                          \
+    a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.root_of, a.perm, a.codes,         \
+        (const TXa *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, \
+        a.R, a.S_pad, a.max_classes, bpi
+    if (a.fp32) {
+        using TXa = float;
+        if (a.roots)
+            hipLaunchKernelGGL((k_prune<float, true>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
+        else
+            hipLaunchKernelGGL((k_prune<float, false>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
+    } else {
+        using TXa = double;
+        if (a.roots)
+            hipLaunchKernelGGL((k_prune<double, true>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
+        else
+            hipLaunchKernelGGL((k_prune<double, false>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
+    }
+#undef SMC_PRUNE_ARGS
 }
 
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,

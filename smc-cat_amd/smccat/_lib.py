@@ -13,6 +13,7 @@ SMC_OK, SMC_ERR_ARG, SMC_ERR_HIP, SMC_ERR_NOMEM, SMC_ERR_STATE = 0, 1, 2, 3, 4
 SMC_PREC_FP64, SMC_PREC_FP32 = 0, 1
 SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC = 0, 1, 2, 3, 4
 KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc")
+SMC_PRUNE_MFMA, SMC_PRUNE_VALU = 0, 1
 
 # Every exported symbol and its (restype, argtypes).
 _P = ctypes.c_void_p
@@ -46,6 +47,7 @@ SIGNATURES = {
     "smc_engine_kernel_stats": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _DP]),
     "smc_engine_reset_stats": (_I, [_P]),
     "smc_engine_device_bytes": (ctypes.c_int64, [_P]),
+    "smc_engine_set_variant": (_I, [_P, _I, _I]),
 }
 
 _lib = None

@@ -25,7 +25,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-I", INCLUDE, "-I", CSRC,
           "-Wall", "-Wno-unused-result"]
 
-SOURCES = ["engine.cpp", "gamma.cpp", "lg_table.cpp", "k_eigen.hip", "k_prune.hip", "k_smc.hip"]
+SOURCES = ["engine.cpp", "gamma.cpp", "lg_table.cpp", "k_eigen.hip", "k_prune.hip", "k_prune_mfma.hip",
+           "k_smc.hip"]
 
 
 def _run(cmd):

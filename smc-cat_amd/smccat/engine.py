@@ -159,6 +159,11 @@ class Engine:
                     "smc_systematic_resample")
         return out
 
+    def set_prune_variant(self, variant):
+        """'mfma' (matrix cores, default) or 'valu'."""
+        v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
+        self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
+
     def set_timing(self, on=True):
         self._check(self.lib.smc_engine_set_timing(self.h, 1 if on else 0), "smc_engine_set_timing")
 

@@ -24,9 +24,14 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _engine_lnl(codes, batch, R, Kmax, precision="fp64"):
+VARIANTS = ["mfma", "valu"]
+RTOL_FP32 = 1e-5   # fast path: FP32 partials and P, FP64 eigen/back-transform and reductions
+
+
+def _engine_lnl(codes, batch, R, Kmax, precision="fp64", variant="mfma"):
     T, S = codes.shape
     with Engine(T, S, R, batch.n, Kmax, precision) as eng:
+        eng.set_prune_variant(variant)
         eng.set_alignment(codes)
         eng.set_particles(batch)
         return eng.refresh_likelihood()
@@ -44,13 +49,14 @@ def _with_gaps(rng, codes, frac=0.03):
     return c
 
 
-def test_c1_full_config(gpu):
-    _say("case c1: 16 taxa x 200 sites, LG+G4, 64 particles")
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_c1_full_config(gpu, variant):
+    _say("case c1: 16 taxa x 200 sites, LG+G4, 64 particles, %s" % variant)
     codes, cfg = sim.simulate_config("c1")
     rng = np.random.default_rng(101)
     batch = sim.prior_particles(rng, 64, 16, 200, num_rates=4, lg=True,
                                 lg_table=sim_lg())
-    lnl, nf = _engine_lnl(codes, batch, 4, 1)
+    lnl, nf = _engine_lnl(codes, batch, 4, 1, variant=variant)
     want = _oracle_lnl(codes, batch, 4)
     assert not nf.any()
     np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
@@ -61,10 +67,11 @@ def sim_lg():
     return lg_base_matrix()
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("taxa,sites,R,invar,seed", [(64, 300, 4, True, 1), (9, 130, 1, True, 2),
                                                      (33, 77, 8, False, 3), (3, 1, 2, True, 4)])
-def test_gtr_gamma_invariant(gpu, taxa, sites, R, invar, seed):
-    _say("case gtr: taxa=%d sites=%d R=%d invar=%s" % (taxa, sites, R, invar))
+def test_gtr_gamma_invariant(gpu, taxa, sites, R, invar, seed, variant):
+    _say("case gtr: taxa=%d sites=%d R=%d invar=%s %s" % (taxa, sites, R, invar, variant))
     rng = np.random.default_rng(seed)
     ch, bl, root = sim.random_tree(rng, taxa)
     codes = rng.integers(0, 20, size=(taxa, sites)).astype(np.int8)
@@ -72,32 +79,34 @@ def test_gtr_gamma_invariant(gpu, taxa, sites, R, invar, seed):
         codes[:, 1] = 5                 # an invariant column
         codes = _with_gaps(rng, codes)
     batch = sim.prior_particles(rng, 12, taxa, sites, num_rates=R, invar=invar)
-    lnl, nf = _engine_lnl(codes, batch, R, 1)
+    lnl, nf = _engine_lnl(codes, batch, R, 1, variant=variant)
     want = _oracle_lnl(codes, batch, R)
     assert not nf.any()
     np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("taxa,sites,R,krange", [(20, 400, 2, (2, 12)), (12, 150, 4, (5, 40)),
                                                  (7, 64, 1, (64, 64))])
-def test_cat_classes_fp64(gpu, taxa, sites, R, krange):
-    _say("case cat: taxa=%d sites=%d R=%d K=%s" % (taxa, sites, R, krange))
+def test_cat_classes_fp64(gpu, taxa, sites, R, krange, variant):
+    _say("case cat: taxa=%d sites=%d R=%d K=%s %s" % (taxa, sites, R, krange, variant))
     rng = np.random.default_rng(taxa + sites)
     codes = _with_gaps(rng, rng.integers(0, 20, size=(taxa, sites)).astype(np.int8))
     batch = sim.prior_particles(rng, 10, taxa, sites, num_rates=R, invar=True, k_range=krange)
-    lnl, nf = _engine_lnl(codes, batch, R, krange[1])
+    lnl, nf = _engine_lnl(codes, batch, R, krange[1], variant=variant)
     want = _oracle_lnl(codes, batch, R)
     assert not nf.any()
     np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
 
 
-def test_nonfinite_flag(gpu):
-    _say("case nonfinite: pinv = 1.5")
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_nonfinite_flag(gpu, variant):
+    _say("case nonfinite: pinv = 1.5, %s" % variant)
     rng = np.random.default_rng(9)
     codes = rng.integers(0, 20, size=(6, 40)).astype(np.int8)
     batch = sim.prior_particles(rng, 3, 6, 40, num_rates=2, invar=True)
     batch.pinv[1] = 1.5          # log of a negative site likelihood, as the reference
-    lnl, nf = _engine_lnl(codes, batch, 2, 1)
+    lnl, nf = _engine_lnl(codes, batch, 2, 1, variant=variant)
     assert nf.tolist() == [0, 1, 0]
     assert math.isnan(lnl[1]) and math.isnan(orc.particle_lnl(codes, batch.particle(1), 2))
 
@@ -120,6 +129,22 @@ def test_partial_refresh_and_reupload(gpu):
     np.testing.assert_allclose(part, _oracle_lnl(codes, b2, R), rtol=RTOL_FP64)
     np.testing.assert_allclose(again[3:7], part, rtol=0, atol=0)
     np.testing.assert_allclose(again[:3], full[:3], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("taxa,sites,R,krange", [(16, 200, 4, (1, 1)), (30, 500, 4, (3, 20)),
+                                                 (64, 300, 2, (1, 1))])
+def test_fp32_fast_path_error(gpu, taxa, sites, R, krange, variant):
+    _say("case fp32: taxa=%d sites=%d R=%d K=%s %s" % (taxa, sites, R, krange, variant))
+    rng = np.random.default_rng(taxa * 7 + sites)
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(taxa, sites)).astype(np.int8))
+    batch = sim.prior_particles(rng, 8, taxa, sites, num_rates=R, invar=True, k_range=krange)
+    lnl32, nf = _engine_lnl(codes, batch, R, krange[1], "fp32", variant)
+    want = _oracle_lnl(codes, batch, R)
+    assert not nf.any()
+    rel = np.abs(lnl32 - want) / np.abs(want)
+    _say("fp32 max rel err %.3e" % rel.max())
+    assert rel.max() <= RTOL_FP32
 
 
 def test_c2_full_size_properties(gpu):
