@@ -78,7 +78,8 @@ struct smc_engine {
     // level schedule cache
     int sched_first = -1, sched_count = -1;
     uint64_t sched_version = 0, topo_version = 1;
-    std::vector<int> level_off;  // items of level L: [level_off[L], level_off[L+1])
+    std::vector<int> bucket_off;  // items of (level L, kind k): [off[3L+k], off[3L+k+1])
+    int max_level = 0;
 
     // timing
     bool timing = false;
@@ -169,34 +170,42 @@ void collect_timings(smc_engine *e)
     e->pending.clear();
 }
 
-// Build the per-level item lists for slots [first, first+count): level L holds
-// the non-root internal nodes of height L (tips have height 0); all roots
+// Build the pruning schedule for slots [first, first+count): bucket (L, k)
+// holds the non-root internal nodes of height L (tips have height 0) whose
+// children are of kind k (0 both tips, 1 one tip, 2 both internal); buckets
+// are stored level-major, so a level is also one contiguous run.  All roots
 // follow in one final list, so the level kernels never carry the root mixture.
 int build_schedule(smc_engine *e, int first, int count)
 {
     if (e->sched_first == first && e->sched_count == count && e->sched_version == e->topo_version)
         return SMC_OK;
-    const int Ti = e->T - 1;
+    const int T = e->T, Ti = T - 1;
+    auto kind_of = [&](int p, int i) {
+        const int *ch = &e->h_children[((size_t)p * Ti + i) * 2];
+        return (ch[0] >= T) + (ch[1] >= T);
+    };
     int maxh = 0;
     for (int p = first; p < first + count; ++p)
         for (int i = 0; i < Ti; ++i)
-            if (e->T + i != e->h_root[p]) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
-    std::vector<int> cnt(maxh + 2, 0);
+            if (T + i != e->h_root[p]) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
+    const int nb = (maxh + 1) * 3;
+    std::vector<int> cnt(nb + 1, 0);
     for (int p = first; p < first + count; ++p)
         for (int i = 0; i < Ti; ++i)
-            if (e->T + i != e->h_root[p]) cnt[e->h_height[(size_t)p * Ti + i]]++;
-    e->level_off.assign(maxh + 2, 0);
-    for (int L = 1; L <= maxh; ++L) e->level_off[L + 1] = e->level_off[L] + cnt[L];
+            if (T + i != e->h_root[p]) cnt[e->h_height[(size_t)p * Ti + i] * 3 + kind_of(p, i)]++;
+    e->bucket_off.assign(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) e->bucket_off[b + 1] = e->bucket_off[b] + cnt[b];
+    e->max_level = maxh;
     std::vector<Item> items((size_t)count * Ti);
-    std::vector<int> fill(e->level_off);
-    const int root_off = e->level_off[maxh + 1];
+    std::vector<int> fill(e->bucket_off);
+    const int root_off = e->bucket_off[nb];
     for (int p = first; p < first + count; ++p)
         for (int i = 0; i < Ti; ++i) {
-            if (e->T + i == e->h_root[p]) {
-                items[root_off + (p - first)] = Item{p, e->T + i};
+            if (T + i == e->h_root[p]) {
+                items[root_off + (p - first)] = Item{p, T + i};
             } else {
-                int h = e->h_height[(size_t)p * Ti + i];
-                items[fill[h]++] = Item{p, e->T + i};
+                const int b = e->h_height[(size_t)p * Ti + i] * 3 + kind_of(p, i);
+                items[fill[b]++] = Item{p, T + i};
             }
         }
     HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
@@ -508,43 +517,48 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                              e->Kmax, first, count, e->fp32},
                     s);
     }
-    const int nlev = (int)e->level_off.size() - 1;
-    for (int L = 1; L <= nlev; ++L) {
-        // levels 1..nlev-1, then the roots of every particle in the range
-        const bool roots = L == nlev;
-        const int off = e->level_off[roots ? nlev : L];
-        const int n_items = roots ? count : e->level_off[L + 1] - e->level_off[L];
-        if (n_items <= 0) continue;
+    PruneArgs a{};
+    a.tiles = e->d_tiles;
+    a.ntiles = e->d_ntiles;
+    a.tile_stride = e->tile_stride;
+    a.children = e->d_children;
+    a.root_of = e->d_root;
+    a.perm = e->d_perm;
+    a.codes = e->d_codes;
+    a.P = e->d_P;
+    a.eig = e->d_eig;
+    a.pinv = e->d_pinv;
+    a.isinv = e->d_isinv;
+    a.invchar = e->d_invchar;
+    a.X = e->d_X;
+    a.E = e->d_E;
+    a.tile_lnl = e->d_tile_lnl;
+    a.T = e->T;
+    a.R = e->R;
+    a.S_pad = e->S_pad;
+    a.max_classes = e->Kmax;
+    a.fp32 = e->fp32;
+    auto run = [&](int off, int n, int kind, bool roots) {
+        if (n <= 0) return;
         TimedLaunch tl(e, SMC_K_PRUNE);
-        PruneArgs a{};
         a.items = e->d_items + off;
-        a.n_items = n_items;
+        a.n_items = n;
+        a.kind = kind;
         a.roots = roots;
-        a.tiles = e->d_tiles;
-        a.ntiles = e->d_ntiles;
-        a.tile_stride = e->tile_stride;
-        a.children = e->d_children;
-        a.root_of = e->d_root;
-        a.perm = e->d_perm;
-        a.codes = e->d_codes;
-        a.P = e->d_P;
-        a.eig = e->d_eig;
-        a.pinv = e->d_pinv;
-        a.isinv = e->d_isinv;
-        a.invchar = e->d_invchar;
-        a.X = e->d_X;
-        a.E = e->d_E;
-        a.tile_lnl = e->d_tile_lnl;
-        a.T = e->T;
-        a.R = e->R;
-        a.S_pad = e->S_pad;
-        a.max_classes = e->Kmax;
-        a.fp32 = e->fp32;
         if (e->prune_variant == SMC_PRUNE_VALU)
             launch_prune(a, s);
         else
             launch_prune_mfma(a, s);
+    };
+    const std::vector<int> &bo = e->bucket_off;
+    for (int L = 1; L <= e->max_level; ++L) {
+        if (e->prune_variant == SMC_PRUNE_VALU) {
+            run(bo[3 * L], bo[3 * L + 3] - bo[3 * L], 3, false);
+        } else {
+            for (int k = 0; k < 3; ++k) run(bo[3 * L + k], bo[3 * L + k + 1] - bo[3 * L + k], k, false);
+        }
     }
+    run(bo[3 * (e->max_level + 1)], count, 3, true);
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
         launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,

@@ -77,6 +77,8 @@ struct PruneArgs {
     int T, R, S_pad, max_classes;
     bool fp32;
     bool roots;              // items are tree roots: fuse the root mixture, store nothing
+    int kind;                // child kinds of every item: 0 tip-tip, 1 tip-internal,
+                             // 2 internal-internal, 3 mixed (decided per item)
 };
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores

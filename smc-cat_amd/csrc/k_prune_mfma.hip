@@ -79,17 +79,28 @@ __device__ __forceinline__ void child_block(bool tip, const TX (&a)[2][5], const
     }
 }
 
-template <typename TX, bool IS_ROOT>
+// Child kinds of a pruning item: both tips, one tip (c0 after a swap) and one
+// internal node, both internal; KIND_ANY decides at run time (used for roots).
+constexpr int KIND_TT = 0, KIND_TI = 1, KIND_II = 2, KIND_ANY = 3;
+
+template <int KIND>
+struct KindTraits {
+    // 16-site column blocks processed together (independent MFMA chains for ILP)
+    static constexpr int U = KIND == KIND_TT ? 4 : (KIND == KIND_TI ? 2 : 1);
+};
+
+template <typename TX, int KIND, bool IS_ROOT>
 __global__ __launch_bounds__(256) void k_prune_mfma(
     const Item *__restrict__ items, const Tile *__restrict__ tiles, const int *__restrict__ ntiles,
-    int tile_stride, const int *__restrict__ children, const int *__restrict__ root_of,
-    const int *__restrict__ perm, const uint8_t *__restrict__ codes, const TX *__restrict__ Pm,
-    const double *__restrict__ eig, const double *__restrict__ pinv,
-    const double *__restrict__ isinv, const int *__restrict__ invchar, TX *__restrict__ X,
-    int *__restrict__ E, double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes,
-    int blocks_per_item)
+    int tile_stride, const int *__restrict__ children, const int *__restrict__ perm,
+    const uint8_t *__restrict__ codes, const TX *__restrict__ Pm, const double *__restrict__ eig,
+    const double *__restrict__ pinv, const double *__restrict__ isinv,
+    const int *__restrict__ invchar, TX *__restrict__ X, int *__restrict__ E,
+    double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes, int blocks_per_item)
 {
     using V4 = typename Mfma<TX>::v4;
+    constexpr int U = KindTraits<KIND>::U;
+    static_assert(!IS_ROOT || U == 1, "root accumulators are indexed per column block");
     const int item = blockIdx.x / blocks_per_item;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int tile = (blockIdx.x % blocks_per_item) * 4 + wave;
@@ -101,11 +112,25 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
     const int c16 = lane & 15, g4 = lane >> 4;
     const size_t g = (size_t)p * max_classes + tl.cls;
     const int nn = 2 * T - 1, nl = n - T;
-    const int c0 = children[((size_t)p * (T - 1) + nl) * 2];
-    const int c1 = children[((size_t)p * (T - 1) + nl) * 2 + 1];
-    const bool tip0 = c0 < T, tip1 = c1 < T;
-    constexpr bool is_root = IS_ROOT;
-    (void)root_of;
+    int c0 = children[((size_t)p * (T - 1) + nl) * 2];
+    int c1 = children[((size_t)p * (T - 1) + nl) * 2 + 1];
+    bool tip0, tip1;
+    if constexpr (KIND == KIND_TT) {
+        tip0 = tip1 = true;
+    } else if constexpr (KIND == KIND_TI) {
+        if (c0 >= T) {
+            const int t = c0;
+            c0 = c1;
+            c1 = t;
+        }
+        tip0 = true;
+        tip1 = false;
+    } else if constexpr (KIND == KIND_II) {
+        tip0 = tip1 = false;
+    } else {
+        tip0 = c0 < T;
+        tip1 = c1 < T;
+    }
     const size_t node_elems = (size_t)R * 20 * S_pad;
     TX *Xp = X + (size_t)p * (T - 1) * node_elems;
     int *Ep = E + (size_t)p * (T - 1) * R * S_pad;
@@ -139,93 +164,102 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
         const int *E1 = Ep + ((size_t)(tip1 ? 0 : c1 - T) * R + r) * S_pad;
 
 #pragma unroll 1
-        for (int sb = 0; sb < nsb; sb++) {
-            const int o = 16 * sb + c16;
-            const bool valid = o < tl.len;
-            const int slot = tl.start + (valid ? o : 0);
-            const int orig = perm[(size_t)p * S_pad + slot];
-            int col0 = 20, col1 = 20;
-            if (tip0) {
-                const int code = codes[(size_t)c0 * S_pad + orig];
-                col0 = code < 20 ? code : 20;
-            }
-            if (tip1) {
-                const int code = codes[(size_t)c1 * S_pad + orig];
-                col1 = code < 20 ? code : 20;
-            }
-            V4 y[2], w[2];
-            child_block<TX>(tip0, a0, P0, X0 + slot, S_pad, col0, valid, g4, y);
-            child_block<TX>(tip1, a1, P1, X1 + slot, S_pad, col1, valid, g4, w);
-            const int e = (tip0 ? 0 : E0[slot]) + (tip1 ? 0 : E1[slot]);
-            // z = y * w and the per-site max over the 20 real rows (4 lane groups x 8 values)
-            TX m = 0;
+        for (int sb0 = 0; sb0 < nsb; sb0 += U) {
+            V4 y[U][2], w[U][2];
+            int slot[U], etot[U];
+            bool valid[U];
 #pragma unroll
-            for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    y[rb][v] *= w[rb][v];
-                    m = fmax(m, y[rb][v]);
+            for (int u = 0; u < U; u++) {
+                const int o = 16 * (sb0 + u) + c16;
+                valid[u] = o < tl.len;
+                slot[u] = tl.start + (valid[u] ? o : 0);
+                const int orig = perm[(size_t)p * S_pad + slot[u]];
+                int col0 = 20, col1 = 20;
+                if (tip0) {
+                    const int code = codes[(size_t)c0 * S_pad + orig];
+                    col0 = code < 20 ? code : 20;
                 }
-            m = fmax(m, __shfl_xor(m, 16, 64));
-            m = fmax(m, __shfl_xor(m, 32, 64));
-            int ex;
-            (void)frexp(m, &ex);
-            const bool ok = (m > (TX)0) && isfinite(m);
-            const TX nanv = (TX)NAN;
+                if (tip1) {
+                    const int code = codes[(size_t)c1 * S_pad + orig];
+                    col1 = code < 20 ? code : 20;
+                }
+                child_block<TX>(tip0, a0, P0, X0 + slot[u], S_pad, col0, valid[u], g4, y[u]);
+                child_block<TX>(tip1, a1, P1, X1 + slot[u], S_pad, col1, valid[u], g4, w[u]);
+                etot[u] = (tip0 ? 0 : E0[slot[u]]) + (tip1 ? 0 : E1[slot[u]]);
+            }
 #pragma unroll
-            for (int rb = 0; rb < 2; rb++)
+            for (int u = 0; u < U; u++) {
+                // z = y * w and the per-site max over the 20 real rows (4 lane groups x 8 values)
+                TX m = 0;
 #pragma unroll
-                for (int v = 0; v < 4; v++) y[rb][v] = ok ? (TX)ldexp(y[rb][v], -ex) : nanv;
-            const int etot = e + (ok ? ex : 0);
-            if constexpr (!is_root) {
-                if (valid) {
-                    TX *xo = Xp + ((size_t)nl * R + r) * 20 * S_pad + slot;
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        y[u][rb][v] *= w[u][rb][v];
+                        m = fmax(m, y[u][rb][v]);
+                    }
+                m = fmax(m, __shfl_xor(m, 16, 64));
+                m = fmax(m, __shfl_xor(m, 32, 64));
+                int ex;
+                (void)frexp(m, &ex);
+                const bool ok = (m > (TX)0) && isfinite(m);
+                const TX nanv = (TX)NAN;
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        y[u][rb][v] = ok ? (TX)ldexp(y[u][rb][v], -ex) : nanv;
+                etot[u] += ok ? ex : 0;
+                if constexpr (!IS_ROOT) {
+                    if (valid[u]) {
+                        TX *xo = Xp + ((size_t)nl * R + r) * 20 * S_pad + slot[u];
+#pragma unroll
+                        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int row = Mfma<TX>::crow(rb, g4, v);
+                                if (row < 20) xo[(size_t)row * S_pad] = y[u][rb][v];
+                            }
+                        if (g4 == 0) Ep[((size_t)nl * R + r) * S_pad + slot[u]] = etot[u];
+                    }
+                } else {
+                    double dot = 0.0;
 #pragma unroll
                     for (int rb = 0; rb < 2; rb++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
                             const int row = Mfma<TX>::crow(rb, g4, v);
-                            if (row < 20) xo[(size_t)row * S_pad] = y[rb][v];
+                            if (row < 20) dot = fma(pi[row], (double)y[u][rb][v], dot);
                         }
-                    if (g4 == 0) Ep[((size_t)nl * R + r) * S_pad + slot] = etot;
+                    dot += __shfl_xor(dot, 16, 64);
+                    dot += __shfl_xor(dot, 32, 64);
+                    const double l = log(dot) + (double)etot[u] * kLn2m;
+                    // the column block is wave-uniform and < 4: select its accumulator
+                    // without dynamic register indexing
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (k == sb0 + u) {
+                            if (l > lse_m[k]) {
+                                lse_s[k] = lse_s[k] * exp(lse_m[k] - l) + 1.0;
+                                lse_m[k] = l;
+                            } else {
+                                lse_s[k] += exp(l - lse_m[k]);
+                            }
+                        }
                 }
-            } else {
-                double dot = 0.0;
-#pragma unroll
-                for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int row = Mfma<TX>::crow(rb, g4, v);
-                        if (row < 20) dot = fma(pi[row], (double)y[rb][v], dot);
-                    }
-                dot += __shfl_xor(dot, 16, 64);
-                dot += __shfl_xor(dot, 32, 64);
-                const double l = log(dot) + (double)etot * kLn2m;
-                // sb is wave-uniform and < 4: select the accumulator without dynamic indexing
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (k == sb) {
-                        if (l > lse_m[k]) {
-                            lse_s[k] = lse_s[k] * exp(lse_m[k] - l) + 1.0;
-                            lse_m[k] = l;
-                        } else {
-                            lse_s[k] += exp(l - lse_m[k]);
-                        }
-                    }
             }
         }
     }
-    if constexpr (is_root) {
+    if constexpr (IS_ROOT) {
         double lf = 0.0;
         const double pv = pinv[p];
-        const double *pic = pi;
 #pragma unroll
         for (int sb = 0; sb < 4; sb++) {
             const int o = 16 * sb + c16;
             if (g4 == 0 && o < tl.len) {
                 const int orig = perm[(size_t)p * S_pad + tl.start + o];
                 const double lsite = lse_m[sb] + log(lse_s[sb]);
-                const double fin = exp(lsite) * (1.0 - pv) + pv * pic[invchar[orig]] * isinv[orig];
+                const double fin = exp(lsite) * (1.0 - pv) + pv * pi[invchar[orig]] * isinv[orig];
                 lf += log(fin);
             }
         }
@@ -242,23 +276,34 @@ void launch_prune_mfma(const PruneArgs &a, hipStream_t s)
     if (a.n_items <= 0) return;
     const int bpi = (a.tile_stride + 3) / 4;
     dim3 grid((unsigned)((size_t)a.n_items * bpi));
-#define SMC_PRUNE_ARGS                                                                          \
-    a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.root_of, a.perm, a.codes,         \
-        (const TXa *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, \
-        a.R, a.S_pad, a.max_classes, bpi
-    if (a.fp32) {
-        using TXa = float;
-        if (a.roots)
-            hipLaunchKernelGGL((k_prune_mfma<float, true>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
-        else
-            hipLaunchKernelGGL((k_prune_mfma<float, false>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
-    } else {
-        using TXa = double;
-        if (a.roots)
-            hipLaunchKernelGGL((k_prune_mfma<double, true>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
-        else
-            hipLaunchKernelGGL((k_prune_mfma<double, false>), grid, dim3(256), 0, s, SMC_PRUNE_ARGS);
-    }
+#define SMC_PRUNE_ARGS                                                                            \
+    a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, (const TXa *)a.P,     \
+        a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, a.R, a.S_pad,        \
+        a.max_classes, bpi
+#define SMC_PRUNE_LAUNCH(TXT)                                                                     \
+    do {                                                                                          \
+        using TXa = TXT;                                                                          \
+        if (a.roots)                                                                              \
+            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_ANY, true>), grid, dim3(256), 0, s,        \
+                               SMC_PRUNE_ARGS);                                                   \
+        else if (a.kind == KIND_TT)                                                               \
+            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_TT, false>), grid, dim3(256), 0, s,        \
+                               SMC_PRUNE_ARGS);                                                   \
+        else if (a.kind == KIND_TI)                                                               \
+            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_TI, false>), grid, dim3(256), 0, s,        \
+                               SMC_PRUNE_ARGS);                                                   \
+        else if (a.kind == KIND_II)                                                               \
+            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_II, false>), grid, dim3(256), 0, s,        \
+                               SMC_PRUNE_ARGS);                                                   \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_ANY, false>), grid, dim3(256), 0, s,       \
+                               SMC_PRUNE_ARGS);                                                   \
+    } while (0)
+    if (a.fp32)
+        SMC_PRUNE_LAUNCH(float);
+    else
+        SMC_PRUNE_LAUNCH(double);
+#undef SMC_PRUNE_LAUNCH
 #undef SMC_PRUNE_ARGS
 }
 
