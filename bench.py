@@ -58,12 +58,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--partial-slots", type=int, default=0,
+                    help="particles whose partials are resident at once (0: all, or a "
+                         "size that fits HBM for c3/c4)")
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--variant", default="mfma", choices=["mfma", "valu"])
+    ap.add_argument("--mixture", default="", choices=["", "reference", "logspace"],
+                    help="root mixture: the reference formula (default for c1/c2) or its "
+                         "log-space form (default for c3/c4, where exp(logSiteLike) underflows)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--fp32-error", type=int, default=64,
+                    help="fp32 runs: particles re-evaluated in FP64 for the error report")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -76,11 +84,21 @@ def main():
     codes, cfg = sim.simulate_config(args.config)
     T, S, R = cfg["taxa"], cfg["sites"], cfg["rates"]
     N = args.particles or cfg["particles"]
+    kr = cfg["k_range"]
     rng = np.random.default_rng(1)
     batch = sim.prior_particles(rng, N, T, S, num_rates=R, invar=cfg["invar"], lg=cfg["lg"],
-                                lg_table=lg_base_matrix() if cfg["lg"] else None)
-    eng = Engine(T, S, R, N, 1, args.precision)
+                                lg_table=lg_base_matrix() if cfg["lg"] else None, k_range=kr)
+    word = 8 if args.precision == "fp64" else 4
+    slots = args.partial_slots
+    if slots == 0:
+        # partials of one wave within ~160 GB of HBM
+        per_particle = (T - 1) * R * S * (20 * word + 4)
+        slots = int(min(N, max(1, 160e9 // per_particle)))
+    eng = Engine(T, S, R, N, kr[1], args.precision, partial_slots=slots,
+                 pmat_classes=int(slots * (kr[0] + kr[1]) // 2 * 1.25) if kr[1] > 1 else 0)
     eng.set_prune_variant(args.variant)
+    mixture = args.mixture or ("logspace" if args.config in ("c3", "c4") else "reference")
+    eng.set_mixture(mixture)
     eng.set_alignment(codes)
     eng.set_particles(batch)
     target = int(0.6 * N)
@@ -105,7 +123,6 @@ def main():
 
     updates = N * (T - 1) * S * R
     value = updates * args.steps / dt
-    word = 8 if args.precision == "fp64" else 4
     bytes_sweep = N * S * R * algorithmic_bytes_per_column(T, word)
     n_prune, ms_prune = stats["prune"]
     launches_per_sweep = n_prune / args.steps
@@ -124,13 +141,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": "%s: %d taxa x %d sites, %s+G%d%s, %d particles; step = full "
+        "config": {"workload": "%s: %d taxa x %d sites, %s+G%d%s%s, %d particles; step = full "
                                "refreshLikelihood(true) of every particle + computeNextStep + "
                                "systematic resampling" %
                                (args.config, T, S, "LG" if cfg["lg"] else "GTR", R,
-                                "+I" if cfg["invar"] else "", N),
-                   "taxa": T, "sites": S, "rates": R, "particles": N,
-                   "precision": args.precision, "prune_kernel": args.variant,
+                                "+I" if cfg["invar"] else "",
+                                " CAT K~U{%d..%d}" % kr if kr[1] > 1 else "", N),
+                   "taxa": T, "sites": S, "rates": R, "particles": N, "partial_slots": slots,
+                   "precision": args.precision, "prune_kernel": args.variant, "mixture": mixture,
                    "updates_per_step": updates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -141,14 +159,32 @@ def main():
                      "algorithmic_bytes_per_step": bytes_sweep},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items()},
         "check": {"lnl_finite": bool(np.all(np.isfinite(lnl)) and not nf.any()),
-                  "lnl_mean": float(np.mean(lnl)), "ess": ess, "temp": temp},
+                  "nonfinite_particles": int(np.sum(~np.isfinite(lnl))),
+                  "lnl_mean_finite": float(np.mean(lnl[np.isfinite(lnl)])) if np.isfinite(lnl).any()
+                  else None, "ess": ess, "temp": temp},
     }
     eng.close()
+    if args.precision == "fp32" and args.fp32_error:
+        # per-particle error of the fast path against the plain FP64 engine (which
+        # matches the FP64 oracle within 1e-10) on a sample of the same particles
+        idx = np.arange(0, N, max(1, N // args.fp32_error))[: args.fp32_error]
+        sub = batch.take(idx)
+        e64 = Engine(T, S, R, len(idx), kr[1], "fp64",
+                     partial_slots=min(len(idx), max(1, slots // 2)))
+        e64.set_mixture(mixture)
+        e64.set_alignment(codes)
+        e64.set_particles(sub)
+        ref, _ = e64.refresh_likelihood()
+        e64.close()
+        rel = np.abs(lnl[idx] - ref) / np.abs(ref)
+        out["fp32_error_vs_fp64"] = {"particles": int(len(idx)), "max_rel": float(rel.max()),
+                                     "median_rel": float(np.median(rel)),
+                                     "max_abs": float(np.abs(lnl[idx] - ref).max())}
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         sample = sim.prior_particles(np.random.default_rng(2), 64, T, S, num_rates=R,
                                      invar=cfg["invar"], lg=cfg["lg"],
-                                     lg_table=lg_base_matrix() if cfg["lg"] else None)
+                                     lg_table=lg_base_matrix() if cfg["lg"] else None, k_range=kr)
         out["cpu_baseline"] = cpu_baseline(codes, sample, R, threads, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None

@@ -53,12 +53,16 @@ typedef struct {
     int32_t max_particles; /* resident particle slots                                */
     int32_t max_classes;   /* CAT classes per particle (1 without CAT)               */
     int32_t precision;     /* SMC_PREC_FP64 (plain) or SMC_PREC_FP32 (fast path)     */
-    int32_t reserved;      /* must be 0                                              */
+    int32_t partial_slots; /* particles whose partials are resident at once; a sweep
+                              over more runs in waves (0: max_particles)             */
+    int64_t pmat_classes;  /* classes whose P(t) are resident at once
+                              (0: partial_slots * max_classes)                       */
 } smc_engine_config;
 
 /* Engine lifetime.  Replaces the per-thread Particle construction of
  * main.cpp:112-115 / Particle::Particle (Particle.cpp:21-40): all device
- * buffers for max_particles resident particles are allocated here. */
+ * buffers are allocated here -- state for max_particles particles, partial
+ * likelihoods for partial_slots of them, P(t) for pmat_classes classes. */
 int smc_engine_create(const smc_engine_config *cfg, smc_engine **out);
 int smc_engine_destroy(smc_engine *e);
 const char *smc_engine_last_error(const smc_engine *e);
@@ -120,6 +124,15 @@ int smc_engine_reset_stats(smc_engine *e);
  * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding. */
 enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1 };
 int smc_engine_set_variant(smc_engine *e, int family, int variant);
+
+/* Site mixture at the root.  SMC_MIX_REFERENCE (default) is the reference's
+ * formula exactly (Particle.cpp:653-670): log(exp(l)(1-p) + p pi[c] isInv), which
+ * underflows to -inf below l = -745 and is NaN when a rate term is -inf, as in
+ * the reference.  SMC_MIX_LOGSPACE computes the same value in log space: equal
+ * to rounding wherever the reference formula is finite, finite where it
+ * underflows.  No reference counterpart. */
+enum { SMC_MIX_REFERENCE = 0, SMC_MIX_LOGSPACE = 1 };
+int smc_engine_set_mixture(smc_engine *e, int mode);
 
 /* Bytes of device memory held by the engine. */
 int64_t smc_engine_device_bytes(const smc_engine *e);

@@ -36,6 +36,8 @@ def load():
         lib.orc_particle_lnl.argtypes = [ctypes.POINTER(OrcParticle)]
         lib.orc_particle_lnl_sites.restype = ctypes.c_double
         lib.orc_particle_lnl_sites.argtypes = [ctypes.POINTER(OrcParticle), _DP]
+        lib.orc_particle_lnl_sites_mode.restype = ctypes.c_double
+        lib.orc_particle_lnl_sites_mode.argtypes = [ctypes.POINTER(OrcParticle), _DP, ctypes.c_int]
         lib.orc_lnl_batch.restype = ctypes.c_int
         lib.orc_lnl_batch.argtypes = [ctypes.POINTER(OrcParticle), ctypes.c_int, _DP, ctypes.c_int]
         lib.orc_ln_prior.restype = ctypes.c_double
@@ -107,6 +109,13 @@ def particle_site_lnl(codes, p, num_rates, rates=None):
     op = make_particle(codes, p, num_rates, rates)
     out = np.empty(codes.shape[1])
     tot = load().orc_particle_lnl_sites(ctypes.byref(op), _d(out))
+    return tot, out
+
+
+def particle_lnl_mode(codes, p, num_rates, logspace, rates=None):
+    op = make_particle(codes, p, num_rates, rates)
+    out = np.empty(codes.shape[1])
+    tot = load().orc_particle_lnl_sites_mode(ctypes.byref(op), _d(out), int(logspace))
     return tot, out
 
 

@@ -273,7 +273,7 @@ typedef struct {
     double A[400], B[400], lam[20], pi[20];
 } orc_class_eig;
 
-double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl)
+double orc_particle_lnl_sites_mode(const orc_particle *p, double *site_lnl, int logspace)
 {
     const int T = p->taxa, S = p->sites, R = p->num_rates, K = p->num_classes;
     const int nn = 2 * T - 1;
@@ -374,11 +374,22 @@ double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl)
                 logs[r] = l;
                 if (l > mx) mx = l;
             }
-            double se = 0.0;
-            for (int r = 0; r < R; r++) se += exp(logs[r] - mx);
-            double lsite = mx + log(se);
-            double fin = exp(lsite) * inv_scaler + p->pinv * pi[inv_char[c]] * is_inv[c];
-            double lf = log(fin);
+            double se = 0.0, lf;
+            if (!logspace) {
+                for (int r = 0; r < R; r++) se += exp(logs[r] - mx);
+                double lsite = mx + log(se);
+                double fin = exp(lsite) * inv_scaler + p->pinv * pi[inv_char[c]] * is_inv[c];
+                lf = log(fin);
+            } else {
+                /* the same quantity evaluated in log space: -inf rate terms drop out and
+                 * log(e^l (1-p) + p pi isInv) = logaddexp(l + log(1-p), log(p pi isInv)) */
+                for (int r = 0; r < R; r++)
+                    if (logs[r] != -INFINITY) se += exp(logs[r] - mx);
+                double a = mx + log(se) + log1p(-p->pinv);
+                double b = is_inv[c] != 0.0 ? log(p->pinv * pi[inv_char[c]] * is_inv[c]) : -INFINITY;
+                double hi = a > b ? a : b, lo = a > b ? b : a;
+                lf = (hi == -INFINITY) ? -INFINITY : hi + log1p(exp(lo - hi));
+            }
             if (site_lnl) site_lnl[c] = lf;
             lnl += lf;
         }
@@ -389,7 +400,12 @@ double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl)
     return lnl;
 }
 
-double orc_particle_lnl(const orc_particle *p) { return orc_particle_lnl_sites(p, NULL); }
+double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl)
+{
+    return orc_particle_lnl_sites_mode(p, site_lnl, 0);
+}
+
+double orc_particle_lnl(const orc_particle *p) { return orc_particle_lnl_sites_mode(p, NULL, 0); }
 
 /* main.cpp:295: one heavy particle per OpenMP thread, dynamic schedule. */
 int orc_lnl_batch(const orc_particle *ps, int n, double *lnl_out, int threads)

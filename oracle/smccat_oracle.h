@@ -73,6 +73,10 @@ double orc_particle_lnl(const orc_particle *p);
  * (the log(finalSiteLike) of Particle.cpp:667-670). */
 double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl);
 
+/* logspace = 1: the same mixture evaluated in log space (finite where
+ * exp(logSiteLike) of Particle.cpp:667 underflows); logspace = 0 is the above. */
+double orc_particle_lnl_sites_mode(const orc_particle *p, double *site_lnl, int logspace);
+
 /* Batch of particles, one particle per OpenMP thread (main.cpp:295 pattern).
  * threads<=0 uses the OpenMP default.  Returns the thread count used. */
 int orc_lnl_batch(const orc_particle *ps, int n, double *lnl_out, int threads);

@@ -1,16 +1,20 @@
 // C-ABI of the particle-likelihood engine: device memory, level scheduling of
 // the per-particle trees and the launch sequence of one likelihood sweep.
 //
-// Device layout (all resident in HBM, sized at create time for max_particles):
+// Device layout (all resident in HBM, sized at create time):
 //   codes    uint8  [T][S_pad]                 alignment, original column order
 //   perm     int32  [P][S_pad]                 slot -> column, class-sorted per particle
 //   tiles    Tile   [P][tile_stride]           <= 64 slots of one class each
 //   eig      f64    [P*Kmax][840]              A | lam | B | pi per class
-//   Pmat     TX     [P*Kmax][2T-1][K_r][20x21] transition matrices (+ row sums)
-//   X        TX     [P][T-1][K_r][20][S_pad]   internal-node partials, state-major
-//   E        int32  [P][T-1][K_r][S_pad]       cumulative binary exponents
+//   Pmat     TX     [C][2T-1][K_r][20x21]      transition matrices (+ row sums) of the
+//                                              C = pmat_classes classes of one wave
+//   X        TX     [W][T-1][K_r][20][S_pad]   internal-node partials, state-major, for the
+//                                              W = partial_slots particles of one wave
+//   E        int32  [W][T-1][K_r][S_pad]       cumulative binary exponents
 //   tile_lnl f64    [P][tile_stride]           per-tile site-lnL sums
-// TX is double on the plain path and float on the fast path.
+// P = max_particles particle states stay resident; a sweep over more particles than W
+// (or with more classes than C) runs in waves.  TX is double on the plain path and
+// float on the fast path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +37,8 @@ struct PendingTiming {
 struct smc_engine {
     smc_engine_config cfg{};
     int T = 0, S = 0, S_pad = 0, R = 0, nn = 0, Pmax = 0, Kmax = 0, tile_stride = 0;
+    int Wmax = 0;          // partial slots (particles per wave)
+    int64_t Cmax = 0;      // classes per wave in Pmat
     bool fp32 = false;
     size_t tx_size = 8;
     hipStream_t stream = nullptr;
@@ -40,6 +46,7 @@ struct smc_engine {
     int64_t device_bytes = 0;
     bool have_alignment = false;
     int prune_variant = SMC_PRUNE_MFMA;
+    int mix_mode = SMC_MIX_REFERENCE;
 
     // device buffers
     uint8_t *d_codes = nullptr;
@@ -74,12 +81,19 @@ struct smc_engine {
     std::vector<int> h_root;      // [P]
     std::vector<int> h_height;    // [P][T-1]
     std::vector<int> h_valid;     // [P] particle uploaded
+    std::vector<int> h_nclass;    // [P] classes per particle
+    int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
 
-    // level schedule cache
+    // pruning schedule cache: waves of particles, each with its level/kind buckets
+    struct Wave {
+        int first, count;
+        int root_off;                 // roots of the wave's particles
+        int max_level;
+        std::vector<int> bucket_off;  // items of (level L, kind k): [off[3L+k], off[3L+k+1])
+    };
     int sched_first = -1, sched_count = -1;
     uint64_t sched_version = 0, topo_version = 1;
-    std::vector<int> bucket_off;  // items of (level L, kind k): [off[3L+k], off[3L+k+1])
-    int max_level = 0;
+    std::vector<Wave> waves;
 
     // timing
     bool timing = false;
@@ -170,11 +184,13 @@ void collect_timings(smc_engine *e)
     e->pending.clear();
 }
 
-// Build the pruning schedule for slots [first, first+count): bucket (L, k)
-// holds the non-root internal nodes of height L (tips have height 0) whose
-// children are of kind k (0 both tips, 1 one tip, 2 both internal); buckets
-// are stored level-major, so a level is also one contiguous run.  All roots
-// follow in one final list, so the level kernels never carry the root mixture.
+// Build the pruning schedule for slots [first, first+count).  Particles are
+// grouped into waves of at most Wmax particles and Cmax classes (partials and P
+// of one wave are resident at a time).  Within a wave, bucket (L, k) holds the
+// non-root internal nodes of height L (tips have height 0) whose children are
+// of kind k (0 both tips, 1 one tip, 2 both internal); buckets are stored
+// level-major, and the wave's roots follow in one final list, so the level
+// kernels never carry the root mixture.
 int build_schedule(smc_engine *e, int first, int count)
 {
     if (e->sched_first == first && e->sched_count == count && e->sched_version == e->topo_version)
@@ -184,31 +200,51 @@ int build_schedule(smc_engine *e, int first, int count)
         const int *ch = &e->h_children[((size_t)p * Ti + i) * 2];
         return (ch[0] >= T) + (ch[1] >= T);
     };
-    int maxh = 0;
-    for (int p = first; p < first + count; ++p)
-        for (int i = 0; i < Ti; ++i)
-            if (T + i != e->h_root[p]) maxh = std::max(maxh, e->h_height[(size_t)p * Ti + i]);
-    const int nb = (maxh + 1) * 3;
-    std::vector<int> cnt(nb + 1, 0);
-    for (int p = first; p < first + count; ++p)
-        for (int i = 0; i < Ti; ++i)
-            if (T + i != e->h_root[p]) cnt[e->h_height[(size_t)p * Ti + i] * 3 + kind_of(p, i)]++;
-    e->bucket_off.assign(nb + 1, 0);
-    for (int b = 0; b < nb; ++b) e->bucket_off[b + 1] = e->bucket_off[b] + cnt[b];
-    e->max_level = maxh;
+    e->waves.clear();
     std::vector<Item> items((size_t)count * Ti);
-    std::vector<int> fill(e->bucket_off);
-    const int root_off = e->bucket_off[nb];
-    for (int p = first; p < first + count; ++p)
-        for (int i = 0; i < Ti; ++i) {
-            if (T + i == e->h_root[p]) {
-                items[root_off + (p - first)] = Item{p, T + i};
-            } else {
-                const int b = e->h_height[(size_t)p * Ti + i] * 3 + kind_of(p, i);
-                items[fill[b]++] = Item{p, T + i};
-            }
+    std::vector<int> pcls((size_t)count, 0);
+    size_t item_pos = 0;
+    int p = first;
+    while (p < first + count) {
+        smc_engine::Wave w;
+        w.first = p;
+        int64_t ncls = 0;
+        while (p < first + count && p - w.first < e->Wmax && ncls + e->h_nclass[p] <= e->Cmax) {
+            pcls[p - first] = (int)ncls;
+            ncls += e->h_nclass[p];
+            ++p;
         }
+        w.count = p - w.first;
+        if (w.count == 0) return fail(e, SMC_ERR_STATE, "a particle has more classes than Pmat holds");
+        int maxh = 0;
+        for (int q = w.first; q < w.first + w.count; ++q)
+            for (int i = 0; i < Ti; ++i)
+                if (T + i != e->h_root[q]) maxh = std::max(maxh, e->h_height[(size_t)q * Ti + i]);
+        const int nb = (maxh + 1) * 3;
+        std::vector<int> cnt(nb + 1, 0);
+        for (int q = w.first; q < w.first + w.count; ++q)
+            for (int i = 0; i < Ti; ++i)
+                if (T + i != e->h_root[q]) cnt[e->h_height[(size_t)q * Ti + i] * 3 + kind_of(q, i)]++;
+        w.bucket_off.assign(nb + 1, (int)item_pos);
+        for (int b = 0; b < nb; ++b) w.bucket_off[b + 1] = w.bucket_off[b] + cnt[b];
+        w.max_level = maxh;
+        std::vector<int> fill(w.bucket_off);
+        w.root_off = w.bucket_off[nb];
+        for (int q = w.first; q < w.first + w.count; ++q)
+            for (int i = 0; i < Ti; ++i) {
+                if (T + i == e->h_root[q]) {
+                    items[w.root_off + (q - w.first)] = Item{q, T + i};
+                } else {
+                    const int b = e->h_height[(size_t)q * Ti + i] * 3 + kind_of(q, i);
+                    items[fill[b]++] = Item{q, T + i};
+                }
+            }
+        item_pos = (size_t)w.root_off + w.count;
+        e->waves.push_back(std::move(w));
+    }
     HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
+                              hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), pcls.size() * sizeof(int),
                               hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     e->sched_first = first;
@@ -227,7 +263,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     *out = nullptr;
     if (cfg->taxa < 3 || cfg->sites < 1 || cfg->num_rates < 1 || cfg->num_rates > 64 ||
         cfg->max_particles < 1 || cfg->max_classes < 1 || cfg->max_classes > 4096 ||
-        (cfg->precision != SMC_PREC_FP64 && cfg->precision != SMC_PREC_FP32) || cfg->reserved != 0)
+        (cfg->precision != SMC_PREC_FP64 && cfg->precision != SMC_PREC_FP32) ||
+        cfg->partial_slots < 0 || cfg->pmat_classes < 0)
         return SMC_ERR_ARG;
     smc_engine *e = new smc_engine();
     e->cfg = *cfg;
@@ -238,6 +275,10 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->nn = 2 * cfg->taxa - 1;
     e->Pmax = cfg->max_particles;
     e->Kmax = cfg->max_classes;
+    e->Wmax = cfg->partial_slots > 0 ? std::min(cfg->partial_slots, cfg->max_particles)
+                                     : cfg->max_particles;
+    e->Cmax = cfg->pmat_classes > 0 ? cfg->pmat_classes : (int64_t)e->Wmax * cfg->max_classes;
+    if (e->Cmax < cfg->max_classes) e->Cmax = cfg->max_classes;
     e->tile_stride = (cfg->sites + kTile - 1) / kTile + cfg->max_classes;
     e->fp32 = cfg->precision == SMC_PREC_FP32;
     e->tx_size = e->fp32 ? 4 : 8;
@@ -276,13 +317,15 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_lnl, P);
     A(&e->d_nonfinite, P);
     A(&e->d_items, P * (T - 1));
-    A(&e->d_E, P * (T - 1) * R * Sp);
+    A(&e->d_pcls_off, P);
+    const size_t W = e->Wmax, C = (size_t)e->Cmax;
+    A(&e->d_E, W * (T - 1) * R * Sp);
     if (e->fp32) {
-        A((float **)&e->d_P, P * K * nn * R * kPStride);
-        A((float **)&e->d_X, P * (T - 1) * R * 20 * Sp);
+        A((float **)&e->d_P, C * nn * R * kPStride);
+        A((float **)&e->d_X, W * (T - 1) * R * 20 * Sp);
     } else {
-        A((double **)&e->d_P, P * K * nn * R * kPStride);
-        A((double **)&e->d_X, P * (T - 1) * R * 20 * Sp);
+        A((double **)&e->d_P, C * nn * R * kPStride);
+        A((double **)&e->d_X, W * (T - 1) * R * 20 * Sp);
     }
     if (rc != SMC_OK) {
         std::string msg = e->err;
@@ -297,6 +340,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_root.assign(P, -1);
     e->h_height.assign(P * (T - 1), 0);
     e->h_valid.assign(P, 0);
+    e->h_nclass.assign(P, 1);
     *out = e;
     return SMC_OK;
 }
@@ -309,7 +353,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
-                    e->d_out2,   e->d_assign};
+                    e->d_out2,   e->d_assign, e->d_pcls_off};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -461,6 +505,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
             }
         ntiles[i] = nt;
         e->h_valid[p] = 1;
+        e->h_nclass[p] = Kp;
     }
     std::vector<int> hch((size_t)count * Ti * 2);
     std::memcpy(hch.data(), children, hch.size() * sizeof(int));
@@ -506,59 +551,66 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
     int rc = build_schedule(e, first, count);
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
-    {
-        TimedLaunch tl(e, SMC_K_EIGEN);
-        launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first, count},
-                     s);
-    }
-    {
-        TimedLaunch tl(e, SMC_K_PMAT);
-        launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_P, e->nn, e->R,
-                             e->Kmax, first, count, e->fp32},
-                    s);
-    }
-    PruneArgs a{};
-    a.tiles = e->d_tiles;
-    a.ntiles = e->d_ntiles;
-    a.tile_stride = e->tile_stride;
-    a.children = e->d_children;
-    a.root_of = e->d_root;
-    a.perm = e->d_perm;
-    a.codes = e->d_codes;
-    a.P = e->d_P;
-    a.eig = e->d_eig;
-    a.pinv = e->d_pinv;
-    a.isinv = e->d_isinv;
-    a.invchar = e->d_invchar;
-    a.X = e->d_X;
-    a.E = e->d_E;
-    a.tile_lnl = e->d_tile_lnl;
-    a.T = e->T;
-    a.R = e->R;
-    a.S_pad = e->S_pad;
-    a.max_classes = e->Kmax;
-    a.fp32 = e->fp32;
-    auto run = [&](int off, int n, int kind, bool roots) {
-        if (n <= 0) return;
-        TimedLaunch tl(e, SMC_K_PRUNE);
-        a.items = e->d_items + off;
-        a.n_items = n;
-        a.kind = kind;
-        a.roots = roots;
-        if (e->prune_variant == SMC_PRUNE_VALU)
-            launch_prune(a, s);
-        else
-            launch_prune_mfma(a, s);
-    };
-    const std::vector<int> &bo = e->bucket_off;
-    for (int L = 1; L <= e->max_level; ++L) {
-        if (e->prune_variant == SMC_PRUNE_VALU) {
-            run(bo[3 * L], bo[3 * L + 3] - bo[3 * L], 3, false);
-        } else {
-            for (int k = 0; k < 3; ++k) run(bo[3 * L + k], bo[3 * L + k + 1] - bo[3 * L + k], k, false);
+    for (const smc_engine::Wave &w : e->waves) {
+        {
+            TimedLaunch tl(e, SMC_K_EIGEN);
+            launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, w.first,
+                                   w.count},
+                         s);
         }
+        {
+            TimedLaunch tl(e, SMC_K_PMAT);
+            launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off,
+                                 e->d_P, e->nn, e->R, e->Kmax, w.first, w.count, e->fp32},
+                        s);
+        }
+        PruneArgs a{};
+        a.tiles = e->d_tiles;
+        a.ntiles = e->d_ntiles;
+        a.tile_stride = e->tile_stride;
+        a.children = e->d_children;
+        a.root_of = e->d_root;
+        a.perm = e->d_perm;
+        a.codes = e->d_codes;
+        a.P = e->d_P;
+        a.pcls_off = e->d_pcls_off;
+        a.eig = e->d_eig;
+        a.pinv = e->d_pinv;
+        a.isinv = e->d_isinv;
+        a.invchar = e->d_invchar;
+        a.X = e->d_X;
+        a.E = e->d_E;
+        a.p_base = w.first;
+        a.tile_lnl = e->d_tile_lnl;
+        a.T = e->T;
+        a.R = e->R;
+        a.S_pad = e->S_pad;
+        a.max_classes = e->Kmax;
+        a.fp32 = e->fp32;
+        a.mix_mode = e->mix_mode;
+        auto run = [&](int off, int n, int kind, bool roots) {
+            if (n <= 0) return;
+            TimedLaunch tl(e, SMC_K_PRUNE);
+            a.items = e->d_items + off;
+            a.n_items = n;
+            a.kind = kind;
+            a.roots = roots;
+            if (e->prune_variant == SMC_PRUNE_VALU)
+                launch_prune(a, s);
+            else
+                launch_prune_mfma(a, s);
+        };
+        const std::vector<int> &bo = w.bucket_off;
+        for (int L = 1; L <= w.max_level; ++L) {
+            if (e->prune_variant == SMC_PRUNE_VALU) {
+                run(bo[3 * L], bo[3 * L + 3] - bo[3 * L], 3, false);
+            } else {
+                for (int k = 0; k < 3; ++k)
+                    run(bo[3 * L + k], bo[3 * L + k + 1] - bo[3 * L + k], k, false);
+            }
+        }
+        run(w.root_off, w.count, 3, true);
     }
-    run(bo[3 * (e->max_level + 1)], count, 3, true);
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
         launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,
@@ -661,6 +713,15 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant)
     if (family != SMC_K_PRUNE || (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU))
         return fail(e, SMC_ERR_ARG, "unknown kernel variant");
     e->prune_variant = variant;
+    return SMC_OK;
+}
+
+int smc_engine_set_mixture(smc_engine *e, int mode)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (mode != SMC_MIX_REFERENCE && mode != SMC_MIX_LOGSPACE)
+        return fail(e, SMC_ERR_ARG, "unknown mixture mode");
+    e->mix_mode = mode;
     return SMC_OK;
 }
 

@@ -50,7 +50,8 @@ struct PmatArgs {
     const double *blen;      // [P][nn]
     const double *rates;     // [P][R]
     const int *num_classes;
-    void *P;                 // [P*Kmax][nn][R][420] (double or float)
+    const int *pcls_off;     // [P] first compact class slot of each particle in the wave
+    void *P;                 // [wave classes][nn][R][420] (double or float)
     int nn, R, max_classes, first, count;
     bool fp32;
 };
@@ -66,17 +67,20 @@ struct PruneArgs {
     const int *root_of;      // [P]
     const int *perm;         // [P][S_pad] slot -> alignment column
     const uint8_t *codes;    // [T][S_pad]
-    const void *P;           // [P*Kmax][nn][R][420]
+    const void *P;           // [wave classes][nn][R][420]
+    const int *pcls_off;     // [P] compact class slot of (particle, class 0) in P
     const double *eig;       // [P*Kmax][840]
     const double *pinv;      // [P]
     const double *isinv;     // [S_pad]
     const int *invchar;      // [S_pad]
-    void *X;                 // partials [P][T-1][R][20][S_pad]
-    int *E;                  // cumulative binary exponents [P][T-1][R][S_pad]
+    void *X;                 // partials [wave slot][T-1][R][20][S_pad]
+    int *E;                  // cumulative binary exponents [wave slot][T-1][R][S_pad]
+    int p_base;              // first particle of the wave (wave slot = particle - p_base)
     double *tile_lnl;        // [P][tile_stride]
     int T, R, S_pad, max_classes;
     bool fp32;
     bool roots;              // items are tree roots: fuse the root mixture, store nothing
+    int mix_mode;            // SMC_MIX_REFERENCE or SMC_MIX_LOGSPACE (roots only)
     int kind;                // child kinds of every item: 0 tip-tip, 1 tip-internal,
                              // 2 internal-internal, 3 mixed (decided per item)
 };

@@ -192,6 +192,7 @@ __global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
                                               const double *__restrict__ blen,
                                               const double *__restrict__ rates,
                                               const int *__restrict__ num_classes,
+                                              const int *__restrict__ pcls_off,
                                               TP *__restrict__ Pout, int nn, int R,
                                               int max_classes, int first)
 {
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
 #pragma unroll
     for (int kk = 0; kk < 20; kk++) e[kk] = exp(lam[kk] * t * rate);
 
-    TP *out = Pout + ((g * nn + n) * R + r) * kPStride;
+    const size_t gp = (size_t)pcls_off[p] + k;  // compact class slot of the current wave
+    TP *out = Pout + ((gp * nn + n) * R + r) * kPStride;
     for (int i = 0; i < 20; i++) {
         double a[20];
 #pragma unroll
@@ -245,10 +247,12 @@ void launch_pmat(const PmatArgs &a, hipStream_t s)
     dim3 grid(a.count * a.max_classes, (a.nn * a.R + 63) / 64);
     if (a.fp32)
         hipLaunchKernelGGL(k_pmat<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
-                           a.num_classes, (float *)a.P, a.nn, a.R, a.max_classes, a.first);
+                           a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes,
+                           a.first);
     else
         hipLaunchKernelGGL(k_pmat<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
-                           a.num_classes, (double *)a.P, a.nn, a.R, a.max_classes, a.first);
+                           a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
+                           a.first);
 }
 
 }  // namespace smccat

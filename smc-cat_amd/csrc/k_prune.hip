@@ -20,6 +20,7 @@
 // sum over all nodes.  At the root the tile's per-site log-likelihoods are
 // summed (wavefront butterfly, fixed order) into one value per tile.
 #include "internal.hpp"
+#include "root_mixture.hpp"
 
 namespace smccat {
 
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                const int *__restrict__ invchar,
                                                TX *__restrict__ X, int *__restrict__ E,
                                                double *__restrict__ tile_lnl, int T, int R,
-                                               int S_pad, int max_classes, int blocks_per_item)
+                                               int S_pad, int max_classes, int blocks_per_item,
+                                               const int *__restrict__ pcls_off, int p_base, int mix_mode)
 {
     // 1-D grid: consecutive blocks walk the tiles of one item, 4 tiles (one per
     // wavefront) per block, so concurrently running waves stream contiguous
@@ -143,9 +145,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     (void)root_of;
 
     const size_t node_elems = (size_t)R * 20 * S_pad;
-    TX *Xp = X + (size_t)p * (T - 1) * node_elems;
-    int *Ep = E + (size_t)p * (T - 1) * R * S_pad;
-    const TX *Pg = Pm + g * nn * R * kPStride;
+    const size_t lp = (size_t)(p - p_base);  // slot of the particle in the wave's partials
+    TX *Xp = X + lp * (T - 1) * node_elems;
+    int *Ep = E + lp * (T - 1) * R * S_pad;
+    const TX *Pg = Pm + ((size_t)pcls_off[p] + tl.cls) * nn * R * kPStride;
     const double *pi = eig + g * kEigStride + kEigPi;
 
     double lse_m = -INFINITY, lse_s = 0.0;
@@ -180,21 +183,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             for (int i = 0; i < 20; i++) dot = fma(pi[i], (double)y0[i], dot);
             double l = log(dot) + (double)etot * kLn2;
             // running log-sum-exp over rates (rate weights are 1, Particle.cpp:661-665)
-            if (l > lse_m) {
-                lse_s = lse_s * exp(lse_m - l) + 1.0;
-                lse_m = l;
-            } else {
-                lse_s += exp(l - lse_m);
-            }
+            lse_push(lse_m, lse_s, l, mix_mode);
         }
     }
     if constexpr (is_root) {
         double lf = 0.0;
         if (active) {
-            double lsite = lse_m + log(lse_s);
-            double pv = pinv[p];
-            double fin = exp(lsite) * (1.0 - pv) + pv * pi[invchar[orig]] * isinv[orig];
-            lf = log(fin);
+            lf = site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig],
+                                     mix_mode);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) lf += __shfl_xor(lf, off, 64);
@@ -227,7 +223,7 @@ void launch_prune(const PruneArgs &a, hipStream_t s)
 #define SMC_PRUNE_ARGS                                                                          \
     a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.root_of, a.perm, a.codes,         \
         (const TXa *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, \
-        a.R, a.S_pad, a.max_classes, bpi
+        a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode
     if (a.fp32) {
         using TXa = float;
         if (a.roots)

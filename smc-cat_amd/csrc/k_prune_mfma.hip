@@ -19,6 +19,7 @@
 // children are not multiplied: their P x is a column (or the row sums) of P,
 // gathered straight into the C layout.
 #include "internal.hpp"
+#include "root_mixture.hpp"
 
 namespace smccat {
 
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
     const uint8_t *__restrict__ codes, const TX *__restrict__ Pm, const double *__restrict__ eig,
     const double *__restrict__ pinv, const double *__restrict__ isinv,
     const int *__restrict__ invchar, TX *__restrict__ X, int *__restrict__ E,
-    double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes, int blocks_per_item)
+    double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes, int blocks_per_item,
+    const int *__restrict__ pcls_off, int p_base, int mix_mode)
 {
     using V4 = typename Mfma<TX>::v4;
     constexpr int U = KindTraits<KIND>::U;
@@ -132,9 +134,10 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
         tip1 = c1 < T;
     }
     const size_t node_elems = (size_t)R * 20 * S_pad;
-    TX *Xp = X + (size_t)p * (T - 1) * node_elems;
-    int *Ep = E + (size_t)p * (T - 1) * R * S_pad;
-    const TX *Pg = Pm + g * nn * R * kPStride;
+    const size_t lp = (size_t)(p - p_base);  // slot of the particle in the wave's partials
+    TX *Xp = X + lp * (T - 1) * node_elems;
+    int *Ep = E + lp * (T - 1) * R * S_pad;
+    const TX *Pg = Pm + ((size_t)pcls_off[p] + tl.cls) * nn * R * kPStride;
     const double *pi = eig + g * kEigStride + kEigPi;
     const int nsb = (tl.len + 15) >> 4;  // 16-site column blocks in this tile (wave-uniform)
 
@@ -238,14 +241,7 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
                     // without dynamic register indexing
 #pragma unroll
                     for (int k = 0; k < 4; k++)
-                        if (k == sb0 + u) {
-                            if (l > lse_m[k]) {
-                                lse_s[k] = lse_s[k] * exp(lse_m[k] - l) + 1.0;
-                                lse_m[k] = l;
-                            } else {
-                                lse_s[k] += exp(l - lse_m[k]);
-                            }
-                        }
+                        if (k == sb0 + u) lse_push(lse_m[k], lse_s[k], l, mix_mode);
                 }
             }
         }
@@ -258,9 +254,8 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
             const int o = 16 * sb + c16;
             if (g4 == 0 && o < tl.len) {
                 const int orig = perm[(size_t)p * S_pad + tl.start + o];
-                const double lsite = lse_m[sb] + log(lse_s[sb]);
-                const double fin = exp(lsite) * (1.0 - pv) + pv * pi[invchar[orig]] * isinv[orig];
-                lf += log(fin);
+                lf += site_log_likelihood(lse_m[sb], lse_s[sb], pv, pi[invchar[orig]], isinv[orig],
+                                          mix_mode);
             }
         }
 #pragma unroll
@@ -279,7 +274,7 @@ void launch_prune_mfma(const PruneArgs &a, hipStream_t s)
 #define SMC_PRUNE_ARGS                                                                            \
     a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, (const TXa *)a.P,     \
         a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, a.R, a.S_pad,        \
-        a.max_classes, bpi
+        a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode
 #define SMC_PRUNE_LAUNCH(TXT)                                                                     \
     do {                                                                                          \
         using TXa = TXT;                                                                          \

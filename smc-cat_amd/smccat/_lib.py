@@ -14,6 +14,7 @@ SMC_PREC_FP64, SMC_PREC_FP32 = 0, 1
 SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC = 0, 1, 2, 3, 4
 KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc")
 SMC_PRUNE_MFMA, SMC_PRUNE_VALU = 0, 1
+SMC_MIX_REFERENCE, SMC_MIX_LOGSPACE = 0, 1
 
 # Every exported symbol and its (restype, argtypes).
 _P = ctypes.c_void_p
@@ -29,7 +30,7 @@ class EngineConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("taxa", ctypes.c_int32), ("sites", ctypes.c_int32),
                 ("num_rates", ctypes.c_int32), ("max_particles", ctypes.c_int32),
                 ("max_classes", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("partial_slots", ctypes.c_int32), ("pmat_classes", ctypes.c_int64)]
 
 
 SIGNATURES = {
@@ -48,6 +49,7 @@ SIGNATURES = {
     "smc_engine_reset_stats": (_I, [_P]),
     "smc_engine_device_bytes": (ctypes.c_int64, [_P]),
     "smc_engine_set_variant": (_I, [_P, _I, _I]),
+    "smc_engine_set_mixture": (_I, [_P, _I]),
 }
 
 _lib = None

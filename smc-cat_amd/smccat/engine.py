@@ -66,10 +66,11 @@ class EngineError(RuntimeError):
 
 class Engine:
     def __init__(self, taxa, sites, num_rates=1, max_particles=1, max_classes=1,
-                 precision="fp64", device=0):
+                 precision="fp64", device=0, partial_slots=0, pmat_classes=0):
         self.lib = _lib.load()
         cfg = _lib.EngineConfig(device, taxa, sites, num_rates, max_particles, max_classes,
-                                _lib.SMC_PREC_FP64 if precision == "fp64" else _lib.SMC_PREC_FP32, 0)
+                                _lib.SMC_PREC_FP64 if precision == "fp64" else _lib.SMC_PREC_FP32,
+                                partial_slots, pmat_classes)
         h = ctypes.c_void_p()
         rc = self.lib.smc_engine_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -163,6 +164,11 @@ class Engine:
         """'mfma' (matrix cores, default) or 'valu'."""
         v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
+
+    def set_mixture(self, mode):
+        """'reference' (Particle.cpp:653-670 exactly, default) or 'logspace'."""
+        m = {"reference": _lib.SMC_MIX_REFERENCE, "logspace": _lib.SMC_MIX_LOGSPACE}[mode]
+        self._check(self.lib.smc_engine_set_mixture(self.h, m), "smc_engine_set_mixture")
 
     def set_timing(self, on=True):
         self._check(self.lib.smc_engine_set_timing(self.h, 1 if on else 0), "smc_engine_set_timing")

@@ -43,38 +43,30 @@ _COORDS = coords190()
 
 
 def random_tree(rng, taxa, mean_len=0.1):
-    """Returns (children [T-1,2] int32, branch_len [2T-1], root id)."""
+    """Returns (children [T-1,2] int32, branch_len [2T-1], root id).
+
+    The root is internal node T; the internal node created when tip i joins is
+    T + i - 1 and takes children (i, r), r the split tip, in r's old child slot."""
     T = taxa
-    # creation order: root, tip0, tip1, then (internal, tip) pairs
-    parent_of = {}
-    kids = {}
-    internal_order = ["root"]
-    kids["root"] = [("tip", 0), ("tip", 1)]
-    parent_of[("tip", 0)] = "root"
-    parent_of[("tip", 1)] = "root"
-    for i in range(2, T):
-        r = int(rng.random() * i)
-        victim = ("tip", r)
-        anc = parent_of[victim]
-        new_int = "n%d" % i
-        internal_order.append(new_int)
-        kids[new_int] = [("tip", i), victim]
-        parent_of[("tip", i)] = ("int", new_int)
-        parent_of[victim] = ("int", new_int)
-        a = anc if anc == "root" else anc[1]
-        kids[a] = [("int", new_int) if k == victim else k for k in kids[a]]
-        parent_of[("int", new_int)] = anc
-    ids = {name: T + k for k, name in enumerate(internal_order)}
     children = np.zeros((T - 1, 2), dtype=np.int32)
-    for name in internal_order:
-        ch = []
-        for kind, v in kids[name]:
-            ch.append(v if kind == "tip" else ids[v])
-        children[ids[name] - T] = ch
+    parent = np.zeros(2 * T - 1, dtype=np.int64)
+    slot = np.zeros(2 * T - 1, dtype=np.int64)
+    children[0] = (0, 1)
+    parent[0] = parent[1] = T
+    slot[1] = 1
+    u = rng.random(max(T - 2, 0))
+    for i in range(2, T):
+        r = int(u[i - 2] * i)
+        new = T + i - 1
+        a, sl = parent[r], slot[r]
+        children[a - T, sl] = new
+        parent[new], slot[new] = a, sl
+        children[new - T] = (i, r)
+        parent[i], slot[i] = new, 0
+        parent[r], slot[r] = new, 1
     blen = rng.exponential(mean_len, size=2 * T - 1)
-    root = ids["root"]
-    blen[root] = 0.0
-    return children, blen, root
+    blen[T] = 0.0
+    return children, blen, T
 
 
 def build_q(base, logits):

@@ -67,3 +67,48 @@ def lnl(codes, children, blen, base, logits, assignment, pinv, rates):
         ic = int(col[0]) if (inv == 1.0 and col[0] < 20) else 0
         tot += np.log(L * (1.0 - pinv) + pinv * pi[ic] * inv)
     return tot
+
+
+def lnl_logdomain(codes, children, blen, base, logits, assignment, pinv, rates):
+    """The same likelihood enumerated entirely in log space (logsumexp over all
+    internal-state assignments, log P from scipy expm): valid where the site
+    likelihood underflows double precision."""
+    from scipy.special import logsumexp
+    T, S = codes.shape
+    internal = list(range(T, 2 * T - 1))
+    parent = {}
+    for k, n in enumerate(internal):
+        for c in children[k]:
+            parent[int(c)] = n
+    root = [n for n in internal if n not in parent][0]
+    order = internal  # enumerate joint internal states
+    import itertools
+    tot = 0.0
+    for s in range(S):
+        lg = logits[assignment[s]]
+        Q, pi = build_q(base, lg)
+        logpi = lg - logsumexp(lg)
+        col = codes[:, s]
+        terms = []
+        for r in rates:
+            logP = {v: np.log(np.clip(scipy.linalg.expm(Q * blen[v] * r), 1e-320, None))
+                    for v in parent}
+            vals = []
+            for states in itertools.product(range(NS), repeat=len(order)):
+                st = dict(zip(order, states))
+                acc = logpi[st[root]]
+                for v, u in parent.items():
+                    if v >= T:
+                        acc += logP[v][st[u], st[v]]
+                    else:
+                        c = int(col[v])
+                        acc += logP[v][st[u], c] if c < 20 else logsumexp(logP[v][st[u]])
+                vals.append(acc)
+            terms.append(logsumexp(vals))
+        l = logsumexp(terms)
+        inv = 1.0 if np.all(col == col[0]) else 0.0
+        ic = int(col[0]) if (inv == 1.0 and col[0] < 20) else 0
+        a = l + np.log1p(-pinv)
+        b = np.log(pinv * np.exp(logpi[ic]) * inv) if inv and pinv > 0 else -np.inf
+        tot += np.logaddexp(a, b)
+    return tot

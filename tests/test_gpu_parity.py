@@ -147,6 +147,54 @@ def test_fp32_fast_path_error(gpu, taxa, sites, R, krange, variant):
     assert rel.max() <= RTOL_FP32
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_waves_of_particles_and_classes(gpu, precision):
+    """Partials for 3 particles and P for 14 classes at a time: 11 CAT particles
+    run in several waves and must give the single-wave results."""
+    _say("case waves: 11 particles, partial_slots=3, pmat_classes=14, %s" % precision)
+    rng = np.random.default_rng(77)
+    T, S, R = 13, 210, 4
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(T, S)).astype(np.int8))
+    batch = sim.prior_particles(rng, 11, T, S, num_rates=R, invar=True, k_range=(2, 9))
+    with Engine(T, S, R, 11, 9, precision, partial_slots=3, pmat_classes=14) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        waved, nf = eng.refresh_likelihood()
+        part, _ = eng.refresh_likelihood(4, 5)
+    one, _ = _engine_lnl(codes, batch, R, 9, precision)
+    assert not nf.any()
+    np.testing.assert_allclose(waved, one, rtol=0, atol=0)
+    np.testing.assert_allclose(part, one[4:9], rtol=0, atol=0)
+    if precision == "fp64":
+        np.testing.assert_allclose(waved, _oracle_lnl(codes, batch, R), rtol=RTOL_FP64)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_logspace_mixture(gpu, variant, precision):
+    """200 taxa on long branches push most sites below log L = -745 (P stays well
+    conditioned, every tip costs ~log pi): the reference formula gives -inf there, the
+    log-space mixture a finite value equal to the oracle's."""
+    _say("case logspace mixture: %s %s" % (variant, precision))
+    rng = np.random.default_rng(55)
+    T, S, R = 200, 96, 4
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(T, S)).astype(np.int8))
+    codes[:, 5] = 3                      # invariant columns keep finite reference terms
+    batch = sim.prior_particles(rng, 4, T, S, num_rates=R, invar=True, k_range=(1, 4))
+    batch.branch_len *= 3.0
+    batch.logits *= 2.0
+    with Engine(T, S, R, batch.n, 4, precision) as eng:
+        eng.set_prune_variant(variant)
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        ref_mode, _ = eng.refresh_likelihood()
+        eng.set_mixture("logspace")
+        log_mode, nf = eng.refresh_likelihood()
+    want = np.array([orc.particle_lnl_mode(codes, batch.particle(i), R, 1)[0] for i in range(batch.n)])
+    assert np.all(np.isneginf(ref_mode)) and not nf.any()
+    np.testing.assert_allclose(log_mode, want, rtol=RTOL_FP64 if precision == "fp64" else RTOL_FP32)
+
+
 def test_c2_full_size_properties(gpu):
     _say("case c2 full size: 64 taxa x 2000 sites, GTR+G4+I, 1024 particles")
     codes, cfg = sim.simulate_config("c2")

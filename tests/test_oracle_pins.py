@@ -250,3 +250,32 @@ def test_site_class_lnl_vs_bruteforce():
         want = bruteforce.lnl(codes[:, 4:5], p["children"], p["branch_len"], p["base"], cands[c:c + 1],
                               np.zeros(1, dtype=int), 0.1, rates)
         assert got[c] == pytest.approx(tau * want, rel=1e-12)
+
+
+def test_logspace_mixture_pins():
+    """logspace mode equals the reference formula where that is finite, and equals a
+    log-domain brute-force enumeration where exp(logSiteLike) underflows."""
+    rng = np.random.default_rng(12)
+    codes, p = _tiny_state(rng, 4, 5, K=1, R=2, pinv=0.15)
+    ref, _ = orc.particle_lnl_mode(codes, p, 2, 0)
+    log, _ = orc.particle_lnl_mode(codes, p, 2, 1)
+    assert log == pytest.approx(ref, rel=1e-13)
+    # extreme stationary frequencies: pi_i down to e^-300, so variable sites fall below -745
+    q = dict(p)
+    q["logits"] = np.zeros((1, 20))
+    q["logits"][0, :10] = -300.0
+    codes2 = codes.copy()
+    codes2[:, 0] = [0, 1, 2, 3]          # rare states everywhere: deep underflow
+    codes2[:, 1] = 4                     # invariant rare column
+    codes2[:, 2] = 15                    # invariant common column
+    ref2, sites_ref = orc.particle_lnl_mode(codes2, q, 2, 0)
+    log2, sites_log = orc.particle_lnl_mode(codes2, q, 2, 1)
+    assert np.isneginf(sites_ref[0]) and np.isfinite(sites_log[0]) and sites_log[0] < -745
+    rates = orc.discretize_gamma(q["shape"], 2)
+    want = bruteforce.lnl_logdomain(codes2, q["children"], q["branch_len"], q["base"], q["logits"],
+                                    q["assignment"], q["pinv"], rates)
+    # transition probabilities into states with pi ~ e^-300 carry the absolute (not
+    # relative) accuracy of expm / the eigen back-transform, hence 1e-9 here
+    assert log2 == pytest.approx(want, rel=1e-9)
+    finite = np.isfinite(sites_ref)
+    np.testing.assert_allclose(sites_log[finite], sites_ref[finite], rtol=1e-12)
