@@ -40,7 +40,8 @@ enum {
     SMC_K_PRUNE = 2,   /* level-ordered pruning, root mixture fused          */
     SMC_K_REDUCE = 3,  /* per-particle log-likelihood reduction              */
     SMC_K_SMC = 4,     /* reweight / ESS bisection / resampling              */
-    SMC_K_COUNT = 5
+    SMC_K_SITE = 5,    /* single-site x candidate-profile scoring            */
+    SMC_K_COUNT = 6
 };
 
 typedef struct smc_engine smc_engine;
@@ -104,6 +105,19 @@ int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const doubl
 /* Systematic resampling (main.cpp:179-195) on the GPU: u in [0,1). */
 int smc_systematic_resample(smc_engine *e, int n, const double *normalized, double u,
                             int32_t *assignments);
+
+/* Tempered single-site log-likelihoods under candidate CAT profiles: the
+ * per-site, per-class conditional of Particle::gibbsPartitionMove
+ * (Particle.cpp:857-941, with the per-rate tip index and rescale stride of
+ * Q15 corrected), batched over entries.  Entry i scores alignment column
+ * sites[i] of particle slot particles[i] (its tree, branch lengths, base
+ * matrix, rates and pInvar) under the ncand[i] <= n_cand profiles
+ * cand_logits[i][c][20]:
+ *   out[i][c] = tempering * log L_site(profile c)   (NaN for c >= ncand[i]).
+ * The root mixture follows smc_engine_set_mixture. */
+int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const int32_t *sites,
+                       int n_cand, const int32_t *ncand, const double *cand_logits,
+                       double tempering, double *out);
 
 /* discretizeGamma (Miscellaneous.hpp:22-37): out[num] mean-one gamma rates. */
 int smc_discretize_gamma(double shape, int num, double *out);

@@ -83,6 +83,12 @@ struct smc_engine {
     std::vector<int> h_valid;     // [P] particle uploaded
     std::vector<int> h_nclass;    // [P] classes per particle
     int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
+    int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
+    // site x candidate scoring scratch (grown on demand)
+    int64_t sc_cap = 0;           // entries * candidates
+    int sc_ent_cap = 0;
+    int *d_sc_ent = nullptr;      // [3][entries]: particle, site, candidate count
+    double *d_sc_logits = nullptr, *d_sc_eig = nullptr, *d_sc_rate = nullptr, *d_sc_out = nullptr;
 
     // pruning schedule cache: waves of particles, each with its level/kind buckets
     struct Wave {
@@ -318,6 +324,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_nonfinite, P);
     A(&e->d_items, P * (T - 1));
     A(&e->d_pcls_off, P);
+    A(&e->d_prog, P * (T - 1));
     const size_t W = e->Wmax, C = (size_t)e->Cmax;
     A(&e->d_E, W * (T - 1) * R * Sp);
     if (e->fp32) {
@@ -353,7 +360,8 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
-                    e->d_out2,   e->d_assign, e->d_pcls_off};
+                    e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent,
+                    e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -421,6 +429,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     std::vector<double> blen((size_t)count * nn), rates((size_t)count * R);
     std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
     std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
+    std::vector<int> progs((size_t)count * Ti);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const int32_t *ch = children + (size_t)i * Ti * 2;
@@ -464,6 +473,44 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
                 int n = order[k];
                 h[n] = 1 + std::max(h[ch[2 * (n - T)]], h[ch[2 * (n - T) + 1]]);
             }
+        }
+        // evaluation program for single-site scoring: post-order that descends into
+        // the child with the larger Strahler number first, so at most
+        // max-Strahler + 1 internal partials are live; checked against kSiteStack
+        {
+            std::vector<int> st(nn, 0);
+            for (int k = Ti - 1; k >= 0; --k) {
+                const int n = order[k];
+                const int a = st[ch[2 * (n - T)]], b = st[ch[2 * (n - T) + 1]];
+                st[n] = a == b ? a + 1 : std::max(a, b);
+            }
+            std::vector<int> prog;
+            prog.reserve(Ti);
+            std::vector<std::pair<int, int>> stk{{root, 0}};
+            while (!stk.empty()) {
+                auto &[n, state] = stk.back();
+                if (state == 0) {
+                    state = 1;
+                    int a = ch[2 * (n - T)], b = ch[2 * (n - T) + 1];
+                    if (st[b] > st[a]) std::swap(a, b);
+                    // push b first so a (the larger subtree) is evaluated first
+                    if (b >= T) stk.push_back({b, 0});
+                    if (a >= T) stk.push_back({a, 0});
+                } else {
+                    prog.push_back(n);
+                    stk.pop_back();
+                }
+            }
+            int depth = 0, maxd = 0;
+            for (int n : prog) {
+                depth -= (ch[2 * (n - T)] >= T) + (ch[2 * (n - T) + 1] >= T);
+                if (depth < 0) return fail(e, SMC_ERR_ARG, "internal error: scoring program");
+                if (n != root) maxd = std::max(maxd, ++depth);
+            }
+            if ((int)prog.size() != Ti || maxd > kSiteStack)
+                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
+                                                ": tree too deep for single-site scoring");
+            std::copy(prog.begin(), prog.end(), progs.begin() + (size_t)i * Ti);
         }
         for (int k = 0; k < Ti; ++k) {
             e->h_height[(size_t)p * Ti + k] = h[T + k];
@@ -532,6 +579,8 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     HIP_TRY(e, hipMemcpyAsync(e->d_tiles + P0 * ts, tiles.data(), tiles.size() * sizeof(Tile),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_ntiles + P0, ntiles.data(), ntiles.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * Ti, progs.data(), progs.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     e->topo_version++;
@@ -690,6 +739,90 @@ int smc_systematic_resample(smc_engine *e, int n, const double *normalized, doub
     }
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipMemcpyAsync(assignments, e->d_assign, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    return SMC_OK;
+}
+
+int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const int32_t *sites,
+                       int n_cand, const int32_t *ncand, const double *cand_logits,
+                       double tempering, double *out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (count < 1 || n_cand < 1 || !particles || !sites || !ncand || !cand_logits || !out)
+        return fail(e, SMC_ERR_ARG, "invalid site-class arguments");
+    if (!e->have_alignment) return fail(e, SMC_ERR_STATE, "no alignment uploaded");
+    for (int i = 0; i < count; ++i) {
+        if (particles[i] < 0 || particles[i] >= e->Pmax || !e->h_valid[particles[i]])
+            return fail(e, SMC_ERR_ARG, "entry " + std::to_string(i) + ": particle not uploaded");
+        if (sites[i] < 0 || sites[i] >= e->S)
+            return fail(e, SMC_ERR_ARG, "entry " + std::to_string(i) + ": site out of range");
+        if (ncand[i] < 0 || ncand[i] > n_cand)
+            return fail(e, SMC_ERR_ARG, "entry " + std::to_string(i) + ": candidate count");
+    }
+    const int64_t need = (int64_t)count * n_cand;
+    if (need > e->sc_cap || count > e->sc_ent_cap) {
+        void *bufs[] = {e->d_sc_ent, e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+        e->d_sc_ent = nullptr;
+        e->d_sc_logits = e->d_sc_eig = e->d_sc_rate = e->d_sc_out = nullptr;
+        e->sc_cap = 0;
+        e->sc_ent_cap = 0;
+        int rc = dev_alloc(e, &e->d_sc_ent, (size_t)3 * count);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_logits, (size_t)need * 20);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_eig, (size_t)need * kEigStride);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_rate, (size_t)need * e->R);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_out, (size_t)need);
+        if (rc != SMC_OK) return rc;
+        e->sc_cap = need;
+        e->sc_ent_cap = count;
+    }
+    hipStream_t s = e->stream;
+    std::vector<int> ent((size_t)3 * count);
+    for (int i = 0; i < count; ++i) {
+        ent[i] = particles[i];
+        ent[count + i] = sites[i];
+        ent[2 * count + i] = ncand[i];
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->d_sc_ent, ent.data(), ent.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    {
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count,
+                          n_cand, e->d_sc_eig, s);
+    }
+    {
+        TimedLaunch tl(e, SMC_K_SITE);
+        SiteClassArgs a{};
+        a.ent_particle = e->d_sc_ent;
+        a.ent_site = e->d_sc_ent + count;
+        a.ent_ncand = e->d_sc_ent + 2 * count;
+        a.n_cand = n_cand;
+        a.cand_eig = e->d_sc_eig;
+        a.children = e->d_children;
+        a.prog = e->d_prog;
+        a.codes = e->d_codes;
+        a.blen = e->d_blen;
+        a.rates = e->d_rates;
+        a.pinv = e->d_pinv;
+        a.isinv = e->d_isinv;
+        a.invchar = e->d_invchar;
+        a.rate_lnl = e->d_sc_rate;
+        a.out = e->d_sc_out;
+        a.count = count;
+        a.T = e->T;
+        a.R = e->R;
+        a.S_pad = e->S_pad;
+        a.mix_mode = e->mix_mode;
+        a.tempering = tempering;
+        launch_site_class(a, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipMemcpyAsync(out, e->d_sc_out, (size_t)need * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     collect_timings(e);
     return SMC_OK;

@@ -87,6 +87,34 @@ struct PruneArgs {
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores
 
+// Single-site x candidate-profile scoring (Gibbs / learned allocation proposal).
+constexpr int kSiteStack = 32;  // LDS stack entries for live internal partials
+struct SiteClassArgs {
+    const int *ent_particle;  // [count] particle of each entry
+    const int *ent_site;      // [count] alignment column of each entry
+    const int *ent_ncand;     // [count] candidates of each entry (<= n_cand)
+    int n_cand;
+    const double *cand_eig;   // [count][n_cand][840]
+    const int *children;      // [P][T-1][2]
+    const int *prog;          // [P][T-1] internal nodes, Strahler-ordered post-order
+    const uint8_t *codes;     // [T][S_pad]
+    const double *blen;       // [P][nn]
+    const double *rates;      // [P][R]
+    const double *pinv;
+    const double *isinv;
+    const int *invchar;
+    double *rate_lnl;         // [count][n_cand][R] scratch
+    double *out;              // [count][n_cand] tempering * log L
+    int count, T, R, S_pad, mix_mode;
+    double tempering;
+};
+void launch_site_class(const SiteClassArgs &a, hipStream_t s);
+
+// Eigen-decomposition of candidate profiles: entry e uses the base matrix of
+// particle ent_particle[e] and logits[e][c][20] for c < ent_ncand[e].
+void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
+                       const double *logits, int count, int n_cand, double *eig, hipStream_t s);
+
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s);
 

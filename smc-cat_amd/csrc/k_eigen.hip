@@ -35,11 +35,10 @@ __device__ __forceinline__ void rr_pair(int r, int k, int &p, int &q)
     q = a < b ? b : a;
 }
 
-__global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ base,
-                                                   const double *__restrict__ logits,
-                                                   const int *__restrict__ num_classes,
-                                                   double *__restrict__ eig, int max_classes,
-                                                   int first)
+// Eigen-decomposition of one class by the calling 64-lane block: b = the 190
+// log-exchangeabilities, lg = 20 logits, out = A | lam | B | pi (840 doubles).
+__device__ void eigen_block(const double *__restrict__ b, const double *__restrict__ lg,
+                            double *__restrict__ out)
 {
     __shared__ double S[20][21];
     __shared__ double U[20][21];
@@ -49,12 +48,6 @@ __global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ bas
     __shared__ double red[64];
 
     const int lane = threadIdx.x;
-    const int p = first + blockIdx.x / max_classes;
-    const int k = blockIdx.x % max_classes;
-    if (k >= num_classes[p]) return;  // whole block exits together
-    const size_t g = (size_t)p * max_classes + k;
-    const double *b = base + (size_t)p * 190;
-    const double *lg = logits + g * 20;
 
     // pi = softmax(logits), TransitionProbabilityClass.cpp:101-109
     if (lane < 20) pi_s[lane] = exp(lg[lane]);
@@ -173,7 +166,6 @@ __global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ bas
         }
     }
 
-    double *out = eig + g * kEigStride;
     for (int idx = lane; idx < 400; idx += 64) {
         int i = idx / 20, kk = idx % 20;
         out[i * 20 + kk] = U[i][kk] / sp_s[i];          // A = Pi^{-1/2} U
@@ -183,6 +175,33 @@ __global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ bas
         out[kEigLam + lane] = S[lane][lane];
         out[kEigPi + lane] = pi_s[lane];
     }
+}
+
+__global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ base,
+                                                   const double *__restrict__ logits,
+                                                   const int *__restrict__ num_classes,
+                                                   double *__restrict__ eig, int max_classes,
+                                                   int first)
+{
+    const int p = first + blockIdx.x / max_classes;
+    const int k = blockIdx.x % max_classes;
+    if (k >= num_classes[p]) return;  // whole block exits together
+    const size_t g = (size_t)p * max_classes + k;
+    eigen_block(base + (size_t)p * 190, logits + g * 20, eig + g * kEigStride);
+}
+
+// Candidate profiles: block e * n_cand + c decomposes logits[e][c] with the base
+// matrix of particle ent_particle[e].
+__global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ base,
+                                                    const int *__restrict__ ent_particle,
+                                                    const int *__restrict__ ent_ncand,
+                                                    const double *__restrict__ logits, int n_cand,
+                                                    double *__restrict__ eig)
+{
+    const int e = blockIdx.x / n_cand, c = blockIdx.x % n_cand;
+    if (c >= ent_ncand[e]) return;
+    eigen_block(base + (size_t)ent_particle[e] * 190, logits + (size_t)blockIdx.x * 20,
+                eig + (size_t)blockIdx.x * kEigStride);
 }
 
 // One thread = one whole P matrix (particle, class, node, rate).  Every lane of
@@ -239,6 +258,14 @@ void launch_eigen(const EigenArgs &a, hipStream_t s)
     if (a.count <= 0) return;
     hipLaunchKernelGGL(k_eigen_f64, dim3(a.count * a.max_classes), dim3(64), 0, s, a.base,
                        a.logits, a.num_classes, a.eig, a.max_classes, a.first);
+}
+
+void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
+                       const double *logits, int count, int n_cand, double *eig, hipStream_t s)
+{
+    if (count <= 0 || n_cand <= 0) return;
+    hipLaunchKernelGGL(k_eigen_cand, dim3(count * n_cand), dim3(64), 0, s, base, ent_particle,
+                       ent_ncand, logits, n_cand, eig);
 }
 
 void launch_pmat(const PmatArgs &a, hipStream_t s)

@@ -11,8 +11,8 @@ LIB_PATH = os.path.join(HERE, "libsmccat_hip.so")
 
 SMC_OK, SMC_ERR_ARG, SMC_ERR_HIP, SMC_ERR_NOMEM, SMC_ERR_STATE = 0, 1, 2, 3, 4
 SMC_PREC_FP64, SMC_PREC_FP32 = 0, 1
-SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC = 0, 1, 2, 3, 4
-KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc")
+SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC, SMC_K_SITE = 0, 1, 2, 3, 4, 5
+KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc", "site")
 SMC_PRUNE_MFMA, SMC_PRUNE_VALU = 0, 1
 SMC_MIX_REFERENCE, SMC_MIX_LOGSPACE = 0, 1
 
@@ -50,6 +50,7 @@ SIGNATURES = {
     "smc_engine_device_bytes": (ctypes.c_int64, [_P]),
     "smc_engine_set_variant": (_I, [_P, _I, _I]),
     "smc_engine_set_mixture": (_I, [_P, _I]),
+    "smc_site_class_lnl": (_I, [_P, _I, _IP, _IP, _I, _IP, _DP, _D, _DP]),
 }
 
 _lib = None

@@ -160,6 +160,21 @@ class Engine:
                     "smc_systematic_resample")
         return out
 
+    def site_class_lnl(self, particles, sites, cand_logits, ncand=None, tempering=1.0):
+        """tempering * log L_site under candidate profiles; cand_logits [count, n_cand, 20]."""
+        particles = np.ascontiguousarray(particles, dtype=np.int32)
+        sites = np.ascontiguousarray(sites, dtype=np.int32)
+        cl = np.ascontiguousarray(cand_logits, dtype=np.float64)
+        count, n_cand = cl.shape[0], cl.shape[1]
+        if ncand is None:
+            ncand = np.full(count, n_cand, dtype=np.int32)
+        ncand = np.ascontiguousarray(ncand, dtype=np.int32)
+        out = np.empty((count, n_cand))
+        self._check(self.lib.smc_site_class_lnl(self.h, count, _ip(particles), _ip(sites), n_cand,
+                                                _ip(ncand), _dp(cl), float(tempering), _dp(out)),
+                    "smc_site_class_lnl")
+        return out
+
     def set_prune_variant(self, variant):
         """'mfma' (matrix cores, default) or 'valu'."""
         v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
