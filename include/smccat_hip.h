@@ -119,6 +119,24 @@ int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const
                        int n_cand, const int32_t *ncand, const double *cand_logits,
                        double tempering, double *out);
 
+/* Particle::gibbsPartitionMove(tempering) (Particle.cpp:785-995) for count
+ * particle slots at once: Neal's Algorithm 8 with m = 5 auxiliary profiles
+ * drawn from the logit prior, num_iter random-site updates (<= 0: numChar/2,
+ * :791), CRP concentration dpp_alpha (the reference fixes 0.01,
+ * Particle.hpp:106).  Each iteration scores one site per particle under all
+ * its classes and auxiliaries in one batched smc_site_class_lnl launch.  An
+ * emptied class is erased (:817-821).  Auxiliaries are capped so a particle
+ * never exceeds max_classes.  Random numbers: Philox4x32-10 keyed by
+ * (seed, slot).  The slots are re-uploaded with their new allocations;
+ * stats_out[3] (nullable) = sites moved, classes created, classes erased. */
+int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t *slots, double tempering,
+                             uint64_t seed, int num_iter, double dpp_alpha, int64_t *stats_out);
+
+/* Current allocation state of a slot (Particle::getAssignments / getCategories,
+ * Particle.hpp:73-74): num_classes, logits[max_classes][20], assignment[S]. */
+int smc_get_particle_state(smc_engine *e, int slot, int32_t *num_classes, double *logits,
+                           int32_t *assignment);
+
 /* discretizeGamma (Miscellaneous.hpp:22-37): out[num] mean-one gamma rates. */
 int smc_discretize_gamma(double shape, int num, double *out);
 

@@ -1,0 +1,109 @@
+// Engine state shared by the translation units of the C-ABI (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "smccat_hip.h"
+
+using namespace smccat;
+
+struct PendingTiming {
+    int family;
+    hipEvent_t a, b;
+};
+
+struct smc_engine {
+    smc_engine_config cfg{};
+    int T = 0, S = 0, S_pad = 0, R = 0, nn = 0, Pmax = 0, Kmax = 0, tile_stride = 0;
+    int Wmax = 0;          // partial slots (particles per wave)
+    int64_t Cmax = 0;      // classes per wave in Pmat
+    bool fp32 = false;
+    size_t tx_size = 8;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t device_bytes = 0;
+    bool have_alignment = false;
+    int prune_variant = SMC_PRUNE_MFMA;
+    int mix_mode = SMC_MIX_REFERENCE;
+
+    // device buffers
+    uint8_t *d_codes = nullptr;
+    double *d_isinv = nullptr;
+    int *d_invchar = nullptr;
+    int *d_children = nullptr;
+    double *d_blen = nullptr;
+    double *d_base = nullptr;
+    double *d_logits = nullptr;
+    int *d_nclass = nullptr;
+    double *d_pinv = nullptr;
+    double *d_rates = nullptr;
+    int *d_root = nullptr;
+    int *d_perm = nullptr;
+    Tile *d_tiles = nullptr;
+    int *d_ntiles = nullptr;
+    double *d_eig = nullptr;
+    void *d_P = nullptr;
+    void *d_X = nullptr;
+    int *d_E = nullptr;
+    double *d_tile_lnl = nullptr;
+    double *d_lnl = nullptr;
+    uint8_t *d_nonfinite = nullptr;
+    Item *d_items = nullptr;
+    // SMC scratch (grown on demand)
+    int smc_cap = 0;
+    double *d_w = nullptr, *d_l = nullptr, *d_norm = nullptr, *d_out2 = nullptr;
+    int *d_assign = nullptr;
+
+    // host mirrors
+    std::vector<int> h_children;  // [P][T-1][2]
+    std::vector<int> h_root;      // [P]
+    std::vector<int> h_height;    // [P][T-1]
+    std::vector<int> h_valid;     // [P] particle uploaded
+    std::vector<int> h_nclass;    // [P] classes per particle
+    // host copies of the uploaded particle state (for moves that edit it)
+    std::vector<double> h_blen;   // [P][nn]
+    std::vector<double> h_base;   // [P][190]
+    std::vector<double> h_logits; // [P][Kmax][20]
+    std::vector<int> h_assign;    // [P][S]
+    std::vector<double> h_pinv, h_shape;  // [P]
+    int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
+    int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
+    // site x candidate scoring scratch (grown on demand)
+    int64_t sc_cap = 0;           // entries * candidates
+    int sc_ent_cap = 0;
+    int *d_sc_ent = nullptr;      // [3][entries]: particle, site, candidate count
+    double *d_sc_logits = nullptr, *d_sc_eig = nullptr, *d_sc_rate = nullptr, *d_sc_out = nullptr;
+
+    // pruning schedule cache: waves of particles, each with its level/kind buckets
+    struct Wave {
+        int first, count;
+        int root_off;                 // roots of the wave's particles
+        int max_level;
+        std::vector<int> bucket_off;  // items of (level L, kind k): [off[3L+k], off[3L+k+1])
+    };
+    int sched_first = -1, sched_count = -1;
+    uint64_t sched_version = 0, topo_version = 1;
+    std::vector<Wave> waves;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<PendingTiming> pending;
+    int64_t launches[SMC_K_COUNT] = {};
+    double total_ms[SMC_K_COUNT] = {};
+};
+
+int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+#define HIP_TRY(e, call)                                                                      \
+    do {                                                                                      \
+        hipError_t _st = (call);                                                              \
+        if (_st != hipSuccess)                                                                \
+            return smc_fail((e), SMC_ERR_HIP,                                                 \
+                            std::string(#call) + " failed: " + hipGetErrorString(_st));      \
+    } while (0)

@@ -51,6 +51,9 @@ SIGNATURES = {
     "smc_engine_set_variant": (_I, [_P, _I, _I]),
     "smc_engine_set_mixture": (_I, [_P, _I]),
     "smc_site_class_lnl": (_I, [_P, _I, _IP, _IP, _I, _IP, _DP, _D, _DP]),
+    "smc_gibbs_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D,
+                                      ctypes.POINTER(ctypes.c_int64)]),
+    "smc_get_particle_state": (_I, [_P, _I, _IP, _DP, _IP]),
 }
 
 _lib = None

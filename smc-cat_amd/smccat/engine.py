@@ -175,6 +175,24 @@ class Engine:
                     "smc_site_class_lnl")
         return out
 
+    def gibbs_partition_move(self, slots, tempering, seed, num_iter=0, dpp_alpha=0.01):
+        """Particle::gibbsPartitionMove for several slots; returns (moved, created, erased)."""
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        stats = (ctypes.c_int64 * 3)()
+        self._check(self.lib.smc_gibbs_partition_move(self.h, len(slots), _ip(slots), float(tempering),
+                                                      int(seed), int(num_iter), float(dpp_alpha), stats),
+                    "smc_gibbs_partition_move")
+        return tuple(int(v) for v in stats)
+
+    def particle_state(self, slot):
+        """(num_classes, logits [K, 20], assignment [S]) of a slot."""
+        nc = ctypes.c_int32()
+        lg = np.zeros((self.max_classes, 20))
+        asg = np.zeros(self.sites, dtype=np.int32)
+        self._check(self.lib.smc_get_particle_state(self.h, int(slot), ctypes.byref(nc), _dp(lg), _ip(asg)),
+                    "smc_get_particle_state")
+        return nc.value, lg[:nc.value].copy(), asg
+
     def set_prune_variant(self, variant):
         """'mfma' (matrix cores, default) or 'valu'."""
         v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
