@@ -58,6 +58,9 @@ typedef struct {
                               over more runs in waves (0: max_particles)             */
     int64_t pmat_classes;  /* classes whose P(t) are resident at once
                               (0: partial_slots * max_classes)                       */
+    int32_t workspaces;    /* partial workspaces per particle: 2 enables the
+                              incremental refresh with accept/reject (0 or 1: one)    */
+    int32_t reserved;      /* must be 0                                              */
 } smc_engine_config;
 
 /* Engine lifetime.  Replaces the per-thread Particle construction of
@@ -95,6 +98,34 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
  * when the particle's lnL is NaN/Inf (the reference's NaN guard, main.cpp:158-174). */
 int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                            uint8_t *nonfinite_out);
+
+/* Incremental likelihood with accept/reject (engines created with workspaces = 2).
+ * The slots' partials must be resident together: count <= partial_slots and
+ * their classes fit pmat_classes.
+ *
+ * smc_update_params: new topology / branch lengths / log-exchangeabilities /
+ * pInvar / gamma shape for slots [first, first+count) (classes untouched) --
+ * what a tree, rate-matrix, shape or invariant move changes (Particle.cpp:677-783,
+ * Tree.cpp:553-811).
+ *
+ * smc_refresh_incremental: Particle::refreshLikelihood(false) (Particle.cpp:530-675).
+ * node_flags[count][2T-1]: bit 0 = updateCL, bit 1 = updateTP (Tree.hpp:26-27);
+ * eig_flags[count] (nullable): the exchangeabilities changed, so every class is
+ * re-decomposed and every P recomputed.  Dirty internal nodes (flagged, parents
+ * of changed branches, and their ancestors) write their other workspace
+ * (Particle.cpp:566-571); the rest keep theirs; P is recomputed for particles
+ * with any changed branch.
+ *
+ * smc_accept_reject: Particle::accept / reject (Particle.cpp:343-478) per slot:
+ * accept[i] != 0 commits flags, lnL and parameters; 0 returns to the last
+ * committed state (flags, lnL, parameters, eigensystems and P).  A forced
+ * refresh (smc_refresh_likelihood) also commits. */
+int smc_update_params(smc_engine *e, int first, int count, const int32_t *children,
+                      const double *branch_len, const double *base, const double *pinv,
+                      const double *shape);
+int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
+                            const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out);
+int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept);
 
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */

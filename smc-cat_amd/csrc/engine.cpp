@@ -172,6 +172,111 @@ int build_schedule(smc_engine *e, int first, int count)
 
 }  // namespace
 
+void smc_commit_range(smc_engine *e, int first, int count)
+{
+    const int Ti = e->T - 1, nn = e->nn;
+    for (int p = first; p < first + count; ++p) {
+        std::copy(e->h_xflag.begin() + (size_t)p * Ti, e->h_xflag.begin() + (size_t)(p + 1) * Ti,
+                  e->h_xflag_old.begin() + (size_t)p * Ti);
+        e->h_lnl_old[p] = e->h_lnl[p];
+        e->h_dirty[p] = 0;
+        smc_engine::Committed &c = e->committed[p];
+        c.children.assign(e->h_children.begin() + (size_t)p * Ti * 2,
+                          e->h_children.begin() + (size_t)(p + 1) * Ti * 2);
+        c.blen.assign(e->h_blen.begin() + (size_t)p * nn, e->h_blen.begin() + (size_t)(p + 1) * nn);
+        c.base.assign(e->h_base.begin() + (size_t)p * 190, e->h_base.begin() + (size_t)(p + 1) * 190);
+        c.pinv = e->h_pinv[p];
+        c.shape = e->h_shape[p];
+    }
+}
+
+int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *heights, int *prog_out)
+{
+    const int T = e->T, Ti = T - 1, nn = e->nn;
+    // topology checks: every non-root node has exactly one parent
+    std::vector<int> parent(nn, -1);
+    for (int k = 0; k < Ti; ++k)
+        for (int c = 0; c < 2; ++c) {
+            int d = ch[2 * k + c];
+            if (d < 0 || d >= nn || d == T + k || parent[d] != -1)
+                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
+                                                ": invalid children array");
+            parent[d] = T + k;
+        }
+    int root = -1, nroots = 0;
+    for (int n = T; n < nn; ++n)
+        if (parent[n] == -1) {
+            root = n;
+            nroots++;
+        }
+    for (int t = 0; t < T; ++t)
+        if (parent[t] == -1) nroots = -1;
+    if (nroots != 1)
+        return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": not a rooted binary tree");
+    // heights (tips 0); iterate in an order where children come first
+    std::vector<int> h(nn, 0), order;
+    order.reserve(Ti);
+    {
+        std::vector<int> st{root};
+        while (!st.empty()) {
+            int n = st.back();
+            st.pop_back();
+            order.push_back(n);
+            for (int c = 0; c < 2; ++c) {
+                int d = ch[2 * (n - T) + c];
+                if (d >= T) st.push_back(d);
+            }
+        }
+        if ((int)order.size() != Ti)
+            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": disconnected tree");
+        for (int k = Ti - 1; k >= 0; --k) {
+            int n = order[k];
+            h[n] = 1 + std::max(h[ch[2 * (n - T)]], h[ch[2 * (n - T) + 1]]);
+        }
+    }
+    // evaluation program for single-site scoring: post-order that descends into
+    // the child with the larger Strahler number first, so at most
+    // max-Strahler + 1 internal partials are live; checked against kSiteStack
+    {
+        std::vector<int> st(nn, 0);
+        for (int k = Ti - 1; k >= 0; --k) {
+            const int n = order[k];
+            const int a = st[ch[2 * (n - T)]], b = st[ch[2 * (n - T) + 1]];
+            st[n] = a == b ? a + 1 : std::max(a, b);
+        }
+        std::vector<int> prog;
+        prog.reserve(Ti);
+        std::vector<std::pair<int, int>> stk{{root, 0}};
+        while (!stk.empty()) {
+            auto &[n, state] = stk.back();
+            if (state == 0) {
+                state = 1;
+                int a = ch[2 * (n - T)], b = ch[2 * (n - T) + 1];
+                if (st[b] > st[a]) std::swap(a, b);
+                // push b first so a (the larger subtree) is evaluated first
+                if (b >= T) stk.push_back({b, 0});
+                if (a >= T) stk.push_back({a, 0});
+            } else {
+                prog.push_back(n);
+                stk.pop_back();
+            }
+        }
+        int depth = 0, maxd = 0;
+        for (int n : prog) {
+            depth -= (ch[2 * (n - T)] >= T) + (ch[2 * (n - T) + 1] >= T);
+            if (depth < 0) return fail(e, SMC_ERR_ARG, "internal error: scoring program");
+            if (n != root) maxd = std::max(maxd, ++depth);
+        }
+        if ((int)prog.size() != Ti || maxd > kSiteStack)
+            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
+                                            ": tree too deep for single-site scoring");
+        std::copy(prog.begin(), prog.end(), prog_out);
+    }
+    *root_out = root;
+    for (int k = 0; k < Ti; ++k) heights[k] = h[T + k];
+    return SMC_OK;
+}
+
 extern "C" {
 
 int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
@@ -181,7 +286,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     if (cfg->taxa < 3 || cfg->sites < 1 || cfg->num_rates < 1 || cfg->num_rates > 64 ||
         cfg->max_particles < 1 || cfg->max_classes < 1 || cfg->max_classes > 4096 ||
         (cfg->precision != SMC_PREC_FP64 && cfg->precision != SMC_PREC_FP32) ||
-        cfg->partial_slots < 0 || cfg->pmat_classes < 0)
+        cfg->partial_slots < 0 || cfg->pmat_classes < 0 || cfg->workspaces < 0 ||
+        cfg->workspaces > 2 || cfg->reserved != 0)
         return SMC_ERR_ARG;
     smc_engine *e = new smc_engine();
     e->cfg = *cfg;
@@ -192,6 +298,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->nn = 2 * cfg->taxa - 1;
     e->Pmax = cfg->max_particles;
     e->Kmax = cfg->max_classes;
+    e->WS = cfg->workspaces == 2 ? 2 : 1;
     e->Wmax = cfg->partial_slots > 0 ? std::min(cfg->partial_slots, cfg->max_particles)
                                      : cfg->max_particles;
     e->Cmax = cfg->pmat_classes > 0 ? cfg->pmat_classes : (int64_t)e->Wmax * cfg->max_classes;
@@ -236,14 +343,17 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_items, P * (T - 1));
     A(&e->d_pcls_off, P);
     A(&e->d_prog, P * (T - 1));
+    A(&e->d_plist, 2 * P);
     const size_t W = e->Wmax, C = (size_t)e->Cmax;
-    A(&e->d_E, W * (T - 1) * R * Sp);
+    const size_t WS = e->WS;
+    A(&e->d_E, W * WS * (T - 1) * R * Sp);
+    A(&e->d_xflag, P * (T - 1));
     if (e->fp32) {
         A((float **)&e->d_P, C * nn * R * kPStride);
-        A((float **)&e->d_X, W * (T - 1) * R * 20 * Sp);
+        A((float **)&e->d_X, W * WS * (T - 1) * R * 20 * Sp);
     } else {
         A((double **)&e->d_P, C * nn * R * kPStride);
-        A((double **)&e->d_X, W * (T - 1) * R * 20 * Sp);
+        A((double **)&e->d_X, W * WS * (T - 1) * R * 20 * Sp);
     }
     if (rc != SMC_OK) {
         std::string msg = e->err;
@@ -253,6 +363,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     }
     (void)hipMemsetAsync(e->d_ntiles, 0, P * sizeof(int), e->stream);
     (void)hipMemsetAsync(e->d_nclass, 0, P * sizeof(int), e->stream);
+    (void)hipMemsetAsync(e->d_xflag, 0, P * (T - 1), e->stream);
     (void)hipStreamSynchronize(e->stream);
     e->h_children.assign(P * (T - 1) * 2, 0);
     e->h_root.assign(P, -1);
@@ -265,6 +376,12 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_assign.assign(P * (size_t)e->S, 0);
     e->h_pinv.assign(P, 0.0);
     e->h_shape.assign(P, 1.0);
+    e->h_xflag.assign(P * (T - 1), 0);
+    e->h_xflag_old.assign(P * (T - 1), 0);
+    e->h_lnl.assign(P, 0.0);
+    e->h_lnl_old.assign(P, 0.0);
+    e->h_dirty.assign(P, 0);
+    e->committed.resize(P);
     *out = e;
     return SMC_OK;
 }
@@ -277,7 +394,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
-                    e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent,
+                    e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (void *b : bufs)
@@ -350,84 +467,13 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const int32_t *ch = children + (size_t)i * Ti * 2;
-        // topology checks: every non-root node has exactly one parent
-        std::vector<int> parent(nn, -1);
-        for (int k = 0; k < Ti; ++k)
-            for (int c = 0; c < 2; ++c) {
-                int d = ch[2 * k + c];
-                if (d < 0 || d >= nn || d == T + k || parent[d] != -1)
-                    return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
-                                                    ": invalid children array");
-                parent[d] = T + k;
-            }
-        int root = -1, nroots = 0;
-        for (int n = T; n < nn; ++n)
-            if (parent[n] == -1) {
-                root = n;
-                nroots++;
-            }
-        for (int t = 0; t < T; ++t)
-            if (parent[t] == -1) nroots = -1;
-        if (nroots != 1)
-            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": not a rooted binary tree");
-        // heights (tips 0); iterate in an order where children come first
-        std::vector<int> h(nn, 0), order;
-        order.reserve(Ti);
+        int root = -1;
+        std::vector<int> h(nn, 0);
         {
-            std::vector<int> st{root};
-            while (!st.empty()) {
-                int n = st.back();
-                st.pop_back();
-                order.push_back(n);
-                for (int c = 0; c < 2; ++c) {
-                    int d = ch[2 * (n - T) + c];
-                    if (d >= T) st.push_back(d);
-                }
-            }
-            if ((int)order.size() != Ti)
-                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": disconnected tree");
-            for (int k = Ti - 1; k >= 0; --k) {
-                int n = order[k];
-                h[n] = 1 + std::max(h[ch[2 * (n - T)]], h[ch[2 * (n - T) + 1]]);
-            }
-        }
-        // evaluation program for single-site scoring: post-order that descends into
-        // the child with the larger Strahler number first, so at most
-        // max-Strahler + 1 internal partials are live; checked against kSiteStack
-        {
-            std::vector<int> st(nn, 0);
-            for (int k = Ti - 1; k >= 0; --k) {
-                const int n = order[k];
-                const int a = st[ch[2 * (n - T)]], b = st[ch[2 * (n - T) + 1]];
-                st[n] = a == b ? a + 1 : std::max(a, b);
-            }
-            std::vector<int> prog;
-            prog.reserve(Ti);
-            std::vector<std::pair<int, int>> stk{{root, 0}};
-            while (!stk.empty()) {
-                auto &[n, state] = stk.back();
-                if (state == 0) {
-                    state = 1;
-                    int a = ch[2 * (n - T)], b = ch[2 * (n - T) + 1];
-                    if (st[b] > st[a]) std::swap(a, b);
-                    // push b first so a (the larger subtree) is evaluated first
-                    if (b >= T) stk.push_back({b, 0});
-                    if (a >= T) stk.push_back({a, 0});
-                } else {
-                    prog.push_back(n);
-                    stk.pop_back();
-                }
-            }
-            int depth = 0, maxd = 0;
-            for (int n : prog) {
-                depth -= (ch[2 * (n - T)] >= T) + (ch[2 * (n - T) + 1] >= T);
-                if (depth < 0) return fail(e, SMC_ERR_ARG, "internal error: scoring program");
-                if (n != root) maxd = std::max(maxd, ++depth);
-            }
-            if ((int)prog.size() != Ti || maxd > kSiteStack)
-                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
-                                                ": tree too deep for single-site scoring");
-            std::copy(prog.begin(), prog.end(), progs.begin() + (size_t)i * Ti);
+            std::vector<int> hh(Ti);
+            int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * Ti);
+            if (rc != SMC_OK) return rc;
+            for (int k = 0; k < Ti; ++k) h[T + k] = hh[k];
         }
         for (int k = 0; k < Ti; ++k) {
             e->h_height[(size_t)p * Ti + k] = h[T + k];
@@ -526,6 +572,14 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
     int rc = build_schedule(e, first, count);
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
+    const int Ti = e->T - 1;
+    if (e->WS == 2) {
+        // forced refresh: every internal node writes its other workspace
+        // (Particle.cpp:566-571), so the committed partials stay intact until commit
+        for (size_t k = (size_t)first * Ti; k < (size_t)(first + count) * Ti; ++k) e->h_xflag[k] ^= 1;
+        HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
+                                  (size_t)count * Ti, hipMemcpyHostToDevice, s));
+    }
     for (const smc_engine::Wave &w : e->waves) {
         {
             TimedLaunch tl(e, SMC_K_EIGEN);
@@ -563,6 +617,8 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
         a.max_classes = e->Kmax;
         a.fp32 = e->fp32;
         a.mix_mode = e->mix_mode;
+        a.xflag = e->WS == 2 ? e->d_xflag : nullptr;
+        a.WS = e->WS;
         auto run = [&](int off, int n, int kind, bool roots) {
             if (n <= 0) return;
             TimedLaunch tl(e, SMC_K_PRUNE);
@@ -592,12 +648,316 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                           e->d_nonfinite, s);
     }
     HIP_TRY(e, hipGetLastError());
-    if (lnl_out)
-        HIP_TRY(e, hipMemcpyAsync(lnl_out, e->d_lnl + first, (size_t)count * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipMemcpyAsync(e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
     if (nonfinite_out)
         HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
                                   hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
+    smc_commit_range(e, first, count);
+    return SMC_OK;
+}
+
+// ------------------------------------------------------------ incremental path
+namespace {
+
+// Compact class slots of one resident wave [first, first+count); fails if the
+// wave does not fit the partial slots or the P buffer.
+int set_single_wave(smc_engine *e, int first, int count)
+{
+    if (count > e->Wmax) return fail(e, SMC_ERR_ARG, "range exceeds partial_slots");
+    std::vector<int> pcls(count);
+    int64_t ncls = 0;
+    for (int i = 0; i < count; ++i) {
+        pcls[i] = (int)ncls;
+        ncls += e->h_nclass[first + i];
+    }
+    if (ncls > e->Cmax) return fail(e, SMC_ERR_ARG, "range exceeds pmat_classes");
+    HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), count * sizeof(int),
+                              hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));  // pcls is a local buffer
+    return SMC_OK;
+}
+
+// Eigen-decompositions and/or P matrices of the particles in the lists (the
+// lists must stay alive until the caller synchronises the stream).
+int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
+                     const std::vector<int> &p_list)
+{
+    hipStream_t s = e->stream;
+    if (!eig_list.empty()) {
+        HIP_TRY(e, hipMemcpyAsync(e->d_plist, eig_list.data(), eig_list.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, s));
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        EigenArgs ea{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first,
+                     (int)eig_list.size(), e->d_plist};
+        launch_eigen(ea, s);
+    }
+    if (!p_list.empty()) {
+        int *dl = e->d_plist + e->Pmax;  // second half: the eigen launch may still read the first
+        HIP_TRY(e, hipMemcpyAsync(dl, p_list.data(), p_list.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, s));
+        TimedLaunch tl(e, SMC_K_PMAT);
+        PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
+                    e->R, e->Kmax, first, (int)p_list.size(), e->fp32, dl};
+        launch_pmat(pa, s);
+    }
+    return SMC_OK;
+}
+
+// Uploads new topology/branch lengths/exchangeabilities/pInvar/shape for slots
+// [first, first+count) without touching their CAT classes.
+int upload_params(smc_engine *e, int first, int count, const int32_t *children,
+                  const double *branch_len, const double *base, const double *pinv,
+                  const double *shape)
+{
+    const int T = e->T, Ti = T - 1, nn = e->nn, R = e->R;
+    std::vector<double> blen((size_t)count * nn), rates((size_t)count * R, 1.0);
+    std::vector<int> progs((size_t)count * Ti), roots(count);
+    for (int i = 0; i < count; ++i) {
+        const int p = first + i;
+        if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
+        const int32_t *ch = children + (size_t)i * Ti * 2;
+        int root = -1;
+        std::vector<int> hh(Ti);
+        int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * Ti);
+        if (rc != SMC_OK) return rc;
+        if (R > 1) {
+            if (!(shape[i] > 0.0)) return fail(e, SMC_ERR_ARG, "shape must be > 0");
+            discretize_gamma(shape[i], R, &rates[(size_t)i * R]);
+        }
+        for (int k = 0; k < Ti; ++k) {
+            e->h_height[(size_t)p * Ti + k] = hh[k];
+            e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
+            e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
+        }
+        e->h_root[p] = root;
+        roots[i] = root;
+        for (int n = 0; n < nn; ++n) blen[(size_t)i * nn + n] = n == root ? 0.0 : branch_len[(size_t)i * nn + n];
+        std::copy(blen.begin() + (size_t)i * nn, blen.begin() + (size_t)(i + 1) * nn,
+                  e->h_blen.begin() + (size_t)p * nn);
+        std::copy(base + (size_t)i * 190, base + (size_t)(i + 1) * 190, e->h_base.begin() + (size_t)p * 190);
+        e->h_pinv[p] = pinv[i];
+        e->h_shape[p] = shape[i];
+    }
+    hipStream_t s = e->stream;
+    const size_t P0 = first;
+    HIP_TRY(e, hipMemcpyAsync(e->d_children + P0 * Ti * 2, e->h_children.data() + P0 * Ti * 2,
+                              (size_t)count * Ti * 2 * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_blen + P0 * nn, blen.data(), blen.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_base + P0 * 190, e->h_base.data() + P0 * 190,
+                              (size_t)count * 190 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_pinv + P0, e->h_pinv.data() + P0, (size_t)count * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_rates + P0 * R, rates.data(), rates.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_root + P0, roots.data(), roots.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * Ti, progs.data(), progs.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    e->topo_version++;
+    return SMC_OK;
+}
+
+}  // namespace
+
+int smc_update_params(smc_engine *e, int first, int count, const int32_t *children,
+                      const double *branch_len, const double *base, const double *pinv,
+                      const double *shape)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (first < 0 || count < 1 || first + count > e->Pmax || !children || !branch_len || !base ||
+        !pinv || !shape)
+        return fail(e, SMC_ERR_ARG, "invalid parameter update");
+    return upload_params(e, first, count, children, branch_len, base, pinv, shape);
+}
+
+int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
+                            const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (e->WS != 2) return fail(e, SMC_ERR_STATE, "incremental refresh needs workspaces = 2");
+    if (first < 0 || count < 1 || first + count > e->Pmax || !node_flags)
+        return fail(e, SMC_ERR_ARG, "invalid incremental refresh arguments");
+    for (int p = first; p < first + count; ++p)
+        if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
+    int rc = set_single_wave(e, first, count);
+    if (rc != SMC_OK) return rc;
+    const int T = e->T, Ti = T - 1, nn = e->nn;
+    std::vector<int> eig_list, p_list;
+    std::vector<std::vector<Item>> buckets;  // (level * 3 + kind)
+    std::vector<Item> roots;
+    std::vector<uint8_t> dirty(nn);
+    std::vector<int> parent(nn);
+    for (int i = 0; i < count; ++i) {
+        const int p = first + i;
+        const uint8_t *nf = node_flags + (size_t)i * nn;
+        const bool eig = eig_flags && eig_flags[i];
+        bool any_tp = eig;
+        for (int n = 0; n < nn; ++n) any_tp |= (nf[n] & 2) != 0;
+        if (eig) eig_list.push_back(p);
+        if (any_tp) p_list.push_back(p);
+        e->h_dirty[p] |= (any_tp ? 1 : 0) | (eig ? 2 : 0);
+        // dirty internal nodes: flagged, parents of changed branches, and every
+        // ancestor of a dirty node (Tree.cpp marks the path to the root the same way)
+        const int *ch = &e->h_children[(size_t)p * Ti * 2];
+        std::fill(parent.begin(), parent.end(), -1);
+        for (int k = 0; k < Ti; ++k) {
+            parent[ch[2 * k]] = T + k;
+            parent[ch[2 * k + 1]] = T + k;
+        }
+        std::fill(dirty.begin(), dirty.end(), 0);
+        for (int n = 0; n < nn; ++n) {
+            if (eig || (nf[n] & 2)) {
+                if (parent[n] >= 0) dirty[parent[n]] = 1;
+                if (eig && n >= T) dirty[n] = 1;
+            }
+            if ((nf[n] & 1) && n >= T) dirty[n] = 1;
+        }
+        for (int n = T; n < nn; ++n)
+            if (dirty[n])
+                for (int q = parent[n]; q >= 0 && !dirty[q]; q = parent[q]) dirty[q] = 1;
+        const int root = e->h_root[p];
+        for (int n = T; n < nn; ++n) {
+            if (!dirty[n] || n == root) continue;
+            e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
+            const int L = e->h_height[(size_t)p * Ti + (n - T)];
+            const int kind = (ch[2 * (n - T)] >= T) + (ch[2 * (n - T) + 1] >= T);
+            if ((int)buckets.size() < 3 * (L + 1)) buckets.resize(3 * (L + 1));
+            buckets[3 * L + kind].push_back(Item{p, n});
+        }
+        roots.push_back(Item{p, root});
+    }
+    rc = recompute_models(e, first, eig_list, p_list);
+    if (rc != SMC_OK) return rc;
+    hipStream_t s = e->stream;
+    HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
+                              (size_t)count * Ti, hipMemcpyHostToDevice, s));
+    std::vector<Item> items;
+    std::vector<int> off;
+    for (auto &b : buckets) {
+        off.push_back((int)items.size());
+        items.insert(items.end(), b.begin(), b.end());
+    }
+    off.push_back((int)items.size());
+    const int root_off = (int)items.size();
+    items.insert(items.end(), roots.begin(), roots.end());
+    HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
+                              hipMemcpyHostToDevice, s));
+    e->sched_first = -1;  // the item buffer no longer holds the full schedule
+    PruneArgs a{};
+    a.tiles = e->d_tiles;
+    a.ntiles = e->d_ntiles;
+    a.tile_stride = e->tile_stride;
+    a.children = e->d_children;
+    a.root_of = e->d_root;
+    a.perm = e->d_perm;
+    a.codes = e->d_codes;
+    a.P = e->d_P;
+    a.pcls_off = e->d_pcls_off;
+    a.eig = e->d_eig;
+    a.pinv = e->d_pinv;
+    a.isinv = e->d_isinv;
+    a.invchar = e->d_invchar;
+    a.X = e->d_X;
+    a.E = e->d_E;
+    a.p_base = first;
+    a.tile_lnl = e->d_tile_lnl;
+    a.T = T;
+    a.R = e->R;
+    a.S_pad = e->S_pad;
+    a.max_classes = e->Kmax;
+    a.fp32 = e->fp32;
+    a.mix_mode = e->mix_mode;
+    a.xflag = e->d_xflag;
+    a.WS = 2;
+    auto run = [&](int o, int n, int kind, bool is_roots) {
+        if (n <= 0) return;
+        TimedLaunch tl(e, SMC_K_PRUNE);
+        a.items = e->d_items + o;
+        a.n_items = n;
+        a.kind = kind;
+        a.roots = is_roots;
+        if (e->prune_variant == SMC_PRUNE_VALU)
+            launch_prune(a, s);
+        else
+            launch_prune_mfma(a, s);
+    };
+    for (size_t b = 0; b + 1 < off.size(); ++b)
+        run(off[b], off[b + 1] - off[b], e->prune_variant == SMC_PRUNE_VALU ? 3 : (int)(b % 3), false);
+    run(root_off, (int)roots.size(), 3, true);
+    {
+        TimedLaunch tl(e, SMC_K_REDUCE);
+        launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,
+                          e->d_nonfinite, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipMemcpyAsync(e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    if (nonfinite_out)
+        HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
+                                  hipMemcpyDeviceToHost, s));
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
+    return SMC_OK;
+}
+
+int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (first < 0 || count < 1 || first + count > e->Pmax || !accept)
+        return fail(e, SMC_ERR_ARG, "invalid accept/reject arguments");
+    const int T = e->T, Ti = T - 1, nn = e->nn;
+    std::vector<int> eig_list, p_list, rej;
+    for (int i = 0; i < count; ++i) {
+        const int p = first + i;
+        if (accept[i]) {
+            smc_commit_range(e, p, 1);
+            continue;
+        }
+        rej.push_back(p);
+        std::copy(e->h_xflag_old.begin() + (size_t)p * Ti, e->h_xflag_old.begin() + (size_t)(p + 1) * Ti,
+                  e->h_xflag.begin() + (size_t)p * Ti);
+        e->h_lnl[p] = e->h_lnl_old[p];
+        if (e->h_dirty[p] & 2) eig_list.push_back(p);
+        if (e->h_dirty[p] & 3) p_list.push_back(p);
+    }
+    hipStream_t s = e->stream;
+    // restore the committed parameters of rejected particles (contiguous runs)
+    for (size_t k = 0; k < rej.size();) {
+        size_t j = k;
+        while (j + 1 < rej.size() && rej[j + 1] == rej[j] + 1) ++j;
+        const int r0 = rej[k], rn = (int)(j - k + 1);
+        std::vector<int32_t> ch((size_t)rn * Ti * 2);
+        std::vector<double> bl((size_t)rn * nn), base((size_t)rn * 190), pv(rn), sh(rn);
+        for (int q = 0; q < rn; ++q) {
+            const smc_engine::Committed &c = e->committed[r0 + q];
+            std::copy(c.children.begin(), c.children.end(), ch.begin() + (size_t)q * Ti * 2);
+            std::copy(c.blen.begin(), c.blen.end(), bl.begin() + (size_t)q * nn);
+            std::copy(c.base.begin(), c.base.end(), base.begin() + (size_t)q * 190);
+            pv[q] = c.pinv;
+            sh[q] = c.shape;
+        }
+        int rc = upload_params(e, r0, rn, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
+        if (rc != SMC_OK) return rc;
+        k = j + 1;
+    }
+    if (!p_list.empty()) {
+        int rc = set_single_wave(e, first, count);
+        if (rc != SMC_OK) return rc;
+        rc = recompute_models(e, first, eig_list, p_list);
+        if (rc != SMC_OK) return rc;
+    }
+    for (int p : rej) e->h_dirty[p] = 0;
+    HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
+                              (size_t)count * Ti, hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_lnl + first, e->h_lnl.data() + first, (size_t)count * sizeof(double),
+                              hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     collect_timings(e);
     return SMC_OK;

@@ -21,6 +21,7 @@ struct smc_engine {
     smc_engine_config cfg{};
     int T = 0, S = 0, S_pad = 0, R = 0, nn = 0, Pmax = 0, Kmax = 0, tile_stride = 0;
     int Wmax = 0;          // partial slots (particles per wave)
+    int WS = 1;            // workspaces per particle: 2 enables accept/reject
     int64_t Cmax = 0;      // classes per wave in Pmat
     bool fp32 = false;
     size_t tx_size = 8;
@@ -73,6 +74,18 @@ struct smc_engine {
     std::vector<double> h_pinv, h_shape;  // [P]
     int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
     int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
+    uint8_t *d_xflag = nullptr;   // [P][T-1] current workspace of each internal node
+    int *d_plist = nullptr;       // [P] particle list scratch for partial recomputes
+    // accept/reject state (Particle.cpp:343-478): committed flags, lnL and parameters
+    std::vector<uint8_t> h_xflag, h_xflag_old;  // [P][T-1]
+    std::vector<double> h_lnl, h_lnl_old;       // [P]
+    std::vector<uint8_t> h_dirty;               // [P] bit0 P changed, bit1 eigen changed
+    struct Committed {
+        std::vector<int32_t> children;
+        std::vector<double> blen, base;
+        double pinv = 0.0, shape = 1.0;
+    };
+    std::vector<Committed> committed;           // [P] parameters at the last commit
     // site x candidate scoring scratch (grown on demand)
     int64_t sc_cap = 0;           // entries * candidates
     int sc_ent_cap = 0;
@@ -99,6 +112,15 @@ struct smc_engine {
 };
 
 int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+// Commits the current state of slots [first, first+count): workspace flags,
+// lnL and parameters become the ones a rejected proposal returns to.
+void smc_commit_range(smc_engine *e, int first, int count);
+
+// Validates the children array of particle p ([T-1][2]) and derives its root,
+// the heights of its internal nodes and the Strahler-ordered scoring program.
+int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *heights,
+                 int *prog_out);
 
 #define HIP_TRY(e, call)                                                                      \
     do {                                                                                      \

@@ -206,7 +206,7 @@ ParticleEngine::ParticleEngine(const Alignment &a, int numRates, int maxParticle
     : aln(a), rates(numRates), kmax(maxClasses), lg(usingLG)
 {
     smc_engine_config cfg{device, a.getNumTaxa(), a.getNumChar(), numRates, maxParticles,
-                          maxClasses, precision, 0, 0};
+                          maxClasses, precision, 0, 0, 1, 0};
     check(nullptr, smc_engine_create(&cfg, &eng), "smc_engine_create");
     check(eng, smc_set_alignment(eng, a.data().data()), "smc_set_alignment");
     slots.resize(maxParticles);

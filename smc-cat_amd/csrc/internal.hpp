@@ -42,6 +42,7 @@ struct EigenArgs {
     int max_classes;
     int first;               // first particle
     int count;               // particles
+    const int *plist;        // optional [count] particle list (else first..first+count-1)
 };
 void launch_eigen(const EigenArgs &a, hipStream_t s);
 
@@ -54,6 +55,7 @@ struct PmatArgs {
     void *P;                 // [wave classes][nn][R][420] (double or float)
     int nn, R, max_classes, first, count;
     bool fp32;
+    const int *plist;        // optional [count] particle list (else first..first+count-1)
 };
 void launch_pmat(const PmatArgs &a, hipStream_t s);
 
@@ -76,6 +78,8 @@ struct PruneArgs {
     void *X;                 // partials [wave slot][T-1][R][20][S_pad]
     int *E;                  // cumulative binary exponents [wave slot][T-1][R][S_pad]
     int p_base;              // first particle of the wave (wave slot = particle - p_base)
+    const uint8_t *xflag;    // [P][T-1] current workspace of each internal node (null: 0)
+    int WS;                  // workspaces per particle (1, or 2 for accept/reject)
     double *tile_lnl;        // [P][tile_stride]
     int T, R, S_pad, max_classes;
     bool fp32;

@@ -181,9 +181,9 @@ __global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ bas
                                                    const double *__restrict__ logits,
                                                    const int *__restrict__ num_classes,
                                                    double *__restrict__ eig, int max_classes,
-                                                   int first)
+                                                   int first, const int *__restrict__ plist)
 {
-    const int p = first + blockIdx.x / max_classes;
+    const int p = plist ? plist[blockIdx.x / max_classes] : first + blockIdx.x / max_classes;
     const int k = blockIdx.x % max_classes;
     if (k >= num_classes[p]) return;  // whole block exits together
     const size_t g = (size_t)p * max_classes + k;
@@ -213,9 +213,10 @@ __global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
                                               const int *__restrict__ num_classes,
                                               const int *__restrict__ pcls_off,
                                               TP *__restrict__ Pout, int nn, int R,
-                                              int max_classes, int first)
+                                              int max_classes, int first,
+                                              const int *__restrict__ plist)
 {
-    const int p = first + blockIdx.x / max_classes;
+    const int p = plist ? plist[blockIdx.x / max_classes] : first + blockIdx.x / max_classes;
     const int k = blockIdx.x % max_classes;
     if (k >= num_classes[p]) return;
     const int idx = blockIdx.y * 64 + threadIdx.x;
@@ -257,7 +258,7 @@ void launch_eigen(const EigenArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return;
     hipLaunchKernelGGL(k_eigen_f64, dim3(a.count * a.max_classes), dim3(64), 0, s, a.base,
-                       a.logits, a.num_classes, a.eig, a.max_classes, a.first);
+                       a.logits, a.num_classes, a.eig, a.max_classes, a.first, a.plist);
 }
 
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
@@ -275,11 +276,11 @@ void launch_pmat(const PmatArgs &a, hipStream_t s)
     if (a.fp32)
         hipLaunchKernelGGL(k_pmat<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes,
-                           a.first);
+                           a.first, a.plist);
     else
         hipLaunchKernelGGL(k_pmat<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
-                           a.first);
+                           a.first, a.plist);
 }
 
 }  // namespace smccat

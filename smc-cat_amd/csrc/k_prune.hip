@@ -43,7 +43,7 @@ struct PruneTypes<float> {
 // y = P x for the first child.  Tips: x is one-hot (code < 20) or all ones, so
 // P x is column `code` of P or its row sums (the stored 21st column).
 template <typename TX>
-__device__ __forceinline__ void child_vector(int child, int T, int r, int s, int orig,
+__device__ __forceinline__ void child_vector(int child, int cslot, int T, int r, int s, int orig,
                                              const TX *__restrict__ Pc,
                                              const uint8_t *__restrict__ codes, int S_pad,
                                              const TX *__restrict__ Xp, const int *__restrict__ Ep,
@@ -56,7 +56,7 @@ __device__ __forceinline__ void child_vector(int child, int T, int r, int s, int
         for (int i = 0; i < 20; i++) y[i] = Pc[i * 21 + col];
         e = 0;
     } else {
-        const size_t cl = (size_t)(child - T);
+        const size_t cl = (size_t)cslot;
         const TX *x = Xp + ((cl * R + r) * 20) * (size_t)S_pad + s;
         TX xv[20];
 #pragma unroll
@@ -74,7 +74,7 @@ __device__ __forceinline__ void child_vector(int child, int T, int r, int s, int
 
 // y *= P x for the second child, row by row, so only y and x are live.
 template <typename TX>
-__device__ __forceinline__ void child_multiply(int child, int T, int r, int s, int orig,
+__device__ __forceinline__ void child_multiply(int child, int cslot, int T, int r, int s, int orig,
                                                const TX *__restrict__ Pc,
                                                const uint8_t *__restrict__ codes, int S_pad,
                                                const TX *__restrict__ Xp,
@@ -88,7 +88,7 @@ __device__ __forceinline__ void child_multiply(int child, int T, int r, int s, i
         for (int i = 0; i < 20; i++) y[i] *= Pc[i * 21 + col];
         e = 0;
     } else {
-        const size_t cl = (size_t)(child - T);
+        const size_t cl = (size_t)cslot;
         const TX *x = Xp + ((cl * R + r) * 20) * (size_t)S_pad + s;
         TX xv[20];
 #pragma unroll
@@ -120,7 +120,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                TX *__restrict__ X, int *__restrict__ E,
                                                double *__restrict__ tile_lnl, int T, int R,
                                                int S_pad, int max_classes, int blocks_per_item,
-                                               const int *__restrict__ pcls_off, int p_base, int mix_mode)
+                                               const int *__restrict__ pcls_off, int p_base, int mix_mode,
+                                               const uint8_t *__restrict__ xflag, int WS)
 {
     // 1-D grid: consecutive blocks walk the tiles of one item, 4 tiles (one per
     // wavefront) per block, so concurrently running waves stream contiguous
@@ -146,8 +147,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
     const size_t node_elems = (size_t)R * 20 * S_pad;
     const size_t lp = (size_t)(p - p_base);  // slot of the particle in the wave's partials
-    TX *Xp = X + lp * (T - 1) * node_elems;
-    int *Ep = E + lp * (T - 1) * R * S_pad;
+    TX *Xp = X + lp * WS * (T - 1) * node_elems;
+    int *Ep = E + lp * WS * (T - 1) * R * S_pad;
+    const uint8_t *xf = xflag ? xflag + (size_t)p * (T - 1) : nullptr;
+    auto nslot = [&](int node) { return (xf ? (int)xf[node - T] * (T - 1) : 0) + node - T; };
+    const int cs0 = c0 >= T ? nslot(c0) : 0, cs1 = c1 >= T ? nslot(c1) : 0;
     const TX *Pg = Pm + ((size_t)pcls_off[p] + tl.cls) * nn * R * kPStride;
     const double *pi = eig + g * kEigStride + kEigPi;
 
@@ -155,9 +159,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int r = 0; r < R; r++) {
         TX y0[20];
         int e0, e1;
-        child_vector<TX>(c0, T, r, s, orig, Pg + ((size_t)c0 * R + r) * kPStride, codes, S_pad,
+        child_vector<TX>(c0, cs0, T, r, s, orig, Pg + ((size_t)c0 * R + r) * kPStride, codes, S_pad,
                          Xp, Ep, R, y0, e0);
-        child_multiply<TX>(c1, T, r, s, orig, Pg + ((size_t)c1 * R + r) * kPStride, codes, S_pad,
+        child_multiply<TX>(c1, cs1, T, r, s, orig, Pg + ((size_t)c1 * R + r) * kPStride, codes, S_pad,
                            Xp, Ep, R, y0, e1);
         TX mx = 0;
 #pragma unroll
@@ -172,10 +176,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int etot = e0 + e1 + (ok ? ex : 0);
         if constexpr (!is_root) {
             if (active) {
-                TX *xo = Xp + (((size_t)nl * R + r) * 20) * S_pad + s;
+                const int ns = nslot(n);
+                TX *xo = Xp + (((size_t)ns * R + r) * 20) * S_pad + s;
 #pragma unroll
                 for (int i = 0; i < 20; i++) xo[(size_t)i * S_pad] = y0[i];
-                Ep[((size_t)nl * R + r) * S_pad + s] = etot;
+                Ep[((size_t)ns * R + r) * S_pad + s] = etot;
             }
         } else {
             double dot = 0.0;
@@ -223,7 +228,7 @@ void launch_prune(const PruneArgs &a, hipStream_t s)
 #define SMC_PRUNE_ARGS                                                                          \
     a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.root_of, a.perm, a.codes,         \
         (const TXa *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, \
-        a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode
+        a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS
     if (a.fp32) {
         using TXa = float;
         if (a.roots)

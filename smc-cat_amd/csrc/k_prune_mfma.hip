@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
     const double *__restrict__ pinv, const double *__restrict__ isinv,
     const int *__restrict__ invchar, TX *__restrict__ X, int *__restrict__ E,
     double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes, int blocks_per_item,
-    const int *__restrict__ pcls_off, int p_base, int mix_mode)
+    const int *__restrict__ pcls_off, int p_base, int mix_mode, const uint8_t *__restrict__ xflag,
+    int WS)
 {
     using V4 = typename Mfma<TX>::v4;
     constexpr int U = KindTraits<KIND>::U;
@@ -135,8 +136,12 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
     }
     const size_t node_elems = (size_t)R * 20 * S_pad;
     const size_t lp = (size_t)(p - p_base);  // slot of the particle in the wave's partials
-    TX *Xp = X + lp * (T - 1) * node_elems;
-    int *Ep = E + lp * (T - 1) * R * S_pad;
+    // partial slots of a particle: WS workspaces x (T-1) internal nodes; a node's
+    // current workspace is its flag (accept/reject double buffering)
+    TX *Xp = X + lp * WS * (T - 1) * node_elems;
+    int *Ep = E + lp * WS * (T - 1) * R * S_pad;
+    const uint8_t *xf = xflag ? xflag + (size_t)p * (T - 1) : nullptr;
+    auto nslot = [&](int node) { return (xf ? (int)xf[node - T] * (T - 1) : 0) + node - T; };
     const TX *Pg = Pm + ((size_t)pcls_off[p] + tl.cls) * nn * R * kPStride;
     const double *pi = eig + g * kEigStride + kEigPi;
     const int nsb = (tl.len + 15) >> 4;  // 16-site column blocks in this tile (wave-uniform)
@@ -161,10 +166,11 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
                 a0[rb][kb] = (!tip0 && row < 20) ? P0[row * 21 + 4 * kb + g4] : (TX)0;
                 a1[rb][kb] = (!tip1 && row < 20) ? P1[row * 21 + 4 * kb + g4] : (TX)0;
             }
-        const TX *X0 = tip0 ? Xp : Xp + ((size_t)(c0 - T) * R + r) * 20 * S_pad;
-        const TX *X1 = tip1 ? Xp : Xp + ((size_t)(c1 - T) * R + r) * 20 * S_pad;
-        const int *E0 = Ep + ((size_t)(tip0 ? 0 : c0 - T) * R + r) * S_pad;
-        const int *E1 = Ep + ((size_t)(tip1 ? 0 : c1 - T) * R + r) * S_pad;
+        const int cs0 = tip0 ? 0 : nslot(c0), cs1 = tip1 ? 0 : nslot(c1);
+        const TX *X0 = Xp + ((size_t)cs0 * R + r) * 20 * S_pad;
+        const TX *X1 = Xp + ((size_t)cs1 * R + r) * 20 * S_pad;
+        const int *E0 = Ep + ((size_t)cs0 * R + r) * S_pad;
+        const int *E1 = Ep + ((size_t)cs1 * R + r) * S_pad;
 
 #pragma unroll 1
         for (int sb0 = 0; sb0 < nsb; sb0 += U) {
@@ -215,7 +221,8 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
                 etot[u] += ok ? ex : 0;
                 if constexpr (!IS_ROOT) {
                     if (valid[u]) {
-                        TX *xo = Xp + ((size_t)nl * R + r) * 20 * S_pad + slot[u];
+                        const int ns = nslot(n);
+                        TX *xo = Xp + ((size_t)ns * R + r) * 20 * S_pad + slot[u];
 #pragma unroll
                         for (int rb = 0; rb < 2; rb++)
 #pragma unroll
@@ -223,7 +230,7 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
                                 const int row = Mfma<TX>::crow(rb, g4, v);
                                 if (row < 20) xo[(size_t)row * S_pad] = y[u][rb][v];
                             }
-                        if (g4 == 0) Ep[((size_t)nl * R + r) * S_pad + slot[u]] = etot[u];
+                        if (g4 == 0) Ep[((size_t)ns * R + r) * S_pad + slot[u]] = etot[u];
                     }
                 } else {
                     double dot = 0.0;
@@ -274,7 +281,7 @@ void launch_prune_mfma(const PruneArgs &a, hipStream_t s)
 #define SMC_PRUNE_ARGS                                                                            \
     a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, (const TXa *)a.P,     \
         a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, a.R, a.S_pad,        \
-        a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode
+        a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS
 #define SMC_PRUNE_LAUNCH(TXT)                                                                     \
     do {                                                                                          \
         using TXa = TXT;                                                                          \

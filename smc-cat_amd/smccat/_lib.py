@@ -30,7 +30,8 @@ class EngineConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("taxa", ctypes.c_int32), ("sites", ctypes.c_int32),
                 ("num_rates", ctypes.c_int32), ("max_particles", ctypes.c_int32),
                 ("max_classes", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("partial_slots", ctypes.c_int32), ("pmat_classes", ctypes.c_int64)]
+                ("partial_slots", ctypes.c_int32), ("pmat_classes", ctypes.c_int64),
+                ("workspaces", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 SIGNATURES = {
@@ -54,6 +55,9 @@ SIGNATURES = {
     "smc_gibbs_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D,
                                       ctypes.POINTER(ctypes.c_int64)]),
     "smc_get_particle_state": (_I, [_P, _I, _IP, _DP, _IP]),
+    "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
+    "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
+    "smc_accept_reject": (_I, [_P, _I, _I, _BP]),
 }
 
 _lib = None

@@ -66,11 +66,11 @@ class EngineError(RuntimeError):
 
 class Engine:
     def __init__(self, taxa, sites, num_rates=1, max_particles=1, max_classes=1,
-                 precision="fp64", device=0, partial_slots=0, pmat_classes=0):
+                 precision="fp64", device=0, partial_slots=0, pmat_classes=0, workspaces=1):
         self.lib = _lib.load()
         cfg = _lib.EngineConfig(device, taxa, sites, num_rates, max_particles, max_classes,
                                 _lib.SMC_PREC_FP64 if precision == "fp64" else _lib.SMC_PREC_FP32,
-                                partial_slots, pmat_classes)
+                                partial_slots, pmat_classes, workspaces, 0)
         h = ctypes.c_void_p()
         rc = self.lib.smc_engine_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -192,6 +192,37 @@ class Engine:
         self._check(self.lib.smc_get_particle_state(self.h, int(slot), ctypes.byref(nc), _dp(lg), _ip(asg)),
                     "smc_get_particle_state")
         return nc.value, lg[:nc.value].copy(), asg
+
+    def update_params(self, batch, first=0):
+        """New topology / branch lengths / base / pinv / shape (classes untouched)."""
+        n, T = batch.n, self.taxa
+        ch = np.ascontiguousarray(batch.children, dtype=np.int32).reshape(n, T - 1, 2)
+        bl = np.ascontiguousarray(batch.branch_len, dtype=np.float64).reshape(n, 2 * T - 1)
+        base = np.ascontiguousarray(batch.base, dtype=np.float64).reshape(n, 190)
+        pinv = np.ascontiguousarray(batch.pinv, dtype=np.float64).reshape(n)
+        shape = np.ascontiguousarray(batch.shape, dtype=np.float64).reshape(n)
+        self._check(self.lib.smc_update_params(self.h, first, n, _ip(ch), _dp(bl), _dp(base), _dp(pinv),
+                                               _dp(shape)), "smc_update_params")
+
+    def refresh_incremental(self, node_flags, eig_flags=None, first=0):
+        """refreshLikelihood(false): node_flags [count, 2T-1] bit0 updateCL, bit1 updateTP."""
+        nf = np.ascontiguousarray(node_flags, dtype=np.uint8)
+        count = nf.shape[0]
+        ef = None if eig_flags is None else np.ascontiguousarray(eig_flags, dtype=np.uint8)
+        lnl = np.empty(count)
+        bad = np.empty(count, dtype=np.uint8)
+        bp = ctypes.POINTER(ctypes.c_uint8)
+        self._check(self.lib.smc_refresh_incremental(self.h, first, count, nf.ctypes.data_as(bp),
+                                                     ef.ctypes.data_as(bp) if ef is not None else None,
+                                                     _dp(lnl), bad.ctypes.data_as(bp)),
+                    "smc_refresh_incremental")
+        return lnl, bad
+
+    def accept_reject(self, accept, first=0):
+        acc = np.ascontiguousarray(accept, dtype=np.uint8)
+        self._check(self.lib.smc_accept_reject(self.h, first, len(acc),
+                                               acc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+                    "smc_accept_reject")
 
     def set_prune_variant(self, variant):
         """'mfma' (matrix cores, default) or 'valu'."""

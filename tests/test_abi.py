@@ -53,7 +53,7 @@ def test_discretize_gamma_rejects_bad_args():
 def test_engine_create_rejects_bad_config():
     lib = _lib.load()
     h = ctypes.c_void_p()
-    cfg = _lib.EngineConfig(0, 2, 10, 1, 1, 1, 0, 0, 0)      # taxa < 3
+    cfg = _lib.EngineConfig(0, 2, 10, 1, 1, 1, 0, 0, 0, 1, 0)      # taxa < 3
     assert lib.smc_engine_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.SMC_ERR_ARG
-    cfg = _lib.EngineConfig(0, 8, 10, 1, 1, 1, 7, 0, 0)      # bad precision
+    cfg = _lib.EngineConfig(0, 8, 10, 1, 1, 1, 7, 0, 0, 1, 0)      # bad precision
     assert lib.smc_engine_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.SMC_ERR_ARG
