@@ -127,6 +127,37 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                             const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out);
 int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept);
 
+/* Rejuvenation settings (main.cpp:54-105, Particle.hpp:87-94, Settings.hpp:19-29). */
+typedef struct {
+    int32_t iterations;          /* MCMC iterations r per particle (Settings::rejuvenationIterations) */
+    int32_t using_lg;            /* LG particles: no rate-matrix moves and no GTR prior           */
+    int32_t invariant;           /* propose pInvar moves                                          */
+    int32_t reserved;            /* must be 0                                                     */
+    uint64_t seed;               /* Philox stream key (streams per slot)                          */
+    double scale_delta;          /* Particle::scaleDelta (1.0)                                    */
+    double subtree_scale_delta;  /* Particle::subtreeScaleDelta (1.0)                             */
+    double shape_delta;          /* Particle::shapeDelta (1.0)                                    */
+    double rate_delta;           /* Particle::rateMatrixDelta (0.25)                              */
+    double rate_full_scale;      /* diagonal of Particle::rateMatrixCholesky (0.5)                */
+    double invar_alpha;          /* Particle::invarAlpha (100)                                    */
+    double gibbs_probability;    /* Settings::alg8Probability (0.1); needs CAT to matter          */
+    double dpp_alpha;            /* CRP concentration (Particle.hpp:106: 0.01)                    */
+} smc_rejuvenation_options;
+
+/* Rejuvenation of slots [first, first+count) at tempering T, replacing the
+ * per-particle loop of main.cpp:296-311: a forced refresh of the (copied)
+ * particles, `iterations` MCMC iterations (Mcmc::operator(), Mcmc.cpp:25-73:
+ * move by weight -- tree 3, shape 1, invariant 1, rate matrix 1 -- repeated
+ * T*0.25 / 2 / 2 / 5 times, each repetition a Metropolis-Hastings step on
+ * T lnL + lnPrior with the move's Hastings term), then with probability
+ * gibbs_probability the CAT Gibbs move and a forced refresh.  All particles
+ * step in lockstep: one incremental refresh evaluates every proposal of a
+ * round.  Needs workspaces = 2 and the range resident (count <= partial_slots).
+ * lnl_out[count] (nullable): final lnL; stats_out[8] (nullable): proposals and
+ * acceptances per move kind (tree, shape, invariant, rate matrix). */
+int smc_rejuvenate(smc_engine *e, int first, int count, double tempering,
+                   const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out);
+
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */
 int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,
@@ -167,6 +198,12 @@ int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t *slots, dou
  * Particle.hpp:73-74): num_classes, logits[max_classes][20], assignment[S]. */
 int smc_get_particle_state(smc_engine *e, int slot, int32_t *num_classes, double *logits,
                            int32_t *assignment);
+
+/* Current (last uploaded or committed) parameters of a slot: children[T-1][2],
+ * branch_len[2T-1], base[190], pinv, shape (what writeToSerialized reads,
+ * Particle.cpp:302-320).  Any pointer may be NULL. */
+int smc_get_particle_params(smc_engine *e, int slot, int32_t *children, double *branch_len,
+                            double *base, double *pinv, double *shape);
 
 /* discretizeGamma (Miscellaneous.hpp:22-37): out[num] mean-one gamma rates. */
 int smc_discretize_gamma(double shape, int num, double *out);

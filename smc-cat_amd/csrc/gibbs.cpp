@@ -240,3 +240,24 @@ extern "C" int smc_get_particle_state(smc_engine *e, int slot, int32_t *num_clas
                   assignment);
     return SMC_OK;
 }
+
+extern "C" int smc_get_particle_params(smc_engine *e, int slot, int32_t *children, double *branch_len,
+                                       double *base, double *pinv, double *shape)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (slot < 0 || slot >= e->Pmax || !e->h_valid[slot])
+        return smc_fail(e, SMC_ERR_ARG, "slot not uploaded");
+    const int Ti = e->T - 1, nn = e->nn;
+    if (children)
+        std::copy(e->h_children.begin() + (size_t)slot * Ti * 2,
+                  e->h_children.begin() + (size_t)(slot + 1) * Ti * 2, children);
+    if (branch_len)
+        std::copy(e->h_blen.begin() + (size_t)slot * nn, e->h_blen.begin() + (size_t)(slot + 1) * nn,
+                  branch_len);
+    if (base)
+        std::copy(e->h_base.begin() + (size_t)slot * 190, e->h_base.begin() + (size_t)(slot + 1) * 190,
+                  base);
+    if (pinv) *pinv = e->h_pinv[slot];
+    if (shape) *shape = e->h_shape[slot];
+    return SMC_OK;
+}

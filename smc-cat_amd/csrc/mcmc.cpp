@@ -1,0 +1,385 @@
+// Batched MCMC rejuvenation of particles at a tempering T.
+//
+// Reference: the rejuvenation loop of main.cpp:293-313 (per particle: MCMC of
+// r iterations, then with probability alg8Probability the CAT Gibbs move and a
+// forced refresh), Mcmc::operator() (Mcmc.cpp:25-73: move chosen by weight,
+// repeated numGibbs times, each repetition a Metropolis-Hastings step with
+// ratio (T lnL' + prior') - (T lnL + prior) + log Hastings, accepted iff
+// log u <= ratio), the move table of main.cpp:54-105, the moves of
+// Particle.cpp:677-783 / Tree.cpp:553-811 and Particle::lnPrior
+// (Particle.cpp:484-517).
+//
+// All particles of the range step in lockstep: each MH step of every particle
+// that still has repetitions left is one proposal; one incremental refresh
+// (smc_refresh_incremental) evaluates all proposals at once, one
+// smc_accept_reject commits or reverts them.  Differences from the reference
+// (each stated where it happens): adaptive NNI proposes a plain NNI (the split
+// posteriors it is biased by live outside this engine); SPR marks the old and
+// the new path to the root dirty (the reference marks only the new one,
+// Tree.cpp:795-800); rate-matrix moves recompute the eigensystem (the
+// reference keeps stale eigenvectors, Q13); the stationary move, a no-op in
+// the reference (Q14), is skipped.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+
+#include "engine_state.hpp"
+
+namespace {
+
+// Philox4x32-10 (Salmon et al., SC 2011), counter-based: one stream per slot.
+struct Rng {
+    uint32_t key[2];
+    uint64_t counter = 0;
+    Rng(uint64_t seed, uint64_t stream)
+    {
+        key[0] = (uint32_t)(seed ^ (stream * 0xD1B54A32D192ED03ull));
+        key[1] = (uint32_t)((seed >> 32) ^ (stream * 0x9E3779B9u) ^ 0x6d63u);
+    }
+    double uniform()
+    {
+        uint32_t c[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), 0x3c6ef372u, 0x13579bdu};
+        uint32_t k0 = key[0], k1 = key[1];
+        for (int r = 0; r < 10; r++) {
+            const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+            const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+            const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+            const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+            c[0] = n0;
+            c[1] = (uint32_t)p1;
+            c[2] = n2;
+            c[3] = (uint32_t)p0;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        counter++;
+        return (double)(((uint64_t)c[0] << 32 | c[1]) >> 11) * (1.0 / 9007199254740992.0);
+    }
+    double normal()
+    {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log1p(-u1)) * std::cos(6.283185307179586 * u2);
+    }
+    // Gamma(a, 1): Marsaglia & Tsang (2000), with the a < 1 boost U^(1/a)
+    double gamma(double a)
+    {
+        if (a < 1.0) return gamma(a + 1.0) * std::pow(uniform(), 1.0 / a);
+        const double d = a - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+        for (;;) {
+            double x, v;
+            do {
+                x = normal();
+                v = 1.0 + c * x;
+            } while (v <= 0.0);
+            v = v * v * v;
+            const double u = uniform();
+            if (u < 1.0 - 0.0331 * x * x * x * x) return d * v;
+            if (std::log(u) < 0.5 * x * x + d * (1.0 - v + std::log(v))) return d * v;
+        }
+    }
+    double beta(double a, double b)
+    {
+        const double x = gamma(a), y = gamma(b);
+        return x / (x + y);
+    }
+};
+
+double log_gamma_pdf(double k, double theta, double x)
+{
+    return (k - 1.0) * std::log(x) - x / theta - std::lgamma(k) - k * std::log(theta);
+}
+double log_beta_pdf(double a, double b, double x)
+{
+    return (a - 1.0) * std::log(x) + (b - 1.0) * std::log1p(-x) + std::lgamma(a + b) - std::lgamma(a) -
+           std::lgamma(b);
+}
+double log_std_normal(double z) { return -0.5 * z * z - 0.9189385332046727; }
+
+enum Move { TREE = 0, SHAPE = 1, INVAR = 2, RATES = 3, N_MOVES = 4 };
+
+struct Params {
+    std::vector<int32_t> ch;
+    std::vector<double> bl, base;
+    double pinv, shape;
+};
+
+// Particle::lnPrior (Particle.cpp:484-517); the CRP/class terms are included so
+// the value matches the reference's, although they cancel in these moves.
+double ln_prior(const smc_engine *e, int p, const Params &q, bool using_lg, double dpp_alpha)
+{
+    const int T = e->T, nn = e->nn, S = e->S, Kmax = e->Kmax;
+    double len = 0.0;
+    for (int n = 0; n < nn; ++n) len += q.bl[n];  // the root's length is 0
+    double lp = log_gamma_pdf((double)(nn - 1), 0.1, len);
+    if (e->R > 1) lp += log_gamma_pdf(3.0, 2.0, q.shape);
+    if (!using_lg)
+        for (double b : q.base) lp += log_std_normal(b);
+    const int K = e->h_nclass[p];
+    lp += std::log(dpp_alpha) * K;
+    std::vector<int> sz(K, 0);
+    for (int s = 0; s < S; ++s) sz[e->h_assign[(size_t)p * S + s]]++;
+    for (int k = 0; k < K; ++k) {
+        for (int j = 0; j < 20; ++j) lp += log_std_normal(e->h_logits[((size_t)p * Kmax + k) * 20 + j]);
+        lp += std::lgamma((double)sz[k]);
+    }
+    lp += std::lgamma(dpp_alpha) - std::lgamma(dpp_alpha + S);
+    (void)T;
+    return lp;
+}
+
+void parents(const std::vector<int32_t> &ch, int T, std::vector<int> &par)
+{
+    par.assign(2 * T - 1, -1);
+    for (int k = 0; k < T - 1; ++k) {
+        par[ch[2 * k]] = T + k;
+        par[ch[2 * k + 1]] = T + k;
+    }
+}
+
+void mark_path(std::vector<uint8_t> &f, const std::vector<int> &par, int n)
+{
+    for (; n >= 0; n = par[n]) f[n] |= 1;
+}
+
+// One tree move (Particle::treeMove mixture, Particle.cpp:680-701).  Returns the
+// log Hastings ratio; fills updateCL / updateTP flags.
+double tree_move(Rng &g, Params &q, int T, std::vector<uint8_t> &f, const smc_rejuvenation_options &o)
+{
+    const int nn = 2 * T - 1;
+    std::vector<int> par;
+    parents(q.ch, T, par);
+    int root = -1;
+    for (int n = T; n < nn; ++n)
+        if (par[n] < 0) root = n;
+    auto pick = [&](bool internal) {
+        for (;;) {
+            const int n = std::min(nn - 1, (int)(g.uniform() * nn));
+            if (n != root && (!internal || n >= T)) return n;
+        }
+    };
+    const double u = g.uniform();
+    if (u < 0.4) {  // NNI 0.1 and adaptive NNI 0.3 (Tree.cpp:625-666)
+        const int p = pick(true), a = par[p];
+        const int n1 = q.ch[2 * (p - T) + (g.uniform() < 0.5 ? 0 : 1)];
+        const int n2 = q.ch[2 * (a - T)] == p ? q.ch[2 * (a - T) + 1] : q.ch[2 * (a - T)];
+        for (int c = 0; c < 2; ++c) {
+            if (q.ch[2 * (p - T) + c] == n1) q.ch[2 * (p - T) + c] = n2;
+            if (q.ch[2 * (a - T) + c] == n2) q.ch[2 * (a - T) + c] = n1;
+        }
+        parents(q.ch, T, par);
+        mark_path(f, par, p);
+        return 0.0;
+    }
+    if (u < 0.5) {  // SPR (Tree.cpp:753-811)
+        const int p = pick(true);
+        std::vector<char> in_sub(nn, 0);
+        std::vector<int> st{p};
+        while (!st.empty()) {
+            const int x = st.back();
+            st.pop_back();
+            in_sub[x] = 1;
+            if (x >= T) {
+                st.push_back(q.ch[2 * (x - T)]);
+                st.push_back(q.ch[2 * (x - T) + 1]);
+            }
+        }
+        int tgt = -1;
+        for (int tries = 0; tries < 64 && tgt < 0; ++tries) {
+            const int a = std::min(nn - 1, (int)(g.uniform() * nn));
+            if (!in_sub[a] && a != root) tgt = a;
+        }
+        if (tgt < 0) return 0.0;
+        const int s = q.ch[2 * (p - T) + (g.uniform() < 0.5 ? 0 : 1)];
+        const int sib = q.ch[2 * (p - T)] == s ? q.ch[2 * (p - T) + 1] : q.ch[2 * (p - T)];
+        const int old = par[p];
+        if (tgt == sib) return 0.0;  // regrafting onto its own sibling changes nothing
+        for (int c = 0; c < 2; ++c)
+            if (q.ch[2 * (old - T) + c] == p) q.ch[2 * (old - T) + c] = sib;
+        const int tpar = par[tgt] == p ? old : par[tgt];
+        q.ch[2 * (p - T)] = s;
+        q.ch[2 * (p - T) + 1] = tgt;
+        for (int c = 0; c < 2; ++c)
+            if (q.ch[2 * (tpar - T) + c] == tgt) q.ch[2 * (tpar - T) + c] = p;
+        parents(q.ch, T, par);
+        mark_path(f, par, p);
+        mark_path(f, par, old);
+        return 0.0;
+    }
+    if (u < 0.9) {  // branch-length scale (Tree.cpp:553-577)
+        const int n = pick(false);
+        const double sc = std::exp(o.scale_delta * (g.uniform() - 0.5));
+        q.bl[n] *= sc;
+        f[n] |= 2;
+        mark_path(f, par, par[n]);
+        return std::log(sc);
+    }
+    // subtree scale (Tree.cpp:579-612)
+    const int n = pick(false);
+    const double sc = std::exp(o.subtree_scale_delta * (g.uniform() - 0.5));
+    int nb = 0;
+    std::vector<int> st{n};
+    while (!st.empty()) {
+        const int x = st.back();
+        st.pop_back();
+        q.bl[x] *= sc;
+        f[x] |= 3;
+        nb++;
+        if (x >= T) {
+            st.push_back(q.ch[2 * (x - T)]);
+            st.push_back(q.ch[2 * (x - T) + 1]);
+        }
+    }
+    mark_path(f, par, par[n]);
+    return nb * std::log(sc);
+}
+
+}  // namespace
+
+extern "C" int smc_rejuvenate(smc_engine *e, int first, int count, double tempering,
+                              const smc_rejuvenation_options *opt, double *lnl_out,
+                              int64_t *stats_out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (!opt || first < 0 || count < 1 || first + count > e->Pmax || opt->iterations < 0)
+        return smc_fail(e, SMC_ERR_ARG, "invalid rejuvenation arguments");
+    if (e->WS != 2) return smc_fail(e, SMC_ERR_STATE, "rejuvenation needs workspaces = 2");
+    const smc_rejuvenation_options &o = *opt;
+    const int T = e->T, Ti = T - 1, nn = e->nn;
+    // full refresh of the freshly copied particles (copyFromSerialized, Particle.cpp:276)
+    int rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+    if (rc != SMC_OK) return rc;
+
+    // the move table of main.cpp:54-105 (stationary moves are no-ops and skipped)
+    double w[N_MOVES] = {3.0, e->R > 1 ? 1.0 : 0.0, o.invariant ? 1.0 : 0.0, o.using_lg ? 0.0 : 1.0};
+    const double w_stat = 1.0;
+    int reps[N_MOVES] = {std::max(1, (int)(T * 0.25)), 2, 2, 5};
+    double wtot = w_stat;
+    for (double x : w) wtot += x;
+
+    std::vector<Rng> rng;
+    rng.reserve(count);
+    for (int i = 0; i < count; ++i) rng.emplace_back(o.seed, (uint64_t)(first + i));
+    std::vector<double> lnl(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count);
+    std::vector<double> prior(count);
+    auto load = [&](int p) {
+        const smc_engine::Committed &c = e->committed[p];
+        return Params{c.children, c.blen, c.base, c.pinv, c.shape};
+    };
+    for (int i = 0; i < count; ++i) prior[i] = ln_prior(e, first + i, load(first + i), o.using_lg, o.dpp_alpha);
+
+    std::vector<int32_t> ch((size_t)count * Ti * 2);
+    std::vector<double> bl((size_t)count * nn), base((size_t)count * 190), pv(count), sh(count);
+    std::vector<uint8_t> flags((size_t)count * nn), eig(count), acc(count);
+    std::vector<double> hastings(count), new_prior(count), lnl_new(count);
+    std::vector<int> move(count), left(count);
+    int64_t proposed[N_MOVES] = {}, accepted[N_MOVES] = {};
+
+    for (int it = 0; it < o.iterations; ++it) {
+        int rounds = 0;
+        for (int i = 0; i < count; ++i) {
+            double u = rng[i].uniform() * wtot, c = 0.0;
+            move[i] = -1;  // stationary (no-op)
+            for (int k = 0; k < N_MOVES; ++k) {
+                c += w[k];
+                if (u < c) {
+                    move[i] = k;
+                    break;
+                }
+            }
+            left[i] = move[i] >= 0 ? reps[move[i]] : 0;
+            rounds = std::max(rounds, left[i]);
+        }
+        for (int r = 0; r < rounds; ++r) {
+            for (int i = 0; i < count; ++i) {
+                const int p = first + i;
+                Params q = load(p);
+                std::vector<uint8_t> f(nn, 0);
+                eig[i] = 0;
+                hastings[i] = 0.0;
+                if (left[i] > 0) {
+                    Rng &g = rng[i];
+                    switch (move[i]) {
+                        case TREE:
+                            hastings[i] = tree_move(g, q, T, f, o);
+                            break;
+                        case SHAPE: {  // Particle.cpp:768-783
+                            const double sc = std::exp(o.shape_delta * (g.uniform() - 0.5));
+                            q.shape *= sc;
+                            std::fill(f.begin(), f.end(), 3);
+                            hastings[i] = std::log(sc);
+                        } break;
+                        case INVAR: {  // Particle.cpp:744-766
+                            const double a = o.invar_alpha + 1.0, b = o.invar_alpha / q.pinv - a + 2.0;
+                            const double np = g.beta(a, b);
+                            const double a2 = o.invar_alpha + 1.0, b2 = o.invar_alpha / np - a2 + 2.0;
+                            hastings[i] = log_beta_pdf(a2, b2, q.pinv) - log_beta_pdf(a, b, np);
+                            q.pinv = np;
+                        } break;
+                        case RATES: {  // Particle.cpp:706-732, eigensystems recomputed (Q13 fixed)
+                            if (g.uniform() < 0.25) {
+                                for (double &b : q.base) b += o.rate_full_scale * g.normal();
+                            } else {
+                                const int idx = std::min(189, (int)(190 * g.uniform()));
+                                q.base[idx] += g.normal() * o.rate_delta;
+                            }
+                            eig[i] = 1;
+                        } break;
+                    }
+                }
+                std::copy(q.ch.begin(), q.ch.end(), ch.begin() + (size_t)i * Ti * 2);
+                std::copy(q.bl.begin(), q.bl.end(), bl.begin() + (size_t)i * nn);
+                std::copy(q.base.begin(), q.base.end(), base.begin() + (size_t)i * 190);
+                pv[i] = q.pinv;
+                sh[i] = q.shape;
+                std::copy(f.begin(), f.end(), flags.begin() + (size_t)i * nn);
+                new_prior[i] = left[i] > 0 ? ln_prior(e, p, q, o.using_lg, o.dpp_alpha) : prior[i];
+            }
+            rc = smc_update_params(e, first, count, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
+            if (rc != SMC_OK) return rc;
+            rc = smc_refresh_incremental(e, first, count, flags.data(), eig.data(), lnl_new.data(), nullptr);
+            if (rc != SMC_OK) return rc;
+            for (int i = 0; i < count; ++i) {
+                if (left[i] <= 0) {
+                    acc[i] = 1;  // unchanged particle: commit is a no-op
+                    continue;
+                }
+                const double ratio = (tempering * lnl_new[i] + new_prior[i]) -
+                                     (tempering * lnl[i] + prior[i]) + hastings[i];
+                const double lu = std::log(rng[i].uniform());
+                acc[i] = (lu <= ratio) ? 1 : 0;  // Mcmc.cpp:62
+                proposed[move[i]]++;
+                if (acc[i]) {
+                    accepted[move[i]]++;
+                    lnl[i] = lnl_new[i];
+                    prior[i] = new_prior[i];
+                }
+                left[i]--;
+            }
+            rc = smc_accept_reject(e, first, count, acc.data());
+            if (rc != SMC_OK) return rc;
+        }
+    }
+    // with probability gibbs_probability: the CAT Gibbs move and a forced refresh
+    // (main.cpp:303-305)
+    if (o.gibbs_probability > 0.0) {
+        std::vector<int32_t> slots;
+        for (int i = 0; i < count; ++i)
+            if (rng[i].uniform() < o.gibbs_probability) slots.push_back(first + i);
+        if (!slots.empty()) {
+            rc = smc_gibbs_partition_move(e, (int)slots.size(), slots.data(), tempering, o.seed ^ 0x61626273ull,
+                                          0, o.dpp_alpha, nullptr);
+            if (rc != SMC_OK) return rc;
+            rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+            if (rc != SMC_OK) return rc;
+            std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl.begin());
+        }
+    }
+    if (lnl_out) std::copy(lnl.begin(), lnl.end(), lnl_out);
+    if (stats_out)
+        for (int k = 0; k < N_MOVES; ++k) {
+            stats_out[2 * k] = proposed[k];
+            stats_out[2 * k + 1] = accepted[k];
+        }
+    return SMC_OK;
+}

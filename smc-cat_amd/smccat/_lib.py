@@ -34,6 +34,16 @@ class EngineConfig(ctypes.Structure):
                 ("workspaces", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class RejuvenationOptions(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("using_lg", ctypes.c_int32),
+                ("invariant", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("scale_delta", ctypes.c_double),
+                ("subtree_scale_delta", ctypes.c_double), ("shape_delta", ctypes.c_double),
+                ("rate_delta", ctypes.c_double), ("rate_full_scale", ctypes.c_double),
+                ("invar_alpha", ctypes.c_double), ("gibbs_probability", ctypes.c_double),
+                ("dpp_alpha", ctypes.c_double)]
+
+
 SIGNATURES = {
     "smc_engine_create": (_I, [ctypes.POINTER(EngineConfig), ctypes.POINTER(_P)]),
     "smc_engine_destroy": (_I, [_P]),
@@ -55,9 +65,12 @@ SIGNATURES = {
     "smc_gibbs_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D,
                                       ctypes.POINTER(ctypes.c_int64)]),
     "smc_get_particle_state": (_I, [_P, _I, _IP, _DP, _IP]),
+    "smc_get_particle_params": (_I, [_P, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
     "smc_accept_reject": (_I, [_P, _I, _I, _BP]),
+    "smc_rejuvenate": (_I, [_P, _I, _I, _D, ctypes.POINTER(RejuvenationOptions), _DP,
+                            ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None

@@ -224,6 +224,49 @@ class Engine:
                                                acc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
                     "smc_accept_reject")
 
+    def rejuvenate(self, tempering, iterations, seed, first=0, count=None, using_lg=False,
+                   invariant=True, gibbs_probability=0.0, dpp_alpha=0.01, **deltas):
+        """MCMC rejuvenation of slots [first, first+count); returns (lnl, stats[4, 2])."""
+        if count is None:
+            count = self.max_particles - first
+        o = _lib.RejuvenationOptions(int(iterations), int(using_lg), int(invariant), 0, int(seed),
+                                     deltas.get("scale_delta", 1.0),
+                                     deltas.get("subtree_scale_delta", 1.0),
+                                     deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
+                                     deltas.get("rate_full_scale", 0.5),
+                                     deltas.get("invar_alpha", 100.0), float(gibbs_probability),
+                                     float(dpp_alpha))
+        lnl = np.empty(count)
+        stats = (ctypes.c_int64 * 8)()
+        self._check(self.lib.smc_rejuvenate(self.h, first, count, float(tempering), ctypes.byref(o),
+                                            _dp(lnl), stats), "smc_rejuvenate")
+        return lnl, np.array(list(stats)).reshape(4, 2)
+
+    def particle_params(self, slot):
+        """(children [T-1, 2], branch_len [2T-1], base [190], pinv, shape) of a slot."""
+        T = self.taxa
+        ch = np.zeros((T - 1, 2), dtype=np.int32)
+        bl = np.zeros(2 * T - 1)
+        base = np.zeros(190)
+        pv, sh = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.smc_get_particle_params(self.h, int(slot), _ip(ch), _dp(bl), _dp(base),
+                                                     ctypes.byref(pv), ctypes.byref(sh)),
+                    "smc_get_particle_params")
+        return ch, bl, base, pv.value, sh.value
+
+    def snapshot(self, slots, batch_template):
+        """ParticleBatch of the engine's current state for the given slots."""
+        out = batch_template.take(np.zeros(len(slots), dtype=int))
+        for j, s in enumerate(slots):
+            ch, bl, base, pv, sh = self.particle_params(s)
+            K, lg, asg = self.particle_state(s)
+            out.children[j], out.branch_len[j], out.base[j] = ch, bl, base
+            out.pinv[j], out.shape[j], out.num_classes[j] = pv, sh, K
+            out.logits[j] = 0.0
+            out.logits[j, :K] = lg
+            out.assignment[j] = asg
+        return out
+
     def set_prune_variant(self, variant):
         """'mfma' (matrix cores, default) or 'valu'."""
         v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
