@@ -142,6 +142,9 @@ typedef struct {
     double invar_alpha;          /* Particle::invarAlpha (100)                                    */
     double gibbs_probability;    /* Settings::alg8Probability (0.1); needs CAT to matter          */
     double dpp_alpha;            /* CRP concentration (Particle.hpp:106: 0.01)                    */
+    const double *rate_cholesky; /* [190][190] row-major lower factor of the full rate-matrix move
+                                    (Particle::rateMatrixCholesky, Particle.cpp:710-716); NULL:
+                                    rate_full_scale * identity                                    */
 } smc_rejuvenation_options;
 
 /* Rejuvenation of slots [first, first+count) at tempering T, replacing the
@@ -157,6 +160,47 @@ typedef struct {
  * acceptances per move kind (tree, shape, invariant, rate matrix). */
 int smc_rejuvenate(smc_engine *e, int first, int count, double tempering,
                    const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out);
+
+/* The SMC sampler loop of main.cpp:143-336 over slots [0, num_particles),
+ * which must hold the initial population (main.cpp:129-139): per iteration
+ * computeNextStep, and when ESS <= threshold * N (and T < 1) systematic
+ * resampling, proposal tuning from the resampled population (main.cpp:197-289),
+ * the copy of each parent into its new slot (on the engine, exact values), and
+ * the rejuvenation of every slot (smc_rejuvenate) in lockstep batches of
+ * `batch` particles.  Differences: copies are exact (the reference rounds
+ * branch lengths to 6 decimals through Newick text, Tree.cpp:429,433); split
+ * posteriors and consensus (main.cpp:203-207,338-425) are not computed. */
+typedef struct {
+    int32_t num_particles;          /* N (Settings::numParticles)                                */
+    int32_t rejuvenation_iterations;/* r (Settings::rejuvenationIterations)                      */
+    double rejuvenation_threshold;  /* resample when ESS <= threshold * N (0.6)                  */
+    double alg8_probability;        /* CAT Gibbs move probability per particle (0.1)             */
+    int32_t using_lg;               /* -LG                                                       */
+    int32_t invariant;              /* -I                                                        */
+    int32_t cat;                    /* CAT classes (Gibbs move enabled)                          */
+    int32_t batch;                  /* particles per lockstep rejuvenation batch; 0 = as many as
+                                       the engine's partial slots hold; 1 = one particle at a
+                                       time, the reference's per-thread shape                    */
+    int32_t max_iterations;         /* stop after this many SMC iterations; 0 = until T = 1      */
+    int32_t reserved;               /* must be 0                                                 */
+    uint64_t seed;                  /* Philox key of the sampler                                 */
+    double dpp_alpha;               /* CRP concentration                                         */
+} smc_sampler_options;
+
+typedef struct {
+    int32_t iterations;             /* SMC iterations run                                        */
+    int32_t resamples;              /* resample + rejuvenation steps                             */
+    double final_temp;              /* tempering reached                                         */
+    double final_ess;               /* ESS of the last computeNextStep (main.cpp:154-156)        */
+    double ess_sum;                 /* sum of the ESS after each reweight                        */
+    double seconds;                 /* wall time of the loop, initial likelihoods included       */
+    double rejuvenation_seconds;    /* of which rejuvenation                                     */
+    int64_t move_stats[8];          /* proposed / accepted: tree, shape, invariant, rate matrix  */
+} smc_sampler_result;
+
+/* lnl_out[N], weights_out[N] (nullable): final raw lnL and raw log weights. */
+int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, smc_sampler_result *result,
+                    double *lnl_out, double *weights_out);
 
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */

@@ -1,0 +1,187 @@
+// The SMC sampler loop: adaptive tempering, resampling, proposal tuning and
+// batched rejuvenation of the whole population on the engine.
+//
+// Reference: main.cpp:143-336 (loop, ESS check, systematic resampling,
+// tuning, rejuvenation), computeNextStep (Miscellaneous.hpp:42-109).  The
+// per-thread Particle objects of the reference become slot ranges of one
+// engine; the serialised copy of a parent into its child slot
+// (copyFromSerialized, Particle.cpp:218-276) becomes a gather of the parent's
+// parameters followed by one upload of the population.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "engine_state.hpp"
+#include "host_rng.hpp"
+
+namespace {
+
+// population variance (boost accumulators' tag::variance: mean of squares minus
+// squared mean)
+struct Var {
+    double n = 0.0, s = 0.0, s2 = 0.0;
+    void push(double x)
+    {
+        n += 1.0;
+        s += x;
+        s2 += x * x;
+    }
+    double get() const
+    {
+        if (n == 0.0) return 0.0;
+        const double m = s / n;
+        return s2 / n - m * m;
+    }
+};
+
+// In-place Cholesky of a symmetric positive-definite [n][n] row-major matrix
+// (lower factor, upper part zeroed).  Returns false when not positive definite.
+bool cholesky(std::vector<double> &a, int n)
+{
+    for (int j = 0; j < n; ++j) {
+        double d = a[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= a[(size_t)j * n + k] * a[(size_t)j * n + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        a[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double v = a[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= a[(size_t)i * n + k] * a[(size_t)j * n + k];
+            a[(size_t)i * n + j] = v / d;
+        }
+        for (int k = j + 1; k < n; ++k) a[(size_t)j * n + k] = 0.0;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, smc_sampler_result *res,
+                               double *lnl_out, double *weights_out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (!opt || !res) return smc_fail(e, SMC_ERR_ARG, "null sampler options or result");
+    const smc_sampler_options &o = *opt;
+    const int N = o.num_particles;
+    if (N < 1 || N > e->Pmax || o.rejuvenation_iterations < 0 || o.batch < 0 || o.reserved != 0 ||
+        !(o.rejuvenation_threshold >= 0.0))
+        return smc_fail(e, SMC_ERR_ARG, "invalid sampler options");
+    if (e->WS != 2) return smc_fail(e, SMC_ERR_STATE, "the sampler needs workspaces = 2");
+    for (int p = 0; p < N; ++p)
+        if (!e->h_valid[p]) return smc_fail(e, SMC_ERR_STATE, "slot without an initial particle");
+    const int T = e->T, Ti = T - 1, nn = e->nn, S = e->S, Kmax = e->Kmax;
+    const int batch = std::min(o.batch > 0 ? o.batch : e->Wmax, e->Wmax);
+    *res = smc_sampler_result{};
+
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    double rejuv_s = 0.0;
+    smccat::Rng g(o.seed, 0xFFFFFFFFull);
+
+    std::vector<double> lnl(N), w(N, -std::log((double)N)), norm(N);
+    int rc = smc_refresh_likelihood(e, 0, N, lnl.data(), nullptr);
+    if (rc != SMC_OK) return rc;
+
+    // host copies of the resampled population, uploaded in one call
+    std::vector<int32_t> ch((size_t)N * Ti * 2), asg((size_t)N * S), nk(N), assign(N);
+    std::vector<double> bl((size_t)N * nn), base((size_t)N * 190), lg((size_t)N * Kmax * 20), pv(N), sh(N);
+    std::vector<double> chol;
+    const double rej_threshold = o.rejuvenation_threshold * N;
+    const int target = (int)rej_threshold;  // bound to const int& in the reference (Q17)
+    double last = 0.0, ess = 0.0;
+    int it = 0;
+    for (it = 1; last < 1.0; ++it) {
+        if (o.max_iterations > 0 && it > o.max_iterations) break;
+        double cur = last;
+        rc = smc_compute_next_step(e, N, w.data(), lnl.data(), norm.data(), &ess, target, &cur, last);
+        if (rc != SMC_OK) return rc;
+        res->iterations = it;
+        res->ess_sum += ess;
+        if (std::isnan(ess)) return smc_fail(e, SMC_ERR_STATE, "NaN ESS (a particle likelihood failed)");
+        if (ess <= rej_threshold && cur != 1.0) {
+            rc = smc_systematic_resample(e, N, norm.data(), g.uniform(), assign.data());
+            if (rc != SMC_OK) return rc;
+            // tuning from the resampled population (main.cpp:213-289)
+            Var rate_var, branch_var;
+            double pinv_mean = 0.0;
+            for (int n = 0; n < N; ++n) {
+                const int p = assign[n];
+                rate_var.push(std::log(e->h_shape[p]));
+                pinv_mean += e->h_pinv[p];
+                for (int v = 0; v < nn; ++v)
+                    if (v != e->h_root[p]) branch_var.push(std::log(e->h_blen[(size_t)p * nn + v]));
+            }
+            pinv_mean /= N;
+            smc_rejuvenation_options ro{};
+            ro.iterations = o.rejuvenation_iterations;
+            ro.using_lg = o.using_lg;
+            ro.invariant = o.invariant;
+            ro.seed = o.seed * 0x9E3779B97F4A7C15ull + (uint64_t)it;
+            ro.shape_delta = std::max(0.5 * rate_var.get(), 0.01);
+            ro.scale_delta = std::max(branch_var.get(), 0.01);
+            ro.subtree_scale_delta = ro.scale_delta;
+            ro.invar_alpha = pinv_mean * 0.5;
+            ro.gibbs_probability = o.cat ? o.alg8_probability : 0.0;
+            ro.dpp_alpha = o.dpp_alpha;
+            ro.rate_full_scale = 0.0;
+            if (!o.using_lg) {
+                // uncentred second moment of the base matrices (Q21), scaled,
+                // shrunk towards its mean diagonal, then factored
+                chol.assign(190 * 190, 0.0);
+                for (int n = 0; n < N; ++n) {
+                    const double *b = &e->h_base[(size_t)assign[n] * 190];
+                    for (int i = 0; i < 190; ++i)
+                        for (int j = 0; j <= i; ++j) chol[i * 190 + j] += b[i] * b[j];
+                }
+                const double f = (2.38 * 2.38) / 190.0 * 0.21, shrink = 0.2;
+                double diag = 0.0;
+                for (int i = 0; i < 190; ++i) diag += chol[i * 190 + i] * f;
+                diag = diag / 190.0 * shrink;
+                for (int i = 0; i < 190; ++i) {
+                    for (int j = 0; j <= i; ++j) chol[i * 190 + j] *= f * (1.0 - shrink);
+                    chol[i * 190 + i] += diag + 1e-6;
+                    for (int j = 0; j < i; ++j) chol[j * 190 + i] = chol[i * 190 + j];
+                }
+                if (!cholesky(chol, 190))
+                    return smc_fail(e, SMC_ERR_STATE, "rate-matrix proposal covariance is not positive definite");
+                ro.rate_cholesky = chol.data();
+                ro.rate_delta = std::max(diag, 0.01);
+            }
+            // copy every parent into its new slot
+            for (int n = 0; n < N; ++n) {
+                const int p = assign[n];
+                std::copy_n(&e->h_children[(size_t)p * Ti * 2], Ti * 2, &ch[(size_t)n * Ti * 2]);
+                std::copy_n(&e->h_blen[(size_t)p * nn], nn, &bl[(size_t)n * nn]);
+                std::copy_n(&e->h_base[(size_t)p * 190], 190, &base[(size_t)n * 190]);
+                std::copy_n(&e->h_logits[(size_t)p * Kmax * 20], Kmax * 20, &lg[(size_t)n * Kmax * 20]);
+                std::copy_n(&e->h_assign[(size_t)p * S], S, &asg[(size_t)n * S]);
+                nk[n] = e->h_nclass[p];
+                pv[n] = e->h_pinv[p];
+                sh[n] = e->h_shape[p];
+            }
+            rc = smc_set_particles(e, 0, N, ch.data(), bl.data(), base.data(), nk.data(), lg.data(),
+                                   asg.data(), pv.data(), sh.data());
+            if (rc != SMC_OK) return rc;
+            const auto r0 = clk::now();
+            for (int first = 0; first < N; first += batch) {
+                const int cnt = std::min(batch, N - first);
+                int64_t st[8];
+                rc = smc_rejuvenate(e, first, cnt, cur, &ro, lnl.data() + first, st);
+                if (rc != SMC_OK) return rc;
+                for (int k = 0; k < 8; ++k) res->move_stats[k] += st[k];
+            }
+            rejuv_s += std::chrono::duration<double>(clk::now() - r0).count();
+            std::fill(w.begin(), w.end(), -std::log((double)N));
+            res->resamples++;
+        }
+        last = cur;
+    }
+    res->final_temp = last;
+    res->final_ess = ess;
+    res->seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    res->rejuvenation_seconds = rejuv_s;
+    if (lnl_out) std::copy(lnl.begin(), lnl.end(), lnl_out);
+    if (weights_out) std::copy(w.begin(), w.end(), weights_out);
+    return SMC_OK;
+}

@@ -41,7 +41,22 @@ class RejuvenationOptions(ctypes.Structure):
                 ("subtree_scale_delta", ctypes.c_double), ("shape_delta", ctypes.c_double),
                 ("rate_delta", ctypes.c_double), ("rate_full_scale", ctypes.c_double),
                 ("invar_alpha", ctypes.c_double), ("gibbs_probability", ctypes.c_double),
-                ("dpp_alpha", ctypes.c_double)]
+                ("dpp_alpha", ctypes.c_double), ("rate_cholesky", ctypes.c_void_p)]
+
+
+class SamplerOptions(ctypes.Structure):
+    _fields_ = [("num_particles", ctypes.c_int32), ("rejuvenation_iterations", ctypes.c_int32),
+                ("rejuvenation_threshold", ctypes.c_double), ("alg8_probability", ctypes.c_double),
+                ("using_lg", ctypes.c_int32), ("invariant", ctypes.c_int32), ("cat", ctypes.c_int32),
+                ("batch", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64), ("dpp_alpha", ctypes.c_double)]
+
+
+class SamplerResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("resamples", ctypes.c_int32),
+                ("final_temp", ctypes.c_double), ("final_ess", ctypes.c_double),
+                ("ess_sum", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("rejuvenation_seconds", ctypes.c_double), ("move_stats", ctypes.c_int64 * 8)]
 
 
 SIGNATURES = {
@@ -66,6 +81,7 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_int64)]),
     "smc_get_particle_state": (_I, [_P, _I, _IP, _DP, _IP]),
     "smc_get_particle_params": (_I, [_P, _I, _IP, _DP, _DP, _DP, _DP]),
+    "smc_run_sampler": (_I, [_P, ctypes.POINTER(SamplerOptions), ctypes.POINTER(SamplerResult), _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
     "smc_accept_reject": (_I, [_P, _I, _I, _BP]),

@@ -235,12 +235,28 @@ class Engine:
                                      deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
                                      deltas.get("rate_full_scale", 0.5),
                                      deltas.get("invar_alpha", 100.0), float(gibbs_probability),
-                                     float(dpp_alpha))
+                                     float(dpp_alpha), None)
         lnl = np.empty(count)
         stats = (ctypes.c_int64 * 8)()
         self._check(self.lib.smc_rejuvenate(self.h, first, count, float(tempering), ctypes.byref(o),
                                             _dp(lnl), stats), "smc_rejuvenate")
         return lnl, np.array(list(stats)).reshape(4, 2)
+
+    def run_sampler(self, num_particles, iterations, seed, using_lg=False, invariant=True, cat=False,
+                    threshold=0.6, alg8_probability=0.1, batch=0, max_iterations=0, dpp_alpha=0.01):
+        """The SMC loop over slots [0, num_particles) (which hold the initial
+        population).  Returns (result dict, final lnl, final raw log weights)."""
+        o = _lib.SamplerOptions(int(num_particles), int(iterations), float(threshold),
+                                float(alg8_probability), int(using_lg), int(invariant), int(cat),
+                                int(batch), int(max_iterations), 0, int(seed), float(dpp_alpha))
+        r = _lib.SamplerResult()
+        lnl = np.empty(num_particles)
+        w = np.empty(num_particles)
+        self._check(self.lib.smc_run_sampler(self.h, ctypes.byref(o), ctypes.byref(r), _dp(lnl), _dp(w)),
+                    "smc_run_sampler")
+        res = {k: getattr(r, k) for k, _ in _lib.SamplerResult._fields_ if k != "move_stats"}
+        res["move_stats"] = np.array(list(r.move_stats)).reshape(4, 2)
+        return res, lnl, w
 
     def particle_params(self, slot):
         """(children [T-1, 2], branch_len [2T-1], base [190], pinv, shape) of a slot."""
@@ -257,6 +273,7 @@ class Engine:
     def snapshot(self, slots, batch_template):
         """ParticleBatch of the engine's current state for the given slots."""
         out = batch_template.take(np.zeros(len(slots), dtype=int))
+        out.logits = np.zeros((len(slots), self.max_classes, 20))
         for j, s in enumerate(slots):
             ch, bl, base, pv, sh = self.particle_params(s)
             K, lg, asg = self.particle_state(s)
