@@ -14,6 +14,7 @@ prior).  Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -31,6 +32,24 @@ def algorithmic_bytes_per_column(T, word):
     T-2 internal non-root children read 20 words, T tip children read a 1-byte
     code, T-1 internal nodes write 20 words + an 8-byte log-scale."""
     return (T - 2) * 20 * word + T * 1 + (T - 1) * (20 * word + 8)
+
+
+def profiled_traffic(config, precision, variant, particles, default_particles):
+    """HBM bytes per pruning launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (calibrated against a plain copy) for this exact workload, or None."""
+    if config != "c2" or variant != "mfma" or particles != default_particles:
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "prune_mfma_c2_64taxa_2000sites_1024particles.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes_per_launch\[(\w+)\] = ([0-9.e+]+)", line.strip())
+                if m and m.group(1) == precision:
+                    return float(m.group(2))
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(codes, batch, R, threads, budget_s=10.0):
@@ -151,7 +170,9 @@ def main():
                    "precision": args.precision, "prune_kernel": args.variant, "mixture": mixture,
                    "updates_per_step": updates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.config, args.precision, args.variant, N,
+                                                 cfg["particles"]),
                      "kernel": "%s<%s>" % ("k_prune_mfma" if args.variant == "mfma" else "k_prune",
                                            "double" if word == 8 else "float"),
                      "launches_per_step": launches_per_sweep,
