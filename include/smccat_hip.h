@@ -202,6 +202,53 @@ typedef struct {
 int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, smc_sampler_result *result,
                     double *lnl_out, double *weights_out);
 
+/* ---- Learned CAT-allocation proposal (no counterpart in the reference) ----
+ * A 40 -> 128 -> 128 (GELU) network scores the candidates of Neal's Algorithm 8
+ * for one site (the context of Particle::gibbsPartitionMove, Particle.cpp:785-995):
+ *   x = [site amino-acid frequencies (20), size-weighted mean class profile (20)],
+ *   l_c = h2 . (U log pi_c + u), l_new = h2 . v + b_new,
+ *   logits tau * l_c + log n_c and tau * l_new + log alpha over K + 1 slots.
+ * Trained on the GPU (hand-written forward, backward and Adam kernels) on the
+ * Gibbs move's own draws; used by smc_learned_partition_move as a
+ * Metropolis-Hastings proposal under the exact site likelihood.
+ * Example layout: header[44] = x[40], tau, log alpha, 0, 0; class entries[24] =
+ * log pi[20], log n, 0, 0, 0; label in [0, K] (K = a new class). */
+typedef struct smc_mlp smc_mlp;
+
+int smc_mlp_num_params(void);
+int smc_mlp_create(smc_engine *e, uint64_t seed, smc_mlp **out);
+int smc_mlp_destroy(smc_mlp *m);
+int smc_mlp_get_params(smc_mlp *m, float *theta);
+int smc_mlp_set_params(smc_mlp *m, const float *theta);   /* resets the optimiser state */
+/* Appends n examples; cls_off[i] indexes the class entries cls[n_cls][24]. */
+int smc_mlp_add_examples(smc_mlp *m, int n, const float *header, const int32_t *num_classes,
+                         const int32_t *label, const int64_t *cls_off, const float *cls, int64_t n_cls);
+/* Runs the Gibbs allocation move (smc_gibbs_partition_move) on the slots and
+ * records every draw as an example. */
+int smc_mlp_collect_gibbs(smc_mlp *m, int count, const int32_t *slots, double tempering, uint64_t seed,
+                          int num_iter, double dpp_alpha, int64_t *stats_out);
+int64_t smc_mlp_num_examples(smc_mlp *m);
+int smc_mlp_clear_examples(smc_mlp *m);
+/* header[44], *num_classes, *label, cls[K][24] of example i (any may be NULL). */
+int smc_mlp_get_example(smc_mlp *m, int64_t i, float *header, int32_t *num_classes, int32_t *label,
+                        float *cls);
+/* Mean cross-entropy of examples [first, first+n) and its gradient (no update). */
+int smc_mlp_loss_grad(smc_mlp *m, int first, int n, double *loss, float *grad);
+/* log q over the K + 1 slots of examples [first, first+n): logq[n][out_stride]. */
+int smc_mlp_infer(smc_mlp *m, int first, int n, int out_stride, float *logq);
+/* Adam epochs over all examples (mini-batches of `batch`, reshuffled each
+ * epoch); epoch_loss[epochs] (nullable) receives each epoch's mean loss. */
+int smc_mlp_train(smc_mlp *m, int epochs, int batch, double lr, uint64_t seed, double *epoch_loss);
+/* The learned allocation move on the engine's slots: num_iter (0: S/2) random
+ * sites per particle, each one Metropolis-Hastings step of Algorithm 8 (one
+ * auxiliary class) with the network's proposal, tempered by
+ * proposal_tempering (< 0: the target's), accepted under the exact tempered
+ * site likelihood.  stats_out[5]: site updates, proposals of another slot,
+ * accepted, classes created, classes erased. */
+int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, double tempering,
+                               uint64_t seed, int num_iter, double dpp_alpha, double proposal_tempering,
+                               int64_t *stats_out);
+
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */
 int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,

@@ -437,6 +437,16 @@ int smc_set_alignment(smc_engine *e, const int8_t *codes)
         inv[s] = same ? 1.0 : 0.0;
         ich[s] = (same && c1 < 20) ? c1 : 0;
     }
+    // amino-acid frequencies of each column (codes >= 20 carry no state)
+    e->h_sitefreq.assign((size_t)S * 20, 0.f);
+    for (int s = 0; s < S; ++s) {
+        int obs = 0;
+        for (int t = 0; t < T; ++t) obs += hc[(size_t)t * Sp + s] < 20;
+        for (int t = 0; t < T; ++t) {
+            const int c = hc[(size_t)t * Sp + s];
+            if (c < 20) e->h_sitefreq[(size_t)s * 20 + c] += 1.f / (float)obs;
+        }
+    }
     HIP_TRY(e, hipMemcpyAsync(e->d_codes, hc.data(), hc.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(e->d_isinv, inv.data(), Sp * sizeof(double), hipMemcpyHostToDevice,
                               e->stream));

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -72,6 +73,7 @@ struct smc_engine {
     std::vector<double> h_logits; // [P][Kmax][20]
     std::vector<int> h_assign;    // [P][S]
     std::vector<double> h_pinv, h_shape;  // [P]
+    std::vector<float> h_sitefreq;        // [S][20] amino-acid frequencies of each column
     int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
     int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
     uint8_t *d_xflag = nullptr;   // [P][T-1] current workspace of each internal node
@@ -112,6 +114,22 @@ struct smc_engine {
 };
 
 int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+// Receives every draw of the Gibbs allocation move: the site, the classes
+// (logits) and sizes without the site, and the chosen slot (K = a new class).
+struct AllocationSink {
+    virtual ~AllocationSink() = default;
+    virtual void add(int site, const std::vector<std::array<double, 20>> &logits,
+                     const std::vector<int> &sizes, int label, double tempering, double dpp_alpha) = 0;
+};
+
+// Replaces the classes (K, logits [Kmax][20]) and allocation of one uploaded
+// slot, keeping its other parameters.
+int smc_upload_partition(smc_engine *e, int slot, int K, const double *logits, const int32_t *assign);
+
+// The Gibbs allocation move (smc_gibbs_partition_move) with an optional sink.
+int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double tempering, uint64_t seed,
+                   int num_iter, double dpp_alpha, int64_t *stats_out, AllocationSink *sink);
 
 // Commits the current state of slots [first, first+count): workspace flags,
 // lnL and parameters become the ones a rejected proposal returns to.

@@ -79,6 +79,12 @@ extern "C" int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t 
                                         double tempering, uint64_t seed, int num_iter,
                                         double dpp_alpha, int64_t *stats_out)
 {
+    return smc_gibbs_move(e, count, slots, tempering, seed, num_iter, dpp_alpha, stats_out, nullptr);
+}
+
+int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double tempering, uint64_t seed,
+                   int num_iter, double dpp_alpha, int64_t *stats_out, AllocationSink *sink)
+{
     if (!e) return SMC_ERR_ARG;
     if (count < 1 || !slots || !(dpp_alpha >= 0.0) || !(tempering >= 0.0))
         return smc_fail(e, SMC_ERR_ARG, "invalid Gibbs move arguments");
@@ -174,6 +180,7 @@ extern "C" int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t 
                 }
             }
             const int was = -2 - ps.assign[site];
+            if (sink) sink->add(site, ps.logits, ps.size, std::min(pick, ps.K), tempering, dpp_alpha);
             if (pick >= ps.K) {  // a new class from the chosen auxiliary
                 std::array<double, 20> lg;
                 std::copy(&cand[((size_t)i * n_cand + pick) * 20],
@@ -189,31 +196,11 @@ extern "C" int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t 
         }
     }
     // write the new allocations back and re-upload the particles
-    const int T = e->T, nn = e->nn;
-    std::vector<int32_t> ch((size_t)count * (T - 1) * 2), nc(count), asg((size_t)count * S);
-    std::vector<double> bl((size_t)count * nn), base((size_t)count * 190),
-        lg((size_t)count * Kmax * 20, 0.0), pinv(count), shape(count);
+    std::vector<double> lg((size_t)Kmax * 20);
     for (int i = 0; i < count; ++i) {
-        const int p = slots[i];
-        std::copy(e->h_children.begin() + (size_t)p * (T - 1) * 2,
-                  e->h_children.begin() + (size_t)(p + 1) * (T - 1) * 2, ch.begin() + (size_t)i * (T - 1) * 2);
-        std::copy(e->h_blen.begin() + (size_t)p * nn, e->h_blen.begin() + (size_t)(p + 1) * nn,
-                  bl.begin() + (size_t)i * nn);
-        std::copy(e->h_base.begin() + (size_t)p * 190, e->h_base.begin() + (size_t)(p + 1) * 190,
-                  base.begin() + (size_t)i * 190);
-        nc[i] = st[i].K;
-        for (int k = 0; k < st[i].K; ++k)
-            std::copy(st[i].logits[k].begin(), st[i].logits[k].end(),
-                      lg.begin() + ((size_t)i * Kmax + k) * 20);
-        std::copy(st[i].assign.begin(), st[i].assign.end(), asg.begin() + (size_t)i * S);
-        pinv[i] = e->h_pinv[p];
-        shape[i] = e->h_shape[p];
-    }
-    // slots may be scattered: upload one at a time (each call is host work plus small copies)
-    for (int i = 0; i < count; ++i) {
-        int rc = smc_set_particles(e, slots[i], 1, &ch[(size_t)i * (T - 1) * 2], &bl[(size_t)i * nn],
-                                   &base[(size_t)i * 190], &nc[i], &lg[(size_t)i * Kmax * 20],
-                                   &asg[(size_t)i * S], &pinv[i], &shape[i]);
+        std::fill(lg.begin(), lg.end(), 0.0);
+        for (int k = 0; k < st[i].K; ++k) std::copy(st[i].logits[k].begin(), st[i].logits[k].end(), &lg[(size_t)k * 20]);
+        int rc = smc_upload_partition(e, slots[i], st[i].K, lg.data(), st[i].assign.data());
         if (rc != SMC_OK) return rc;
     }
     if (stats_out) {
@@ -222,6 +209,17 @@ extern "C" int smc_gibbs_partition_move(smc_engine *e, int count, const int32_t 
         stats_out[2] = erased;
     }
     return SMC_OK;
+}
+
+int smc_upload_partition(smc_engine *e, int slot, int K, const double *logits, const int32_t *assign)
+{
+    const int T = e->T, nn = e->nn;
+    std::vector<int32_t> ch(e->h_children.begin() + (size_t)slot * (T - 1) * 2,
+                            e->h_children.begin() + (size_t)(slot + 1) * (T - 1) * 2);
+    const double pinv = e->h_pinv[slot], shape = e->h_shape[slot];
+    std::vector<double> bl(e->h_blen.begin() + (size_t)slot * nn, e->h_blen.begin() + (size_t)(slot + 1) * nn);
+    std::vector<double> base(e->h_base.begin() + (size_t)slot * 190, e->h_base.begin() + (size_t)(slot + 1) * 190);
+    return smc_set_particles(e, slot, 1, ch.data(), bl.data(), base.data(), &K, logits, assign, &pinv, &shape);
 }
 
 extern "C" int smc_get_particle_state(smc_engine *e, int slot, int32_t *num_classes,

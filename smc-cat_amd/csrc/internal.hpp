@@ -126,4 +126,31 @@ void launch_next_step(int n, double *w, const double *l, double *norm, double *o
                       int target_ess, double last_temp, hipStream_t s);
 void launch_resample(int n, const double *norm, double u, int *assign, hipStream_t s);
 
+// Proposal-network batch (mlp_layout.hpp): B examples, row r uses example
+// idx[r] (idx may be null: row r is example r).  Activations are per row.
+struct MlpArgs {
+    int B = 0;
+    const int *idx = nullptr;
+    const float *hdr = nullptr;     // [examples][kHdr]
+    const int *K = nullptr;         // [examples] classes
+    const int *label = nullptr;     // [examples] chosen slot (training)
+    const int64_t *coff = nullptr;  // [examples] first class entry
+    const float *cls = nullptr;     // [entries][kCls]
+    const float *theta = nullptr;
+    float *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *dH2 = nullptr;  // [B][kH]
+    float *loss = nullptr;          // [B]
+    float *part = nullptr;          // [ceil(B / kRows)][kNParam]
+    float *logq = nullptr;          // [B][out_stride] (inference)
+    int out_stride = 0;
+    float inv_B = 0.f;
+};
+
+void launch_mlp_trunk(const MlpArgs &a, bool train, hipStream_t s);
+void launch_mlp_head(const MlpArgs &a, bool train, hipStream_t s);
+void launch_mlp_backward(const MlpArgs &a, hipStream_t s);
+void launch_mlp_reduce(int rows, const float *part, float *grad, const float *loss, int B,
+                       double *loss_acc, hipStream_t s);
+void launch_mlp_adam(float *theta, float *m, float *v, const float *g, float lr, float b1, float b2,
+                     float eps, float c1, float c2, hipStream_t s);
+
 }  // namespace smccat

@@ -1,2 +1,3 @@
 """smccat: MI355X-native particle-likelihood engine for Smc-CAT (Python view of the C-ABI)."""
-from .engine import Engine, EngineError, ParticleBatch, discretize_gamma, lg_base_matrix  # noqa: F401
+from .engine import (Engine, EngineError, ParticleBatch, ProposalNetwork, discretize_gamma,  # noqa: F401
+                     lg_base_matrix)

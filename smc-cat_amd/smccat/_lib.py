@@ -24,6 +24,8 @@ _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int32)
 _BP = ctypes.POINTER(ctypes.c_uint8)
 _CP = ctypes.POINTER(ctypes.c_int8)
+_FP = ctypes.POINTER(ctypes.c_float)
+_LP = ctypes.POINTER(ctypes.c_int64)
 
 
 class EngineConfig(ctypes.Structure):
@@ -81,6 +83,20 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_int64)]),
     "smc_get_particle_state": (_I, [_P, _I, _IP, _DP, _IP]),
     "smc_get_particle_params": (_I, [_P, _I, _IP, _DP, _DP, _DP, _DP]),
+    "smc_mlp_num_params": (_I, []),
+    "smc_mlp_create": (_I, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "smc_mlp_destroy": (_I, [_P]),
+    "smc_mlp_get_params": (_I, [_P, _FP]),
+    "smc_mlp_set_params": (_I, [_P, _FP]),
+    "smc_mlp_add_examples": (_I, [_P, _I, _FP, _IP, _IP, _LP, _FP, ctypes.c_int64]),
+    "smc_mlp_collect_gibbs": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D, _LP]),
+    "smc_mlp_num_examples": (ctypes.c_int64, [_P]),
+    "smc_mlp_clear_examples": (_I, [_P]),
+    "smc_mlp_get_example": (_I, [_P, ctypes.c_int64, _FP, _IP, _IP, _FP]),
+    "smc_mlp_loss_grad": (_I, [_P, _I, _I, _DP, _FP]),
+    "smc_mlp_infer": (_I, [_P, _I, _I, _I, _FP]),
+    "smc_mlp_train": (_I, [_P, _I, _I, _D, ctypes.c_uint64, _DP]),
+    "smc_learned_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D, _D, _LP]),
     "smc_run_sampler": (_I, [_P, ctypes.POINTER(SamplerOptions), ctypes.POINTER(SamplerResult), _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),

@@ -311,6 +311,120 @@ class Engine:
         return out
 
 
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _lp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class ProposalNetwork:
+    """The learned CAT-allocation proposal of one engine (smc_mlp_* C-ABI).
+
+    Examples are (header [44], K, label, class entries [K, 24]) as in
+    include/smccat_hip.h; collect_gibbs records the Gibbs move's own draws."""
+
+    HDR, CLS = 44, 24
+
+    def __init__(self, engine, seed=0):
+        self.engine = engine
+        self.lib = engine.lib
+        self.h = ctypes.c_void_p()
+        engine._check(self.lib.smc_mlp_create(engine.h, int(seed), ctypes.byref(self.h)), "smc_mlp_create")
+        self.num_params = int(self.lib.smc_mlp_num_params())
+
+    def close(self):
+        if self.h:
+            self.lib.smc_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        self.engine._check(rc, what)
+
+    @property
+    def params(self):
+        th = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.lib.smc_mlp_get_params(self.h, _fp(th)), "smc_mlp_get_params")
+        return th
+
+    @params.setter
+    def params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.shape == (self.num_params,)
+        self._check(self.lib.smc_mlp_set_params(self.h, _fp(th)), "smc_mlp_set_params")
+
+    def add_examples(self, header, num_classes, label, cls_list):
+        header = np.ascontiguousarray(header, dtype=np.float32)
+        K = np.ascontiguousarray(num_classes, dtype=np.int32)
+        lab = np.ascontiguousarray(label, dtype=np.int32)
+        off = np.concatenate([[0], np.cumsum([len(c) for c in cls_list])[:-1]]).astype(np.int64)
+        cls = np.ascontiguousarray(np.concatenate(cls_list), dtype=np.float32)
+        self._check(self.lib.smc_mlp_add_examples(self.h, len(K), _fp(header), _ip(K), _ip(lab), _lp(off),
+                                                  _fp(cls), len(cls)), "smc_mlp_add_examples")
+
+    def collect_gibbs(self, slots, tempering, seed, num_iter=0, dpp_alpha=0.01):
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        st = np.zeros(3, dtype=np.int64)
+        self._check(self.lib.smc_mlp_collect_gibbs(self.h, len(sl), _ip(sl), float(tempering), int(seed),
+                                                   int(num_iter), float(dpp_alpha), _lp(st)),
+                    "smc_mlp_collect_gibbs")
+        return st
+
+    @property
+    def num_examples(self):
+        return int(self.lib.smc_mlp_num_examples(self.h))
+
+    def clear_examples(self):
+        self._check(self.lib.smc_mlp_clear_examples(self.h), "smc_mlp_clear_examples")
+
+    def example(self, i):
+        hdr = np.empty(self.HDR, dtype=np.float32)
+        K, lab = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.smc_mlp_get_example(self.h, int(i), _fp(hdr), ctypes.byref(K), ctypes.byref(lab),
+                                                 None), "smc_mlp_get_example")
+        cls = np.empty((K.value, self.CLS), dtype=np.float32)
+        self._check(self.lib.smc_mlp_get_example(self.h, int(i), None, None, None, _fp(cls)),
+                    "smc_mlp_get_example")
+        return hdr, K.value, lab.value, cls
+
+    def loss_grad(self, first, n):
+        g = np.empty(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.smc_mlp_loss_grad(self.h, int(first), int(n), ctypes.byref(loss), _fp(g)),
+                    "smc_mlp_loss_grad")
+        return loss.value, g
+
+    def infer(self, first, n, out_stride):
+        out = np.empty((n, out_stride), dtype=np.float32)
+        self._check(self.lib.smc_mlp_infer(self.h, int(first), int(n), int(out_stride), _fp(out)),
+                    "smc_mlp_infer")
+        return out
+
+    def train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        losses = np.empty(max(epochs, 1))
+        self._check(self.lib.smc_mlp_train(self.h, int(epochs), int(batch), float(lr), int(seed), _dp(losses)),
+                    "smc_mlp_train")
+        return losses[:epochs]
+
+    def move(self, slots, tempering, seed, num_iter=0, dpp_alpha=0.01, proposal_tempering=-1.0):
+        """The learned allocation move; returns stats [updates, proposed, accepted,
+        created, erased]."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        st = np.zeros(5, dtype=np.int64)
+        self._check(self.lib.smc_learned_partition_move(self.h, len(sl), _ip(sl), float(tempering), int(seed),
+                                                        int(num_iter), float(dpp_alpha),
+                                                        float(proposal_tempering), _lp(st)),
+                    "smc_learned_partition_move")
+        return st
+
+
 def discretize_gamma(shape, num):
     lib = _lib.load()
     out = np.empty(num)
