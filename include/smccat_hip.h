@@ -127,6 +127,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                             const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out);
 int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept);
 
+typedef struct smc_mlp smc_mlp;  /* learned CAT-allocation proposal (below) */
+
 /* Rejuvenation settings (main.cpp:54-105, Particle.hpp:87-94, Settings.hpp:19-29). */
 typedef struct {
     int32_t iterations;          /* MCMC iterations r per particle (Settings::rejuvenationIterations) */
@@ -145,6 +147,12 @@ typedef struct {
     const double *rate_cholesky; /* [190][190] row-major lower factor of the full rate-matrix move
                                     (Particle::rateMatrixCholesky, Particle.cpp:710-716); NULL:
                                     rate_full_scale * identity                                    */
+    smc_mlp *allocation_proposal;/* non-NULL: the CAT move is the learned Metropolis-Hastings move
+                                    (smc_learned_partition_move) with this network, or, with
+                                    record_allocations, the Gibbs move recording its draws into
+                                    the network's examples; NULL: the Gibbs move                 */
+    int32_t record_allocations;
+    int32_t reserved2;           /* must be 0                                                     */
 } smc_rejuvenation_options;
 
 /* Rejuvenation of slots [first, first+count) at tempering T, replacing the
@@ -185,6 +193,14 @@ typedef struct {
     int32_t reserved;               /* must be 0                                                 */
     uint64_t seed;                  /* Philox key of the sampler                                 */
     double dpp_alpha;               /* CRP concentration                                         */
+    smc_mlp *allocation_proposal;   /* non-NULL: the learned CAT move; the first
+                                       proposal_warmup resampling steps run the Gibbs move and
+                                       record its draws, the network is trained on them
+                                       (train_epochs Adam epochs at train_lr) and later steps
+                                       use the learned move                                      */
+    int32_t proposal_warmup;
+    int32_t train_epochs;
+    double train_lr;
 } smc_sampler_options;
 
 typedef struct {
@@ -195,7 +211,9 @@ typedef struct {
     double ess_sum;                 /* sum of the ESS after each reweight                        */
     double seconds;                 /* wall time of the loop, initial likelihoods included       */
     double rejuvenation_seconds;    /* of which rejuvenation                                     */
+    double training_seconds;        /* of which training the proposal network                    */
     int64_t move_stats[8];          /* proposed / accepted: tree, shape, invariant, rate matrix  */
+    int64_t cat_stats[3];           /* CAT move: site updates, site moves, Gibbs examples kept   */
 } smc_sampler_result;
 
 /* lnl_out[N], weights_out[N] (nullable): final raw lnL and raw log weights. */
@@ -213,8 +231,6 @@ int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, smc_sampler_r
  * Metropolis-Hastings proposal under the exact site likelihood.
  * Example layout: header[44] = x[40], tau, log alpha, 0, 0; class entries[24] =
  * log pi[20], log n, 0, 0, 0; label in [0, K] (K = a new class). */
-typedef struct smc_mlp smc_mlp;
-
 int smc_mlp_num_params(void);
 int smc_mlp_create(smc_engine *e, uint64_t seed, smc_mlp **out);
 int smc_mlp_destroy(smc_mlp *m);

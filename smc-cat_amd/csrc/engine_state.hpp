@@ -127,6 +127,12 @@ struct AllocationSink {
 // slot, keeping its other parameters.
 int smc_upload_partition(smc_engine *e, int slot, int K, const double *logits, const int32_t *assign);
 
+// smc_rejuvenate that also adds the CAT move's site updates and site moves to
+// cat_stats[0..1] (nullable).
+int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
+                      const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out,
+                      int64_t *cat_stats);
+
 // The Gibbs allocation move (smc_gibbs_partition_move) with an optional sink.
 int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double tempering, uint64_t seed,
                    int num_iter, double dpp_alpha, int64_t *stats_out, AllocationSink *sink);

@@ -184,6 +184,13 @@ extern "C" int smc_rejuvenate(smc_engine *e, int first, int count, double temper
                               const smc_rejuvenation_options *opt, double *lnl_out,
                               int64_t *stats_out)
 {
+    return smc_rejuvenate_ex(e, first, count, tempering, opt, lnl_out, stats_out, nullptr);
+}
+
+int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
+                      const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out,
+                      int64_t *cat_stats)
+{
     if (!e) return SMC_ERR_ARG;
     if (!opt || first < 0 || count < 1 || first + count > e->Pmax || opt->iterations < 0)
         return smc_fail(e, SMC_ERR_ARG, "invalid rejuvenation arguments");
@@ -321,8 +328,26 @@ extern "C" int smc_rejuvenate(smc_engine *e, int first, int count, double temper
         for (int i = 0; i < count; ++i)
             if (rng[i].uniform() < o.gibbs_probability) slots.push_back(first + i);
         if (!slots.empty()) {
-            rc = smc_gibbs_partition_move(e, (int)slots.size(), slots.data(), tempering, o.seed ^ 0x61626273ull,
-                                          0, o.dpp_alpha, nullptr);
+            const int n = (int)slots.size();
+            const uint64_t seed = o.seed ^ 0x61626273ull;
+            int64_t st[5] = {};
+            if (o.allocation_proposal && !o.record_allocations) {
+                rc = smc_learned_partition_move(o.allocation_proposal, n, slots.data(), tempering, seed, 0,
+                                                o.dpp_alpha, -1.0, st);
+                if (cat_stats) {
+                    cat_stats[0] += st[0];
+                    cat_stats[1] += st[2];
+                }
+            } else {
+                rc = o.allocation_proposal
+                         ? smc_mlp_collect_gibbs(o.allocation_proposal, n, slots.data(), tempering, seed, 0,
+                                                 o.dpp_alpha, st)
+                         : smc_gibbs_partition_move(e, n, slots.data(), tempering, seed, 0, o.dpp_alpha, st);
+                if (cat_stats) {
+                    cat_stats[0] += (int64_t)n * (int64_t)(e->S / 2.0);
+                    cat_stats[1] += st[0];
+                }
+            }
             if (rc != SMC_OK) return rc;
             rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
             if (rc != SMC_OK) return rc;

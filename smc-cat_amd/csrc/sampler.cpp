@@ -62,6 +62,9 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
 {
     if (!e) return SMC_ERR_ARG;
     if (!opt || !res) return smc_fail(e, SMC_ERR_ARG, "null sampler options or result");
+    if (opt->allocation_proposal && (opt->proposal_warmup < 0 || opt->train_epochs < 0 ||
+                                     (opt->train_epochs > 0 && !(opt->train_lr > 0.0))))
+        return smc_fail(e, SMC_ERR_ARG, "invalid proposal training options");
     const smc_sampler_options &o = *opt;
     const int N = o.num_particles;
     if (N < 1 || N > e->Pmax || o.rejuvenation_iterations < 0 || o.batch < 0 || o.reserved != 0 ||
@@ -125,6 +128,10 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
             ro.gibbs_probability = o.cat ? o.alg8_probability : 0.0;
             ro.dpp_alpha = o.dpp_alpha;
             ro.rate_full_scale = 0.0;
+            // the learned CAT move: Gibbs with recording during the warm-up steps
+            const bool learned = o.cat && o.allocation_proposal;
+            ro.allocation_proposal = learned ? o.allocation_proposal : nullptr;
+            ro.record_allocations = learned && res->resamples < o.proposal_warmup;
             if (!o.using_lg) {
                 // uncentred second moment of the base matrices (Q21), scaled,
                 // shrunk towards its mean diagonal, then factored
@@ -167,11 +174,21 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
             for (int first = 0; first < N; first += batch) {
                 const int cnt = std::min(batch, N - first);
                 int64_t st[8];
-                rc = smc_rejuvenate(e, first, cnt, cur, &ro, lnl.data() + first, st);
+                rc = smc_rejuvenate_ex(e, first, cnt, cur, &ro, lnl.data() + first, st, res->cat_stats);
                 if (rc != SMC_OK) return rc;
                 for (int k = 0; k < 8; ++k) res->move_stats[k] += st[k];
             }
             rejuv_s += std::chrono::duration<double>(clk::now() - r0).count();
+            if (ro.record_allocations && res->resamples + 1 == o.proposal_warmup) {
+                const auto t1 = clk::now();
+                res->cat_stats[2] = smc_mlp_num_examples(o.allocation_proposal);
+                if (res->cat_stats[2] > 0 && o.train_epochs > 0) {
+                    rc = smc_mlp_train(o.allocation_proposal, o.train_epochs, 4096, o.train_lr, o.seed ^ 0x6c6561726eull,
+                                       nullptr);
+                    if (rc != SMC_OK) return rc;
+                }
+                res->training_seconds += std::chrono::duration<double>(clk::now() - t1).count();
+            }
             std::fill(w.begin(), w.end(), -std::log((double)N));
             res->resamples++;
         }

@@ -43,7 +43,9 @@ class RejuvenationOptions(ctypes.Structure):
                 ("subtree_scale_delta", ctypes.c_double), ("shape_delta", ctypes.c_double),
                 ("rate_delta", ctypes.c_double), ("rate_full_scale", ctypes.c_double),
                 ("invar_alpha", ctypes.c_double), ("gibbs_probability", ctypes.c_double),
-                ("dpp_alpha", ctypes.c_double), ("rate_cholesky", ctypes.c_void_p)]
+                ("dpp_alpha", ctypes.c_double), ("rate_cholesky", ctypes.c_void_p),
+                ("allocation_proposal", ctypes.c_void_p), ("record_allocations", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32)]
 
 
 class SamplerOptions(ctypes.Structure):
@@ -51,14 +53,17 @@ class SamplerOptions(ctypes.Structure):
                 ("rejuvenation_threshold", ctypes.c_double), ("alg8_probability", ctypes.c_double),
                 ("using_lg", ctypes.c_int32), ("invariant", ctypes.c_int32), ("cat", ctypes.c_int32),
                 ("batch", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64), ("dpp_alpha", ctypes.c_double)]
+                ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64), ("dpp_alpha", ctypes.c_double),
+                ("allocation_proposal", ctypes.c_void_p), ("proposal_warmup", ctypes.c_int32),
+                ("train_epochs", ctypes.c_int32), ("train_lr", ctypes.c_double)]
 
 
 class SamplerResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("resamples", ctypes.c_int32),
                 ("final_temp", ctypes.c_double), ("final_ess", ctypes.c_double),
                 ("ess_sum", ctypes.c_double), ("seconds", ctypes.c_double),
-                ("rejuvenation_seconds", ctypes.c_double), ("move_stats", ctypes.c_int64 * 8)]
+                ("rejuvenation_seconds", ctypes.c_double), ("training_seconds", ctypes.c_double),
+                ("move_stats", ctypes.c_int64 * 8), ("cat_stats", ctypes.c_int64 * 3)]
 
 
 SIGNATURES = {

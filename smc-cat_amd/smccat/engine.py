@@ -235,7 +235,7 @@ class Engine:
                                      deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
                                      deltas.get("rate_full_scale", 0.5),
                                      deltas.get("invar_alpha", 100.0), float(gibbs_probability),
-                                     float(dpp_alpha), None)
+                                     float(dpp_alpha), None, None, 0, 0)
         lnl = np.empty(count)
         stats = (ctypes.c_int64 * 8)()
         self._check(self.lib.smc_rejuvenate(self.h, first, count, float(tempering), ctypes.byref(o),
@@ -243,19 +243,24 @@ class Engine:
         return lnl, np.array(list(stats)).reshape(4, 2)
 
     def run_sampler(self, num_particles, iterations, seed, using_lg=False, invariant=True, cat=False,
-                    threshold=0.6, alg8_probability=0.1, batch=0, max_iterations=0, dpp_alpha=0.01):
+                    threshold=0.6, alg8_probability=0.1, batch=0, max_iterations=0, dpp_alpha=0.01,
+                    proposal=None, proposal_warmup=1, train_epochs=8, train_lr=2e-3):
         """The SMC loop over slots [0, num_particles) (which hold the initial
-        population).  Returns (result dict, final lnl, final raw log weights)."""
+        population).  proposal: a ProposalNetwork for the learned CAT move.
+        Returns (result dict, final lnl, final raw log weights)."""
         o = _lib.SamplerOptions(int(num_particles), int(iterations), float(threshold),
                                 float(alg8_probability), int(using_lg), int(invariant), int(cat),
-                                int(batch), int(max_iterations), 0, int(seed), float(dpp_alpha))
+                                int(batch), int(max_iterations), 0, int(seed), float(dpp_alpha),
+                                proposal.h if proposal is not None else None, int(proposal_warmup),
+                                int(train_epochs), float(train_lr))
         r = _lib.SamplerResult()
         lnl = np.empty(num_particles)
         w = np.empty(num_particles)
         self._check(self.lib.smc_run_sampler(self.h, ctypes.byref(o), ctypes.byref(r), _dp(lnl), _dp(w)),
                     "smc_run_sampler")
-        res = {k: getattr(r, k) for k, _ in _lib.SamplerResult._fields_ if k != "move_stats"}
+        res = {k: getattr(r, k) for k, _ in _lib.SamplerResult._fields_ if k not in ("move_stats", "cat_stats")}
         res["move_stats"] = np.array(list(r.move_stats)).reshape(4, 2)
+        res["cat_stats"] = np.array(list(r.cat_stats))
         return res, lnl, w
 
     def particle_params(self, slot):

@@ -81,3 +81,26 @@ def test_sampler_cat_with_gibbs(gpu):
                                       alg8_probability=0.5, dpp_alpha=1.0)
         assert res["final_temp"] == 1.0
         _check_final(eng, batch, codes, R, lnl, n_check=N)
+
+
+def test_sampler_cat_learned_proposal(gpu):
+    _say("case sampler learned: 12 taxa x 120 sites LG+G4 CAT K<=6, 24 particles, warm-up 1")
+    from smccat import ProposalNetwork
+    rng = np.random.default_rng(8)
+    T, S, R, N = 12, 120, 4, 24
+    codes, _ = sim.simulate_config("c3", taxa=T, sites=S)
+    batch = sim.prior_particles(rng, N, T, S, num_rates=R, lg=True, lg_table=lg_base_matrix(),
+                                k_range=(2, 6), max_classes=16)
+    with Engine(T, S, R, N, 16, workspaces=2) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        with ProposalNetwork(eng, seed=1) as net:
+            res, lnl, w = eng.run_sampler(N, 2, seed=4, using_lg=True, invariant=False, cat=True,
+                                          alg8_probability=0.5, dpp_alpha=1.0, proposal=net,
+                                          proposal_warmup=2, train_epochs=3)
+            _say("  %s" % res)
+            assert res["final_temp"] == 1.0 and res["resamples"] > 2
+            assert res["cat_stats"][2] > 0 and net.num_examples == res["cat_stats"][2]
+            assert res["cat_stats"][0] > res["cat_stats"][2] and res["cat_stats"][1] > 0
+            assert res["training_seconds"] > 0
+        _check_final(eng, batch, codes, R, lnl, n_check=N)
