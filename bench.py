@@ -72,6 +72,159 @@ def cpu_baseline(codes, batch, R, threads, budget_s=10.0):
                       (done, T, S, R, dt)}
 
 
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def ess_run(codes, batch, R, precision, batch_size, learned, iterations, seed, dpp_alpha, kmax,
+            alg8=0.1, warmup=1, epochs=8):
+    """One SMC run (main.cpp:143-336) on the engine; returns its summary."""
+    from smccat import Engine, ProposalNetwork
+    T, S = codes.shape
+    N = batch.n
+    eng = Engine(T, S, R, N, kmax, precision, workspaces=2)
+    net = None
+    try:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        if learned:
+            net = ProposalNetwork(eng, seed=seed)
+        t0 = time.perf_counter()
+        last = [t0]
+
+        def tick(it, temp, ess):
+            now = time.perf_counter()
+            if now - last[0] > 20.0:
+                progress("  iteration %d T=%.5f ESS %.1f (%.0f s)" % (it, temp, ess, now - t0))
+                last[0] = now
+        res, lnl, _ = eng.run_sampler(N, iterations, seed=seed, using_lg=True, invariant=False, cat=True,
+                                      alg8_probability=alg8, batch=batch_size, dpp_alpha=dpp_alpha,
+                                      proposal=net, proposal_warmup=warmup, train_epochs=epochs,
+                                      on_iteration=tick)
+        phases = {k: round(v[0] / 1e3, 3) for k, v in eng.phase_stats().items()}
+    finally:
+        if net is not None:
+            net.close()
+        eng.close()
+    return {"seconds": res["seconds"], "final_ess": res["final_ess"], "host_phase_seconds": phases,
+            "ess_per_sec": res["final_ess"] / res["seconds"],
+            "mean_ess": res["ess_sum"] / max(res["iterations"], 1),
+            "iterations": res["iterations"], "resamples": res["resamples"],
+            "rejuvenation_seconds": res["rejuvenation_seconds"],
+            "training_seconds": res["training_seconds"],
+            "cat_site_updates": int(res["cat_stats"][0]), "cat_site_moves": int(res["cat_stats"][1]),
+            "training_examples": int(res["cat_stats"][2]),
+            "lnl_median": float(np.median(lnl))}
+
+
+def ess_compare(args):
+    """SMC ESS/sec of the plain FP64 path (one particle at a time, as the
+    reference's per-thread loop; Gibbs CAT move) against the FP32 + batched +
+    learned-proposal path on the same CAT data and initial population."""
+    from smccat import lg_base_matrix
+    from smccat import simulate as sim
+    T, S, N = args.ess_taxa, args.ess_sites, args.ess_particles
+    codes, cfg = sim.simulate_config("c3", taxa=T, sites=S)
+    R = cfg["rates"]
+    kr = (5, 60)
+    batch = sim.prior_particles(np.random.default_rng(11), N, T, S, num_rates=R, lg=True,
+                                lg_table=lg_base_matrix(), k_range=kr)
+    out = {"workload": "CAT data as c3 (DP-CAT, 20 profiles ~ Dirichlet(0.5), LG+G4) at %d taxa x %d "
+                       "sites; %d particles, K ~ U{%d..%d}; %d rejuvenation iterations; CAT move "
+                       "probability 0.1; DP alpha %.1f" % (T, S, N, kr[0], kr[1], args.ess_iterations,
+                                                           1.0),
+           "metric": "ESS at T = 1 / wall seconds of the SMC loop (initial likelihoods included)"}
+    runs = {"plain_fp64": ("fp64", 1, False), "fp64_batched": ("fp64", 0, False),
+            "fp32_batched_learned": ("fp32", 0, True)}
+    for name, (prec, bs, learned) in runs.items():
+        if name == "fp64_batched" and not args.ess_attribution:
+            continue
+        vals = []
+        for seed in range(1, args.ess_seeds + 1):
+            vals.append(ess_run(codes, batch, R, prec, bs, learned, args.ess_iterations, seed, 1.0, kr[1]))
+            progress("ess %s seed %d: %.1f s, final ESS %.1f, %d iterations" %
+                     (name, seed, vals[-1]["seconds"], vals[-1]["final_ess"], vals[-1]["iterations"]))
+        out[name] = {k: float(np.mean([v[k] for v in vals])) for k in vals[0] if k != "host_phase_seconds"}
+        out[name]["host_phase_seconds"] = vals[0]["host_phase_seconds"]
+        out[name]["seeds"] = args.ess_seeds
+    out["ratio_ess_per_sec"] = out["fp32_batched_learned"]["ess_per_sec"] / out["plain_fp64"]["ess_per_sec"]
+    return out
+
+
+def c5_bench(args):
+    """Learned CAT-allocation proposal on c3 data: Gibbs draws recorded as
+    training pairs, GPU training, and the learned move against the Gibbs move."""
+    from smccat import Engine, ProposalNetwork, lg_base_matrix
+    from smccat import simulate as sim
+    codes, cfg = sim.simulate_config("c3", taxa=args.c5_taxa or None, sites=args.c5_sites or None)
+    T, S = codes.shape
+    R, N, kr = cfg["rates"], args.c5_particles or cfg["particles"], cfg["k_range"]
+    batch = sim.prior_particles(np.random.default_rng(5), N, T, S, num_rates=R, lg=True,
+                                lg_table=lg_base_matrix(), k_range=kr)
+    out = {"workload": "c3 data (%d taxa x %d sites, LG+G4, DP-CAT), %d particles K ~ U{%d..%d}, "
+                       "DP alpha 1.0, tempering 1" % (T, S, N, kr[0], kr[1])}
+    eng = Engine(T, S, R, N, kr[1] + 8, "fp32", partial_slots=8)
+    eng.set_alignment(codes)
+    eng.set_particles(batch)
+    net = ProposalNetwork(eng, seed=1)
+    crp = ProposalNetwork(eng, seed=2)
+    crp.params = np.zeros(net.num_params, dtype=np.float32)
+    t0 = time.perf_counter()
+    rounds = 0
+    while net.num_examples < args.c5_examples:
+        net.collect_gibbs(range(N), 1.0, seed=100 + rounds, dpp_alpha=1.0)
+        rounds += 1
+        progress("c5 collected %d examples in %.1f s" % (net.num_examples, time.perf_counter() - t0))
+    t_collect = time.perf_counter() - t0
+    n = net.num_examples
+    hold = max(1, n // 20)
+    n_train = n - hold
+    # held-out examples: the last 5% (the latest Gibbs iterations)
+    before, _ = net.loss_grad(n_train, hold)
+    # train on the first 95%: move the held-out block to a second network's store
+    held = [net.example(i) for i in range(n_train, n)]
+    ev = ProposalNetwork(eng, seed=3)
+    ev.add_examples(np.array([h[0] for h in held]), [h[1] for h in held], [h[2] for h in held],
+                    [h[3] for h in held])
+    crp.add_examples(np.array([h[0] for h in held]), [h[1] for h in held], [h[2] for h in held],
+                     [h[3] for h in held])
+    crp_loss, _ = crp.loss_grad(0, hold)
+    t0 = time.perf_counter()
+    losses = net.train(args.c5_epochs, batch=4096, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    progress("c5 trained %d epochs in %.1f s" % (args.c5_epochs, t_train))
+    ev.params = net.params
+    after, _ = ev.loss_grad(0, hold)
+    out["examples"] = n
+    out["collect_seconds"] = t_collect
+    out["train"] = {"epochs": args.c5_epochs, "seconds": t_train,
+                    "examples_per_sec": n * args.c5_epochs / t_train,
+                    "epoch_loss": [float(x) for x in losses]}
+    out["heldout_loss"] = {"crp_only": crp_loss, "untrained": before, "trained": after}
+    # the moves on the same (Gibbs-updated) particles: site updates per second
+    iters = args.c5_move_iters
+    moves = {}
+    for name, fn in (("gibbs", lambda: eng.gibbs_partition_move(range(N), 1.0, seed=7, num_iter=iters,
+                                                                   dpp_alpha=1.0)),
+                     ("learned", lambda: net.move(range(N), 1.0, seed=7, num_iter=iters, dpp_alpha=1.0)),
+                     ("crp_proposal", lambda: crp.move(range(N), 1.0, seed=7, num_iter=iters,
+                                                       dpp_alpha=1.0))):
+        t0 = time.perf_counter()
+        st = fn()
+        dt = time.perf_counter() - t0
+        progress("c5 move %s: %.1f s" % (name, dt))
+        moves[name] = {"seconds": dt, "site_updates_per_sec": N * iters / dt,
+                       "site_moves": int(st[0] if name == "gibbs" else st[2])}
+        if name != "gibbs":
+            moves[name]["acceptance"] = float(st[2] / max(st[1], 1))
+    out["moves"] = moves
+    out["speedup_site_updates"] = moves["learned"]["site_updates_per_sec"] / moves["gibbs"]["site_updates_per_sec"]
+    for x in (net, crp, ev):
+        x.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,13 +242,34 @@ def main():
                          "log-space form (default for c3/c4, where exp(logSiteLike) underflows)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "c5"],
+                    help="sweep: the updates/s line (with the SMC ESS/sec comparison); c5: the "
+                         "learned-proposal benchmark")
+    ap.add_argument("--no-ess", action="store_true", help="skip the SMC ESS/sec comparison")
+    ap.add_argument("--ess-taxa", type=int, default=16)
+    ap.add_argument("--ess-sites", type=int, default=200)
+    ap.add_argument("--ess-particles", type=int, default=64)
+    ap.add_argument("--ess-iterations", type=int, default=4)
+    ap.add_argument("--ess-seeds", type=int, default=1)
+    ap.add_argument("--ess-attribution", action="store_true",
+                    help="also run FP64 batched with the Gibbs move")
+    ap.add_argument("--c5-taxa", type=int, default=0)
+    ap.add_argument("--c5-sites", type=int, default=0)
+    ap.add_argument("--c5-particles", type=int, default=0)
+    ap.add_argument("--c5-examples", type=int, default=1_000_000)
+    ap.add_argument("--c5-epochs", type=int, default=6)
+    ap.add_argument("--c5-move-iters", type=int, default=500)
     ap.add_argument("--fp32-error", type=int, default=64,
                     help="fp32 runs: particles re-evaluated in FP64 for the error report")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(particles would shard across GPUs by slot; see DESIGN.md)", file=sys.stderr)
+              "(particles would shard across GPUs by slot)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "c5":
+        print(json.dumps({"metric": "learned CAT-allocation proposal: training and move throughput",
+                          "c5": c5_bench(args)}), flush=True)
+        return
 
     from smccat import Engine, lg_base_matrix
     from smccat import simulate as sim
@@ -129,6 +303,7 @@ def main():
         eng.systematic_resample(norm, 0.5)
         return lnl, nf, ess, temp
 
+    progress("sweep workload %s: %d particles uploaded" % (args.config, N))
     for _ in range(args.warmup):
         step()
     eng.set_timing(True)
@@ -139,6 +314,7 @@ def main():
     dt = time.perf_counter() - t0
     stats = eng.kernel_stats()
     eng.set_timing(False)
+    progress("sweep: %d steps in %.2f s" % (args.steps, dt))
 
     updates = N * (T - 1) * S * R
     value = updates * args.steps / dt
@@ -207,8 +383,10 @@ def main():
                                      invar=cfg["invar"], lg=cfg["lg"],
                                      lg_table=lg_base_matrix() if cfg["lg"] else None, k_range=kr)
         out["cpu_baseline"] = cpu_baseline(codes, sample, R, threads, args.cpu_seconds)
+        progress("cpu baseline done")
     else:
         out["cpu_baseline"] = None
+    out["smc_ess"] = None if args.no_ess else ess_compare(args)
     print(json.dumps(out), flush=True)
 
 

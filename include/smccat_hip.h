@@ -201,6 +201,9 @@ typedef struct {
     int32_t proposal_warmup;
     int32_t train_epochs;
     double train_lr;
+    /* called after every computeNextStep (nullable): iteration, tempering, ESS */
+    void (*on_iteration)(void *user, int32_t iteration, double temp, double ess);
+    void *user;
 } smc_sampler_options;
 
 typedef struct {
@@ -264,6 +267,23 @@ int smc_mlp_train(smc_mlp *m, int epochs, int batch, double lr, uint64_t seed, d
 int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, double tempering,
                                uint64_t seed, int num_iter, double dpp_alpha, double proposal_tempering,
                                int64_t *stats_out);
+
+/* Host wall time spent in each entry point (diagnostics; not reset by
+ * smc_engine_reset_stats).  ms[SMC_PHASE_COUNT], calls[SMC_PHASE_COUNT]; either
+ * may be NULL.  Nested calls count in both phases. */
+enum {
+    SMC_PHASE_SET_PARTICLES = 0,
+    SMC_PHASE_REFRESH = 1,
+    SMC_PHASE_UPDATE_PARAMS = 2,
+    SMC_PHASE_REFRESH_INCREMENTAL = 3,
+    SMC_PHASE_ACCEPT_REJECT = 4,
+    SMC_PHASE_PROPOSE = 5,      /* host proposals and priors of smc_rejuvenate   */
+    SMC_PHASE_CAT_MOVE = 6,     /* Gibbs or learned allocation moves              */
+    SMC_PHASE_SITE_CLASS = 7,
+    SMC_PHASE_COUNT = 8
+};
+int smc_engine_phase_stats(smc_engine *e, double *ms, int64_t *calls);
+int smc_engine_reset_phase_stats(smc_engine *e);
 
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */

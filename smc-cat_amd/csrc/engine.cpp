@@ -376,6 +376,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_assign.assign(P * (size_t)e->S, 0);
     e->h_pinv.assign(P, 0.0);
     e->h_shape.assign(P, 1.0);
+    e->h_rates.assign((size_t)P * e->R, 1.0);
     e->h_xflag.assign(P * (T - 1), 0);
     e->h_xflag_old.assign(P * (T - 1), 0);
     e->h_lnl.assign(P, 0.0);
@@ -395,7 +396,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
-                    e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out};
+                    e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -463,6 +464,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
                       const double *shape)
 {
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_SET_PARTICLES);
     if (first < 0 || count < 1 || first + count > e->Pmax)
         return fail(e, SMC_ERR_ARG, "particle range out of bounds");
     if (!children || !branch_len || !base || !num_classes || !logits || !assignment || !pinv ||
@@ -504,6 +506,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
         } else {
             rates[(size_t)i] = 1.0;
         }
+        std::copy_n(&rates[(size_t)i * R], R, &e->h_rates[(size_t)p * R]);
         // class-sorted slot order and tiles (each tile is one class)
         const int Kp = num_classes[i];
         if (Kp < 1 || Kp > K)
@@ -573,6 +576,7 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                            uint8_t *nonfinite_out)
 {
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_REFRESH);
     if (first < 0 || count < 1 || first + count > e->Pmax)
         return fail(e, SMC_ERR_ARG, "particle range out of bounds");
     if (!e->have_alignment) return fail(e, SMC_ERR_STATE, "no alignment uploaded");
@@ -736,7 +740,9 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
         if (rc != SMC_OK) return rc;
         if (R > 1) {
             if (!(shape[i] > 0.0)) return fail(e, SMC_ERR_ARG, "shape must be > 0");
-            discretize_gamma(shape[i], R, &rates[(size_t)i * R]);
+            if (shape[i] != e->h_shape[p])  // rates of an unchanged shape are kept
+                discretize_gamma(shape[i], R, &e->h_rates[(size_t)p * R]);
+            std::copy_n(&e->h_rates[(size_t)p * R], R, &rates[(size_t)i * R]);
         }
         for (int k = 0; k < Ti; ++k) {
             e->h_height[(size_t)p * Ti + k] = hh[k];
@@ -780,6 +786,7 @@ int smc_update_params(smc_engine *e, int first, int count, const int32_t *childr
                       const double *shape)
 {
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_UPDATE_PARAMS);
     if (first < 0 || count < 1 || first + count > e->Pmax || !children || !branch_len || !base ||
         !pinv || !shape)
         return fail(e, SMC_ERR_ARG, "invalid parameter update");
@@ -790,6 +797,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                             const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out)
 {
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_REFRESH_INCREMENTAL);
     if (e->WS != 2) return fail(e, SMC_ERR_STATE, "incremental refresh needs workspaces = 2");
     if (first < 0 || count < 1 || first + count > e->Pmax || !node_flags)
         return fail(e, SMC_ERR_ARG, "invalid incremental refresh arguments");
@@ -920,6 +928,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
 int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept)
 {
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_ACCEPT_REJECT);
     if (first < 0 || count < 1 || first + count > e->Pmax || !accept)
         return fail(e, SMC_ERR_ARG, "invalid accept/reject arguments");
     const int T = e->T, Ti = T - 1, nn = e->nn;
@@ -1044,7 +1053,16 @@ int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const
                        int n_cand, const int32_t *ncand, const double *cand_logits,
                        double tempering, double *out)
 {
+    return smc_site_class_lnl_ex(e, count, particles, sites, n_cand, ncand, cand_logits, nullptr, tempering,
+                                 out);
+}
+
+int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, const int32_t *sites,
+                          int n_cand, const int32_t *ncand, const double *cand_logits,
+                          const int32_t *cand_class, double tempering, double *out)
+{
     if (!e) return SMC_ERR_ARG;
+    PhaseTimer phase_timer(e, SMC_PHASE_SITE_CLASS);
     if (count < 1 || n_cand < 1 || !particles || !sites || !ncand || !cand_logits || !out)
         return fail(e, SMC_ERR_ARG, "invalid site-class arguments");
     if (!e->have_alignment) return fail(e, SMC_ERR_STATE, "no alignment uploaded");
@@ -1058,21 +1076,25 @@ int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const
     }
     const int64_t need = (int64_t)count * n_cand;
     if (need > e->sc_cap || count > e->sc_ent_cap) {
-        void *bufs[] = {e->d_sc_ent, e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out};
+        void *bufs[] = {e->d_sc_ent, e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
-        e->d_sc_ent = nullptr;
+        e->d_sc_ent = e->d_sc_cls = nullptr;
         e->d_sc_logits = e->d_sc_eig = e->d_sc_rate = e->d_sc_out = nullptr;
+        // grow geometrically: the moves call this with varying batch sizes
+        const int64_t cap = std::max<int64_t>(need, e->sc_cap * 3 / 2);
+        const int ecap = std::max(count, e->sc_ent_cap * 3 / 2);
         e->sc_cap = 0;
         e->sc_ent_cap = 0;
-        int rc = dev_alloc(e, &e->d_sc_ent, (size_t)3 * count);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_logits, (size_t)need * 20);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_eig, (size_t)need * kEigStride);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_rate, (size_t)need * e->R);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_out, (size_t)need);
+        int rc = dev_alloc(e, &e->d_sc_ent, (size_t)3 * ecap);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_logits, (size_t)cap * 20);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_eig, (size_t)cap * kEigStride);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_rate, (size_t)cap * e->R);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_out, (size_t)cap);
+        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_cls, (size_t)cap);
         if (rc != SMC_OK) return rc;
-        e->sc_cap = need;
-        e->sc_ent_cap = count;
+        e->sc_cap = cap;
+        e->sc_ent_cap = ecap;
     }
     hipStream_t s = e->stream;
     std::vector<int> ent((size_t)3 * count);
@@ -1085,10 +1107,12 @@ int smc_site_class_lnl(smc_engine *e, int count, const int32_t *particles, const
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double),
                               hipMemcpyHostToDevice, s));
+    if (cand_class)
+        HIP_TRY(e, hipMemcpyAsync(e->d_sc_cls, cand_class, (size_t)need * sizeof(int), hipMemcpyHostToDevice, s));
     {
         TimedLaunch tl(e, SMC_K_EIGEN);
         launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count,
-                          n_cand, e->d_sc_eig, s);
+                          n_cand, e->d_sc_eig, cand_class ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax, s);
     }
     {
         TimedLaunch tl(e, SMC_K_SITE);
@@ -1181,3 +1205,40 @@ int smc_engine_reset_stats(smc_engine *e)
 }
 
 }  // extern "C"
+
+int smc_engine_phase_stats(smc_engine *e, double *ms, int64_t *calls)
+{
+    if (!e) return SMC_ERR_ARG;
+    for (int k = 0; k < SMC_PHASE_COUNT; ++k) {
+        if (ms) ms[k] = e->phase_ms[k];
+        if (calls) calls[k] = e->phase_calls[k];
+    }
+    return SMC_OK;
+}
+
+int smc_engine_reset_phase_stats(smc_engine *e)
+{
+    if (!e) return SMC_ERR_ARG;
+    for (int k = 0; k < SMC_PHASE_COUNT; ++k) {
+        e->phase_ms[k] = 0.0;
+        e->phase_calls[k] = 0;
+    }
+    return SMC_OK;
+}
+
+int smc_refresh_eigen(smc_engine *e, int count, const int32_t *slots)
+{
+    if (count < 1) return SMC_OK;
+    if (count > e->Pmax) return fail(e, SMC_ERR_ARG, "too many slots");
+    hipStream_t s = e->stream;
+    HIP_TRY(e, hipMemcpyAsync(e->d_plist, slots, (size_t)count * sizeof(int), hipMemcpyHostToDevice, s));
+    {
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        EigenArgs ea{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, 0, count, e->d_plist};
+        launch_eigen(ea, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    return SMC_OK;
+}

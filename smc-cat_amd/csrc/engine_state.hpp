@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -74,6 +75,7 @@ struct smc_engine {
     std::vector<int> h_assign;    // [P][S]
     std::vector<double> h_pinv, h_shape;  // [P]
     std::vector<float> h_sitefreq;        // [S][20] amino-acid frequencies of each column
+    std::vector<double> h_rates;          // [P][R] discrete-gamma rates of h_shape
     int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
     int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
     uint8_t *d_xflag = nullptr;   // [P][T-1] current workspace of each internal node
@@ -92,6 +94,7 @@ struct smc_engine {
     int64_t sc_cap = 0;           // entries * candidates
     int sc_ent_cap = 0;
     int *d_sc_ent = nullptr;      // [3][entries]: particle, site, candidate count
+    int *d_sc_cls = nullptr;      // [entries][candidates] engine class of a candidate or -1
     double *d_sc_logits = nullptr, *d_sc_eig = nullptr, *d_sc_rate = nullptr, *d_sc_out = nullptr;
 
     // pruning schedule cache: waves of particles, each with its level/kind buckets
@@ -105,6 +108,10 @@ struct smc_engine {
     uint64_t sched_version = 0, topo_version = 1;
     std::vector<Wave> waves;
 
+    // host time per entry point (smc_engine_phase_stats)
+    double phase_ms[SMC_PHASE_COUNT] = {};
+    int64_t phase_calls[SMC_PHASE_COUNT] = {};
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -114,6 +121,20 @@ struct smc_engine {
 };
 
 int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+// Adds the wall time of its scope to e->phase_ms[phase].
+struct PhaseTimer {
+    smc_engine *e;
+    int phase;
+    std::chrono::steady_clock::time_point t0;
+    PhaseTimer(smc_engine *ee, int ph) : e(ee), phase(ph), t0(std::chrono::steady_clock::now()) {}
+    ~PhaseTimer()
+    {
+        e->phase_ms[phase] +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        e->phase_calls[phase]++;
+    }
+};
 
 // Receives every draw of the Gibbs allocation move: the site, the classes
 // (logits) and sizes without the site, and the chosen slot (K = a new class).
@@ -132,6 +153,18 @@ int smc_upload_partition(smc_engine *e, int slot, int K, const double *logits, c
 int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                       const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out,
                       int64_t *cat_stats);
+
+// smc_site_class_lnl where cand_class[i][c] >= 0 (nullable) names a class of
+// the entry's particle whose eigensystem the engine holds (its logits are then
+// not read); -1 decomposes cand_logits[i][c].
+extern "C" int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles,
+                                     const int32_t *sites, int n_cand, const int32_t *ncand,
+                                     const double *cand_logits, const int32_t *cand_class, double tempering,
+                                     double *out);
+
+// Recomputes the eigensystems of every class of the listed slots (from the
+// uploaded base matrices and logits), so that cached candidates are current.
+int smc_refresh_eigen(smc_engine *e, int count, const int32_t *slots);
 
 // The Gibbs allocation move (smc_gibbs_partition_move) with an optional sink.
 int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double tempering, uint64_t seed,

@@ -69,6 +69,7 @@ struct Philox {
 struct PState {
     int K;
     std::vector<std::array<double, 20>> logits;
+    std::vector<int> orig;  // class index at upload (its eigensystem is on the engine), -1: new
     std::vector<int> assign;
     std::vector<int> size;
 };
@@ -109,12 +110,18 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
         ps.assign.assign(e->h_assign.begin() + (size_t)p * S, e->h_assign.begin() + (size_t)(p + 1) * S);
         ps.size.assign(ps.K, 0);
         for (int a : ps.assign) ps.size[a]++;
+        ps.orig.resize(ps.K);
+        for (int k = 0; k < ps.K; ++k) ps.orig[k] = k;
         rng.emplace_back(seed, (uint64_t)p);
     }
+    // existing classes are scored with the engine's eigensystems: make them current
+    int rc0 = smc_refresh_eigen(e, count, slots);
+    if (rc0 != SMC_OK) return rc0;
     const double log_aux = std::log(dpp_alpha / m);
     const int n_cand = Kmax + m;
     std::vector<int32_t> ent_p(count), ent_s(count), ent_n(count);
     std::vector<double> cand((size_t)count * n_cand * 20, 0.0), ll((size_t)count * n_cand);
+    std::vector<int32_t> cand_cls((size_t)count * n_cand, -1);
     int64_t moved = 0, created = 0, erased = 0;
 
     for (int it = 0; it < iters; ++it) {
@@ -129,6 +136,7 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             if (ps.size[k0] == 0) {  // erase the emptied class (Particle.cpp:819-821)
                 was = -1;
                 ps.logits.erase(ps.logits.begin() + k0);
+                ps.orig.erase(ps.orig.begin() + k0);
                 ps.size.erase(ps.size.begin() + k0);
                 for (int &a : ps.assign)
                     if (a > k0) a--;
@@ -141,16 +149,21 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             ent_s[i] = site;
             ent_n[i] = ps.K + n_aux;
             double *c = &cand[(size_t)i * n_cand * 20];
-            for (int k = 0; k < ps.K; ++k) std::copy(ps.logits[k].begin(), ps.logits[k].end(), c + 20 * k);
+            int32_t *cc = &cand_cls[(size_t)i * n_cand];
+            for (int k = 0; k < ps.K; ++k) {
+                std::copy(ps.logits[k].begin(), ps.logits[k].end(), c + 20 * k);
+                cc[k] = ps.orig[k];
+            }
             for (int a = 0; a < n_aux; ++a) {
+                cc[ps.K + a] = -1;
                 double *lg = c + 20 * (ps.K + a);
                 for (int j = 0; j < 19; ++j) lg[j] = g.normal();
                 lg[19] = 0.0;
             }
             ps.assign[site] = -2 - was;  // remembers the previous class until the draw
         }
-        int rc = smc_site_class_lnl(e, count, ent_p.data(), ent_s.data(), n_cand, ent_n.data(),
-                                    cand.data(), tempering, ll.data());
+        int rc = smc_site_class_lnl_ex(e, count, ent_p.data(), ent_s.data(), n_cand, ent_n.data(),
+                                       cand.data(), cand_cls.data(), tempering, ll.data());
         if (rc != SMC_OK) return rc;
         for (int i = 0; i < count; ++i) {
             PState &ps = st[i];
@@ -186,6 +199,7 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
                 std::copy(&cand[((size_t)i * n_cand + pick) * 20],
                           &cand[((size_t)i * n_cand + pick) * 20] + 20, lg.begin());
                 ps.logits.push_back(lg);
+                ps.orig.push_back(-1);
                 ps.size.push_back(0);
                 pick = ps.K++;
                 created++;

@@ -116,8 +116,11 @@ void launch_site_class(const SiteClassArgs &a, hipStream_t s);
 
 // Eigen-decomposition of candidate profiles: entry e uses the base matrix of
 // particle ent_particle[e] and logits[e][c][20] for c < ent_ncand[e].
+// cand_cls (nullable) [count][n_cand]: >= 0 copies the class's eigensystem from
+// eig_cache ([particles][max_classes] layout) instead of decomposing the logits.
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
-                       const double *logits, int count, int n_cand, double *eig, hipStream_t s);
+                       const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
+                       const double *eig_cache, int max_classes, hipStream_t s);
 
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s);

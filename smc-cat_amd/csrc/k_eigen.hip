@@ -191,15 +191,25 @@ __global__ __launch_bounds__(64) void k_eigen_f64(const double *__restrict__ bas
 }
 
 // Candidate profiles: block e * n_cand + c decomposes logits[e][c] with the base
-// matrix of particle ent_particle[e].
+// matrix of particle ent_particle[e], or, when cand_cls[e][c] >= 0, copies the
+// eigensystem the engine already holds for that class of the particle.
 __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ base,
                                                     const int *__restrict__ ent_particle,
                                                     const int *__restrict__ ent_ncand,
                                                     const double *__restrict__ logits, int n_cand,
-                                                    double *__restrict__ eig)
+                                                    double *__restrict__ eig,
+                                                    const int *__restrict__ cand_cls,
+                                                    const double *__restrict__ eig_cache, int max_classes)
 {
     const int e = blockIdx.x / n_cand, c = blockIdx.x % n_cand;
     if (c >= ent_ncand[e]) return;
+    const int k = cand_cls ? cand_cls[blockIdx.x] : -1;
+    if (k >= 0) {  // block-uniform
+        const double *src = eig_cache + ((size_t)ent_particle[e] * max_classes + k) * kEigStride;
+        double *dst = eig + (size_t)blockIdx.x * kEigStride;
+        for (int i = threadIdx.x; i < kEigStride; i += 64) dst[i] = src[i];
+        return;
+    }
     eigen_block(base + (size_t)ent_particle[e] * 190, logits + (size_t)blockIdx.x * 20,
                 eig + (size_t)blockIdx.x * kEigStride);
 }
@@ -262,11 +272,12 @@ void launch_eigen(const EigenArgs &a, hipStream_t s)
 }
 
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
-                       const double *logits, int count, int n_cand, double *eig, hipStream_t s)
+                       const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
+                       const double *eig_cache, int max_classes, hipStream_t s)
 {
     if (count <= 0 || n_cand <= 0) return;
     hipLaunchKernelGGL(k_eigen_cand, dim3(count * n_cand), dim3(64), 0, s, base, ent_particle,
-                       ent_ncand, logits, n_cand, eig);
+                       ent_ncand, logits, n_cand, eig, cand_cls, eig_cache, max_classes);
 }
 
 void launch_pmat(const PmatArgs &a, hipStream_t s)

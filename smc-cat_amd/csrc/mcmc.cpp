@@ -23,6 +23,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <optional>
 
 #include "engine_state.hpp"
 #include "host_rng.hpp"
@@ -48,19 +49,27 @@ struct Params {
     double pinv, shape;
 };
 
-// Particle::lnPrior (Particle.cpp:484-517); the CRP/class terms are included so
-// the value matches the reference's, although they cancel in these moves.
-double ln_prior(const smc_engine *e, int p, const Params &q, bool using_lg, double dpp_alpha)
+// Particle::lnPrior (Particle.cpp:484-517), split in two: the terms of the
+// tree, shape and base matrix, which the moves change, and the CRP and class
+// terms, which they do not (computed once per particle and added back so the
+// value matches the reference's).
+double ln_prior_params(const smc_engine *e, const Params &q, bool using_lg)
 {
-    const int T = e->T, nn = e->nn, S = e->S, Kmax = e->Kmax;
+    const int nn = e->nn;
     double len = 0.0;
     for (int n = 0; n < nn; ++n) len += q.bl[n];  // the root's length is 0
     double lp = log_gamma_pdf((double)(nn - 1), 0.1, len);
     if (e->R > 1) lp += log_gamma_pdf(3.0, 2.0, q.shape);
     if (!using_lg)
         for (double b : q.base) lp += log_std_normal(b);
+    return lp;
+}
+
+double ln_prior_classes(const smc_engine *e, int p, double dpp_alpha)
+{
+    const int S = e->S, Kmax = e->Kmax;
     const int K = e->h_nclass[p];
-    lp += std::log(dpp_alpha) * K;
+    double lp = std::log(dpp_alpha) * K;
     std::vector<int> sz(K, 0);
     for (int s = 0; s < S; ++s) sz[e->h_assign[(size_t)p * S + s]]++;
     for (int k = 0; k < K; ++k) {
@@ -68,7 +77,6 @@ double ln_prior(const smc_engine *e, int p, const Params &q, bool using_lg, doub
         lp += std::lgamma((double)sz[k]);
     }
     lp += std::lgamma(dpp_alpha) - std::lgamma(dpp_alpha + S);
-    (void)T;
     return lp;
 }
 
@@ -217,7 +225,11 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         const smc_engine::Committed &c = e->committed[p];
         return Params{c.children, c.blen, c.base, c.pinv, c.shape};
     };
-    for (int i = 0; i < count; ++i) prior[i] = ln_prior(e, first + i, load(first + i), o.using_lg, o.dpp_alpha);
+    std::vector<double> prior_cls(count);
+    for (int i = 0; i < count; ++i) {
+        prior_cls[i] = ln_prior_classes(e, first + i, o.dpp_alpha);
+        prior[i] = ln_prior_params(e, load(first + i), o.using_lg) + prior_cls[i];
+    }
 
     std::vector<int32_t> ch((size_t)count * Ti * 2);
     std::vector<double> bl((size_t)count * nn), base((size_t)count * 190), pv(count), sh(count);
@@ -242,6 +254,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
             rounds = std::max(rounds, left[i]);
         }
         for (int r = 0; r < rounds; ++r) {
+            std::optional<PhaseTimer> propose_timer(std::in_place, e, SMC_PHASE_PROPOSE);
             for (int i = 0; i < count; ++i) {
                 const int p = first + i;
                 Params q = load(p);
@@ -294,8 +307,9 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                 pv[i] = q.pinv;
                 sh[i] = q.shape;
                 std::copy(f.begin(), f.end(), flags.begin() + (size_t)i * nn);
-                new_prior[i] = left[i] > 0 ? ln_prior(e, p, q, o.using_lg, o.dpp_alpha) : prior[i];
+                new_prior[i] = left[i] > 0 ? ln_prior_params(e, q, o.using_lg) + prior_cls[i] : prior[i];
             }
+            propose_timer.reset();
             rc = smc_update_params(e, first, count, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
             if (rc != SMC_OK) return rc;
             rc = smc_refresh_incremental(e, first, count, flags.data(), eig.data(), lnl_new.data(), nullptr);
@@ -328,6 +342,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         for (int i = 0; i < count; ++i)
             if (rng[i].uniform() < o.gibbs_probability) slots.push_back(first + i);
         if (!slots.empty()) {
+            PhaseTimer cat_timer(e, SMC_PHASE_CAT_MOVE);
             const int n = (int)slots.size();
             const uint64_t seed = o.seed ^ 0x61626273ull;
             int64_t st[5] = {};

@@ -445,18 +445,31 @@ namespace {
 struct MvState {
     std::vector<std::array<double, 20>> logits;
     std::vector<std::array<double, 20>> pi;  // softmax of logits
+    std::vector<std::array<float, 20>> lpi;  // log pi, as the network reads it
+    std::vector<int> orig;                   // class index at upload, -1: created by the move
     std::vector<int> size;
     std::vector<int32_t> assign;
     double meansum[20];                      // sum over sites of pi of their class
 };
 
-void softmax20(const std::array<double, 20> &lg, std::array<double, 20> &pi)
+void add_class(MvState &ps, const std::array<double, 20> &lg, int orig)
 {
+    ps.orig.push_back(orig);
     double mx = lg[0];
     for (int a = 1; a < 20; ++a) mx = std::max(mx, lg[a]);
     double z = 0.0;
-    for (int a = 0; a < 20; ++a) z += (pi[a] = std::exp(lg[a] - mx));
-    for (int a = 0; a < 20; ++a) pi[a] /= z;
+    for (int a = 0; a < 20; ++a) z += std::exp(lg[a] - mx);
+    const double lz = mx + std::log(z);
+    std::array<double, 20> pi;
+    std::array<float, 20> lpi;
+    for (int a = 0; a < 20; ++a) {
+        pi[a] = std::exp(lg[a] - lz);
+        lpi[a] = (float)(lg[a] - lz);
+    }
+    ps.logits.push_back(lg);
+    ps.pi.push_back(pi);
+    ps.lpi.push_back(lpi);
+    ps.size.push_back(0);
 }
 
 }  // namespace
@@ -487,14 +500,12 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         const int p = slots[i];
         MvState &ps = st[i];
         const int K = e->h_nclass[p];
-        ps.logits.resize(K);
-        ps.pi.resize(K);
         for (int k = 0; k < K; ++k) {
-            for (int a = 0; a < 20; ++a) ps.logits[k][a] = e->h_logits[((size_t)p * Kmax + k) * 20 + a];
-            softmax20(ps.logits[k], ps.pi[k]);
+            std::array<double, 20> lg;
+            for (int a = 0; a < 20; ++a) lg[a] = e->h_logits[((size_t)p * Kmax + k) * 20 + a];
+            add_class(ps, lg, k);
         }
         ps.assign.assign(e->h_assign.begin() + (size_t)p * S, e->h_assign.begin() + (size_t)(p + 1) * S);
-        ps.size.assign(K, 0);
         std::fill(ps.meansum, ps.meansum + 20, 0.0);
         for (int a : ps.assign) {
             ps.size[a]++;
@@ -502,7 +513,10 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         }
         rng.emplace_back(seed, (uint64_t)p);
     }
-    int rc = ensure_batch(m, count);
+    // existing classes are scored with the engine's eigensystems: make them current
+    int rc = smc_refresh_eigen(e, count, slots);
+    if (rc != SMC_OK) return rc;
+    rc = ensure_batch(m, count);
     if (rc != SMC_OK) return rc;
     const int stride = Kmax + 1;
     HIP_TRY(e, grow(m->d_mv_hdr, m->cap_mv_hdr, (int64_t)count * kHdr));
@@ -512,9 +526,13 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     HIP_TRY(e, grow(m->d_logq, m->cap_logq, (int64_t)count * stride));
 
     std::vector<float> hdr((size_t)count * kHdr), cls((size_t)count * Kmax * kCls), logq((size_t)count * stride);
+    std::vector<float> logn(S + 1, 0.f);  // log of class sizes
+    for (int n = 1; n <= S; ++n) logn[n] = (float)std::log((double)n);
     std::vector<int32_t> Kv(count), site(count), cur(count);
     std::vector<int64_t> coff(count);
     std::vector<std::array<double, 20>> phi(count), phi_new(count);
+    std::vector<int> phi_orig(count, -1);
+    std::vector<int32_t> cand_cls;
     std::vector<uint8_t> was_single(count);
     std::vector<int32_t> prop(count), ent_p, ent_s, ent_n, ent_i;
     std::vector<double> cand, ll;
@@ -533,8 +551,11 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             was_single[i] = ps.size[k0] == 0;
             if (was_single[i]) {  // the site's profile becomes the auxiliary class
                 phi[i] = ps.logits[k0];
+                phi_orig[i] = ps.orig[k0];
+                ps.orig.erase(ps.orig.begin() + k0);
                 ps.logits.erase(ps.logits.begin() + k0);
                 ps.pi.erase(ps.pi.begin() + k0);
+                ps.lpi.erase(ps.lpi.begin() + k0);
                 ps.size.erase(ps.size.begin() + k0);
                 for (int32_t &a : ps.assign)
                     if (a > k0) a--;
@@ -543,7 +564,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             ps.assign[s] = -1;
             cur[i] = was_single[i] ? K : k0;
             Kv[i] = K;
-            coff[i] = (int64_t)i * Kmax;
+            coff[i] = i ? coff[i - 1] + Kv[i - 1] : 0;
             float *h = &hdr[(size_t)i * kHdr];
             for (int a = 0; a < 20; ++a) {
                 h[a] = e->h_sitefreq[(size_t)s * 20 + a];
@@ -552,17 +573,18 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             h[40] = (float)tau_q;
             h[41] = (float)log_alpha;
             h[42] = h[43] = 0.f;
-            for (int k = 0; k < K; ++k) {
-                float *c = &cls[((size_t)i * Kmax + k) * kCls];
-                for (int a = 0; a < 20; ++a) c[a] = (float)std::log(ps.pi[k][a]);
-                c[20] = (float)std::log((double)ps.size[k]);
+            for (int k = 0; k < K; ++k) {  // compact class table: entries coff[i] ..
+                float *c = &cls[(size_t)(coff[i] + k) * kCls];
+                std::copy(ps.lpi[k].begin(), ps.lpi[k].end(), c);
+                c[20] = logn[ps.size[k]];
                 c[21] = c[22] = c[23] = 0.f;
             }
         }
         HIP_TRY(e, hipMemcpyAsync(m->d_mv_hdr, hdr.data(), hdr.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(e, hipMemcpyAsync(m->d_mv_K, Kv.data(), (size_t)count * 4, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(e, hipMemcpyAsync(m->d_mv_coff, coff.data(), (size_t)count * 8, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(e, hipMemcpyAsync(m->d_mv_cls, cls.data(), cls.size() * 4, hipMemcpyHostToDevice, e->stream));
+        const size_t n_ent = (size_t)(coff[count - 1] + Kv[count - 1]);
+        HIP_TRY(e, hipMemcpyAsync(m->d_mv_cls, cls.data(), n_ent * kCls * 4, hipMemcpyHostToDevice, e->stream));
         MlpArgs a = batch_args(m, count, nullptr);
         a.hdr = m->d_mv_hdr;
         a.K = m->d_mv_K;
@@ -582,6 +604,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         ent_n.clear();
         ent_i.clear();
         cand.clear();
+        cand_cls.clear();
         for (int i = 0; i < count; ++i) {
             MvState &ps = st[i];
             smccat::Rng &g = rng[i];
@@ -611,19 +634,22 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             ent_n.push_back(2);
             const std::array<double, 20> &pc = was_single[i] ? phi[i] : ps.logits[cur[i]];
             cand.insert(cand.end(), pc.begin(), pc.end());
+            cand_cls.push_back(was_single[i] ? phi_orig[i] : ps.orig[cur[i]]);
             if (pick == K) {  // a fresh profile from the logit prior
                 for (int a = 0; a < 19; ++a) phi_new[i][a] = g.normal();
                 phi_new[i][19] = 0.0;
                 cand.insert(cand.end(), phi_new[i].begin(), phi_new[i].end());
+                cand_cls.push_back(-1);
             } else {
                 cand.insert(cand.end(), ps.logits[pick].begin(), ps.logits[pick].end());
+                cand_cls.push_back(ps.orig[pick]);
             }
         }
         const int ne = (int)ent_i.size();
         if (ne > 0) {
             ll.resize((size_t)ne * 2);
-            rc = smc_site_class_lnl(e, ne, ent_p.data(), ent_s.data(), 2, ent_n.data(), cand.data(), tempering,
-                                    ll.data());
+            rc = smc_site_class_lnl_ex(e, ne, ent_p.data(), ent_s.data(), 2, ent_n.data(), cand.data(),
+                                       cand_cls.data(), tempering, ll.data());
             if (rc != SMC_OK) return rc;
         }
         // accept or reject, then put the site back
@@ -647,10 +673,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 }
             }
             if (dest == K) {  // the auxiliary class becomes a real class
-                ps.logits.push_back(fresh ? phi_new[i] : phi[i]);
-                ps.pi.emplace_back();
-                softmax20(ps.logits.back(), ps.pi.back());
-                ps.size.push_back(0);
+                add_class(ps, fresh ? phi_new[i] : phi[i], fresh ? -1 : phi_orig[i]);
                 if (fresh) created++;
             } else if (was_single[i]) {
                 erased++;

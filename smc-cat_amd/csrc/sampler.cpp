@@ -102,6 +102,7 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
         res->iterations = it;
         res->ess_sum += ess;
         if (std::isnan(ess)) return smc_fail(e, SMC_ERR_STATE, "NaN ESS (a particle likelihood failed)");
+        if (o.on_iteration) o.on_iteration(o.user, it, cur, ess);
         if (ess <= rej_threshold && cur != 1.0) {
             rc = smc_systematic_resample(e, N, norm.data(), g.uniform(), assign.data());
             if (rc != SMC_OK) return rc;

@@ -55,7 +55,12 @@ class SamplerOptions(ctypes.Structure):
                 ("batch", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("seed", ctypes.c_uint64), ("dpp_alpha", ctypes.c_double),
                 ("allocation_proposal", ctypes.c_void_p), ("proposal_warmup", ctypes.c_int32),
-                ("train_epochs", ctypes.c_int32), ("train_lr", ctypes.c_double)]
+                ("train_epochs", ctypes.c_int32), ("train_lr", ctypes.c_double),
+                ("on_iteration", ctypes.c_void_p), ("user", ctypes.c_void_p)]
+
+
+# void (*on_iteration)(void *user, int32_t iteration, double temp, double ess)
+ITERATION_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double)
 
 
 class SamplerResult(ctypes.Structure):
@@ -102,6 +107,8 @@ SIGNATURES = {
     "smc_mlp_infer": (_I, [_P, _I, _I, _I, _FP]),
     "smc_mlp_train": (_I, [_P, _I, _I, _D, ctypes.c_uint64, _DP]),
     "smc_learned_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D, _D, _LP]),
+    "smc_engine_phase_stats": (_I, [_P, _DP, _LP]),
+    "smc_engine_reset_phase_stats": (_I, [_P]),
     "smc_run_sampler": (_I, [_P, ctypes.POINTER(SamplerOptions), ctypes.POINTER(SamplerResult), _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),

@@ -244,15 +244,19 @@ class Engine:
 
     def run_sampler(self, num_particles, iterations, seed, using_lg=False, invariant=True, cat=False,
                     threshold=0.6, alg8_probability=0.1, batch=0, max_iterations=0, dpp_alpha=0.01,
-                    proposal=None, proposal_warmup=1, train_epochs=8, train_lr=2e-3):
+                    proposal=None, proposal_warmup=1, train_epochs=8, train_lr=2e-3, on_iteration=None):
         """The SMC loop over slots [0, num_particles) (which hold the initial
-        population).  proposal: a ProposalNetwork for the learned CAT move.
+        population).  proposal: a ProposalNetwork for the learned CAT move;
+        on_iteration(iteration, temp, ess) is called after every reweighting.
         Returns (result dict, final lnl, final raw log weights)."""
+        cb = _lib.ITERATION_CALLBACK((lambda u, it, t, e: on_iteration(it, t, e)) if on_iteration
+                                     else (lambda u, it, t, e: None))
         o = _lib.SamplerOptions(int(num_particles), int(iterations), float(threshold),
                                 float(alg8_probability), int(using_lg), int(invariant), int(cat),
                                 int(batch), int(max_iterations), 0, int(seed), float(dpp_alpha),
                                 proposal.h if proposal is not None else None, int(proposal_warmup),
-                                int(train_epochs), float(train_lr))
+                                int(train_epochs), float(train_lr),
+                                ctypes.cast(cb, ctypes.c_void_p) if on_iteration else None, None)
         r = _lib.SamplerResult()
         lnl = np.empty(num_particles)
         w = np.empty(num_particles)
@@ -262,6 +266,19 @@ class Engine:
         res["move_stats"] = np.array(list(r.move_stats)).reshape(4, 2)
         res["cat_stats"] = np.array(list(r.cat_stats))
         return res, lnl, w
+
+    PHASES = ("set_particles", "refresh", "update_params", "refresh_incremental", "accept_reject",
+              "propose", "cat_move", "site_class")
+
+    def phase_stats(self, reset=False):
+        """{phase: (host ms, calls)} of the engine's entry points."""
+        ms = np.zeros(len(self.PHASES))
+        calls = np.zeros(len(self.PHASES), dtype=np.int64)
+        self._check(self.lib.smc_engine_phase_stats(self.h, _dp(ms), calls.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int64))), "smc_engine_phase_stats")
+        if reset:
+            self._check(self.lib.smc_engine_reset_phase_stats(self.h), "smc_engine_reset_phase_stats")
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.PHASES)}
 
     def particle_params(self, slot):
         """(children [T-1, 2], branch_len [2T-1], base [190], pinv, shape) of a slot."""
