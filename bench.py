@@ -246,6 +246,8 @@ def main():
                     help="sweep: the updates/s line (with the SMC ESS/sec comparison); c5: the "
                          "learned-proposal benchmark")
     ap.add_argument("--no-ess", action="store_true", help="skip the SMC ESS/sec comparison")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--ess-taxa", type=int, default=16)
     ap.add_argument("--ess-sites", type=int, default=200)
     ap.add_argument("--ess-particles", type=int, default=64)
@@ -304,24 +306,35 @@ def main():
         return lnl, nf, ess, temp
 
     progress("sweep workload %s: %d particles uploaded" % (args.config, N))
+    tune = None
+    if args.variant == "mfma" and not args.no_autotune:
+        tune = eng.autotune(0, N, sweeps=2)
+        progress("autotune: %s" % tune)
     for _ in range(args.warmup):
         step()
-    eng.set_timing(True)
-    eng.reset_stats()
+    # timed steps: each sweep is one replay of its captured graph
+    eng.set_graphs(not args.no_graphs)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         lnl, nf, ess, temp = step()
     dt = time.perf_counter() - t0
+    progress("sweep: %d steps in %.2f s" % (args.steps, dt))
+    # per-kernel durations (HIP events on the engine's stream): the same kernels
+    # and launch shapes, launched eagerly so each launch can be timed
+    n_timed = max(1, min(args.steps, 3))
+    eng.set_timing(True)
+    eng.reset_stats()
+    for _ in range(n_timed):
+        step()
     stats = eng.kernel_stats()
     eng.set_timing(False)
-    progress("sweep: %d steps in %.2f s" % (args.steps, dt))
 
     updates = N * (T - 1) * S * R
     value = updates * args.steps / dt
     bytes_sweep = N * S * R * algorithmic_bytes_per_column(T, word)
     n_prune, ms_prune = stats["prune"]
-    launches_per_sweep = n_prune / args.steps
-    prune_ms_per_sweep = ms_prune / args.steps
+    launches_per_sweep = n_prune / n_timed
+    prune_ms_per_sweep = ms_prune / n_timed
     achieved = bytes_sweep / (prune_ms_per_sweep * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -354,7 +367,9 @@ def main():
                      "launches_per_step": launches_per_sweep,
                      "avg_launch_ms": ms_prune / max(n_prune, 1),
                      "algorithmic_bytes_per_step": bytes_sweep},
-        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items()},
+        "kernel_ms_per_step": {k: v[1] / n_timed for k, v in stats.items()},
+        "launch_shape": tune or eng.tuning(),
+        "graphs": eng.graph_stats(),
         "check": {"lnl_finite": bool(np.all(np.isfinite(lnl)) and not nf.any()),
                   "nonfinite_particles": int(np.sum(~np.isfinite(lnl))),
                   "lnl_mean_finite": float(np.mean(lnl[np.isfinite(lnl)])) if np.isfinite(lnl).any()

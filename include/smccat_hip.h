@@ -268,6 +268,34 @@ int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, doub
                                uint64_t seed, int num_iter, double dpp_alpha, double proposal_tempering,
                                int64_t *stats_out);
 
+/* Launch shape of the full sweep (no counterpart in the reference: the
+ * reference runs one particle per OpenMP thread).  The pruning launches of a
+ * sweep are persistent grid-stride kernels (blocks_per_cu workgroups per
+ * compute unit) that read their item ranges from the device, so a whole sweep
+ * (eigen, P, every level bucket, roots, lnL reduction) is captured once as a
+ * hipGraph and replayed while the waves and level counts stay the same.
+ * unroll[k] is the number of 16-site column blocks a wavefront carries at
+ * once for items whose children are tip-tip (k=0), tip-internal (1) and
+ * internal-internal (2). */
+typedef struct {
+    int32_t blocks_per_cu;   /* 1..64 (default 32)         */
+    int32_t unroll[3];       /* 1, 2 or 4 (default 4, 2, 1) */
+    double ms_per_sweep;     /* autotune: best measured     */
+    int32_t configs_tried;   /* autotune: candidates timed  */
+    int32_t reserved;
+} smc_tuning;
+
+int smc_engine_get_tuning(smc_engine *e, smc_tuning *t);
+int smc_engine_set_tuning(smc_engine *e, const smc_tuning *t);
+/* Times candidate launch shapes on full sweeps of slots [first, first+count)
+ * (coordinate descent over blocks_per_cu, then each unroll; `sweeps` replays per
+ * candidate) and keeps the fastest.  Recomputes those particles' lnL. */
+int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tuning *out);
+/* enable = 0 runs sweeps as eager launches (the default captures graphs);
+ * per-family kernel timing also runs eagerly. */
+int smc_engine_set_graphs(smc_engine *e, int enable);
+int smc_engine_graph_stats(smc_engine *e, int64_t *captures, int64_t *launches);
+
 /* Host wall time spent in each entry point (diagnostics; not reset by
  * smc_engine_reset_stats).  ms[SMC_PHASE_COUNT], calls[SMC_PHASE_COUNT]; either
  * may be NULL.  Nested calls count in both phases. */

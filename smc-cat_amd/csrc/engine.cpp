@@ -137,14 +137,19 @@ int build_schedule(smc_engine *e, int first, int count)
         for (int q = w.first; q < w.first + w.count; ++q)
             for (int i = 0; i < Ti; ++i)
                 if (T + i != e->h_root[q]) maxh = std::max(maxh, e->h_height[(size_t)q * Ti + i]);
-        const int nb = (maxh + 1) * 3;
+        // a captured sweep walks graph_levels levels: rounding keeps it valid while the
+        // population's tallest tree varies a little
+        const int glev = std::min(T - 1, (maxh + 3) / 4 * 4);
+        const int nb = (glev + 1) * 3;
         std::vector<int> cnt(nb + 1, 0);
         for (int q = w.first; q < w.first + w.count; ++q)
             for (int i = 0; i < Ti; ++i)
                 if (T + i != e->h_root[q]) cnt[e->h_height[(size_t)q * Ti + i] * 3 + kind_of(q, i)]++;
-        w.bucket_off.assign(nb + 1, (int)item_pos);
+        w.bucket_off.assign(nb + 2, (int)item_pos);
         for (int b = 0; b < nb; ++b) w.bucket_off[b + 1] = w.bucket_off[b] + cnt[b];
+        w.bucket_off[nb + 1] = w.bucket_off[nb] + w.count;
         w.max_level = maxh;
+        w.graph_levels = glev;
         std::vector<int> fill(w.bucket_off);
         w.root_off = w.bucket_off[nb];
         for (int q = w.first; q < w.first + w.count; ++q)
@@ -159,6 +164,26 @@ int build_schedule(smc_engine *e, int first, int count)
         item_pos = (size_t)w.root_off + w.count;
         e->waves.push_back(std::move(w));
     }
+    std::vector<int> boffs;
+    for (smc_engine::Wave &w : e->waves) {
+        w.boff = (int)boffs.size();
+        boffs.insert(boffs.end(), w.bucket_off.begin(), w.bucket_off.end());
+    }
+    if ((int64_t)boffs.size() > e->bucket_cap) {
+        if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
+        e->d_bucket_off = nullptr;
+        e->bucket_cap = 0;
+        const int64_t cap = std::max<int64_t>((int64_t)boffs.size(), 1024);
+        HIP_TRY(e, hipMalloc((void **)&e->d_bucket_off, (size_t)cap * sizeof(int)));
+        e->bucket_cap = cap;
+        if (e->graph_exec) {  // the captured sweep reads the old buffer
+            (void)hipGraphExecDestroy(e->graph_exec);
+            e->graph_exec = nullptr;
+            e->graph_key.clear();
+        }
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->d_bucket_off, boffs.data(), boffs.size() * sizeof(int),
+                              hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
                               hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), pcls.size() * sizeof(int),
@@ -317,6 +342,12 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
         delete e;
         return SMC_ERR_HIP;
     }
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess &&
+            ncu > 0)
+            e->num_cus = ncu;
+    }
     const size_t P = e->Pmax, T = e->T, Sp = e->S_pad, R = e->R, K = e->Kmax, nn = e->nn;
     int rc = SMC_OK;
     auto A = [&](auto **ptr, size_t n) {
@@ -398,6 +429,8 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &pt : e->pending) {
@@ -572,6 +605,167 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     return SMC_OK;
 }
 
+namespace {
+
+PruneArgs prune_args(smc_engine *e, int p_base)
+{
+    PruneArgs a{};
+    a.tiles = e->d_tiles;
+    a.ntiles = e->d_ntiles;
+    a.tile_stride = e->tile_stride;
+    a.children = e->d_children;
+    a.root_of = e->d_root;
+    a.perm = e->d_perm;
+    a.codes = e->d_codes;
+    a.P = e->d_P;
+    a.pcls_off = e->d_pcls_off;
+    a.eig = e->d_eig;
+    a.pinv = e->d_pinv;
+    a.isinv = e->d_isinv;
+    a.invchar = e->d_invchar;
+    a.X = e->d_X;
+    a.E = e->d_E;
+    a.p_base = p_base;
+    a.tile_lnl = e->d_tile_lnl;
+    a.T = e->T;
+    a.R = e->R;
+    a.S_pad = e->S_pad;
+    a.max_classes = e->Kmax;
+    a.fp32 = e->fp32;
+    a.mix_mode = e->mix_mode;
+    a.xflag = e->WS == 2 ? e->d_xflag : nullptr;
+    a.WS = e->WS;
+    a.grid = e->tune.blocks_per_cu > 0 ? e->num_cus * e->tune.blocks_per_cu : 0;
+    return a;
+}
+
+// The launches of a full sweep over the scheduled waves.  `device_buckets`: the
+// pruning launches read their item ranges from d_bucket_off (persistent grid, no
+// host-known sizes), as a captured graph needs; otherwise each launch covers its
+// bucket exactly.
+void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
+{
+    hipStream_t s = e->stream;
+    const bool mfma = e->prune_variant != SMC_PRUNE_VALU;
+    for (const smc_engine::Wave &w : e->waves) {
+        {
+            TimedLaunch tl(e, SMC_K_EIGEN);
+            launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, w.first,
+                                   w.count},
+                         s);
+        }
+        {
+            TimedLaunch tl(e, SMC_K_PMAT);
+            launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off,
+                                 e->d_P, e->nn, e->R, e->Kmax, w.first, w.count, e->fp32},
+                        s);
+        }
+        PruneArgs a = prune_args(e, w.first);
+        const std::vector<int> &bo = w.bucket_off;
+        const int nb = 3 * (w.graph_levels + 1);
+        auto run = [&](int b, int kind, bool roots) {
+            const int off = bo[b], n = bo[b + 1] - bo[b];
+            if (!device_buckets && n <= 0) return;
+            TimedLaunch tl(e, SMC_K_PRUNE);
+            a.kind = kind;
+            a.roots = roots;
+            a.unroll = kind < 3 ? e->tune.unroll[kind] : 1;
+            if (device_buckets) {
+                a.items = e->d_items;
+                a.bucket_off = e->d_bucket_off + w.boff;
+                a.bucket = b;
+            } else {
+                a.items = e->d_items + off;
+                a.item_base = 0;
+                a.n_items = n;
+                a.bucket_off = nullptr;
+            }
+            if (mfma)
+                launch_prune_mfma(a, s);
+            else
+                launch_prune(a, s);
+        };
+        const int levels = device_buckets ? w.graph_levels : w.max_level;
+        for (int L = 1; L <= levels; ++L) {
+            if (!mfma) {
+                a.items = e->d_items + bo[3 * L];
+                a.n_items = bo[3 * L + 3] - bo[3 * L];
+                if (a.n_items > 0) {
+                    TimedLaunch tl(e, SMC_K_PRUNE);
+                    a.kind = 3;
+                    a.roots = false;
+                    launch_prune(a, s);
+                }
+                continue;
+            }
+            // tip-tip nodes have height 1; nodes above have an internal child
+            if (L == 1)
+                run(3 * L + 0, 0, false);
+            else
+                for (int k = 1; k < 3; ++k) run(3 * L + k, k, false);
+        }
+        if (!mfma) {
+            a.items = e->d_items + w.root_off;
+            a.n_items = w.count;
+            a.roots = true;
+            a.kind = 3;
+            TimedLaunch tl(e, SMC_K_PRUNE);
+            launch_prune(a, s);
+        } else {
+            run(nb, 3, true);
+        }
+    }
+    TimedLaunch tl(e, SMC_K_REDUCE);
+    launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl, e->d_nonfinite,
+                      s);
+}
+
+// One full sweep: replayed from a captured graph when the launch sequence is
+// the one captured (same waves, level counts and settings), else captured anew;
+// eager when timing per kernel family or with the VALU variant.
+int run_sweep(smc_engine *e, int first, int count)
+{
+    hipStream_t s = e->stream;
+    const bool graph = e->use_graph && !e->timing && e->prune_variant != SMC_PRUNE_VALU;
+    if (!graph) {
+        issue_sweep(e, first, count, false);
+        return SMC_OK;
+    }
+    std::vector<int64_t> key = {first, count, e->fp32, e->mix_mode, e->WS, e->tune.blocks_per_cu,
+                                e->tune.unroll[0], e->tune.unroll[1], e->tune.unroll[2],
+                                (int64_t)(intptr_t)e->d_bucket_off};
+    for (const smc_engine::Wave &w : e->waves) {
+        key.push_back(w.first);
+        key.push_back(w.count);
+        key.push_back(w.graph_levels);
+        key.push_back(w.boff);
+    }
+    if (!e->graph_exec || key != e->graph_key) {
+        if (e->graph_exec) {
+            HIP_TRY(e, hipGraphExecDestroy(e->graph_exec));
+            e->graph_exec = nullptr;
+        }
+        e->graph_key.clear();
+        hipGraph_t g = nullptr;
+        HIP_TRY(e, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        issue_sweep(e, first, count, true);
+        HIP_TRY(e, hipStreamEndCapture(s, &g));
+        hipError_t st = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (st != hipSuccess) {
+            e->graph_exec = nullptr;
+            return smc_fail(e, SMC_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(st));
+        }
+        e->graph_key = key;
+        e->graph_captures++;
+    }
+    HIP_TRY(e, hipGraphLaunch(e->graph_exec, s));
+    e->graph_launches++;
+    return SMC_OK;
+}
+
+}  // namespace
+
 int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
                            uint8_t *nonfinite_out)
 {
@@ -594,73 +788,8 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
         HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
                                   (size_t)count * Ti, hipMemcpyHostToDevice, s));
     }
-    for (const smc_engine::Wave &w : e->waves) {
-        {
-            TimedLaunch tl(e, SMC_K_EIGEN);
-            launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, w.first,
-                                   w.count},
-                         s);
-        }
-        {
-            TimedLaunch tl(e, SMC_K_PMAT);
-            launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off,
-                                 e->d_P, e->nn, e->R, e->Kmax, w.first, w.count, e->fp32},
-                        s);
-        }
-        PruneArgs a{};
-        a.tiles = e->d_tiles;
-        a.ntiles = e->d_ntiles;
-        a.tile_stride = e->tile_stride;
-        a.children = e->d_children;
-        a.root_of = e->d_root;
-        a.perm = e->d_perm;
-        a.codes = e->d_codes;
-        a.P = e->d_P;
-        a.pcls_off = e->d_pcls_off;
-        a.eig = e->d_eig;
-        a.pinv = e->d_pinv;
-        a.isinv = e->d_isinv;
-        a.invchar = e->d_invchar;
-        a.X = e->d_X;
-        a.E = e->d_E;
-        a.p_base = w.first;
-        a.tile_lnl = e->d_tile_lnl;
-        a.T = e->T;
-        a.R = e->R;
-        a.S_pad = e->S_pad;
-        a.max_classes = e->Kmax;
-        a.fp32 = e->fp32;
-        a.mix_mode = e->mix_mode;
-        a.xflag = e->WS == 2 ? e->d_xflag : nullptr;
-        a.WS = e->WS;
-        auto run = [&](int off, int n, int kind, bool roots) {
-            if (n <= 0) return;
-            TimedLaunch tl(e, SMC_K_PRUNE);
-            a.items = e->d_items + off;
-            a.n_items = n;
-            a.kind = kind;
-            a.roots = roots;
-            if (e->prune_variant == SMC_PRUNE_VALU)
-                launch_prune(a, s);
-            else
-                launch_prune_mfma(a, s);
-        };
-        const std::vector<int> &bo = w.bucket_off;
-        for (int L = 1; L <= w.max_level; ++L) {
-            if (e->prune_variant == SMC_PRUNE_VALU) {
-                run(bo[3 * L], bo[3 * L + 3] - bo[3 * L], 3, false);
-            } else {
-                for (int k = 0; k < 3; ++k)
-                    run(bo[3 * L + k], bo[3 * L + k + 1] - bo[3 * L + k], k, false);
-            }
-        }
-        run(w.root_off, w.count, 3, true);
-    }
-    {
-        TimedLaunch tl(e, SMC_K_REDUCE);
-        launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,
-                          e->d_nonfinite, s);
-    }
+    rc = run_sweep(e, first, count);
+    if (rc != SMC_OK) return rc;
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipMemcpyAsync(e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double),
                               hipMemcpyDeviceToHost, s));
@@ -867,30 +996,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
                               hipMemcpyHostToDevice, s));
     e->sched_first = -1;  // the item buffer no longer holds the full schedule
-    PruneArgs a{};
-    a.tiles = e->d_tiles;
-    a.ntiles = e->d_ntiles;
-    a.tile_stride = e->tile_stride;
-    a.children = e->d_children;
-    a.root_of = e->d_root;
-    a.perm = e->d_perm;
-    a.codes = e->d_codes;
-    a.P = e->d_P;
-    a.pcls_off = e->d_pcls_off;
-    a.eig = e->d_eig;
-    a.pinv = e->d_pinv;
-    a.isinv = e->d_isinv;
-    a.invchar = e->d_invchar;
-    a.X = e->d_X;
-    a.E = e->d_E;
-    a.p_base = first;
-    a.tile_lnl = e->d_tile_lnl;
-    a.T = T;
-    a.R = e->R;
-    a.S_pad = e->S_pad;
-    a.max_classes = e->Kmax;
-    a.fp32 = e->fp32;
-    a.mix_mode = e->mix_mode;
+    PruneArgs a = prune_args(e, first);
     a.xflag = e->d_xflag;
     a.WS = 2;
     auto run = [&](int o, int n, int kind, bool is_roots) {
@@ -900,6 +1006,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.n_items = n;
         a.kind = kind;
         a.roots = is_roots;
+        a.unroll = kind < 3 ? e->tune.unroll[kind] : 1;
         if (e->prune_variant == SMC_PRUNE_VALU)
             launch_prune(a, s);
         else
@@ -1240,5 +1347,127 @@ int smc_refresh_eigen(smc_engine *e, int count, const int32_t *slots)
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipStreamSynchronize(s));
     collect_timings(e);
+    return SMC_OK;
+}
+
+int smc_engine_set_graphs(smc_engine *e, int enable)
+{
+    if (!e) return SMC_ERR_ARG;
+    e->use_graph = enable != 0;
+    return SMC_OK;
+}
+
+int smc_engine_graph_stats(smc_engine *e, int64_t *captures, int64_t *launches)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (captures) *captures = e->graph_captures;
+    if (launches) *launches = e->graph_launches;
+    return SMC_OK;
+}
+
+int smc_engine_get_tuning(smc_engine *e, smc_tuning *t)
+{
+    if (!e || !t) return SMC_ERR_ARG;
+    t->blocks_per_cu = e->tune.blocks_per_cu;
+    for (int k = 0; k < 3; ++k) t->unroll[k] = e->tune.unroll[k];
+    t->ms_per_sweep = 0.0;
+    t->configs_tried = 0;
+    return SMC_OK;
+}
+
+int smc_engine_set_tuning(smc_engine *e, const smc_tuning *t)
+{
+    if (!e || !t) return SMC_ERR_ARG;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(e, SMC_ERR_ARG, "blocks_per_cu out of 1..64");
+    for (int k = 0; k < 3; ++k)
+        if (t->unroll[k] != 1 && t->unroll[k] != 2 && t->unroll[k] != 4)
+            return fail(e, SMC_ERR_ARG, "unroll must be 1, 2 or 4");
+    e->tune.blocks_per_cu = t->blocks_per_cu;
+    for (int k = 0; k < 3; ++k) e->tune.unroll[k] = t->unroll[k];
+    return SMC_OK;
+}
+
+// Measured search over the launch shape of the full sweep: the persistent grid
+// (workgroups per CU) and the column-block unroll of each child kind, by
+// coordinate descent from the current setting.  Each candidate is timed over
+// `sweeps` replays of its captured sweep of slots [first, first+count) after one
+// untimed replay; the fastest is kept.  The particles' lnL are recomputed
+// (the results do not depend on the launch shape).
+int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tuning *out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (sweeps < 1 || first < 0 || count < 1 || first + count > e->Pmax)
+        return fail(e, SMC_ERR_ARG, "invalid autotune arguments");
+    if (e->prune_variant == SMC_PRUNE_VALU) return fail(e, SMC_ERR_STATE, "autotune needs the MFMA variant");
+    for (int p = first; p < first + count; ++p)
+        if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
+    const bool timing = e->timing, graphs = e->use_graph;
+    e->timing = false;
+    e->use_graph = true;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(e, hipEventCreate(&t0));
+    HIP_TRY(e, hipEventCreate(&t1));
+    int tried = 0;
+    auto measure = [&](const smc_engine::Tune &t, double *ms) -> int {
+        e->tune = t;
+        int rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);  // captures, warms
+        if (rc != SMC_OK) return rc;
+        HIP_TRY(e, hipEventRecord(t0, e->stream));
+        for (int k = 0; k < sweeps; ++k) {
+            rc = run_sweep(e, first, count);
+            if (rc != SMC_OK) return rc;
+        }
+        HIP_TRY(e, hipEventRecord(t1, e->stream));
+        HIP_TRY(e, hipEventSynchronize(t1));
+        float f = 0.f;
+        HIP_TRY(e, hipEventElapsedTime(&f, t0, t1));
+        *ms = (double)f / sweeps;
+        tried++;
+        return SMC_OK;
+    };
+    smc_engine::Tune best = e->tune;
+    double best_ms = 0.0;
+    int rc = measure(best, &best_ms);
+    const int grids[] = {2, 4, 8, 16, 32, 64};
+    const int unrolls[] = {1, 2, 4};
+    if (rc == SMC_OK)
+        for (int g : grids) {
+            if (g == best.blocks_per_cu) continue;
+            smc_engine::Tune t = best;
+            t.blocks_per_cu = g;
+            double ms;
+            if ((rc = measure(t, &ms)) != SMC_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    for (int k = 0; k < 3 && rc == SMC_OK; ++k)
+        for (int u : unrolls) {
+            if (u == best.unroll[k]) continue;
+            smc_engine::Tune t = best;
+            t.unroll[k] = u;
+            double ms;
+            if ((rc = measure(t, &ms)) != SMC_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    e->tune = best;
+    e->timing = timing;
+    e->use_graph = graphs;
+    if (rc != SMC_OK) return rc;
+    // leave the particles as a sweep with the chosen shape computed them
+    rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+    if (rc != SMC_OK) return rc;
+    if (out) {
+        out->blocks_per_cu = best.blocks_per_cu;
+        for (int k = 0; k < 3; ++k) out->unroll[k] = best.unroll[k];
+        out->ms_per_sweep = best_ms;
+        out->configs_tried = tried;
+    }
     return SMC_OK;
 }

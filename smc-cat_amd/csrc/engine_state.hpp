@@ -102,8 +102,26 @@ struct smc_engine {
         int first, count;
         int root_off;                 // roots of the wave's particles
         int max_level;
-        std::vector<int> bucket_off;  // items of (level L, kind k): [off[3L+k], off[3L+k+1])
+        int graph_levels;             // max_level rounded up: the levels a captured sweep walks
+        int boff;                     // first entry of this wave's offsets in d_bucket_off
+        // items of (level L, kind k): [off[3L+k], off[3L+k+1]) for L <= graph_levels, then
+        // the roots: [off[nb], off[nb+1]) with nb = 3 (graph_levels + 1)
+        std::vector<int> bucket_off;
     };
+    int *d_bucket_off = nullptr;      // the waves' bucket offsets on the device
+    int64_t bucket_cap = 0;
+
+    // launch tuning (smc_engine_autotune) and the captured full sweep
+    struct Tune {
+        int blocks_per_cu = 32;       // persistent grid of the pruning launches: CUs x this
+        int unroll[3] = {4, 2, 1};    // 16-site column blocks per step: tip-tip, tip-int, int-int
+    };
+    Tune tune;
+    int num_cus = 256;
+    bool use_graph = true;
+    std::vector<int64_t> graph_key;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_launches = 0, graph_captures = 0;
     int sched_first = -1, sched_count = -1;
     uint64_t sched_version = 0, topo_version = 1;
     std::vector<Wave> waves;

@@ -87,6 +87,14 @@ struct PruneArgs {
     int mix_mode;            // SMC_MIX_REFERENCE or SMC_MIX_LOGSPACE (roots only)
     int kind;                // child kinds of every item: 0 tip-tip, 1 tip-internal,
                              // 2 internal-internal, 3 mixed (decided per item)
+    // matrix-core kernel only: items [item_base, item_base + n_items) of `items`,
+    // or, when bucket_off is set, the range bucket_off[bucket] .. [bucket + 1]
+    // read on the device (a launch that does not depend on the topology)
+    int item_base = 0;
+    const int *bucket_off = nullptr;
+    int bucket = 0;
+    int grid = 0;            // persistent grid (workgroups); 0: one per unit of work
+    int unroll = 1;          // 16-site column blocks per step (1, 2 or 4)
 };
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores

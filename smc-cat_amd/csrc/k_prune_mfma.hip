@@ -21,6 +21,9 @@
 #include "internal.hpp"
 #include "root_mixture.hpp"
 
+#include <algorithm>
+#include <cstdint>
+
 namespace smccat {
 
 namespace {
@@ -84,32 +87,60 @@ __device__ __forceinline__ void child_block(bool tip, const TX (&a)[2][5], const
 // internal node, both internal; KIND_ANY decides at run time (used for roots).
 constexpr int KIND_TT = 0, KIND_TI = 1, KIND_II = 2, KIND_ANY = 3;
 
-template <int KIND>
-struct KindTraits {
-    // 16-site column blocks processed together (independent MFMA chains for ILP)
-    static constexpr int U = KIND == KIND_TT ? 4 : (KIND == KIND_TI ? 2 : 1);
+// Kernel arguments (one struct, passed by value).
+template <typename TX>
+struct KArgs {
+    const Item *items;
+    const Tile *tiles;
+    const int *ntiles;
+    int tile_stride;
+    const int *children, *perm;
+    const uint8_t *codes;
+    const TX *Pm;
+    const double *eig, *pinv, *isinv;
+    const int *invchar;
+    TX *X;
+    int *E;
+    double *tile_lnl;
+    int T, R, S_pad, max_classes, blocks_per_item;
+    const int *pcls_off;
+    int p_base, mix_mode;
+    const uint8_t *xflag;
+    int WS;
 };
 
-template <typename TX, int KIND, bool IS_ROOT>
-__global__ __launch_bounds__(256) void k_prune_mfma(
-    const Item *__restrict__ items, const Tile *__restrict__ tiles, const int *__restrict__ ntiles,
-    int tile_stride, const int *__restrict__ children, const int *__restrict__ perm,
-    const uint8_t *__restrict__ codes, const TX *__restrict__ Pm, const double *__restrict__ eig,
-    const double *__restrict__ pinv, const double *__restrict__ isinv,
-    const int *__restrict__ invchar, TX *__restrict__ X, int *__restrict__ E,
-    double *__restrict__ tile_lnl, int T, int R, int S_pad, int max_classes, int blocks_per_item,
-    const int *__restrict__ pcls_off, int p_base, int mix_mode, const uint8_t *__restrict__ xflag,
-    int WS)
+// One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
+// item `item`.  U is the number of 16-site column blocks processed together
+// (independent MFMA chains for ILP); roots use U = 1.
+template <typename TX, int KIND, bool IS_ROOT, int U>
+__device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int tile_group)
 {
     using V4 = typename Mfma<TX>::v4;
-    constexpr int U = KindTraits<KIND>::U;
     static_assert(!IS_ROOT || U == 1, "root accumulators are indexed per column block");
-    const int item = blockIdx.x / blocks_per_item;
+    const Item *__restrict__ items = ka.items;
+    const Tile *__restrict__ tiles = ka.tiles;
+    const int *__restrict__ ntiles = ka.ntiles;
+    const int tile_stride = ka.tile_stride;
+    const int *__restrict__ children = ka.children;
+    const int *__restrict__ perm = ka.perm;
+    const uint8_t *__restrict__ codes = ka.codes;
+    const TX *__restrict__ Pm = ka.Pm;
+    const double *__restrict__ eig = ka.eig;
+    const double *__restrict__ pinv = ka.pinv;
+    const double *__restrict__ isinv = ka.isinv;
+    const int *__restrict__ invchar = ka.invchar;
+    TX *__restrict__ X = ka.X;
+    int *__restrict__ E = ka.E;
+    double *__restrict__ tile_lnl = ka.tile_lnl;
+    const int T = ka.T, R = ka.R, S_pad = ka.S_pad, max_classes = ka.max_classes;
+    const int *__restrict__ pcls_off = ka.pcls_off;
+    const int p_base = ka.p_base, mix_mode = ka.mix_mode, WS = ka.WS;
+    const uint8_t *__restrict__ xflag = ka.xflag;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int tile = (blockIdx.x % blocks_per_item) * 4 + wave;
+    const int tile = tile_group * 4 + wave;
     const Item it = items[item];
     const int p = it.particle, n = it.node;
-    if (tile >= ntiles[p]) return;  // whole wavefront exits; no block barrier follows
+    if (tile >= ntiles[p]) return;  // this wavefront has no tile here; no block barrier follows
     const Tile tl = tiles[(size_t)p * tile_stride + tile];
     const int lane = threadIdx.x & 63;
     const int c16 = lane & 15, g4 = lane >> 4;
@@ -271,42 +302,80 @@ __global__ __launch_bounds__(256) void k_prune_mfma(
     }
 }
 
+// Persistent grid-stride form: the launch walks the units of items
+// [base, base + n), read from the device (bucket_off[bucket], [bucket + 1]) when
+// bucket_off is given, so one launch configuration serves every topology and a
+// captured graph can be replayed after the schedule changes.
+template <typename TX, int KIND, bool IS_ROOT, int U>
+__global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__restrict__ bucket_off,
+                                                    int bucket, int base, int n)
+{
+    if (bucket_off) {
+        base = bucket_off[bucket];
+        n = bucket_off[bucket + 1] - base;
+    }
+    const int bpi = ka.blocks_per_item;
+    const int n_work = n * bpi;
+    for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const int q = w / bpi;
+        prune_unit<TX, KIND, IS_ROOT, U>(ka, base + q, w - q * bpi);
+    }
+}
+
+}  // namespace
+
+namespace {
+
+template <typename TX, int KIND, bool IS_ROOT, int U>
+void launch_one(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_prune_mfma<TX, KIND, IS_ROOT, U>), dim3(grid), dim3(256), 0, s, ka, a.bucket_off,
+                       a.bucket, a.item_base, a.n_items);
+}
+
+template <typename TX, int KIND>
+void launch_kind(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
+{
+    if (a.unroll >= 4)
+        launch_one<TX, KIND, false, 4>(a, ka, grid, s);
+    else if (a.unroll == 2)
+        launch_one<TX, KIND, false, 2>(a, ka, grid, s);
+    else
+        launch_one<TX, KIND, false, 1>(a, ka, grid, s);
+}
+
+template <typename TX>
+void launch_typed(const PruneArgs &a, hipStream_t s)
+{
+    const int bpi = (a.tile_stride + 3) / 4;
+    KArgs<TX> ka{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes,
+                 (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
+                 a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS};
+    // without device bucket offsets the work is known: launch exactly what it needs,
+    // capped at the persistent grid
+    int64_t units = a.bucket_off ? INT64_MAX : (int64_t)a.n_items * bpi;
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
+    if (a.roots)
+        launch_one<TX, KIND_ANY, true, 1>(a, ka, grid, s);
+    else if (a.kind == KIND_TT)
+        launch_kind<TX, KIND_TT>(a, ka, grid, s);
+    else if (a.kind == KIND_TI)
+        launch_kind<TX, KIND_TI>(a, ka, grid, s);
+    else if (a.kind == KIND_II)
+        launch_kind<TX, KIND_II>(a, ka, grid, s);
+    else
+        launch_one<TX, KIND_ANY, false, 1>(a, ka, grid, s);
+}
+
 }  // namespace
 
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s)
 {
-    if (a.n_items <= 0) return;
-    const int bpi = (a.tile_stride + 3) / 4;
-    dim3 grid((unsigned)((size_t)a.n_items * bpi));
-#define SMC_PRUNE_ARGS                                                                            \
-    a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, (const TXa *)a.P,     \
-        a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, a.R, a.S_pad,        \
-        a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS
-#define SMC_PRUNE_LAUNCH(TXT)                                                                     \
-    do {                                                                                          \
-        using TXa = TXT;                                                                          \
-        if (a.roots)                                                                              \
-            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_ANY, true>), grid, dim3(256), 0, s,        \
-                               SMC_PRUNE_ARGS);                                                   \
-        else if (a.kind == KIND_TT)                                                               \
-            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_TT, false>), grid, dim3(256), 0, s,        \
-                               SMC_PRUNE_ARGS);                                                   \
-        else if (a.kind == KIND_TI)                                                               \
-            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_TI, false>), grid, dim3(256), 0, s,        \
-                               SMC_PRUNE_ARGS);                                                   \
-        else if (a.kind == KIND_II)                                                               \
-            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_II, false>), grid, dim3(256), 0, s,        \
-                               SMC_PRUNE_ARGS);                                                   \
-        else                                                                                      \
-            hipLaunchKernelGGL((k_prune_mfma<TXT, KIND_ANY, false>), grid, dim3(256), 0, s,       \
-                               SMC_PRUNE_ARGS);                                                   \
-    } while (0)
+    if (!a.bucket_off && a.n_items <= 0) return;
     if (a.fp32)
-        SMC_PRUNE_LAUNCH(float);
+        launch_typed<float>(a, s);
     else
-        SMC_PRUNE_LAUNCH(double);
-#undef SMC_PRUNE_LAUNCH
-#undef SMC_PRUNE_ARGS
+        launch_typed<double>(a, s);
 }
 
 }  // namespace smccat

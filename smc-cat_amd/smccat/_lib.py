@@ -48,6 +48,12 @@ class RejuvenationOptions(ctypes.Structure):
                 ("reserved2", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("unroll", ctypes.c_int32 * 3),
+                ("ms_per_sweep", ctypes.c_double), ("configs_tried", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class SamplerOptions(ctypes.Structure):
     _fields_ = [("num_particles", ctypes.c_int32), ("rejuvenation_iterations", ctypes.c_int32),
                 ("rejuvenation_threshold", ctypes.c_double), ("alg8_probability", ctypes.c_double),
@@ -109,6 +115,11 @@ SIGNATURES = {
     "smc_learned_partition_move": (_I, [_P, _I, _IP, _D, ctypes.c_uint64, _I, _D, _D, _LP]),
     "smc_engine_phase_stats": (_I, [_P, _DP, _LP]),
     "smc_engine_reset_phase_stats": (_I, [_P]),
+    "smc_engine_get_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
+    "smc_engine_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
+    "smc_engine_autotune": (_I, [_P, _I, _I, _I, ctypes.POINTER(Tuning)]),
+    "smc_engine_set_graphs": (_I, [_P, _I]),
+    "smc_engine_graph_stats": (_I, [_P, _LP, _LP]),
     "smc_run_sampler": (_I, [_P, ctypes.POINTER(SamplerOptions), ctypes.POINTER(SamplerResult), _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),

@@ -267,6 +267,34 @@ class Engine:
         res["cat_stats"] = np.array(list(r.cat_stats))
         return res, lnl, w
 
+    def tuning(self):
+        t = _lib.Tuning()
+        self._check(self.lib.smc_engine_get_tuning(self.h, ctypes.byref(t)), "smc_engine_get_tuning")
+        return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll)}
+
+    def set_tuning(self, blocks_per_cu, unroll):
+        t = _lib.Tuning(int(blocks_per_cu), (ctypes.c_int32 * 3)(*unroll), 0.0, 0, 0)
+        self._check(self.lib.smc_engine_set_tuning(self.h, ctypes.byref(t)), "smc_engine_set_tuning")
+
+    def autotune(self, first=0, count=None, sweeps=3):
+        """Measured search over the sweep's launch shape; returns the chosen shape."""
+        if count is None:
+            count = self.max_particles - first
+        t = _lib.Tuning()
+        self._check(self.lib.smc_engine_autotune(self.h, int(first), int(count), int(sweeps), ctypes.byref(t)),
+                    "smc_engine_autotune")
+        return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll), "ms_per_sweep": t.ms_per_sweep,
+                "configs_tried": t.configs_tried}
+
+    def set_graphs(self, enable):
+        self._check(self.lib.smc_engine_set_graphs(self.h, int(bool(enable))), "smc_engine_set_graphs")
+
+    def graph_stats(self):
+        c, l = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.smc_engine_graph_stats(self.h, ctypes.byref(c), ctypes.byref(l)),
+                    "smc_engine_graph_stats")
+        return {"captures": c.value, "launches": l.value}
+
     PHASES = ("set_particles", "refresh", "update_params", "refresh_incremental", "accept_reject",
               "propose", "cat_move", "site_class")
 

@@ -1,0 +1,86 @@
+"""Captured sweeps and launch tuning: a sweep replayed from its hipGraph gives
+exactly the eager result, on every replay, with eager work between replays
+and after the trees change; every launch shape the autotuner may pick gives the
+same per-particle lnL bit for bit, and the tuned engine matches the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+from smccat import Engine
+from smccat import simulate as sim
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _setup(precision="fp64", N=16, T=24, S=300, R=4, kmax=1, seed=3, workspaces=1):
+    codes, _ = sim.simulate_config("c2", taxa=T, sites=S)
+    batch = sim.prior_particles(np.random.default_rng(seed), N, T, S, num_rates=R, invar=True,
+                                k_range=(1, kmax))
+    eng = Engine(T, S, R, N, kmax, precision, workspaces=workspaces)
+    eng.set_alignment(codes)
+    eng.set_particles(batch)
+    return eng, codes, batch
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replays_equal_eager(gpu, precision):
+    _say("case graph replay %s: 16 particles, 24 taxa x 300 sites" % precision)
+    eng, codes, batch = _setup(precision)
+    with eng:
+        eng.set_graphs(False)
+        want, _ = eng.refresh_likelihood()
+        eng.set_graphs(True)
+        for k in range(4):
+            got, _ = eng.refresh_likelihood()
+            assert np.array_equal(got, want), k
+            eng.set_graphs(False)
+            part, _ = eng.refresh_likelihood(2, 5)  # eager work between replays
+            assert np.array_equal(part, want[2:7])
+            eng.set_graphs(True)
+        st = eng.graph_stats()
+        assert st["launches"] >= 4 and st["captures"] <= 2
+        # new trees of the same sizes: the captured launches read the new schedule
+        other = sim.prior_particles(np.random.default_rng(99), batch.n, 24, 300, num_rates=4, invar=True)
+        eng.set_particles(other)
+        got, _ = eng.refresh_likelihood()
+        eng.set_graphs(False)
+        again, _ = eng.refresh_likelihood()
+        assert np.array_equal(got, again)
+    if precision == "fp64":
+        ref = np.array([orc.particle_lnl(codes, other.particle(i), 4) for i in range(4)])
+        np.testing.assert_allclose(got[:4], ref, rtol=1e-10)
+
+
+def test_every_launch_shape_gives_identical_lnl(gpu):
+    _say("case launch shapes: CAT K<=6, 12 particles")
+    eng, codes, batch = _setup(N=12, kmax=6)
+    with eng:
+        eng.set_graphs(False)
+        want, _ = eng.refresh_likelihood()
+        for g in (1, 3, 8):
+            for u in ((1, 1, 1), (2, 4, 2), (4, 2, 4)):
+                eng.set_tuning(g, u)
+                for graphs in (False, True):
+                    eng.set_graphs(graphs)
+                    got, _ = eng.refresh_likelihood()
+                    assert np.array_equal(got, want), (g, u, graphs)
+
+
+def test_autotune_then_oracle(gpu):
+    _say("case autotune: 32 particles")
+    eng, codes, batch = _setup(N=32, workspaces=2)
+    with eng:
+        res = eng.autotune(sweeps=2)
+        _say("  %s" % res)
+        assert res["configs_tried"] > 5 and res["ms_per_sweep"] > 0
+        assert eng.tuning()["blocks_per_cu"] == res["blocks_per_cu"]
+        got, _ = eng.refresh_likelihood()
+    ref = np.array([orc.particle_lnl(codes, batch.particle(i), 4) for i in range(6)])
+    np.testing.assert_allclose(got[:6], ref, rtol=1e-10)
