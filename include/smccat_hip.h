@@ -275,11 +275,11 @@ int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, doub
  * (eigen, P, every level bucket, roots, lnL reduction) is captured once as a
  * hipGraph and replayed while the waves and level counts stay the same.
  * unroll[k] is the number of 16-site column blocks a wavefront carries at
- * once for items whose children are tip-tip (k=0), tip-internal (1) and
- * internal-internal (2). */
+ * once for items with k children read as stored partials (tips and cherries,
+ * internal nodes with two tip children, are built in registers). */
 typedef struct {
     int32_t blocks_per_cu;   /* 1..64 (default 32)         */
-    int32_t unroll[3];       /* 1, 2 or 4 (default 4, 2, 1) */
+    int32_t unroll[3];       /* 1, 2 or 4 (default 2, 1, 1) */
     double ms_per_sweep;     /* autotune: best measured     */
     int32_t configs_tried;   /* autotune: candidates timed  */
     int32_t reserved;

@@ -105,18 +105,14 @@ void collect_timings(smc_engine *e)
 // grouped into waves of at most Wmax particles and Cmax classes (partials and P
 // of one wave are resident at a time).  Within a wave, bucket (L, k) holds the
 // non-root internal nodes of height L (tips have height 0) whose children are
-// of kind k (0 both tips, 1 one tip, 2 both internal); buckets are stored
-// level-major, and the wave's roots follow in one final list, so the level
-// kernels never carry the root mixture.
+// of PRUNE_* kind k; buckets are stored level-major, and the wave's roots follow
+// in one final list, so the level kernels never carry the root mixture.
 int build_schedule(smc_engine *e, int first, int count)
 {
     if (e->sched_first == first && e->sched_count == count && e->sched_version == e->topo_version)
         return SMC_OK;
     const int T = e->T, Ti = T - 1;
-    auto kind_of = [&](int p, int i) {
-        const int *ch = &e->h_children[((size_t)p * Ti + i) * 2];
-        return (ch[0] >= T) + (ch[1] >= T);
-    };
+    auto kind_of = [&](int p, int i) { return prune_kind(&e->h_children[(size_t)p * Ti * 2], T, T + i); };
     e->waves.clear();
     std::vector<Item> items((size_t)count * Ti);
     std::vector<int> pcls((size_t)count, 0);
@@ -140,11 +136,11 @@ int build_schedule(smc_engine *e, int first, int count)
         // a captured sweep walks graph_levels levels: rounding keeps it valid while the
         // population's tallest tree varies a little
         const int glev = std::min(T - 1, (maxh + 3) / 4 * 4);
-        const int nb = (glev + 1) * 3;
+        const int nb = (glev + 1) * PRUNE_KINDS;
         std::vector<int> cnt(nb + 1, 0);
         for (int q = w.first; q < w.first + w.count; ++q)
             for (int i = 0; i < Ti; ++i)
-                if (T + i != e->h_root[q]) cnt[e->h_height[(size_t)q * Ti + i] * 3 + kind_of(q, i)]++;
+                if (T + i != e->h_root[q]) cnt[e->h_height[(size_t)q * Ti + i] * PRUNE_KINDS + kind_of(q, i)]++;
         w.bucket_off.assign(nb + 2, (int)item_pos);
         for (int b = 0; b < nb; ++b) w.bucket_off[b + 1] = w.bucket_off[b] + cnt[b];
         w.bucket_off[nb + 1] = w.bucket_off[nb] + w.count;
@@ -157,7 +153,7 @@ int build_schedule(smc_engine *e, int first, int count)
                 if (T + i == e->h_root[q]) {
                     items[w.root_off + (q - w.first)] = Item{q, T + i};
                 } else {
-                    const int b = e->h_height[(size_t)q * Ti + i] * 3 + kind_of(q, i);
+                    const int b = e->h_height[(size_t)q * Ti + i] * PRUNE_KINDS + kind_of(q, i);
                     items[fill[b]++] = Item{q, T + i};
                 }
             }
@@ -662,14 +658,14 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
         }
         PruneArgs a = prune_args(e, w.first);
         const std::vector<int> &bo = w.bucket_off;
-        const int nb = 3 * (w.graph_levels + 1);
+        const int nb = PRUNE_KINDS * (w.graph_levels + 1);
         auto run = [&](int b, int kind, bool roots) {
             const int off = bo[b], n = bo[b + 1] - bo[b];
             if (!device_buckets && n <= 0) return;
             TimedLaunch tl(e, SMC_K_PRUNE);
             a.kind = kind;
             a.roots = roots;
-            a.unroll = kind < 3 ? e->tune.unroll[kind] : 1;
+            a.unroll = roots ? 1 : e->tune.unroll[prune_stored(kind)];
             if (device_buckets) {
                 a.items = e->d_items;
                 a.bucket_off = e->d_bucket_off + w.boff;
@@ -688,31 +684,30 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
         const int levels = device_buckets ? w.graph_levels : w.max_level;
         for (int L = 1; L <= levels; ++L) {
             if (!mfma) {
-                a.items = e->d_items + bo[3 * L];
-                a.n_items = bo[3 * L + 3] - bo[3 * L];
+                a.items = e->d_items + bo[PRUNE_KINDS * L];
+                a.n_items = bo[PRUNE_KINDS * (L + 1)] - bo[PRUNE_KINDS * L];
                 if (a.n_items > 0) {
                     TimedLaunch tl(e, SMC_K_PRUNE);
-                    a.kind = 3;
+                    a.kind = PRUNE_ANY;
                     a.roots = false;
                     launch_prune(a, s);
                 }
                 continue;
             }
-            // tip-tip nodes have height 1; nodes above have an internal child
-            if (L == 1)
-                run(3 * L + 0, 0, false);
-            else
-                for (int k = 1; k < 3; ++k) run(3 * L + k, k, false);
+            // cherries (tip-tip, all of height 1) are built by their parents; every
+            // node above height 1 has an internal child
+            if (L >= 2)
+                for (int k = PRUNE_TC; k < PRUNE_KINDS; ++k) run(PRUNE_KINDS * L + k, k, false);
         }
         if (!mfma) {
             a.items = e->d_items + w.root_off;
             a.n_items = w.count;
             a.roots = true;
-            a.kind = 3;
+            a.kind = PRUNE_ANY;
             TimedLaunch tl(e, SMC_K_PRUNE);
             launch_prune(a, s);
         } else {
-            run(nb, 3, true);
+            run(nb, PRUNE_ANY, true);
         }
     }
     TimedLaunch tl(e, SMC_K_REDUCE);
@@ -936,7 +931,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     if (rc != SMC_OK) return rc;
     const int T = e->T, Ti = T - 1, nn = e->nn;
     std::vector<int> eig_list, p_list;
-    std::vector<std::vector<Item>> buckets;  // (level * 3 + kind)
+    std::vector<std::vector<Item>> buckets;  // (level * PRUNE_KINDS + kind)
     std::vector<Item> roots;
     std::vector<uint8_t> dirty(nn);
     std::vector<int> parent(nn);
@@ -973,9 +968,9 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
             const int L = e->h_height[(size_t)p * Ti + (n - T)];
-            const int kind = (ch[2 * (n - T)] >= T) + (ch[2 * (n - T) + 1] >= T);
-            if ((int)buckets.size() < 3 * (L + 1)) buckets.resize(3 * (L + 1));
-            buckets[3 * L + kind].push_back(Item{p, n});
+            const int kind = prune_kind(ch, T, n);
+            if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
+            buckets[PRUNE_KINDS * L + kind].push_back(Item{p, n});
         }
         roots.push_back(Item{p, root});
     }
@@ -1006,15 +1001,20 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.n_items = n;
         a.kind = kind;
         a.roots = is_roots;
-        a.unroll = kind < 3 ? e->tune.unroll[kind] : 1;
+        a.unroll = is_roots ? 1 : e->tune.unroll[prune_stored(kind)];
         if (e->prune_variant == SMC_PRUNE_VALU)
             launch_prune(a, s);
         else
             launch_prune_mfma(a, s);
     };
-    for (size_t b = 0; b + 1 < off.size(); ++b)
-        run(off[b], off[b + 1] - off[b], e->prune_variant == SMC_PRUNE_VALU ? 3 : (int)(b % 3), false);
-    run(root_off, (int)roots.size(), 3, true);
+    for (size_t b = 0; b + 1 < off.size(); ++b) {
+        const int kind = (int)(b % PRUNE_KINDS);
+        if (e->prune_variant == SMC_PRUNE_VALU)
+            run(off[b], off[b + 1] - off[b], PRUNE_ANY, false);
+        else if (kind != PRUNE_TT)  // cherries are built by their parents
+            run(off[b], off[b + 1] - off[b], kind, false);
+    }
+    run(root_off, (int)roots.size(), PRUNE_ANY, true);
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
         launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,

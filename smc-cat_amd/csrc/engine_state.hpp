@@ -114,7 +114,7 @@ struct smc_engine {
     // launch tuning (smc_engine_autotune) and the captured full sweep
     struct Tune {
         int blocks_per_cu = 32;       // persistent grid of the pruning launches: CUs x this
-        int unroll[3] = {4, 2, 1};    // 16-site column blocks per step: tip-tip, tip-int, int-int
+        int unroll[3] = {2, 1, 1};    // 16-site column blocks per step, by stored children (0, 1, 2)
     };
     Tune tune;
     int num_cus = 256;

@@ -1,6 +1,8 @@
 // Internal declarations shared by the engine's translation units.
 #pragma once
 
+#include <utility>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -59,6 +61,34 @@ struct PmatArgs {
 };
 void launch_pmat(const PmatArgs &a, hipStream_t s);
 
+// Child modes of a pruning item (sorted pair): a child is a tip (T), a cherry
+// (C: an internal node whose children are both tips) or a stored partial (S).
+// The matrix-core kernel never stores cherries (their parents build them);
+// the VALU kernel stores them like any internal node.
+enum { PRUNE_TT = 0, PRUNE_TC = 1, PRUNE_TS = 2, PRUNE_CC = 3, PRUNE_CS = 4, PRUNE_SS = 5, PRUNE_KINDS = 6,
+       PRUNE_ANY = 6 };
+
+// PRUNE_* kind of internal node `node` of a children array [T-1][2].
+inline int prune_kind(const int *ch, int T, int node)
+{
+    auto mode = [&](int c) {
+        if (c < T) return 0;
+        return (ch[2 * (c - T)] < T && ch[2 * (c - T) + 1] < T) ? 1 : 2;
+    };
+    int a = mode(ch[2 * (node - T)]), b = mode(ch[2 * (node - T) + 1]);
+    if (a > b) std::swap(a, b);
+    static const int table[3][3] = {{PRUNE_TT, PRUNE_TC, PRUNE_TS}, {PRUNE_TC, PRUNE_CC, PRUNE_CS},
+                                    {PRUNE_TS, PRUNE_CS, PRUNE_SS}};
+    return table[a][b];
+}
+
+// Number of children of a PRUNE_* kind read as stored partials (0, 1, 2).
+inline int prune_stored(int kind)
+{
+    static const int n[PRUNE_KINDS] = {0, 0, 1, 0, 1, 2};
+    return kind < PRUNE_KINDS ? n[kind] : 1;
+}
+
 struct PruneArgs {
     const Item *items;       // this level's items
     int n_items;
@@ -85,8 +115,7 @@ struct PruneArgs {
     bool fp32;
     bool roots;              // items are tree roots: fuse the root mixture, store nothing
     int mix_mode;            // SMC_MIX_REFERENCE or SMC_MIX_LOGSPACE (roots only)
-    int kind;                // child kinds of every item: 0 tip-tip, 1 tip-internal,
-                             // 2 internal-internal, 3 mixed (decided per item)
+    int kind;                // PRUNE_* child modes of every item (PRUNE_ANY: decided per item)
     // matrix-core kernel only: items [item_base, item_base + n_items) of `items`,
     // or, when bucket_off is set, the range bucket_off[bucket] .. [bucket + 1]
     // read on the device (a launch that does not depend on the topology)

@@ -18,6 +18,14 @@
 // as 16-lane contiguous runs of the state-major [state][site] layout.  Tip
 // children are not multiplied: their P x is a column (or the row sums) of P,
 // gathered straight into the C layout.
+//
+// Cherries (internal nodes whose children are both tips, a third of the nodes
+// of a random tree) are never stored: the parent builds the cherry's partial
+// -- the product of two tip columns of P, scaled exactly as a stored partial
+// would be -- straight into its B operand, which saves the cherry's write and
+// the parent's read (about a third of a sweep's HBM bytes) and gives the same
+// bits.  Each child is therefore a tip, a cherry or a stored partial; items are
+// launched per (child mode, child mode) pair, roots decide at run time.
 #include "internal.hpp"
 #include "root_mixture.hpp"
 
@@ -54,38 +62,92 @@ struct Mfma<float> {
     static __device__ __forceinline__ int crow(int rb, int g4, int v) { return 16 * rb + 4 * g4 + v; }
 };
 
-// Y (C layout, [rb]) of one child over one 16-site column block.
+// Child modes; MODE_RT decides per item at run time (roots).
+constexpr int MODE_TIP = 0, MODE_CHERRY = 1, MODE_STORED = 2, MODE_RT = 3;
+
+// Y (C layout) of a tip child: column `col` of P (row sums for col 20).
 template <typename TX>
-__device__ __forceinline__ void child_block(bool tip, const TX (&a)[2][5], const TX *__restrict__ Pc,
-                                            const TX *__restrict__ xcol, int S_pad, int col,
-                                            bool valid, int g4, typename Mfma<TX>::v4 (&y)[2])
+__device__ __forceinline__ void tip_block(const TX *__restrict__ Pc, int col, int g4,
+                                          typename Mfma<TX>::v4 (&y)[2])
+{
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = Mfma<TX>::crow(rb, g4, v);
+            y[rb][v] = (row < 20) ? Pc[row * 21 + col] : (TX)0;
+        }
+}
+
+// B operand (states 4 kb + g4 of this lane's site) of a cherry: the product of
+// its tips' P columns, scaled by 2^-e with e the frexp exponent of the max over
+// the 20 states (NaN when the max is 0 or not finite), as a stored partial is.
+// Every lane must call it (cross-lane max).  Returns e (0 when not finite).
+template <typename TX>
+__device__ __forceinline__ int cherry_b(const TX *__restrict__ Pa, const TX *__restrict__ Pb, int ca, int cb,
+                                        bool valid, int g4, TX (&b)[5])
+{
+    TX m = 0;
+#pragma unroll
+    for (int kb = 0; kb < 5; kb++) {
+        const int row = 4 * kb + g4;
+        b[kb] = valid ? Pa[row * 21 + ca] * Pb[row * 21 + cb] : (TX)0;
+        m = fmax(m, b[kb]);
+    }
+    m = fmax(m, __shfl_xor(m, 16, 64));
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    int ex;
+    (void)frexp(m, &ex);
+    const bool ok = (m > (TX)0) && isfinite(m);
+#pragma unroll
+    for (int kb = 0; kb < 5; kb++) b[kb] = ok ? (TX)ldexp(b[kb], -ex) : (TX)NAN;
+    return ok ? ex : 0;
+}
+
+template <typename TX>
+__device__ __forceinline__ void stored_b(const TX *__restrict__ xcol, int S_pad, bool valid, int g4, TX (&b)[5])
+{
+#pragma unroll
+    for (int kb = 0; kb < 5; kb++) b[kb] = valid ? xcol[(size_t)(4 * kb + g4) * S_pad] : (TX)0;
+}
+
+template <typename TX>
+__device__ __forceinline__ void mma_block(const TX (&a)[2][5], const TX (&b)[5], typename Mfma<TX>::v4 (&y)[2])
 {
     using V4 = typename Mfma<TX>::v4;
-    if (tip) {
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++)
+    for (int rb = 0; rb < 2; rb++) {
+        V4 acc = {0, 0, 0, 0};
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = Mfma<TX>::crow(rb, g4, v);
-                y[rb][v] = (row < 20) ? Pc[row * 21 + col] : (TX)0;
-            }
-    } else {
-        TX b[5];
-#pragma unroll
-        for (int kb = 0; kb < 5; kb++) b[kb] = valid ? xcol[(size_t)(4 * kb + g4) * S_pad] : (TX)0;
-#pragma unroll
-        for (int rb = 0; rb < 2; rb++) {
-            V4 acc = {0, 0, 0, 0};
-#pragma unroll
-            for (int kb = 0; kb < 5; kb++) acc = Mfma<TX>::mma(a[rb][kb], b[kb], acc);
-            y[rb] = acc;
-        }
+        for (int kb = 0; kb < 5; kb++) acc = Mfma<TX>::mma(a[rb][kb], b[kb], acc);
+        y[rb] = acc;
     }
 }
 
-// Child kinds of a pruning item: both tips, one tip (c0 after a swap) and one
-// internal node, both internal; KIND_ANY decides at run time (used for roots).
-constexpr int KIND_TT = 0, KIND_TI = 1, KIND_II = 2, KIND_ANY = 3;
+// One child over one column block: Y (C layout) and the exponent it adds.
+// `mode` is the child's mode (wave-uniform; MODE is it when known at compile time).
+template <typename TX, int MODE>
+__device__ __forceinline__ int child_block(int mode, const TX (&a)[2][5], const TX *__restrict__ Pc,
+                                           const TX *__restrict__ Pa, const TX *__restrict__ Pb,
+                                           const TX *__restrict__ xcol, const int *__restrict__ ecol,
+                                           int S_pad, int col, int ca, int cb, bool valid, int g4,
+                                           typename Mfma<TX>::v4 (&y)[2])
+{
+    if (MODE == MODE_TIP || (MODE == MODE_RT && mode == MODE_TIP)) {
+        tip_block<TX>(Pc, col, g4, y);
+        return 0;
+    }
+    TX b[5];
+    int e = 0;
+    if (MODE == MODE_CHERRY || (MODE == MODE_RT && mode == MODE_CHERRY)) {
+        e = cherry_b<TX>(Pa, Pb, ca, cb, valid, g4, b);
+    } else {
+        stored_b<TX>(xcol, S_pad, valid, g4, b);
+        e = *ecol;
+    }
+    mma_block<TX>(a, b, y);
+    return e;
+}
 
 // Kernel arguments (one struct, passed by value).
 template <typename TX>
@@ -112,7 +174,7 @@ struct KArgs {
 // One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
 // item `item`.  U is the number of 16-site column blocks processed together
 // (independent MFMA chains for ILP); roots use U = 1.
-template <typename TX, int KIND, bool IS_ROOT, int U>
+template <typename TX, int M0, int M1, bool IS_ROOT, int U>
 __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int tile_group)
 {
     using V4 = typename Mfma<TX>::v4;
@@ -146,25 +208,26 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     const int c16 = lane & 15, g4 = lane >> 4;
     const size_t g = (size_t)p * max_classes + tl.cls;
     const int nn = 2 * T - 1, nl = n - T;
-    int c0 = children[((size_t)p * (T - 1) + nl) * 2];
-    int c1 = children[((size_t)p * (T - 1) + nl) * 2 + 1];
-    bool tip0, tip1;
-    if constexpr (KIND == KIND_TT) {
-        tip0 = tip1 = true;
-    } else if constexpr (KIND == KIND_TI) {
-        if (c0 >= T) {
-            const int t = c0;
-            c0 = c1;
-            c1 = t;
-        }
-        tip0 = true;
-        tip1 = false;
-    } else if constexpr (KIND == KIND_II) {
-        tip0 = tip1 = false;
-    } else {
-        tip0 = c0 < T;
-        tip1 = c1 < T;
+    const int *__restrict__ chp = children + (size_t)p * (T - 1) * 2;
+    int c0 = chp[nl * 2], c1 = chp[nl * 2 + 1];
+    auto mode_of = [&](int c) {
+        if (c < T) return MODE_TIP;
+        return (chp[(c - T) * 2] < T && chp[(c - T) * 2 + 1] < T) ? MODE_CHERRY : MODE_STORED;
+    };
+    int m0 = mode_of(c0), m1 = mode_of(c1);
+    if (m0 > m1) {  // canonical order: the launch's (M0, M1) pair is sorted
+        const int tc = c0, tm = m0;
+        c0 = c1;
+        m0 = m1;
+        c1 = tc;
+        m1 = tm;
     }
+    const int md0 = M0 == MODE_RT ? m0 : M0, md1 = M1 == MODE_RT ? m1 : M1;
+    // tips of cherry children
+    const int ga0 = md0 == MODE_CHERRY ? chp[(c0 - T) * 2] : 0;
+    const int gb0 = md0 == MODE_CHERRY ? chp[(c0 - T) * 2 + 1] : 0;
+    const int ga1 = md1 == MODE_CHERRY ? chp[(c1 - T) * 2] : 0;
+    const int gb1 = md1 == MODE_CHERRY ? chp[(c1 - T) * 2 + 1] : 0;
     const size_t node_elems = (size_t)R * 20 * S_pad;
     const size_t lp = (size_t)(p - p_base);  // slot of the particle in the wave's partials
     // partial slots of a particle: WS workspaces x (T-1) internal nodes; a node's
@@ -188,16 +251,18 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
         const TX *P1 = Pg + ((size_t)c1 * R + r) * kPStride;
+        const TX *Pa0 = Pg + ((size_t)ga0 * R + r) * kPStride, *Pb0 = Pg + ((size_t)gb0 * R + r) * kPStride;
+        const TX *Pa1 = Pg + ((size_t)ga1 * R + r) * kPStride, *Pb1 = Pg + ((size_t)gb1 * R + r) * kPStride;
         TX a0[2][5], a1[2][5];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++)
 #pragma unroll
             for (int kb = 0; kb < 5; kb++) {
                 const int row = 16 * rb + c16;
-                a0[rb][kb] = (!tip0 && row < 20) ? P0[row * 21 + 4 * kb + g4] : (TX)0;
-                a1[rb][kb] = (!tip1 && row < 20) ? P1[row * 21 + 4 * kb + g4] : (TX)0;
+                a0[rb][kb] = (md0 != MODE_TIP && row < 20) ? P0[row * 21 + 4 * kb + g4] : (TX)0;
+                a1[rb][kb] = (md1 != MODE_TIP && row < 20) ? P1[row * 21 + 4 * kb + g4] : (TX)0;
             }
-        const int cs0 = tip0 ? 0 : nslot(c0), cs1 = tip1 ? 0 : nslot(c1);
+        const int cs0 = md0 == MODE_STORED ? nslot(c0) : 0, cs1 = md1 == MODE_STORED ? nslot(c1) : 0;
         const TX *X0 = Xp + ((size_t)cs0 * R + r) * 20 * S_pad;
         const TX *X1 = Xp + ((size_t)cs1 * R + r) * 20 * S_pad;
         const int *E0 = Ep + ((size_t)cs0 * R + r) * S_pad;
@@ -214,18 +279,25 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 valid[u] = o < tl.len;
                 slot[u] = tl.start + (valid[u] ? o : 0);
                 const int orig = perm[(size_t)p * S_pad + slot[u]];
-                int col0 = 20, col1 = 20;
-                if (tip0) {
-                    const int code = codes[(size_t)c0 * S_pad + orig];
-                    col0 = code < 20 ? code : 20;
+                auto col_of = [&](int tip) {
+                    const int code = codes[(size_t)tip * S_pad + orig];
+                    return code < 20 ? code : 20;
+                };
+                int col0 = 20, col1 = 20, ca0 = 20, cb0 = 20, ca1 = 20, cb1 = 20;
+                if (md0 == MODE_TIP) col0 = col_of(c0);
+                if (md1 == MODE_TIP) col1 = col_of(c1);
+                if (md0 == MODE_CHERRY) {
+                    ca0 = col_of(ga0);
+                    cb0 = col_of(gb0);
                 }
-                if (tip1) {
-                    const int code = codes[(size_t)c1 * S_pad + orig];
-                    col1 = code < 20 ? code : 20;
+                if (md1 == MODE_CHERRY) {
+                    ca1 = col_of(ga1);
+                    cb1 = col_of(gb1);
                 }
-                child_block<TX>(tip0, a0, P0, X0 + slot[u], S_pad, col0, valid[u], g4, y[u]);
-                child_block<TX>(tip1, a1, P1, X1 + slot[u], S_pad, col1, valid[u], g4, w[u]);
-                etot[u] = (tip0 ? 0 : E0[slot[u]]) + (tip1 ? 0 : E1[slot[u]]);
+                etot[u] = child_block<TX, M0>(md0, a0, P0, Pa0, Pb0, X0 + slot[u], E0 + slot[u], S_pad, col0, ca0,
+                                              cb0, valid[u], g4, y[u]);
+                etot[u] += child_block<TX, M1>(md1, a1, P1, Pa1, Pb1, X1 + slot[u], E1 + slot[u], S_pad, col1,
+                                               ca1, cb1, valid[u], g4, w[u]);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -306,7 +378,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
 // [base, base + n), read from the device (bucket_off[bucket], [bucket + 1]) when
 // bucket_off is given, so one launch configuration serves every topology and a
 // captured graph can be replayed after the schedule changes.
-template <typename TX, int KIND, bool IS_ROOT, int U>
+template <typename TX, int M0, int M1, bool IS_ROOT, int U>
 __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__restrict__ bucket_off,
                                                     int bucket, int base, int n)
 {
@@ -318,7 +390,7 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
     const int n_work = n * bpi;
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
         const int q = w / bpi;
-        prune_unit<TX, KIND, IS_ROOT, U>(ka, base + q, w - q * bpi);
+        prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, w - q * bpi);
     }
 }
 
@@ -326,22 +398,22 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
 
 namespace {
 
-template <typename TX, int KIND, bool IS_ROOT, int U>
+template <typename TX, int M0, int M1, bool IS_ROOT, int U>
 void launch_one(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_prune_mfma<TX, KIND, IS_ROOT, U>), dim3(grid), dim3(256), 0, s, ka, a.bucket_off,
+    hipLaunchKernelGGL((k_prune_mfma<TX, M0, M1, IS_ROOT, U>), dim3(grid), dim3(256), 0, s, ka, a.bucket_off,
                        a.bucket, a.item_base, a.n_items);
 }
 
-template <typename TX, int KIND>
-void launch_kind(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
+template <typename TX, int M0, int M1>
+void launch_pair(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
 {
     if (a.unroll >= 4)
-        launch_one<TX, KIND, false, 4>(a, ka, grid, s);
+        launch_one<TX, M0, M1, false, 4>(a, ka, grid, s);
     else if (a.unroll == 2)
-        launch_one<TX, KIND, false, 2>(a, ka, grid, s);
+        launch_one<TX, M0, M1, false, 2>(a, ka, grid, s);
     else
-        launch_one<TX, KIND, false, 1>(a, ka, grid, s);
+        launch_one<TX, M0, M1, false, 1>(a, ka, grid, s);
 }
 
 template <typename TX>
@@ -355,16 +427,19 @@ void launch_typed(const PruneArgs &a, hipStream_t s)
     // capped at the persistent grid
     int64_t units = a.bucket_off ? INT64_MAX : (int64_t)a.n_items * bpi;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
-    if (a.roots)
-        launch_one<TX, KIND_ANY, true, 1>(a, ka, grid, s);
-    else if (a.kind == KIND_TT)
-        launch_kind<TX, KIND_TT>(a, ka, grid, s);
-    else if (a.kind == KIND_TI)
-        launch_kind<TX, KIND_TI>(a, ka, grid, s);
-    else if (a.kind == KIND_II)
-        launch_kind<TX, KIND_II>(a, ka, grid, s);
-    else
-        launch_one<TX, KIND_ANY, false, 1>(a, ka, grid, s);
+    if (a.roots) {
+        launch_one<TX, MODE_RT, MODE_RT, true, 1>(a, ka, grid, s);
+        return;
+    }
+    switch (a.kind) {
+        case PRUNE_TT: launch_pair<TX, MODE_TIP, MODE_TIP>(a, ka, grid, s); break;
+        case PRUNE_TC: launch_pair<TX, MODE_TIP, MODE_CHERRY>(a, ka, grid, s); break;
+        case PRUNE_TS: launch_pair<TX, MODE_TIP, MODE_STORED>(a, ka, grid, s); break;
+        case PRUNE_CC: launch_pair<TX, MODE_CHERRY, MODE_CHERRY>(a, ka, grid, s); break;
+        case PRUNE_CS: launch_pair<TX, MODE_CHERRY, MODE_STORED>(a, ka, grid, s); break;
+        case PRUNE_SS: launch_pair<TX, MODE_STORED, MODE_STORED>(a, ka, grid, s); break;
+        default: launch_one<TX, MODE_RT, MODE_RT, false, 1>(a, ka, grid, s); break;
+    }
 }
 
 }  // namespace
