@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--unroll", default="", help="fixed launch shape instead of autotuning: U0,U1,U2")
     ap.add_argument("--blocks-per-cu", type=int, default=32)
+    ap.add_argument("--xcd-groups", type=int, default=8)
     ap.add_argument("--no-graphs", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--ess-taxa", type=int, default=16)
     ap.add_argument("--ess-sites", type=int, default=200)
@@ -310,7 +311,7 @@ def main():
     progress("sweep workload %s: %d particles uploaded" % (args.config, N))
     tune = None
     if args.variant == "mfma" and args.unroll:
-        eng.set_tuning(args.blocks_per_cu, [int(x) for x in args.unroll.split(",")])
+        eng.set_tuning(args.blocks_per_cu, [int(x) for x in args.unroll.split(",")], args.xcd_groups)
     elif args.variant == "mfma" and not args.no_autotune:
         tune = eng.autotune(0, N, sweeps=2)
         progress("autotune: %s" % tune)

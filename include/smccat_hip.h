@@ -282,7 +282,9 @@ typedef struct {
     int32_t unroll[3];       /* 1, 2 or 4 (default 2, 1, 1) */
     double ms_per_sweep;     /* autotune: best measured     */
     int32_t configs_tried;   /* autotune: candidates timed  */
-    int32_t reserved;
+    int32_t xcd_groups;      /* 1, 2, 4 or 8 (default 8): the workgroups of one item are
+                                kept on one XCD (workgroup i runs on XCD i mod 8), so the
+                                P matrices and codes it reads are fetched into one L2 */
 } smc_tuning;
 
 int smc_engine_get_tuning(smc_engine *e, smc_tuning *t);

@@ -632,6 +632,7 @@ PruneArgs prune_args(smc_engine *e, int p_base)
     a.xflag = e->WS == 2 ? e->d_xflag : nullptr;
     a.WS = e->WS;
     a.grid = e->tune.blocks_per_cu > 0 ? e->num_cus * e->tune.blocks_per_cu : 0;
+    a.xcds = e->tune.xcd_groups;
     return a;
 }
 
@@ -727,7 +728,7 @@ int run_sweep(smc_engine *e, int first, int count)
         return SMC_OK;
     }
     std::vector<int64_t> key = {first, count, e->fp32, e->mix_mode, e->WS, e->tune.blocks_per_cu,
-                                e->tune.unroll[0], e->tune.unroll[1], e->tune.unroll[2],
+                                e->tune.unroll[0], e->tune.unroll[1], e->tune.unroll[2], e->tune.xcd_groups,
                                 (int64_t)(intptr_t)e->d_bucket_off};
     for (const smc_engine::Wave &w : e->waves) {
         key.push_back(w.first);
@@ -1370,6 +1371,7 @@ int smc_engine_get_tuning(smc_engine *e, smc_tuning *t)
     if (!e || !t) return SMC_ERR_ARG;
     t->blocks_per_cu = e->tune.blocks_per_cu;
     for (int k = 0; k < 3; ++k) t->unroll[k] = e->tune.unroll[k];
+    t->xcd_groups = e->tune.xcd_groups;
     t->ms_per_sweep = 0.0;
     t->configs_tried = 0;
     return SMC_OK;
@@ -1382,8 +1384,11 @@ int smc_engine_set_tuning(smc_engine *e, const smc_tuning *t)
     for (int k = 0; k < 3; ++k)
         if (t->unroll[k] != 1 && t->unroll[k] != 2 && t->unroll[k] != 4)
             return fail(e, SMC_ERR_ARG, "unroll must be 1, 2 or 4");
+    if (t->xcd_groups != 1 && t->xcd_groups != 2 && t->xcd_groups != 4 && t->xcd_groups != 8)
+        return fail(e, SMC_ERR_ARG, "xcd_groups must be 1, 2, 4 or 8");
     e->tune.blocks_per_cu = t->blocks_per_cu;
     for (int k = 0; k < 3; ++k) e->tune.unroll[k] = t->unroll[k];
+    e->tune.xcd_groups = t->xcd_groups;
     return SMC_OK;
 }
 
@@ -1442,6 +1447,17 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
                 best = t;
             }
         }
+    for (int x : {1, 8}) {
+        if (x == best.xcd_groups || rc != SMC_OK) continue;
+        smc_engine::Tune t = best;
+        t.xcd_groups = x;
+        double ms;
+        if ((rc = measure(t, &ms)) != SMC_OK) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
     for (int k = 0; k < 3 && rc == SMC_OK; ++k)
         for (int u : unrolls) {
             if (u == best.unroll[k]) continue;
@@ -1466,6 +1482,7 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
     if (out) {
         out->blocks_per_cu = best.blocks_per_cu;
         for (int k = 0; k < 3; ++k) out->unroll[k] = best.unroll[k];
+        out->xcd_groups = best.xcd_groups;
         out->ms_per_sweep = best_ms;
         out->configs_tried = tried;
     }

@@ -124,6 +124,7 @@ struct PruneArgs {
     int bucket = 0;
     int grid = 0;            // persistent grid (workgroups); 0: one per unit of work
     int unroll = 1;          // 16-site column blocks per step (1, 2 or 4)
+    int xcds = 1;            // keep an item's workgroups on one of this many XCDs
 };
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores

@@ -21,10 +21,9 @@
 //
 // Cherries (internal nodes whose children are both tips, a third of the nodes
 // of a random tree) are never stored: the parent builds the cherry's partial
-// -- the product of two tip columns of P, scaled exactly as a stored partial
-// would be -- straight into its B operand, which saves the cherry's write and
-// the parent's read (about a third of a sweep's HBM bytes) and gives the same
-// bits.  Each child is therefore a tip, a cherry or a stored partial; items are
+// -- the product of two tip columns of P -- straight into its B operand, which
+// saves the cherry's write and the parent's read (about a third of a sweep's
+// HBM bytes).  Each child is therefore a tip, a cherry or a stored partial; items are
 // launched per (child mode, child mode) pair, roots decide at run time.
 #include "internal.hpp"
 #include "root_mixture.hpp"
@@ -80,28 +79,20 @@ __device__ __forceinline__ void tip_block(const TX *__restrict__ Pc, int col, in
 }
 
 // B operand (states 4 kb + g4 of this lane's site) of a cherry: the product of
-// its tips' P columns, scaled by 2^-e with e the frexp exponent of the max over
-// the 20 states (NaN when the max is 0 or not finite), as a stored partial is.
-// Every lane must call it (cross-lane max).  Returns e (0 when not finite).
+// its tips' P columns.  It is not rescaled: the products of two probabilities
+// stay in range, and the parent's own rescale by a power of two makes the
+// result exactly the one a rescaled, stored cherry gives (the exponents only
+// split differently).
 template <typename TX>
 __device__ __forceinline__ int cherry_b(const TX *__restrict__ Pa, const TX *__restrict__ Pb, int ca, int cb,
                                         bool valid, int g4, TX (&b)[5])
 {
-    TX m = 0;
 #pragma unroll
     for (int kb = 0; kb < 5; kb++) {
         const int row = 4 * kb + g4;
         b[kb] = valid ? Pa[row * 21 + ca] * Pb[row * 21 + cb] : (TX)0;
-        m = fmax(m, b[kb]);
     }
-    m = fmax(m, __shfl_xor(m, 16, 64));
-    m = fmax(m, __shfl_xor(m, 32, 64));
-    int ex;
-    (void)frexp(m, &ex);
-    const bool ok = (m > (TX)0) && isfinite(m);
-#pragma unroll
-    for (int kb = 0; kb < 5; kb++) b[kb] = ok ? (TX)ldexp(b[kb], -ex) : (TX)NAN;
-    return ok ? ex : 0;
+    return 0;
 }
 
 template <typename TX>
@@ -169,6 +160,7 @@ struct KArgs {
     int p_base, mix_mode;
     const uint8_t *xflag;
     int WS;
+    int xcds;  // workgroup i runs on XCD i mod xcds: the units of an item share one XCD's L2
 };
 
 // One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
@@ -386,11 +378,15 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
         base = bucket_off[bucket];
         n = bucket_off[bucket + 1] - base;
     }
-    const int bpi = ka.blocks_per_item;
-    const int n_work = n * bpi;
+    // unit w runs on XCD w mod X (the grid is a multiple of X); XCD x takes the
+    // items x, x + X, ..., each with all its tile groups, so the P matrices and
+    // tip codes an item reads are fetched into one L2 only
+    const int bpi = ka.blocks_per_item, X = ka.xcds;
+    const int n_work = (n + X - 1) / X * X * bpi;
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
-        const int q = w / bpi;
-        prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, w - q * bpi);
+        const int l = w / X;
+        const int q = (l / bpi) * X + (w - l * X);
+        if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, l % bpi);
     }
 }
 
@@ -422,10 +418,12 @@ void launch_typed(const PruneArgs &a, hipStream_t s)
     const int bpi = (a.tile_stride + 3) / 4;
     KArgs<TX> ka{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes,
                  (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
-                 a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS};
-    // without device bucket offsets the work is known: launch exactly what it needs,
-    // capped at the persistent grid
-    int64_t units = a.bucket_off ? INT64_MAX : (int64_t)a.n_items * bpi;
+                 a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS,
+                 std::max(1, a.xcds)};
+    // without device bucket offsets the work is known: launch exactly what it needs
+    // (items rounded up to whole XCD groups), capped at the persistent grid
+    const int X = ka.xcds;
+    int64_t units = a.bucket_off ? INT64_MAX : (int64_t)(a.n_items + X - 1) / X * X * bpi;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
     if (a.roots) {
         launch_one<TX, MODE_RT, MODE_RT, true, 1>(a, ka, grid, s);

@@ -64,13 +64,13 @@ def test_every_launch_shape_gives_identical_lnl(gpu):
     with eng:
         eng.set_graphs(False)
         want, _ = eng.refresh_likelihood()
-        for g in (1, 3, 8):
+        for g, x in ((1, 1), (3, 8), (8, 2)):
             for u in ((1, 1, 1), (2, 4, 2), (4, 2, 4)):
-                eng.set_tuning(g, u)
+                eng.set_tuning(g, u, x)
                 for graphs in (False, True):
                     eng.set_graphs(graphs)
                     got, _ = eng.refresh_likelihood()
-                    assert np.array_equal(got, want), (g, u, graphs)
+                    assert np.array_equal(got, want), (g, u, x, graphs)
 
 
 def test_autotune_then_oracle(gpu):
