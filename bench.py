@@ -225,6 +225,77 @@ def c5_bench(args):
     return out
 
 
+def rejuv_bench(args):
+    """Cost of one rejuvenation step per particle on c4-shaped data (200 taxa x
+    10,000 sites, GTR+G4+I, CAT K ~ U{5..120}): the plain FP64 path runs one
+    particle at a time with the Gibbs CAT move, as the reference's per-thread
+    loop does; the fast path runs FP32, all resident particles in lockstep, with
+    the learned CAT move.  Both run the same moves on the same target, so at
+    equal ESS per SMC step the ratio of these costs is the ratio of ESS/sec."""
+    from smccat import Engine, ProposalNetwork
+    from smccat import simulate as sim
+    codes, cfg = sim.simulate_config("c4", taxa=args.rj_taxa or None, sites=args.rj_sites or None)
+    T, S = codes.shape
+    R, kr = cfg["rates"], cfg["k_range"]
+    kmax = kr[1] + 8
+    nf, npl, it, tau = args.rj_particles, args.rj_plain, args.rj_iterations, args.rj_temp
+    batch = sim.prior_particles(np.random.default_rng(4), nf, T, S, num_rates=R, invar=True, k_range=kr,
+                                max_classes=kmax)
+    out = {"workload": "c4 data (%d taxa x %d sites, GTR+G4+I, DP-CAT), K ~ U{%d..%d}, %d MCMC iterations, "
+                       "CAT move probability 0.1, DP alpha 1.0, tempering %.2f" % (T, S, kr[0], kr[1], it, tau)}
+    # fast path: FP32, lockstep batch, learned CAT move (trained beforehand on Gibbs draws)
+    eng = Engine(T, S, R, nf, kmax, "fp32", workspaces=2)
+    eng.set_mixture("logspace")
+    eng.set_alignment(codes)
+    eng.set_particles(batch)
+    net = ProposalNetwork(eng, seed=1)
+    t0 = time.perf_counter()
+    net.collect_gibbs(range(nf), tau, seed=3, num_iter=args.rj_train_iters, dpp_alpha=1.0)
+    t_col = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = net.train(args.rj_epochs, batch=4096, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    progress("rejuv: %d training examples in %.1f s, trained in %.1f s" % (net.num_examples, t_col, t_train))
+    eng.set_particles(batch)
+    eng.phase_stats(reset=True)
+    t0 = time.perf_counter()
+    lnl, stats = eng.rejuvenate(tau, it, seed=5, invariant=True, gibbs_probability=0.1, dpp_alpha=1.0,
+                                proposal=net)
+    t_fast = time.perf_counter() - t0
+    phases = {k: round(v[0] / 1e3, 3) for k, v in eng.phase_stats().items()}
+    progress("rejuv fast: %d particles in %.1f s" % (nf, t_fast))
+    net.close()
+    eng.close()
+    out["fp32_batched_learned"] = {"particles": nf, "seconds": t_fast, "seconds_per_particle": t_fast / nf,
+                                   "host_phase_seconds": phases, "mh_stats": stats.tolist(),
+                                   "collect_seconds": t_col, "train_seconds": t_train,
+                                   "final_train_loss": float(losses[-1]) if len(losses) else None}
+    # plain path: FP64, one particle at a time; the Gibbs move is timed on its own
+    # (one particle, a tenth of its S/2 iterations, scaled) and weighted by its
+    # probability 0.1, so a small sample gives the expected cost
+    sub = batch.take(np.arange(npl))
+    eng = Engine(T, S, R, npl, kmax, "fp64", workspaces=2)
+    eng.set_mixture("logspace")
+    eng.set_alignment(codes)
+    eng.set_particles(sub)
+    t0 = time.perf_counter()
+    for i in range(npl):
+        eng.rejuvenate(tau, it, seed=5, first=i, count=1, invariant=True, gibbs_probability=0.0)
+    t_mcmc = (time.perf_counter() - t0) / npl
+    gi = max(1, S // 20)
+    t0 = time.perf_counter()
+    eng.gibbs_partition_move([0], tau, seed=7, num_iter=gi, dpp_alpha=1.0)
+    eng.refresh_likelihood(0, 1)
+    t_gibbs = (time.perf_counter() - t0) * (S / 2.0) / gi
+    eng.close()
+    per = t_mcmc + 0.1 * t_gibbs
+    progress("rejuv plain: %.2f s MCMC + 0.1 x %.1f s Gibbs per particle" % (t_mcmc, t_gibbs))
+    out["plain_fp64"] = {"particles_sampled": npl, "mcmc_seconds_per_particle": t_mcmc,
+                         "gibbs_seconds_per_move": t_gibbs, "seconds_per_particle": per}
+    out["ratio_per_particle"] = per / out["fp32_batched_learned"]["seconds_per_particle"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,7 +313,15 @@ def main():
                          "log-space form (default for c3/c4, where exp(logSiteLike) underflows)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--mode", default="sweep", choices=["sweep", "c5"],
+    ap.add_argument("--rj-taxa", type=int, default=0)
+    ap.add_argument("--rj-sites", type=int, default=0)
+    ap.add_argument("--rj-particles", type=int, default=100)
+    ap.add_argument("--rj-plain", type=int, default=4)
+    ap.add_argument("--rj-iterations", type=int, default=16)
+    ap.add_argument("--rj-temp", type=float, default=0.5)
+    ap.add_argument("--rj-train-iters", type=int, default=1000)
+    ap.add_argument("--rj-epochs", type=int, default=6)
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "c5", "rejuv"],
                     help="sweep: the updates/s line (with the SMC ESS/sec comparison); c5: the "
                          "learned-proposal benchmark")
     ap.add_argument("--no-ess", action="store_true", help="skip the SMC ESS/sec comparison")
@@ -271,6 +350,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(particles would shard across GPUs by slot)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "rejuv":
+        print(json.dumps({"metric": "rejuvenation step cost per particle, plain FP64 vs FP32 + batched + learned",
+                          "rejuv": rejuv_bench(args)}), flush=True)
+        return
     if args.mode == "c5":
         print(json.dumps({"metric": "learned CAT-allocation proposal: training and move throughput",
                           "c5": c5_bench(args)}), flush=True)

@@ -201,6 +201,7 @@ void smc_commit_range(smc_engine *e, int first, int count)
                   e->h_xflag_old.begin() + (size_t)p * Ti);
         e->h_lnl_old[p] = e->h_lnl[p];
         e->h_dirty[p] = 0;
+        e->h_pnodes[p].clear();
         smc_engine::Committed &c = e->committed[p];
         c.children.assign(e->h_children.begin() + (size_t)p * Ti * 2,
                           e->h_children.begin() + (size_t)(p + 1) * Ti * 2);
@@ -371,6 +372,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_pcls_off, P);
     A(&e->d_prog, P * (T - 1));
     A(&e->d_plist, 2 * P);
+    A(&e->d_pnodes, P * nn);
     const size_t W = e->Wmax, C = (size_t)e->Cmax;
     const size_t WS = e->WS;
     A(&e->d_E, W * WS * (T - 1) * R * Sp);
@@ -409,6 +411,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_lnl.assign(P, 0.0);
     e->h_lnl_old.assign(P, 0.0);
     e->h_dirty.assign(P, 0);
+    e->h_pnodes.assign(P, {});
     e->committed.resize(P);
     *out = e;
     return SMC_OK;
@@ -427,6 +430,7 @@ int smc_engine_destroy(smc_engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
+    if (e->d_pnodes) (void)hipFree(e->d_pnodes);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &pt : e->pending) {
@@ -823,9 +827,19 @@ int set_single_wave(smc_engine *e, int first, int count)
 // Eigen-decompositions and/or P matrices of the particles in the lists (the
 // lists must stay alive until the caller synchronises the stream).
 int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
-                     const std::vector<int> &p_list)
+                     const std::vector<int> &p_list, const std::vector<Item> &p_nodes = {})
 {
     hipStream_t s = e->stream;
+    if (!p_nodes.empty()) {  // P of single (particle, node) pairs: branch-length moves
+        if ((int64_t)p_nodes.size() > (int64_t)e->Pmax * e->nn)
+            return fail(e, SMC_ERR_STATE, "too many P nodes");
+        HIP_TRY(e, hipMemcpyAsync(e->d_pnodes, p_nodes.data(), p_nodes.size() * sizeof(Item),
+                                  hipMemcpyHostToDevice, s));
+        TimedLaunch tl(e, SMC_K_PMAT);
+        PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
+                    e->R, e->Kmax, first, 0, e->fp32, nullptr};
+        launch_pmat_nodes(pa, e->d_pnodes, (int)p_nodes.size(), s);
+    }
     if (!eig_list.empty()) {
         HIP_TRY(e, hipMemcpyAsync(e->d_plist, eig_list.data(), eig_list.size() * sizeof(int),
                                   hipMemcpyHostToDevice, s));
@@ -936,14 +950,26 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<Item> roots;
     std::vector<uint8_t> dirty(nn);
     std::vector<int> parent(nn);
+    std::vector<Item> p_nodes;
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
         const bool eig = eig_flags && eig_flags[i];
+        // P: every node of a particle whose eigensystems change, else the nodes
+        // whose branch length (or rate) changed; kept for a reject to restore
         bool any_tp = eig;
-        for (int n = 0; n < nn; ++n) any_tp |= (nf[n] & 2) != 0;
-        if (eig) eig_list.push_back(p);
-        if (any_tp) p_list.push_back(p);
+        for (int n = 0; n < nn; ++n)
+            if (nf[n] & 2) {
+                any_tp = true;
+                if (!eig) {
+                    p_nodes.push_back(Item{p, n});
+                    e->h_pnodes[p].push_back(n);
+                }
+            }
+        if (eig) {
+            eig_list.push_back(p);
+            p_list.push_back(p);
+        }
         e->h_dirty[p] |= (any_tp ? 1 : 0) | (eig ? 2 : 0);
         // dirty internal nodes: flagged, parents of changed branches, and every
         // ancestor of a dirty node (Tree.cpp marks the path to the root the same way)
@@ -975,7 +1001,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         }
         roots.push_back(Item{p, root});
     }
-    rc = recompute_models(e, first, eig_list, p_list);
+    rc = recompute_models(e, first, eig_list, p_list, p_nodes);
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
     HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
@@ -1041,6 +1067,7 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         return fail(e, SMC_ERR_ARG, "invalid accept/reject arguments");
     const int T = e->T, Ti = T - 1, nn = e->nn;
     std::vector<int> eig_list, p_list, rej;
+    std::vector<Item> p_nodes;
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         if (accept[i]) {
@@ -1051,8 +1078,12 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         std::copy(e->h_xflag_old.begin() + (size_t)p * Ti, e->h_xflag_old.begin() + (size_t)(p + 1) * Ti,
                   e->h_xflag.begin() + (size_t)p * Ti);
         e->h_lnl[p] = e->h_lnl_old[p];
-        if (e->h_dirty[p] & 2) eig_list.push_back(p);
-        if (e->h_dirty[p] & 3) p_list.push_back(p);
+        if (e->h_dirty[p] & 2) {
+            eig_list.push_back(p);
+            p_list.push_back(p);
+        } else if (e->h_dirty[p] & 1) {
+            for (int n : e->h_pnodes[p]) p_nodes.push_back(Item{p, n});
+        }
     }
     hipStream_t s = e->stream;
     // restore the committed parameters of rejected particles (contiguous runs)
@@ -1074,13 +1105,16 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         if (rc != SMC_OK) return rc;
         k = j + 1;
     }
-    if (!p_list.empty()) {
+    if (!p_list.empty() || !p_nodes.empty()) {
         int rc = set_single_wave(e, first, count);
         if (rc != SMC_OK) return rc;
-        rc = recompute_models(e, first, eig_list, p_list);
+        rc = recompute_models(e, first, eig_list, p_list, p_nodes);
         if (rc != SMC_OK) return rc;
     }
-    for (int p : rej) e->h_dirty[p] = 0;
+    for (int p : rej) {
+        e->h_dirty[p] = 0;
+        e->h_pnodes[p].clear();
+    }
     HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
                               (size_t)count * Ti, hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_lnl + first, e->h_lnl.data() + first, (size_t)count * sizeof(double),

@@ -84,6 +84,8 @@ struct smc_engine {
     std::vector<uint8_t> h_xflag, h_xflag_old;  // [P][T-1]
     std::vector<double> h_lnl, h_lnl_old;       // [P]
     std::vector<uint8_t> h_dirty;               // [P] bit0 P changed, bit1 eigen changed
+    std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
+    Item *d_pnodes = nullptr;                   // [P * nn] (particle, node) list for P updates
     struct Committed {
         std::vector<int32_t> children;
         std::vector<double> blen, base;

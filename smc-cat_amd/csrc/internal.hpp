@@ -60,6 +60,8 @@ struct PmatArgs {
     const int *plist;        // optional [count] particle list (else first..first+count-1)
 };
 void launch_pmat(const PmatArgs &a, hipStream_t s);
+// P of the listed (particle, node) items only (every class and rate of each).
+void launch_pmat_nodes(const PmatArgs &a, const Item *items, int n_items, hipStream_t s);
 
 // Child modes of a pruning item (sorted pair): a child is a tip (T), a cherry
 // (C: an internal node whose children are both tips) or a stored partial (S).

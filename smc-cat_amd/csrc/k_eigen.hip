@@ -214,6 +214,34 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
                 eig + (size_t)blockIdx.x * kEigStride);
 }
 
+// P(t r) = A diag(e^{lam t r}) B of one eigensystem (A, lam, B at eg), written
+// row-major with a 21st column of row sums.
+template <typename TP>
+__device__ __forceinline__ void pmat_one(const double *__restrict__ eg, double t, double rate, TP *__restrict__ out)
+{
+    const double *A = eg;
+    const double *lam = eg + kEigLam;
+    const double *B = eg + kEigB;
+    double e[20];
+#pragma unroll
+    for (int kk = 0; kk < 20; kk++) e[kk] = exp(lam[kk] * t * rate);
+    for (int i = 0; i < 20; i++) {
+        double a[20];
+#pragma unroll
+        for (int kk = 0; kk < 20; kk++) a[kk] = A[i * 20 + kk] * e[kk];
+        double rowsum = 0.0;
+#pragma unroll
+        for (int j = 0; j < 20; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 20; kk++) acc = fma(a[kk], B[kk * 20 + j], acc);
+            out[i * 21 + j] = (TP)acc;
+            rowsum += acc;
+        }
+        out[i * 21 + 20] = (TP)rowsum;
+    }
+}
+
 // One thread = one whole P matrix (particle, class, node, rate).  Every lane of
 // a block shares the class, so A, lam and B are wave-uniform scalar loads.
 template <typename TP>
@@ -233,32 +261,29 @@ __global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
     if (idx >= nn * R) return;
     const int n = idx / R, r = idx % R;
     const size_t g = (size_t)p * max_classes + k;
-    const double *A = eig + g * kEigStride;
-    const double *lam = A + kEigLam;
-    const double *B = A + kEigB;
-    const double t = blen[(size_t)p * nn + n];
-    const double rate = rates[(size_t)p * R + r];
+    pmat_one<TP>(eig + g * kEigStride, blen[(size_t)p * nn + n], rates[(size_t)p * R + r],
+                 Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
+}
 
-    double e[20];
-#pragma unroll
-    for (int kk = 0; kk < 20; kk++) e[kk] = exp(lam[kk] * t * rate);
-
-    const size_t gp = (size_t)pcls_off[p] + k;  // compact class slot of the current wave
-    TP *out = Pout + ((gp * nn + n) * R + r) * kPStride;
-    for (int i = 0; i < 20; i++) {
-        double a[20];
-#pragma unroll
-        for (int kk = 0; kk < 20; kk++) a[kk] = A[i * 20 + kk] * e[kk];
-        double rowsum = 0.0;
-#pragma unroll
-        for (int j = 0; j < 20; j++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int kk = 0; kk < 20; kk++) acc = fma(a[kk], B[kk * 20 + j], acc);
-            out[i * 21 + j] = (TP)acc;
-            rowsum += acc;
-        }
-        out[i * 21 + 20] = (TP)rowsum;
+// P of listed (particle, node) items only, every class and rate: block = item,
+// threads over (class, rate) pairs.
+template <typename TP>
+__global__ __launch_bounds__(64) void k_pmat_nodes(const double *__restrict__ eig,
+                                                    const double *__restrict__ blen,
+                                                    const double *__restrict__ rates,
+                                                    const int *__restrict__ num_classes,
+                                                    const int *__restrict__ pcls_off,
+                                                    TP *__restrict__ Pout, int nn, int R, int max_classes,
+                                                    const Item *__restrict__ items)
+{
+    const Item it = items[blockIdx.x];
+    const int p = it.particle, n = it.node;
+    const int pairs = num_classes[p] * R;
+    for (int q = threadIdx.x; q < pairs; q += 64) {
+        const int k = q / R, r = q % R;
+        const size_t g = (size_t)p * max_classes + k;
+        pmat_one<TP>(eig + g * kEigStride, blen[(size_t)p * nn + n], rates[(size_t)p * R + r],
+                     Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
     }
 }
 
@@ -292,6 +317,21 @@ void launch_pmat(const PmatArgs &a, hipStream_t s)
         hipLaunchKernelGGL(k_pmat<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
+}
+
+}  // namespace smccat
+
+namespace smccat {
+
+void launch_pmat_nodes(const PmatArgs &a, const Item *items, int n_items, hipStream_t s)
+{
+    if (n_items <= 0) return;
+    if (a.fp32)
+        hipLaunchKernelGGL(k_pmat_nodes<float>, dim3(n_items), dim3(64), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes, items);
+    else
+        hipLaunchKernelGGL(k_pmat_nodes<double>, dim3(n_items), dim3(64), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes, items);
 }
 
 }  // namespace smccat

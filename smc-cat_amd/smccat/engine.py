@@ -225,8 +225,9 @@ class Engine:
                     "smc_accept_reject")
 
     def rejuvenate(self, tempering, iterations, seed, first=0, count=None, using_lg=False,
-                   invariant=True, gibbs_probability=0.0, dpp_alpha=0.01, **deltas):
-        """MCMC rejuvenation of slots [first, first+count); returns (lnl, stats[4, 2])."""
+                   invariant=True, gibbs_probability=0.0, dpp_alpha=0.01, proposal=None, **deltas):
+        """MCMC rejuvenation of slots [first, first+count); returns (lnl, stats[4, 2]).
+        proposal: a ProposalNetwork makes the CAT move the learned one."""
         if count is None:
             count = self.max_particles - first
         o = _lib.RejuvenationOptions(int(iterations), int(using_lg), int(invariant), 0, int(seed),
@@ -235,7 +236,8 @@ class Engine:
                                      deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
                                      deltas.get("rate_full_scale", 0.5),
                                      deltas.get("invar_alpha", 100.0), float(gibbs_probability),
-                                     float(dpp_alpha), None, None, 0, 0)
+                                     float(dpp_alpha), None, proposal.h if proposal is not None else None,
+                                     0, 0)
         lnl = np.empty(count)
         stats = (ctypes.c_int64 * 8)()
         self._check(self.lib.smc_rejuvenate(self.h, first, count, float(tempering), ctypes.byref(o),
