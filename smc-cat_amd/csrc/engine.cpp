@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -185,6 +186,7 @@ int build_schedule(smc_engine *e, int first, int count)
     HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), pcls.size() * sizeof(int),
                               hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->pcls_first = -1;  // the wave layout of the schedule replaced any single-wave layout
     e->sched_first = first;
     e->sched_count = count;
     e->sched_version = e->topo_version;
@@ -411,6 +413,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_lnl.assign(P, 0.0);
     e->h_lnl_old.assign(P, 0.0);
     e->h_dirty.assign(P, 0);
+    e->h_touched.assign(P, 0);
     e->h_pnodes.assign(P, {});
     e->committed.resize(P);
     *out = e;
@@ -431,6 +434,8 @@ int smc_engine_destroy(smc_engine *e)
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
+    for (void *b : {(void *)e->pool.d_eig, (void *)e->pool.d_logits, (void *)e->pool.d_ent})
+        if (b) (void)hipFree(b);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &pt : e->pending) {
@@ -601,6 +606,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * Ti, progs.data(), progs.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipStreamSynchronize(s));
+    e->class_version++;
     e->topo_version++;
     return SMC_OK;
 }
@@ -811,6 +817,8 @@ namespace {
 int set_single_wave(smc_engine *e, int first, int count)
 {
     if (count > e->Wmax) return fail(e, SMC_ERR_ARG, "range exceeds partial_slots");
+    if (e->pcls_first == first && e->pcls_count == count && e->pcls_version == e->class_version)
+        return SMC_OK;  // the device already holds this layout
     std::vector<int> pcls(count);
     int64_t ncls = 0;
     for (int i = 0; i < count; ++i) {
@@ -821,6 +829,9 @@ int set_single_wave(smc_engine *e, int first, int count)
     HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), count * sizeof(int),
                               hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));  // pcls is a local buffer
+    e->pcls_first = first;
+    e->pcls_count = count;
+    e->pcls_version = e->class_version;
     return SMC_OK;
 }
 
@@ -833,12 +844,31 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
     if (!p_nodes.empty()) {  // P of single (particle, node) pairs: branch-length moves
         if ((int64_t)p_nodes.size() > (int64_t)e->Pmax * e->nn)
             return fail(e, SMC_ERR_STATE, "too many P nodes");
-        HIP_TRY(e, hipMemcpyAsync(e->d_pnodes, p_nodes.data(), p_nodes.size() * sizeof(Item),
-                                  hipMemcpyHostToDevice, s));
+        // grouped by particle (the lists are built particle by particle):
+        // [particles | offsets | nodes] in one upload
+        std::vector<int> &buf = e->h_pnode_buf;
+        buf.clear();
+        std::vector<int> plist, noff{0}, nodes;
+        for (size_t i = 0; i < p_nodes.size(); ++i) {
+            if (i == 0 || p_nodes[i].particle != p_nodes[i - 1].particle) {
+                if (i) noff.push_back((int)nodes.size());
+                plist.push_back(p_nodes[i].particle);
+            }
+            nodes.push_back(p_nodes[i].node);
+        }
+        noff.push_back((int)nodes.size());
+        const int np = (int)plist.size();
+        int max_nodes = 0;
+        for (int i = 0; i < np; ++i) max_nodes = std::max(max_nodes, noff[i + 1] - noff[i]);
+        buf.insert(buf.end(), plist.begin(), plist.end());
+        buf.insert(buf.end(), noff.begin(), noff.end());
+        buf.insert(buf.end(), nodes.begin(), nodes.end());
+        int *d = reinterpret_cast<int *>(e->d_pnodes);  // capacity: 2 ints per (particle, node)
+        HIP_TRY(e, hipMemcpyAsync(d, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice, s));
         TimedLaunch tl(e, SMC_K_PMAT);
         PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
                     e->R, e->Kmax, first, 0, e->fp32, nullptr};
-        launch_pmat_nodes(pa, e->d_pnodes, (int)p_nodes.size(), s);
+        launch_pmat_nodes(pa, d, d + np, d + 2 * np + 1, np, max_nodes, s);
     }
     if (!eig_list.empty()) {
         HIP_TRY(e, hipMemcpyAsync(e->d_plist, eig_list.data(), eig_list.size() * sizeof(int),
@@ -894,6 +924,7 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
         std::copy(blen.begin() + (size_t)i * nn, blen.begin() + (size_t)(i + 1) * nn,
                   e->h_blen.begin() + (size_t)p * nn);
         std::copy(base + (size_t)i * 190, base + (size_t)(i + 1) * 190, e->h_base.begin() + (size_t)p * 190);
+        if (pinv[i] != e->h_pinv[p]) e->h_touched[p] = 1;
         e->h_pinv[p] = pinv[i];
         e->h_shape[p] = shape[i];
     }
@@ -951,6 +982,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<uint8_t> dirty(nn);
     std::vector<int> parent(nn);
     std::vector<Item> p_nodes;
+    std::vector<int> keep;
+    std::vector<double> keep_lnl;
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
@@ -999,7 +1032,15 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
             buckets[PRUNE_KINDS * L + kind].push_back(Item{p, n});
         }
-        roots.push_back(Item{p, root});
+        bool changed = eig || e->h_touched[p];  // pInvar enters at the root only
+        e->h_touched[p] = 0;
+        for (int n = 0; n < nn && !changed; ++n) changed = nf[n] != 0;
+        if (changed) {
+            roots.push_back(Item{p, root});
+        } else {  // nothing to recompute: its lnL stays (its tile sums may be stale)
+            keep.push_back(p);
+            keep_lnl.push_back(e->h_lnl[p]);
+        }
     }
     rc = recompute_models(e, first, eig_list, p_list, p_nodes);
     if (rc != SMC_OK) return rc;
@@ -1034,12 +1075,26 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         else
             launch_prune_mfma(a, s);
     };
+    const int bpi = (e->tile_stride + 3) / 4;
+    const int64_t small = (int64_t)e->num_cus * 8;  // units that do not fill the GPU twice over
     for (size_t b = 0; b + 1 < off.size(); ++b) {
         const int kind = (int)(b % PRUNE_KINDS);
-        if (e->prune_variant == SMC_PRUNE_VALU)
+        if (e->prune_variant == SMC_PRUNE_VALU) {
             run(off[b], off[b + 1] - off[b], PRUNE_ANY, false);
-        else if (kind != PRUNE_TT)  // cherries are built by their parents
+        } else if (kind == PRUNE_TC) {
+            // a level's non-cherry buckets are contiguous: when they are small, one
+            // launch that decides the child modes per item saves four launches
+            const size_t end = std::min(off.size() - 1, b + (PRUNE_KINDS - 1));
+            const int n = off[end] - off[b];
+            if ((int64_t)n * bpi <= small) {
+                run(off[b], n, PRUNE_ANY, false);
+                b = end - 1;
+            } else {
+                run(off[b], off[b + 1] - off[b], kind, false);
+            }
+        } else if (kind != PRUNE_TT) {  // cherries are built by their parents
             run(off[b], off[b + 1] - off[b], kind, false);
+        }
     }
     run(root_off, (int)roots.size(), PRUNE_ANY, true);
     {
@@ -1055,6 +1110,10 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                                   hipMemcpyDeviceToHost, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     collect_timings(e);
+    for (size_t k = 0; k < keep.size(); ++k) {
+        e->h_lnl[keep[k]] = keep_lnl[k];
+        if (nonfinite_out) nonfinite_out[keep[k] - first] = std::isfinite(keep_lnl[k]) ? 0 : 1;
+    }
     if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
     return SMC_OK;
 }
@@ -1086,11 +1145,10 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         }
     }
     hipStream_t s = e->stream;
-    // restore the committed parameters of rejected particles (contiguous runs)
-    for (size_t k = 0; k < rej.size();) {
-        size_t j = k;
-        while (j + 1 < rej.size() && rej[j + 1] == rej[j] + 1) ++j;
-        const int r0 = rej[k], rn = (int)(j - k + 1);
+    // restore the committed parameters of rejected particles: one upload of the
+    // span they cover (accepted particles in it are committed already)
+    if (!rej.empty()) {
+        const int r0 = rej.front(), rn = rej.back() - rej.front() + 1;
         std::vector<int32_t> ch((size_t)rn * Ti * 2);
         std::vector<double> bl((size_t)rn * nn), base((size_t)rn * 190), pv(rn), sh(rn);
         for (int q = 0; q < rn; ++q) {
@@ -1103,7 +1161,6 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         }
         int rc = upload_params(e, r0, rn, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
         if (rc != SMC_OK) return rc;
-        k = j + 1;
     }
     if (!p_list.empty() || !p_nodes.empty()) {
         int rc = set_single_wave(e, first, count);
@@ -1254,7 +1311,8 @@ int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, co
     {
         TimedLaunch tl(e, SMC_K_EIGEN);
         launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count,
-                          n_cand, e->d_sc_eig, cand_class ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax, s);
+                          n_cand, e->d_sc_eig, cand_class ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax,
+                          e->pool.d_eig, s);
     }
     {
         TimedLaunch tl(e, SMC_K_SITE);
@@ -1520,5 +1578,68 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
         out->ms_per_sweep = best_ms;
         out->configs_tried = tried;
     }
+    return SMC_OK;
+}
+
+int smc_pool_refill(smc_engine *e, int count, const int32_t *slots, int q,
+                    const std::function<void(int, double *)> &draw, int fresh)
+{
+    ProfilePool &pl = e->pool;
+    hipStream_t s = e->stream;
+    const int64_t n = (int64_t)count * q;
+    auto grow = [&](auto *&ptr, int64_t &cap, int64_t need) -> int {
+        if (need <= cap) return SMC_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const int64_t c2 = std::max<int64_t>(need, 1024);
+        HIP_TRY(e, hipMalloc((void **)&ptr, (size_t)c2 * sizeof(*ptr)));
+        cap = c2;
+        return SMC_OK;
+    };
+    int rc = grow(pl.d_eig, pl.cap, n * kEigStride);
+    if (rc == SMC_OK) rc = grow(pl.d_logits, pl.lcap, n * 20);
+    if (rc == SMC_OK) rc = grow(pl.d_ent, pl.ecap, 2 * (int64_t)count);
+    if (rc != SMC_OK) return rc;
+    // unconsumed profiles stay first (a slot uses its draws in order, whatever
+    // the batch), then each slot draws as many new ones as it consumed
+    const bool keep = fresh == 0 && pl.count == count && pl.q == q && (int)pl.next.size() == count;
+    std::vector<double> lg((size_t)n * 20, 0.0);
+    for (int i = 0; i < count; ++i) {
+        int j = 0;
+        if (keep)
+            for (int k = pl.next[i]; k < q; ++k, ++j)
+                std::copy_n(&pl.logits[((size_t)i * q + k) * 20], 20, &lg[((size_t)i * q + j) * 20]);
+        for (; j < q; ++j) draw(i, &lg[((size_t)i * q + j) * 20]);
+    }
+    pl.logits.swap(lg);
+    pl.count = count;
+    pl.q = q;
+    pl.next.assign(count, 0);
+    std::vector<int> ent(2 * (size_t)count);
+    for (int i = 0; i < count; ++i) {
+        ent[i] = slots[i];
+        ent[count + i] = q;
+    }
+    HIP_TRY(e, hipMemcpyAsync(pl.d_ent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(pl.d_logits, pl.logits.data(), pl.logits.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    {
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        launch_eigen_cand(e->d_base, pl.d_ent, pl.d_ent + count, pl.d_logits, count, q, pl.d_eig, nullptr,
+                          e->d_eig, e->Kmax, nullptr, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipStreamSynchronize(s));
+    collect_timings(e);
+    return SMC_OK;
+}
+
+int smc_pool_to_class(smc_engine *e, int slot, int pool_index, int cls)
+{
+    if (cls < 0 || cls >= e->Kmax) return fail(e, SMC_ERR_STATE, "no free class slot");
+    HIP_TRY(e, hipMemcpyAsync(e->d_eig + ((size_t)slot * e->Kmax + cls) * kEigStride,
+                              e->pool.d_eig + (size_t)pool_index * kEigStride, kEigStride * sizeof(double),
+                              hipMemcpyDeviceToDevice, e->stream));
     return SMC_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <array>
 #include <chrono>
+#include <functional>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -17,6 +18,18 @@ using namespace smccat;
 struct PendingTiming {
     int family;
     hipEvent_t a, b;
+};
+
+// Fresh candidate profiles drawn ahead and decomposed in one launch: q per slot
+// of a move's batch, consumed in draw order (logits [count][q][20], device
+// eigensystems in the same order).
+struct ProfilePool {
+    int count = 0, q = 0;
+    std::vector<int> next;
+    std::vector<double> logits;
+    double *d_eig = nullptr, *d_logits = nullptr;
+    int *d_ent = nullptr;
+    int64_t cap = 0, lcap = 0, ecap = 0;
 };
 
 struct smc_engine {
@@ -85,7 +98,9 @@ struct smc_engine {
     std::vector<double> h_lnl, h_lnl_old;       // [P]
     std::vector<uint8_t> h_dirty;               // [P] bit0 P changed, bit1 eigen changed
     std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
-    Item *d_pnodes = nullptr;                   // [P * nn] (particle, node) list for P updates
+    std::vector<uint8_t> h_touched;             // [P] pInvar changed by the last parameter upload
+    Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
+    std::vector<int> h_pnode_buf;               // host staging of those lists (kept until the next call)
     struct Committed {
         std::vector<int32_t> children;
         std::vector<double> blen, base;
@@ -97,6 +112,7 @@ struct smc_engine {
     int sc_ent_cap = 0;
     int *d_sc_ent = nullptr;      // [3][entries]: particle, site, candidate count
     int *d_sc_cls = nullptr;      // [entries][candidates] engine class of a candidate or -1
+    ProfilePool pool;             // fresh profiles of the CAT moves
     double *d_sc_logits = nullptr, *d_sc_eig = nullptr, *d_sc_rate = nullptr, *d_sc_out = nullptr;
 
     // pruning schedule cache: waves of particles, each with its level/kind buckets
@@ -126,6 +142,9 @@ struct smc_engine {
     hipGraphExec_t graph_exec = nullptr;
     int64_t graph_launches = 0, graph_captures = 0;
     int sched_first = -1, sched_count = -1;
+    // single-wave class layout last written to d_pcls_off (incremental path)
+    int pcls_first = -1, pcls_count = -1;
+    uint64_t pcls_version = 0, class_version = 1;
     uint64_t sched_version = 0, topo_version = 1;
     std::vector<Wave> waves;
 
@@ -182,6 +201,18 @@ extern "C" int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *pa
                                      const int32_t *sites, int n_cand, const int32_t *ncand,
                                      const double *cand_logits, const int32_t *cand_class, double tempering,
                                      double *out);
+
+// Fills the profile pool: q profiles per listed slot, all decomposed in one
+// launch.  With fresh = 0 (a refill within one move) each slot keeps its
+// unconsumed profiles first and draw(i, logits20) tops it up; fresh = 1 starts
+// a new pool.  A site-class candidate -2 - (i q + j) names profile j of entry i.
+int smc_pool_refill(smc_engine *e, int count, const int32_t *slots, int q,
+                    const std::function<void(int, double *)> &draw, int fresh);
+
+// Copies pool profile pool_index's eigensystem into class slot `cls` of
+// particle `slot` (stream-ordered), so a class a move creates is scored from
+// the engine's eigensystems like the others.
+int smc_pool_to_class(smc_engine *e, int slot, int pool_index, int cls);
 
 // Recomputes the eigensystems of every class of the listed slots (from the
 // uploaded base matrices and logits), so that cached candidates are current.

@@ -69,7 +69,8 @@ struct Philox {
 struct PState {
     int K;
     std::vector<std::array<double, 20>> logits;
-    std::vector<int> orig;  // class index at upload (its eigensystem is on the engine), -1: new
+    std::vector<int> orig;  // engine class slot holding the class's eigensystem
+    std::vector<int> free;  // engine class slots no class uses
     std::vector<int> assign;
     std::vector<int> size;
 };
@@ -112,10 +113,24 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
         for (int a : ps.assign) ps.size[a]++;
         ps.orig.resize(ps.K);
         for (int k = 0; k < ps.K; ++k) ps.orig[k] = k;
+        for (int k = Kmax - 1; k >= ps.K; --k) ps.free.push_back(k);
         rng.emplace_back(seed, (uint64_t)p);
     }
     // existing classes are scored with the engine's eigensystems: make them current
     int rc0 = smc_refresh_eigen(e, count, slots);
+    if (rc0 != SMC_OK) return rc0;
+    // auxiliary profiles come from a pool of prior draws decomposed in batches
+    // (each slot's own stream, consumed in draw order)
+    // (a stream of their own per slot, so refills do not shift the site draws)
+    const int pool_q = m * 16;
+    std::vector<Philox> prof_rng;
+    prof_rng.reserve(count);
+    for (int i = 0; i < count; ++i) prof_rng.emplace_back(seed ^ 0x70726f66696c65ull, (uint64_t)slots[i]);
+    auto draw = [&](int i, double *lg) {
+        for (int j = 0; j < 19; ++j) lg[j] = prof_rng[i].normal();
+        lg[19] = 0.0;
+    };
+    rc0 = smc_pool_refill(e, count, slots, pool_q, draw, 1);
     if (rc0 != SMC_OK) return rc0;
     const double log_aux = std::log(dpp_alpha / m);
     const int n_cand = Kmax + m;
@@ -125,6 +140,12 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
     int64_t moved = 0, created = 0, erased = 0;
 
     for (int it = 0; it < iters; ++it) {
+        for (int i = 0; i < count; ++i)
+            if (e->pool.next[i] + m > pool_q) {  // a slot is running out: top every slot up
+                int rc = smc_pool_refill(e, count, slots, pool_q, draw, 0);
+                if (rc != SMC_OK) return rc;
+                break;
+            }
         for (int i = 0; i < count; ++i) {
             PState &ps = st[i];
             Philox &g = rng[i];
@@ -136,6 +157,7 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             if (ps.size[k0] == 0) {  // erase the emptied class (Particle.cpp:819-821)
                 was = -1;
                 ps.logits.erase(ps.logits.begin() + k0);
+                ps.free.push_back(ps.orig[k0]);
                 ps.orig.erase(ps.orig.begin() + k0);
                 ps.size.erase(ps.size.begin() + k0);
                 for (int &a : ps.assign)
@@ -155,10 +177,9 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
                 cc[k] = ps.orig[k];
             }
             for (int a = 0; a < n_aux; ++a) {
-                cc[ps.K + a] = -1;
-                double *lg = c + 20 * (ps.K + a);
-                for (int j = 0; j < 19; ++j) lg[j] = g.normal();
-                lg[19] = 0.0;
+                const int slot = e->pool.next[i]++;
+                cc[ps.K + a] = -2 - (i * pool_q + slot);
+                std::copy_n(&e->pool.logits[((size_t)i * pool_q + slot) * 20], 20, c + 20 * (ps.K + a));
             }
             ps.assign[site] = -2 - was;  // remembers the previous class until the draw
         }
@@ -199,7 +220,12 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
                 std::copy(&cand[((size_t)i * n_cand + pick) * 20],
                           &cand[((size_t)i * n_cand + pick) * 20] + 20, lg.begin());
                 ps.logits.push_back(lg);
-                ps.orig.push_back(-1);
+                // its eigensystem (a pool profile) moves into a free engine class slot
+                const int f = ps.free.back();
+                ps.free.pop_back();
+                int rc = smc_pool_to_class(e, slots[i], -2 - cand_cls[(size_t)i * n_cand + pick], f);
+                if (rc != SMC_OK) return rc;
+                ps.orig.push_back(f);
                 ps.size.push_back(0);
                 pick = ps.K++;
                 created++;

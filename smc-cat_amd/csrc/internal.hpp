@@ -60,8 +60,10 @@ struct PmatArgs {
     const int *plist;        // optional [count] particle list (else first..first+count-1)
 };
 void launch_pmat(const PmatArgs &a, hipStream_t s);
-// P of the listed (particle, node) items only (every class and rate of each).
-void launch_pmat_nodes(const PmatArgs &a, const Item *items, int n_items, hipStream_t s);
+// P of listed nodes only: particle plist[i] recomputes nodes[noff[i] .. noff[i+1])
+// for every class and rate.
+void launch_pmat_nodes(const PmatArgs &a, const int *plist, const int *noff, const int *nodes, int n_particles,
+                       int max_nodes, hipStream_t s);
 
 // Child modes of a pruning item (sorted pair): a child is a tip (T), a cherry
 // (C: an internal node whose children are both tips) or a stored partial (S).
@@ -157,10 +159,11 @@ void launch_site_class(const SiteClassArgs &a, hipStream_t s);
 // Eigen-decomposition of candidate profiles: entry e uses the base matrix of
 // particle ent_particle[e] and logits[e][c][20] for c < ent_ncand[e].
 // cand_cls (nullable) [count][n_cand]: >= 0 copies the class's eigensystem from
-// eig_cache ([particles][max_classes] layout) instead of decomposing the logits.
+// eig_cache ([particles][max_classes] layout), <= -2 copies pool[-2 - v],
+// -1 decomposes the logits.
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
                        const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
-                       const double *eig_cache, int max_classes, hipStream_t s);
+                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s);
 
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s);

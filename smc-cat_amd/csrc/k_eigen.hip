@@ -199,13 +199,15 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
                                                     const double *__restrict__ logits, int n_cand,
                                                     double *__restrict__ eig,
                                                     const int *__restrict__ cand_cls,
-                                                    const double *__restrict__ eig_cache, int max_classes)
+                                                    const double *__restrict__ eig_cache, int max_classes,
+                                                    const double *__restrict__ pool)
 {
     const int e = blockIdx.x / n_cand, c = blockIdx.x % n_cand;
     if (c >= ent_ncand[e]) return;
     const int k = cand_cls ? cand_cls[blockIdx.x] : -1;
-    if (k >= 0) {  // block-uniform
-        const double *src = eig_cache + ((size_t)ent_particle[e] * max_classes + k) * kEigStride;
+    if (k >= 0 || k <= -2) {  // block-uniform: a copy of a decomposition made earlier
+        const double *src = k >= 0 ? eig_cache + ((size_t)ent_particle[e] * max_classes + k) * kEigStride
+                                   : pool + (size_t)(-2 - k) * kEigStride;
         double *dst = eig + (size_t)blockIdx.x * kEigStride;
         for (int i = threadIdx.x; i < kEigStride; i += 64) dst[i] = src[i];
         return;
@@ -265,8 +267,9 @@ __global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
                  Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
 }
 
-// P of listed (particle, node) items only, every class and rate: block = item,
-// threads over (class, rate) pairs.
+// P of listed nodes only: block = (listed particle, class, chunk of 64 of its
+// (node, rate) pairs), so A, lam and B stay wave-uniform (scalar loads) as in
+// k_pmat.
 template <typename TP>
 __global__ __launch_bounds__(64) void k_pmat_nodes(const double *__restrict__ eig,
                                                     const double *__restrict__ blen,
@@ -274,14 +277,18 @@ __global__ __launch_bounds__(64) void k_pmat_nodes(const double *__restrict__ ei
                                                     const int *__restrict__ num_classes,
                                                     const int *__restrict__ pcls_off,
                                                     TP *__restrict__ Pout, int nn, int R, int max_classes,
-                                                    const Item *__restrict__ items)
+                                                    const int *__restrict__ plist,
+                                                    const int *__restrict__ noff,
+                                                    const int *__restrict__ nodes)
 {
-    const Item it = items[blockIdx.x];
-    const int p = it.particle, n = it.node;
-    const int pairs = num_classes[p] * R;
-    for (int q = threadIdx.x; q < pairs; q += 64) {
-        const int k = q / R, r = q % R;
-        const size_t g = (size_t)p * max_classes + k;
+    const int pi = blockIdx.x / max_classes, k = blockIdx.x % max_classes;
+    const int p = plist[pi];
+    if (k >= num_classes[p]) return;
+    const int n0 = noff[pi], cnt = noff[pi + 1] - n0;
+    const size_t g = (size_t)p * max_classes + k;
+    const int q = blockIdx.y * 64 + threadIdx.x;  // (node, rate) pair of this lane
+    if (q < cnt * R) {
+        const int n = nodes[n0 + q / R], r = q % R;
         pmat_one<TP>(eig + g * kEigStride, blen[(size_t)p * nn + n], rates[(size_t)p * R + r],
                      Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
     }
@@ -298,11 +305,11 @@ void launch_eigen(const EigenArgs &a, hipStream_t s)
 
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
                        const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
-                       const double *eig_cache, int max_classes, hipStream_t s)
+                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s)
 {
     if (count <= 0 || n_cand <= 0) return;
     hipLaunchKernelGGL(k_eigen_cand, dim3(count * n_cand), dim3(64), 0, s, base, ent_particle,
-                       ent_ncand, logits, n_cand, eig, cand_cls, eig_cache, max_classes);
+                       ent_ncand, logits, n_cand, eig, cand_cls, eig_cache, max_classes, pool);
 }
 
 void launch_pmat(const PmatArgs &a, hipStream_t s)
@@ -323,15 +330,17 @@ void launch_pmat(const PmatArgs &a, hipStream_t s)
 
 namespace smccat {
 
-void launch_pmat_nodes(const PmatArgs &a, const Item *items, int n_items, hipStream_t s)
+void launch_pmat_nodes(const PmatArgs &a, const int *plist, const int *noff, const int *nodes, int n_particles,
+                       int max_nodes, hipStream_t s)
 {
-    if (n_items <= 0) return;
+    if (n_particles <= 0 || max_nodes <= 0) return;
+    const dim3 grid((unsigned)n_particles * a.max_classes, (unsigned)((max_nodes * a.R + 63) / 64));
     if (a.fp32)
-        hipLaunchKernelGGL(k_pmat_nodes<float>, dim3(n_items), dim3(64), 0, s, a.eig, a.blen, a.rates,
-                           a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes, items);
+        hipLaunchKernelGGL(k_pmat_nodes<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates, a.num_classes,
+                           a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
     else
-        hipLaunchKernelGGL(k_pmat_nodes<double>, dim3(n_items), dim3(64), 0, s, a.eig, a.blen, a.rates,
-                           a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes, items);
+        hipLaunchKernelGGL(k_pmat_nodes<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates, a.num_classes,
+                           a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
 }
 
 }  // namespace smccat

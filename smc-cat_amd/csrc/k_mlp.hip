@@ -6,8 +6,8 @@
 //                    staged transposed in LDS so the 128 x 128 layer reads it
 //                    conflict-free; keeps z1, h1, z2 (training) and h2.
 //   k_mlp_head       one wavefront per example (4 per wave, 16 per workgroup):
-//                    lane l owns units l and l + 64 and the logits of classes
-//                    l, l + 64, ...; log-softmax over K + 1 slots; in training
+//                    lane l owns units l and l + 64 and scores classes l, l + 64,
+//                    ... in parallel; log-softmax over K + 1 slots; in training
 //                    also the loss, d loss / d h2 and the head's gradient.
 //   k_mlp_trunk_bwd  the trunk's backward pass for one row.
 // Gradients are written per row ("partials", [rows][kNParam]) and summed by
@@ -106,6 +106,11 @@ __device__ __forceinline__ float slot_get(const float (&v)[kMaxCand / 64], int c
     return __shfl(s, c & 63, 64);
 }
 
+// The head works in the class-parallel form l_c = w0 + w . log pi_c with
+// w[a] = sum_j h2[j] U[j][a] and w0 = h2 . u (20 + 1 wave reductions per
+// example); lane l then scores classes l, l + 64, ... independently.  The
+// backward pass uses the same form: with rr = sum_c dl_c log pi_c and
+// sd = sum_c dl_c, dh2 = U rr + u sd + v dl_new and dU = h2 rr^T.
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_mlp_head(int B, const int *__restrict__ idx,
                                                   const float *__restrict__ hdr, const int *__restrict__ Kc,
@@ -145,26 +150,26 @@ __global__ __launch_bounds__(256) void k_mlp_head(int B, const int *__restrict__
         const int K = Kc[ex];
         const float *ce = cls + (size_t)coff[ex] * kCls;
         const float h0 = H2[(size_t)r * kH + j0], h1 = H2[(size_t)r * kH + j1];
+        float w[20];
+#pragma unroll
+        for (int a = 0; a < 20; a++) w[a] = wave_sum(fmaf(h0, U0[a], h1 * U1[a]));
+        const float w0 = wave_sum(fmaf(h0, ub0, h1 * ub1));
+        const float ln = wave_sum(fmaf(h0, v0, h1 * v1)) + bn;
         float lg[NS];
 #pragma unroll
-        for (int k = 0; k < NS; k++) lg[k] = -INFINITY;
-        for (int c = 0; c < K; c++) {
-            const float *e = ce + (size_t)c * kCls;
-            float g0 = ub0, g1 = ub1;
+        for (int k = 0; k < NS; k++) {
+            const int c = 64 * k + lane;
+            lg[k] = -INFINITY;
+            if (c < K) {
+                const float *e = ce + (size_t)c * kCls;
+                float l = w0;
 #pragma unroll
-            for (int a = 0; a < 20; a++) {
-                g0 = fmaf(U0[a], e[a], g0);
-                g1 = fmaf(U1[a], e[a], g1);
+                for (int a = 0; a < 20; a++) l = fmaf(w[a], e[a], l);
+                lg[k] = fmaf(tau, l, e[20]);
+            } else if (c == K) {
+                lg[k] = fmaf(tau, ln, la);
             }
-            const float l = wave_sum(h0 * g0 + h1 * g1);
-#pragma unroll
-            for (int k = 0; k < NS; k++)
-                if (k == (c >> 6) && lane == (c & 63)) lg[k] = fmaf(tau, l, e[20]);
         }
-        const float ln = wave_sum(h0 * v0 + h1 * v1) + bn;
-#pragma unroll
-        for (int k = 0; k < NS; k++)
-            if (k == (K >> 6) && lane == (K & 63)) lg[k] = fmaf(tau, ln, la);
         float m = -INFINITY;
 #pragma unroll
         for (int k = 0; k < NS; k++) m = fmaxf(m, lg[k]);
@@ -183,34 +188,34 @@ __global__ __launch_bounds__(256) void k_mlp_head(int B, const int *__restrict__
             const int lab = label[ex];
             const float llab = slot_get(lg, lab);
             if (lane == 0) loss[r] = lse - llab;
-            float dl[NS];
-#pragma unroll
-            for (int k = 0; k < NS; k++) {
-                const int c = 64 * k + lane;
-                const float d = (c <= K) ? expf(lg[k] - lse) - (c == lab ? 1.f : 0.f) : 0.f;
-                dl[k] = tau * d * inv_B;
-            }
-            float dh0 = 0.f, dh1 = 0.f, sd = 0.f;
             float rr[20];
 #pragma unroll
             for (int a = 0; a < 20; a++) rr[a] = 0.f;
-            for (int c = 0; c < K; c++) {
-                const float *e = ce + (size_t)c * kCls;
-                const float dlc = slot_get(dl, c);
-                float g0 = ub0, g1 = ub1;
+            float sd = 0.f, dln = 0.f;
 #pragma unroll
-                for (int a = 0; a < 20; a++) {
-                    g0 = fmaf(U0[a], e[a], g0);
-                    g1 = fmaf(U1[a], e[a], g1);
-                    rr[a] = fmaf(dlc, e[a], rr[a]);
+            for (int k = 0; k < NS; k++) {
+                const int c = 64 * k + lane;
+                if (c > K) continue;
+                const float dl = tau * (expf(lg[k] - lse) - (c == lab ? 1.f : 0.f)) * inv_B;
+                if (c == K) {
+                    dln = dl;
+                } else {
+                    const float *e = ce + (size_t)c * kCls;
+#pragma unroll
+                    for (int a = 0; a < 20; a++) rr[a] = fmaf(dl, e[a], rr[a]);
+                    sd += dl;
                 }
-                dh0 = fmaf(dlc, g0, dh0);
-                dh1 = fmaf(dlc, g1, dh1);
-                sd += dlc;
             }
-            const float dln = slot_get(dl, K);
-            dh0 = fmaf(dln, v0, dh0);
-            dh1 = fmaf(dln, v1, dh1);
+#pragma unroll
+            for (int a = 0; a < 20; a++) rr[a] = wave_sum(rr[a]);
+            sd = wave_sum(sd);
+            dln = wave_sum(dln);  // held by one lane
+            float dh0 = fmaf(ub0, sd, v0 * dln), dh1 = fmaf(ub1, sd, v1 * dln);
+#pragma unroll
+            for (int a = 0; a < 20; a++) {
+                dh0 = fmaf(U0[a], rr[a], dh0);
+                dh1 = fmaf(U1[a], rr[a], dh1);
+            }
             dH2[(size_t)r * kH + j0] = dh0;
             dH2[(size_t)r * kH + j1] = dh1;
 #pragma unroll

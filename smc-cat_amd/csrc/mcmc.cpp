@@ -238,9 +238,13 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     std::vector<int> move(count), left(count);
     int64_t proposed[N_MOVES] = {}, accepted[N_MOVES] = {};
 
-    for (int it = 0; it < o.iterations; ++it) {
-        int rounds = 0;
-        for (int i = 0; i < count; ++i) {
+    // Each particle walks its own sequence of iterations (move drawn by weight,
+    // then its repetitions); a round advances every particle that still has
+    // steps by one MH step, so a particle never waits for another's longer move
+    // (the chains are the ones a per-particle loop would run).
+    std::vector<int> started(count, 0);
+    auto next_move = [&](int i) {
+        while (left[i] <= 0 && started[i] < o.iterations) {
             double u = rng[i].uniform() * wtot, c = 0.0;
             move[i] = -1;  // stationary (no-op)
             for (int k = 0; k < N_MOVES; ++k) {
@@ -251,9 +255,18 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                 }
             }
             left[i] = move[i] >= 0 ? reps[move[i]] : 0;
-            rounds = std::max(rounds, left[i]);
+            started[i]++;
         }
-        for (int r = 0; r < rounds; ++r) {
+    };
+    for (int i = 0; i < count; ++i) {
+        left[i] = 0;
+        next_move(i);
+    }
+    for (;;) {
+        bool active = false;
+        for (int i = 0; i < count; ++i) active |= left[i] > 0;
+        if (!active) break;
+        {
             std::optional<PhaseTimer> propose_timer(std::in_place, e, SMC_PHASE_PROPOSE);
             for (int i = 0; i < count; ++i) {
                 const int p = first + i;
@@ -333,6 +346,8 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
             }
             rc = smc_accept_reject(e, first, count, acc.data());
             if (rc != SMC_OK) return rc;
+            for (int i = 0; i < count; ++i)
+                if (left[i] <= 0) next_move(i);
         }
     }
     // with probability gibbs_probability: the CAT Gibbs move and a forced refresh

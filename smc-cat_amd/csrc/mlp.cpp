@@ -446,7 +446,8 @@ struct MvState {
     std::vector<std::array<double, 20>> logits;
     std::vector<std::array<double, 20>> pi;  // softmax of logits
     std::vector<std::array<float, 20>> lpi;  // log pi, as the network reads it
-    std::vector<int> orig;                   // class index at upload, -1: created by the move
+    std::vector<int> orig;                   // engine class slot holding the class's eigensystem
+    std::vector<int> free;                   // engine class slots no class uses
     std::vector<int> size;
     std::vector<int32_t> assign;
     double meansum[20];                      // sum over sites of pi of their class
@@ -506,6 +507,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             add_class(ps, lg, k);
         }
         ps.assign.assign(e->h_assign.begin() + (size_t)p * S, e->h_assign.begin() + (size_t)(p + 1) * S);
+        for (int k = Kmax - 1; k >= K; --k) ps.free.push_back(k);
         std::fill(ps.meansum, ps.meansum + 20, 0.0);
         for (int a : ps.assign) {
             ps.size[a]++;
@@ -515,6 +517,18 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     }
     // existing classes are scored with the engine's eigensystems: make them current
     int rc = smc_refresh_eigen(e, count, slots);
+    if (rc != SMC_OK) return rc;
+    // fresh profiles come from a pool of prior draws decomposed in batches (a
+    // stream of their own per slot, consumed in draw order)
+    const int pool_q = 16;
+    std::vector<smccat::Rng> prof_rng;
+    prof_rng.reserve(count);
+    for (int i = 0; i < count; ++i) prof_rng.emplace_back(seed ^ 0x70726f66696c65ull, (uint64_t)slots[i]);
+    auto draw = [&](int i, double *lg) {
+        for (int a = 0; a < 19; ++a) lg[a] = prof_rng[i].normal();
+        lg[19] = 0.0;
+    };
+    rc = smc_pool_refill(e, count, slots, pool_q, draw, 1);
     if (rc != SMC_OK) return rc;
     rc = ensure_batch(m, count);
     if (rc != SMC_OK) return rc;
@@ -539,6 +553,12 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     int64_t updates = 0, proposed = 0, accepted = 0, created = 0, erased = 0;
 
     for (int it = 0; it < iters; ++it) {
+        for (int i = 0; i < count; ++i)
+            if (e->pool.next[i] >= pool_q) {  // a slot used its pool up: top every slot up
+                rc = smc_pool_refill(e, count, slots, pool_q, draw, 0);
+                if (rc != SMC_OK) return rc;
+                break;
+            }
         // remove a random site from its class; build the proposal's example
         for (int i = 0; i < count; ++i) {
             MvState &ps = st[i];
@@ -635,11 +655,11 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             const std::array<double, 20> &pc = was_single[i] ? phi[i] : ps.logits[cur[i]];
             cand.insert(cand.end(), pc.begin(), pc.end());
             cand_cls.push_back(was_single[i] ? phi_orig[i] : ps.orig[cur[i]]);
-            if (pick == K) {  // a fresh profile from the logit prior
-                for (int a = 0; a < 19; ++a) phi_new[i][a] = g.normal();
-                phi_new[i][19] = 0.0;
+            if (pick == K) {  // a fresh profile from the logit prior (the pool)
+                const int slot = e->pool.next[i]++;
+                std::copy_n(&e->pool.logits[((size_t)i * pool_q + slot) * 20], 20, phi_new[i].begin());
                 cand.insert(cand.end(), phi_new[i].begin(), phi_new[i].end());
-                cand_cls.push_back(-1);
+                cand_cls.push_back(-2 - (i * pool_q + slot));
             } else {
                 cand.insert(cand.end(), ps.logits[pick].begin(), ps.logits[pick].end());
                 cand_cls.push_back(ps.orig[pick]);
@@ -673,9 +693,17 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 }
             }
             if (dest == K) {  // the auxiliary class becomes a real class
-                add_class(ps, fresh ? phi_new[i] : phi[i], fresh ? -1 : phi_orig[i]);
-                if (fresh) created++;
+                int slot_k = phi_orig[i];
+                if (fresh) {  // a pool profile: its eigensystem moves into a free class slot
+                    slot_k = ps.free.back();
+                    ps.free.pop_back();
+                    rc = smc_pool_to_class(e, slots[i], -2 - cand_cls[2 * q - 1], slot_k);
+                    if (rc != SMC_OK) return rc;
+                    created++;
+                }
+                add_class(ps, fresh ? phi_new[i] : phi[i], slot_k);
             } else if (was_single[i]) {
+                ps.free.push_back(phi_orig[i]);
                 erased++;
             }
             ps.assign[s] = dest;
