@@ -166,8 +166,23 @@ struct KArgs {
 // One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
 // item `item`.  U is the number of 16-site column blocks processed together
 // (independent MFMA chains for ILP); roots use U = 1.
+// LDS per wavefront for the tips' P matrices of cherry children (2 per cherry).
+template <int M0, int M1>
+struct CherryLds {
+    static constexpr int mats = (M0 == MODE_RT || M1 == MODE_RT) ? 4 : 2 * ((M0 == MODE_CHERRY) + (M1 == MODE_CHERRY));
+    static constexpr int elems = mats > 0 ? mats * kPStride : 1;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename TX, int M0, int M1, bool IS_ROOT, int U>
-__device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int tile_group)
+__device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int tile_group,
+                                           TX (&lds)[4][CherryLds<M0, M1>::elems])
 {
     using V4 = typename Mfma<TX>::v4;
     static_assert(!IS_ROOT || U == 1, "root accumulators are indexed per column block");
@@ -243,8 +258,32 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
         const TX *P1 = Pg + ((size_t)c1 * R + r) * kPStride;
-        const TX *Pa0 = Pg + ((size_t)ga0 * R + r) * kPStride, *Pb0 = Pg + ((size_t)gb0 * R + r) * kPStride;
-        const TX *Pa1 = Pg + ((size_t)ga1 * R + r) * kPStride, *Pb1 = Pg + ((size_t)gb1 * R + r) * kPStride;
+        // the tips' P matrices of cherry children, staged in this wavefront's LDS
+        // (the gathers by tip code then stay out of the vector L1)
+        TX *lw = lds[wave];
+        const TX *Pa0 = lw, *Pb0 = lw + kPStride;
+        const TX *Pa1 = lw + (md0 == MODE_CHERRY ? 2 : 0) * kPStride;
+        const TX *Pb1 = Pa1 + kPStride;
+        if constexpr (CherryLds<M0, M1>::mats > 0) {
+            if (md0 == MODE_CHERRY || md1 == MODE_CHERRY) {
+                wave_sync();  // the previous rate's reads are done
+                const TX *ga = Pg + ((size_t)(md0 == MODE_CHERRY ? ga0 : ga1) * R + r) * kPStride;
+                const TX *gb = Pg + ((size_t)(md0 == MODE_CHERRY ? gb0 : gb1) * R + r) * kPStride;
+                for (int i = lane; i < kPStride; i += 64) {
+                    lw[i] = ga[i];
+                    lw[kPStride + i] = gb[i];
+                }
+                if (md0 == MODE_CHERRY && md1 == MODE_CHERRY) {
+                    const TX *ga_ = Pg + ((size_t)ga1 * R + r) * kPStride;
+                    const TX *gb_ = Pg + ((size_t)gb1 * R + r) * kPStride;
+                    for (int i = lane; i < kPStride; i += 64) {
+                        lw[2 * kPStride + i] = ga_[i];
+                        lw[3 * kPStride + i] = gb_[i];
+                    }
+                }
+                wave_sync();
+            }
+        }
         TX a0[2][5], a1[2][5];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++)
@@ -381,12 +420,13 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
     // unit w runs on XCD w mod X (the grid is a multiple of X); XCD x takes the
     // items x, x + X, ..., each with all its tile groups, so the P matrices and
     // tip codes an item reads are fetched into one L2 only
+    __shared__ TX lds[4][CherryLds<M0, M1>::elems];
     const int bpi = ka.blocks_per_item, X = ka.xcds;
     const int n_work = (n + X - 1) / X * X * bpi;
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
         const int l = w / X;
         const int q = (l / bpi) * X + (w - l * X);
-        if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, l % bpi);
+        if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, l % bpi, lds);
     }
 }
 
