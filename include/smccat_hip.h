@@ -315,6 +315,11 @@ enum {
 int smc_engine_phase_stats(smc_engine *e, double *ms, int64_t *calls);
 int smc_engine_reset_phase_stats(smc_engine *e);
 
+/* Particle::lnPrior (Particle.cpp:484-517) of an uploaded slot: Gamma(nodes-1,
+ * scale 0.1) on the tree length, Gamma(3, 2) on the shape when K_r > 1, N(0,1)
+ * on the 190 exchangeabilities unless using_lg, and the CRP/class terms. */
+int smc_ln_prior(smc_engine *e, int slot, int using_lg, double dpp_alpha, double *out);
+
 /* computeNextStep (Miscellaneous.hpp:42-109) on the GPU.  raw_weights is
  * updated in place, normalized[n] and *ess / *current_temp are written. */
 int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,

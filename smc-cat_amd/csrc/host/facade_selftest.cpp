@@ -3,11 +3,13 @@
 // writeToSerialized, for every particle record of a text file, and prints
 // one line per result for the test suite to compare with the oracle.
 //
-// usage: facade_selftest <fasta> <particles.txt> <num_rates> <max_classes> <lg 0|1> [--cpu]
+// usage: facade_selftest <fasta> <particles.txt> <num_rates> <max_classes> <lg 0|1>
+//                        [--cpu | --moves <out.txt>]
 // particles.txt: blocks of
 //   newick <string> / pinv <v> / shape <v> / base <190 values> / classes <K>
 //   logits <20 values> (K lines) / assign <S values> / end
-// --cpu: Newick parse/write and lnPrior only (no GPU calls).
+// --cpu: Newick parse/write only (no GPU calls).  --moves: also runs the Gibbs
+// move and rejuvenation and writes the resulting records to out.txt.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,6 +105,58 @@ int main(int argc, char **argv)
         printf("resample");
         for (int a : asg) printf(" %d", a);
         printf("\n");
+        if (argc > 7 && std::strcmp(argv[6], "--moves") == 0) {
+            // moves on a second engine with two workspaces: the Gibbs move on slot 0,
+            // then rejuvenation of every slot (main.cpp:296-311); the resulting states
+            // go through writeToSerialized and back into the first engine
+            ParticleEngine eng2(aln, R, (int)recs.size(), K, lg, SMC_PREC_FP64, 0, 2);
+            for (size_t i = 0; i < recs.size(); ++i) Particle(eng2, (int)i).copyFromSerialized(recs[i]);
+            const double g = Particle(eng2, 0).gibbsPartitionMove(0.8);
+            printf("gibbs_return %g\n", g);
+            smc_rejuvenation_options o{};
+            o.iterations = 3;
+            o.using_lg = lg;
+            o.invariant = 1;
+            o.seed = 11;
+            o.scale_delta = o.subtree_scale_delta = o.shape_delta = 1.0;
+            o.rate_delta = 0.25;
+            o.rate_full_scale = 0.5;
+            o.invar_alpha = 100.0;
+            o.gibbs_probability = 0.5;
+            o.dpp_alpha = 0.01;
+            std::vector<double> rl;
+            eng2.rejuvenate(0, (int)recs.size(), 0.5, o, rl);
+            std::ofstream f(argv[7]);
+            for (size_t i = 0; i < recs.size(); ++i) {
+                SerializedParticle back;
+                Particle(eng2, (int)i).writeToSerialized(back);
+                f << "newick " << back.newick << "\npinv ";
+                char buf[64];
+                snprintf(buf, sizeof buf, "%.17g", back.pInvar);
+                f << buf << "\nshape ";
+                snprintf(buf, sizeof buf, "%.17g", back.shape);
+                f << buf << "\nbase";
+                for (double b : back.baseMatrix) {
+                    snprintf(buf, sizeof buf, " %.17g", b);
+                    f << buf;
+                }
+                f << "\nclasses " << back.stationaries.size() << "\n";
+                for (auto &st : back.stationaries) {
+                    f << "logits";
+                    for (double v : st) {
+                        snprintf(buf, sizeof buf, " %.17g", v);
+                        f << buf;
+                    }
+                    f << "\n";
+                }
+                f << "assign";
+                for (int a : back.assignments) f << " " << a;
+                f << "\nend\n";
+                Particle q(eng, (int)i);
+                q.copyFromSerialized(back);
+                printf("rj %zu %.17g %.17g %.17g\n", i, rl[i], q.lnLikelihood(), q.lnPrior());
+            }
+        }
     } catch (const std::exception &ex) {
         fprintf(stderr, "error: %s\n", ex.what());
         return 1;

@@ -202,11 +202,11 @@ std::string writeNewick(const TreeTopology &t)
 
 // ---------------------------------------------------------------- engine
 ParticleEngine::ParticleEngine(const Alignment &a, int numRates, int maxParticles, int maxClasses,
-                               bool usingLG, int precision, int device)
+                               bool usingLG, int precision, int device, int workspaces)
     : aln(a), rates(numRates), kmax(maxClasses), lg(usingLG)
 {
     smc_engine_config cfg{device, a.getNumTaxa(), a.getNumChar(), numRates, maxParticles,
-                          maxClasses, precision, 0, 0, 1, 0};
+                          maxClasses, precision, 0, 0, workspaces, 0};
     check(nullptr, smc_engine_create(&cfg, &eng), "smc_engine_create");
     check(eng, smc_set_alignment(eng, a.data().data()), "smc_set_alignment");
     slots.resize(maxParticles);
@@ -245,6 +245,65 @@ void ParticleEngine::refreshAll(int n, std::vector<double> &out)
     out.resize(n);
     check(eng, smc_refresh_likelihood(eng, 0, n, out.data(), nullptr), "smc_refresh_likelihood");
     for (int i = 0; i < n; ++i) lnls[i] = out[i];
+}
+
+void ParticleEngine::pull(int slot)
+{
+    Slot &sl = slots.at(slot);
+    if (!sl.valid) throw std::runtime_error("slot not uploaded");
+    const int T = aln.getNumTaxa(), S = aln.getNumChar();
+    TreeTopology &t = sl.tree;
+    t.children.assign((size_t)(T - 1) * 2, 0);
+    t.branchLen.assign(2 * T - 1, 0.0);
+    double pinv = 0.0, shape = 1.0;
+    check(eng, smc_get_particle_params(eng, slot, t.children.data(), t.branchLen.data(),
+                                       sl.sp.baseMatrix.data(), &pinv, &shape),
+          "smc_get_particle_params");
+    std::vector<int> parent(2 * T - 1, -1);
+    for (int k = 0; k < T - 1; ++k) {
+        parent[t.children[2 * k]] = T + k;
+        parent[t.children[2 * k + 1]] = T + k;
+    }
+    for (int n = T; n < 2 * T - 1; ++n)
+        if (parent[n] < 0) t.root = n;
+    int32_t K = 0;
+    std::vector<double> logits((size_t)kmax * 20);
+    sl.sp.assignments.assign(S, 0);
+    check(eng, smc_get_particle_state(eng, slot, &K, logits.data(), sl.sp.assignments.data()),
+          "smc_get_particle_state");
+    sl.sp.stationaries.assign(K, {});
+    for (int k = 0; k < K; ++k)
+        for (int i = 0; i < 20; ++i) sl.sp.stationaries[k][i] = logits[(size_t)k * 20 + i];
+    sl.sp.categorySize.assign(K, 0);
+    for (int a : sl.sp.assignments) sl.sp.categorySize[a]++;
+    sl.sp.pInvar = pinv;
+    sl.sp.shape = shape;
+    sl.sp.newick = writeNewick(t);
+}
+
+void ParticleEngine::rejuvenate(int first, int count, double tempering, const smc_rejuvenation_options &opt,
+                                std::vector<double> &out)
+{
+    out.resize(count);
+    check(eng, smc_rejuvenate(eng, first, count, tempering, &opt, out.data(), nullptr), "smc_rejuvenate");
+    for (int i = 0; i < count; ++i) {
+        lnls[first + i] = out[i];
+        pull(first + i);
+    }
+}
+
+smc_sampler_result ParticleEngine::runSampler(const smc_sampler_options &opt, std::vector<double> &out,
+                                              std::vector<double> &rawWeights)
+{
+    smc_sampler_result res{};
+    out.resize(opt.num_particles);
+    rawWeights.resize(opt.num_particles);
+    check(eng, smc_run_sampler(eng, &opt, &res, out.data(), rawWeights.data()), "smc_run_sampler");
+    for (int i = 0; i < opt.num_particles; ++i) {
+        lnls[i] = out[i];
+        pull(i);
+    }
+    return res;
 }
 
 // ---------------------------------------------------------------- Particle
@@ -287,30 +346,18 @@ void Particle::refreshLikelihood(bool)
 
 double Particle::lnPrior()
 {
-    const auto &s = e.slots.at(slot);
-    const int T = (int)s.tree.names.size(), nn = 2 * T - 1;
-    auto log_gamma_pdf = [](double k, double theta, double x) {
-        return (k - 1.0) * std::log(x) - x / theta - std::lgamma(k) - k * std::log(theta);
-    };
-    auto log_norm = [](double z) { return -0.5 * z * z - 0.5 * std::log(2.0 * M_PI); };
-    double len = 0.0;
-    for (int n = 0; n < nn; ++n)
-        if (n != s.tree.root) len += s.tree.branchLen[n];
-    double lnP = log_gamma_pdf((double)(nn - 1), 0.1, len);  // tree prior, rate 10
-    if (e.numRates() > 1) lnP += log_gamma_pdf(3.0, 2.0, s.sp.shape);
-    // GTR prior: LG particles carry the fixed table and add nothing (Particle.cpp:500-505)
-    if (!e.usingLG())
-        for (double b : s.sp.baseMatrix) lnP += log_norm(b);
-    const int K = (int)s.sp.stationaries.size();
-    lnP += std::log(dppAlpha) * K;
-    std::vector<int> sizes(K, 0);
-    for (int a : s.sp.assignments) sizes[a]++;
-    for (int k = 0; k < K; ++k) {
-        for (double z : s.sp.stationaries[k]) lnP += log_norm(z);
-        lnP += std::lgamma((double)sizes[k]);
-    }
-    lnP += std::lgamma(dppAlpha) - std::lgamma(dppAlpha + (double)s.sp.assignments.size());
-    return lnP;
+    double lp = 0.0;
+    check(e.handle(), smc_ln_prior(e.handle(), slot, e.usingLG() ? 1 : 0, dppAlpha, &lp), "smc_ln_prior");
+    return lp;
+}
+
+double Particle::gibbsPartitionMove(double tempering)
+{
+    const int32_t sl = slot;
+    check(e.handle(), smc_gibbs_partition_move(e.handle(), 1, &sl, tempering, e.nextSeed(), 0, dppAlpha, nullptr),
+          "smc_gibbs_partition_move");
+    e.pull(slot);
+    return INFINITY;  // Particle.cpp:994
 }
 
 int Particle::getNumCategories() const { return (int)e.slots.at(slot).sp.stationaries.size(); }

@@ -71,8 +71,11 @@ std::string writeNewick(const TreeTopology &t);
 class ParticleEngine {
 public:
     // usingLG: particles carry the fixed LG table (no GTR prior term), as Particle's `lg`.
+    // workspaces = 2 keeps two partial buffers per particle (accept/reject, as the
+    // reference's CL double buffer) and enables rejuvenate / runSampler.
     ParticleEngine(const Alignment &aln, int numRates, int maxParticles, int maxClasses,
-                   bool usingLG = false, int precision = SMC_PREC_FP64, int device = 0);
+                   bool usingLG = false, int precision = SMC_PREC_FP64, int device = 0,
+                   int workspaces = 1);
     ~ParticleEngine();
     ParticleEngine(const ParticleEngine &) = delete;
     ParticleEngine &operator=(const ParticleEngine &) = delete;
@@ -81,6 +84,16 @@ public:
     void upload(int slot, const SerializedParticle &sp);
     // Full likelihood of slots [0, n) in one batched sweep; lnl resized to n.
     void refreshAll(int n, std::vector<double> &lnl);
+    // mcmc(p, r, T) + optional gibbsPartitionMove for slots [first, first+count)
+    // (main.cpp:296-311) in one batched call; lnl resized to count.
+    void rejuvenate(int first, int count, double tempering, const smc_rejuvenation_options &opt,
+                    std::vector<double> &lnl);
+    // The SMC loop of main.cpp:143-336 over slots [0, opt.num_particles).
+    smc_sampler_result runSampler(const smc_sampler_options &opt, std::vector<double> &lnl,
+                                  std::vector<double> &rawWeights);
+    // Re-reads a slot's state from the engine after moves made there.
+    void pull(int slot);
+    uint64_t nextSeed() { return ++seedCounter; }
     double lnl(int slot) const { return lnls.at(slot); }
 
     smc_engine *handle() { return eng; }
@@ -103,6 +116,7 @@ private:
     smc_engine *eng = nullptr;
     int rates, kmax;
     bool lg;
+    uint64_t seedCounter = 0;
 };
 
 class Particle {
@@ -113,6 +127,9 @@ public:
     void refreshLikelihood(bool forceUpdate = false);  // Particle.cpp:530-675
     double lnLikelihood() { return currentLnLikelihood; }  // Particle.cpp:480-482
     double lnPrior();                                  // Particle.cpp:484-517
+    // Particle::gibbsPartitionMove (Particle.cpp:785-995) on this slot; returns
+    // +infinity as the reference does (the move is always accepted).
+    double gibbsPartitionMove(double tempering);
     int getNumCategories() const;
     int getNumNodes() const { return 2 * e.alignment().getNumTaxa() - 1; }
     int getNumRates() const { return e.numRates(); }

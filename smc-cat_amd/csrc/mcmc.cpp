@@ -392,3 +392,20 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         }
     return SMC_OK;
 }
+
+extern "C" int smc_ln_prior(smc_engine *e, int slot, int using_lg, double dpp_alpha, double *out)
+{
+    if (!e || !out) return SMC_ERR_ARG;
+    if (slot < 0 || slot >= e->Pmax || !e->h_valid[slot])
+        return smc_fail(e, SMC_ERR_ARG, "slot not uploaded");
+    if (!(dpp_alpha > 0.0)) return smc_fail(e, SMC_ERR_ARG, "dpp_alpha must be > 0");
+    const int Ti = e->T - 1, nn = e->nn;
+    Params q{std::vector<int32_t>(e->h_children.begin() + (size_t)slot * Ti * 2,
+                                  e->h_children.begin() + (size_t)(slot + 1) * Ti * 2),
+             std::vector<double>(e->h_blen.begin() + (size_t)slot * nn, e->h_blen.begin() + (size_t)(slot + 1) * nn),
+             std::vector<double>(e->h_base.begin() + (size_t)slot * 190,
+                                 e->h_base.begin() + (size_t)(slot + 1) * 190),
+             e->h_pinv[slot], e->h_shape[slot]};
+    *out = ln_prior_params(e, q, using_lg != 0) + ln_prior_classes(e, slot, dpp_alpha);
+    return SMC_OK;
+}

@@ -120,6 +120,7 @@ SIGNATURES = {
     "smc_engine_autotune": (_I, [_P, _I, _I, _I, ctypes.POINTER(Tuning)]),
     "smc_engine_set_graphs": (_I, [_P, _I]),
     "smc_engine_graph_stats": (_I, [_P, _LP, _LP]),
+    "smc_ln_prior": (_I, [_P, _I, _I, _D, _DP]),
     "smc_run_sampler": (_I, [_P, ctypes.POINTER(SamplerOptions), ctypes.POINTER(SamplerResult), _DP, _DP]),
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),

@@ -310,6 +310,13 @@ class Engine:
             self._check(self.lib.smc_engine_reset_phase_stats(self.h), "smc_engine_reset_phase_stats")
         return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.PHASES)}
 
+    def ln_prior(self, slot, using_lg=False, dpp_alpha=0.01):
+        """Particle::lnPrior of an uploaded slot."""
+        out = ctypes.c_double()
+        self._check(self.lib.smc_ln_prior(self.h, int(slot), int(using_lg), float(dpp_alpha),
+                                          ctypes.byref(out)), "smc_ln_prior")
+        return out.value
+
     def particle_params(self, slot):
         """(children [T-1, 2], branch_len [2T-1], base [190], pinv, shape) of a slot."""
         T = self.taxa
