@@ -4,7 +4,7 @@
 // Device layout (all resident in HBM, sized at create time):
 //   codes    uint8  [T][S_pad]                 alignment, original column order
 //   perm     int32  [P][S_pad]                 slot -> column, class-sorted per particle
-//   tiles    Tile   [P][tile_stride]           <= 64 slots of one class each
+//   tiles    Tile   [P][tile_stride]           one class's slots in an aligned 64-slot window
 //   eig      f64    [P*Kmax][840]              A | lam | B | pi per class
 //   Pmat     TX     [C][2T-1][K_r][20x21]      transition matrices (+ row sums) of the
 //                                              C = pmat_classes classes of one wave
@@ -559,10 +559,14 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
         for (int k = 0; k < Kp; ++k) cnt[k + 1] += cnt[k];
         std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
         for (int s = 0; s < S; ++s) perm[(size_t)i * Sp + fillp[as[s]]++] = s;
+        // tiles: each class's slots cut at the 64-slot window boundaries (at most
+        // ceil(S / 64) windows plus one split window per class boundary)
         int nt = 0;
         for (int k = 0; k < Kp; ++k)
-            for (int st = cnt[k]; st < cnt[k + 1]; st += kTile) {
-                tiles[(size_t)i * ts + nt++] = Tile{st, std::min(kTile, cnt[k + 1] - st), k, 0};
+            for (int w = cnt[k] / kTile * kTile; w < cnt[k + 1]; w += kTile) {
+                const int lo = std::max(cnt[k], w) - w, hi = std::min(cnt[k + 1], w + kTile) - w;
+                if (nt >= ts) return fail(e, SMC_ERR_STATE, "tile table overflow");
+                tiles[(size_t)i * ts + nt++] = Tile{w, hi, k, lo};
             }
         ntiles[i] = nt;
         e->h_valid[p] = 1;
@@ -676,7 +680,7 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
             TimedLaunch tl(e, SMC_K_PRUNE);
             a.kind = kind;
             a.roots = roots;
-            a.unroll = roots ? 1 : e->tune.unroll[prune_stored(kind)];
+            a.unroll = roots ? e->tune.unroll_rt : e->tune.unroll[prune_stored(kind)];
             if (device_buckets) {
                 a.items = e->d_items;
                 a.bucket_off = e->d_bucket_off + w.boff;
@@ -980,7 +984,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<std::vector<Item>> buckets;  // (level * PRUNE_KINDS + kind)
     std::vector<Item> roots;
     std::vector<uint8_t> dirty(nn);
-    std::vector<int> parent(nn);
+    std::vector<int> parent(nn), dlevel(nn, 0), dl_order;
     std::vector<Item> p_nodes;
     std::vector<int> keep;
     std::vector<double> keep_lnl;
@@ -1024,10 +1028,35 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             if (dirty[n])
                 for (int q = parent[n]; q >= 0 && !dirty[q]; q = parent[q]) dirty[q] = 1;
         const int root = e->h_root[p];
+        // launch level of a dirty node: one above the highest level among its dirty
+        // children that are stored partials (a cherry is built by its parent, a tip
+        // needs nothing), so the k-th node from the bottom of every particle's dirty
+        // path shares one launch; a node's height in the tree would spread a path
+        // over as many launches as the tree is tall.  The dirty nodes form a subtree
+        // holding the root (every ancestor of a dirty node is dirty): walk it in
+        // pre-order, then assign levels in reverse, children first.
+        dl_order.clear();
+        if (dirty[root]) dl_order.push_back(root);
+        for (size_t i = 0; i < dl_order.size(); ++i) {
+            const int v = dl_order[i];
+            for (int j = 0; j < 2; ++j) {
+                const int c = ch[2 * (v - T) + j];
+                if (c >= T && dirty[c]) dl_order.push_back(c);
+            }
+        }
+        for (size_t i = dl_order.size(); i-- > 0;) {
+            const int v = dl_order[i];
+            int L = 1;
+            for (int j = 0; j < 2; ++j) {
+                const int c = ch[2 * (v - T) + j];
+                if (c >= T && dirty[c] && prune_kind(ch, T, c) != PRUNE_TT) L = std::max(L, dlevel[c] + 1);
+            }
+            dlevel[v] = L;
+        }
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
-            const int L = e->h_height[(size_t)p * Ti + (n - T)];
+            const int L = dlevel[n];
             const int kind = prune_kind(ch, T, n);
             if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
             buckets[PRUNE_KINDS * L + kind].push_back(Item{p, n});
@@ -1069,7 +1098,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.n_items = n;
         a.kind = kind;
         a.roots = is_roots;
-        a.unroll = is_roots ? 1 : e->tune.unroll[prune_stored(kind)];
+        a.unroll = (is_roots || kind == PRUNE_ANY) ? e->tune.unroll_rt : e->tune.unroll[prune_stored(kind)];
         if (e->prune_variant == SMC_PRUNE_VALU)
             launch_prune(a, s);
         else
@@ -1464,6 +1493,7 @@ int smc_engine_get_tuning(smc_engine *e, smc_tuning *t)
     t->blocks_per_cu = e->tune.blocks_per_cu;
     for (int k = 0; k < 3; ++k) t->unroll[k] = e->tune.unroll[k];
     t->xcd_groups = e->tune.xcd_groups;
+    t->unroll_runtime = e->tune.unroll_rt;
     t->ms_per_sweep = 0.0;
     t->configs_tried = 0;
     return SMC_OK;
@@ -1473,14 +1503,16 @@ int smc_engine_set_tuning(smc_engine *e, const smc_tuning *t)
 {
     if (!e || !t) return SMC_ERR_ARG;
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(e, SMC_ERR_ARG, "blocks_per_cu out of 1..64");
-    for (int k = 0; k < 3; ++k)
-        if (t->unroll[k] != 1 && t->unroll[k] != 2 && t->unroll[k] != 4)
-            return fail(e, SMC_ERR_ARG, "unroll must be 1, 2 or 4");
+    for (int k = 0; k < 4; ++k) {
+        const int u = k < 3 ? t->unroll[k] : t->unroll_runtime;
+        if (u != 1 && u != 2 && u != 4) return fail(e, SMC_ERR_ARG, "unroll must be 1, 2 or 4");
+    }
     if (t->xcd_groups != 1 && t->xcd_groups != 2 && t->xcd_groups != 4 && t->xcd_groups != 8)
         return fail(e, SMC_ERR_ARG, "xcd_groups must be 1, 2, 4 or 8");
     e->tune.blocks_per_cu = t->blocks_per_cu;
     for (int k = 0; k < 3; ++k) e->tune.unroll[k] = t->unroll[k];
     e->tune.xcd_groups = t->xcd_groups;
+    e->tune.unroll_rt = t->unroll_runtime;
     return SMC_OK;
 }
 
@@ -1550,11 +1582,12 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
             best = t;
         }
     }
-    for (int k = 0; k < 3 && rc == SMC_OK; ++k)
+    for (int k = 0; k < 4 && rc == SMC_OK; ++k)
         for (int u : unrolls) {
-            if (u == best.unroll[k]) continue;
+            int &cur = k < 3 ? best.unroll[k] : best.unroll_rt;
+            if (u == cur) continue;
             smc_engine::Tune t = best;
-            t.unroll[k] = u;
+            (k < 3 ? t.unroll[k] : t.unroll_rt) = u;
             double ms;
             if ((rc = measure(t, &ms)) != SMC_OK) break;
             if (ms < best_ms) {
@@ -1575,6 +1608,7 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
         out->blocks_per_cu = best.blocks_per_cu;
         for (int k = 0; k < 3; ++k) out->unroll[k] = best.unroll[k];
         out->xcd_groups = best.xcd_groups;
+        out->unroll_runtime = best.unroll_rt;
         out->ms_per_sweep = best_ms;
         out->configs_tried = tried;
     }

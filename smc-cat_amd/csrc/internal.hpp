@@ -18,12 +18,16 @@ constexpr int kPStride = 420;      // P row-major with a 21st column: P[i][20] =
 
 extern const double kLG190[190];
 
-// A pruning tile: `len` consecutive (class-sorted) site slots of one class.
+// A pruning tile: the slots of class `cls` inside the 64-slot window that starts at
+// slot `start` (a multiple of 64) of the class-sorted slot order, i.e. window
+// offsets [lo, len).  Windows are aligned so that every 16-slot column block starts
+// on a cache-line boundary; a window holding the end of one class and the start of
+// the next is two tiles.
 struct Tile {
     int start;
     int len;
     int cls;
-    int pad;
+    int lo;
 };
 
 // One (particle, internal node) pair of a pruning level.

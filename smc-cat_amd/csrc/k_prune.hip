@@ -134,8 +134,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (tile >= ntiles[p]) return;  // whole wavefront exits; no block barrier follows
     const Tile tl = tiles[(size_t)p * tile_stride + tile];
     const int lane = threadIdx.x & 63;
-    const bool active = lane < tl.len;
-    const int s = tl.start + (active ? lane : 0);
+    const bool active = lane >= tl.lo && lane < tl.len;
+    const int s = tl.start + (active ? lane : tl.lo);
     const int orig = perm[(size_t)p * S_pad + s];
     const size_t g = (size_t)p * max_classes + tl.cls;
     const int nn = 2 * T - 1;

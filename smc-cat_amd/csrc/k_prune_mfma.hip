@@ -6,8 +6,10 @@
 // mixture with rate weights 1 and the invariant term fused at the root).
 //
 // Per (particle, node, rate, child) the contraction is Y[20 x n] = P[20 x 20] X[20 x n]
-// over the n sites of a tile.  One wavefront owns one tile of <= 64 class-sorted
-// site slots and walks it in 16-site column blocks:
+// over the n sites of a tile.  One wavefront owns one tile -- the slots of one
+// class inside a 64-slot aligned window of the class-sorted slot order, so that
+// every column block starts on a cache-line boundary -- and walks it in 16-site
+// column blocks:
 //   A operand: P rows 16*rb + (lane & 15), columns 4*kb + (lane >> 4)   (rb < 2, kb < 5)
 //              rows 20..31 are zero padding, so 62.5% of the MFMA work is useful;
 //   B operand: partial rows 4*kb + (lane >> 4) at site 16*sb + (lane & 15);
@@ -30,6 +32,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 namespace smccat {
 
@@ -93,6 +96,34 @@ __device__ __forceinline__ int cherry_b(const TX *__restrict__ Pa, const TX *__r
         b[kb] = valid ? Pa[row * 21 + ca] * Pb[row * 21 + cb] : (TX)0;
     }
     return 0;
+}
+
+template <typename TX>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using t = __attribute__((ext_vector_type(2))) float;
+};
+template <>
+struct Vec2<double> {
+    using t = __attribute__((ext_vector_type(2))) double;
+};
+using int2v = __attribute__((ext_vector_type(2))) int;
+
+// B operands of two adjacent slots (xpair: the even one) with one vector load per
+// row; a slot outside its tile contributes zeros
+template <typename TX>
+__device__ __forceinline__ void stored_b2(const TX *__restrict__ xpair, int S_pad, bool v0, bool v1, bool any,
+                                          int g4, TX (&b0)[5], TX (&b1)[5])
+{
+    using T2 = typename Vec2<TX>::t;
+#pragma unroll
+    for (int kb = 0; kb < 5; kb++) {
+        T2 x = {(TX)0, (TX)0};
+        if (any) x = *(const T2 *)(xpair + (size_t)(4 * kb + g4) * S_pad);
+        b0[kb] = v0 ? x.x : (TX)0;
+        b1[kb] = v1 ? x.y : (TX)0;
+    }
 }
 
 template <typename TX>
@@ -165,7 +196,7 @@ struct KArgs {
 
 // One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
 // item `item`.  U is the number of 16-site column blocks processed together
-// (independent MFMA chains for ILP); roots use U = 1.
+// (independent MFMA chains and loads in flight); a tile has at most 4 blocks.
 // LDS per wavefront for the tips' P matrices of cherry children (2 per cherry).
 template <int M0, int M1>
 struct CherryLds {
@@ -185,7 +216,6 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                                            TX (&lds)[4][CherryLds<M0, M1>::elems])
 {
     using V4 = typename Mfma<TX>::v4;
-    static_assert(!IS_ROOT || U == 1, "root accumulators are indexed per column block");
     const Item *__restrict__ items = ka.items;
     const Tile *__restrict__ tiles = ka.tiles;
     const int *__restrict__ ntiles = ka.ntiles;
@@ -245,7 +275,18 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     auto nslot = [&](int node) { return (xf ? (int)xf[node - T] * (T - 1) : 0) + node - T; };
     const TX *Pg = Pm + ((size_t)pcls_off[p] + tl.cls) * nn * R * kPStride;
     const double *pi = eig + g * kEigStride + kEigPi;
-    const int nsb = (tl.len + 15) >> 4;  // 16-site column blocks in this tile (wave-uniform)
+    // Column mapping.  Unpaired (U = 1): lane c16 of block sb holds window offset
+    // 16 sb + c16.  Paired (U >= 2): blocks 2j and 2j + 1 of a step hold the even and
+    // odd offsets of one 32-slot run, so a lane owns two adjacent slots and reads and
+    // writes them with one 8-byte (FP32) or 16-byte (FP64) access per row: 16 lanes
+    // then cover a whole 128-byte line (FP32) where single slots cover half of one.
+    constexpr bool PAIRED = U >= 2;
+    using T2 = typename Vec2<TX>::t;
+    auto site_of = [](int sb0, int u, int c) {
+        return PAIRED ? 16 * sb0 + 32 * (u >> 1) + 2 * c + (u & 1) : 16 * (sb0 + u) + c;
+    };
+    const int nsb = (tl.len + 15) >> 4;               // 16-slot column blocks of the window in use
+    const int sb_begin = ((tl.lo >> 4) / U) * U;      // the first step holding a valid slot
 
     // running log-sum-exp over rates of the root's 4 column blocks
     double lse_m[4], lse_s[4];
@@ -300,38 +341,62 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         const int *E1 = Ep + ((size_t)cs1 * R + r) * S_pad;
 
 #pragma unroll 1
-        for (int sb0 = 0; sb0 < nsb; sb0 += U) {
+        for (int sb0 = sb_begin; sb0 < nsb; sb0 += U) {
             V4 y[U][2], w[U][2];
             int slot[U], etot[U];
             bool valid[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int o = 16 * (sb0 + u) + c16;
-                valid[u] = o < tl.len;
-                slot[u] = tl.start + (valid[u] ? o : 0);
-                const int orig = perm[(size_t)p * S_pad + slot[u]];
-                auto col_of = [&](int tip) {
-                    const int code = codes[(size_t)tip * S_pad + orig];
-                    return code < 20 ? code : 20;
-                };
-                int col0 = 20, col1 = 20, ca0 = 20, cb0 = 20, ca1 = 20, cb1 = 20;
-                if (md0 == MODE_TIP) col0 = col_of(c0);
-                if (md1 == MODE_TIP) col1 = col_of(c1);
-                if (md0 == MODE_CHERRY) {
-                    ca0 = col_of(ga0);
-                    cb0 = col_of(gb0);
-                }
-                if (md1 == MODE_CHERRY) {
-                    ca1 = col_of(ga1);
-                    cb1 = col_of(gb1);
-                }
-                etot[u] = child_block<TX, M0>(md0, a0, P0, Pa0, Pb0, X0 + slot[u], E0 + slot[u], S_pad, col0, ca0,
-                                              cb0, valid[u], g4, y[u]);
-                etot[u] += child_block<TX, M1>(md1, a1, P1, Pa1, Pb1, X1 + slot[u], E1 + slot[u], S_pad, col1,
-                                               ca1, cb1, valid[u], g4, w[u]);
+                const int o = site_of(sb0, u, c16);
+                valid[u] = o >= tl.lo && o < tl.len;
+                slot[u] = tl.start + (valid[u] ? o : tl.lo);
             }
+            // one child into acc[U][2]; stored children of paired launches read both
+            // slots of a lane with one vector load per row
+            auto child = [&](auto mode_c, int md, const TX(&a)[2][5], const TX *Pc, const TX *Pa, const TX *Pb,
+                             const TX *Xc, const int *Ec, int c, int ga, int gb, V4(&acc)[U][2], int(&ex)[U]) {
+                constexpr int MC = decltype(mode_c)::value;
+                if constexpr (PAIRED) {
+                    if (MC == MODE_STORED || (MC == MODE_RT && md == MODE_STORED)) {
+#pragma unroll
+                        for (int j = 0; j < U / 2; j++) {
+                            const int pb = tl.start + site_of(sb0, 2 * j, c16);  // even, in the window
+                            TX b0[5], b1[5];
+                            const bool any = valid[2 * j] || valid[2 * j + 1];
+                            stored_b2<TX>(Xc + pb, S_pad, valid[2 * j], valid[2 * j + 1], any, g4, b0, b1);
+                            int2v ev = {0, 0};
+                            if (any) ev = *(const int2v *)(Ec + pb);
+                            mma_block<TX>(a, b0, acc[2 * j]);
+                            mma_block<TX>(a, b1, acc[2 * j + 1]);
+                            ex[2 * j] = ev.x;
+                            ex[2 * j + 1] = ev.y;
+                        }
+                        return;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int orig = perm[(size_t)p * S_pad + slot[u]];
+                    auto col_of = [&](int tip) {
+                        const int code = codes[(size_t)tip * S_pad + orig];
+                        return code < 20 ? code : 20;
+                    };
+                    int col = 20, ca = 20, cb = 20;
+                    if (md == MODE_TIP) col = col_of(c);
+                    if (md == MODE_CHERRY) {
+                        ca = col_of(ga);
+                        cb = col_of(gb);
+                    }
+                    ex[u] = child_block<TX, MC>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, col, ca, cb,
+                                                valid[u], g4, acc[u]);
+                }
+            };
+            int e1[U];
+            child(std::integral_constant<int, M0>{}, md0, a0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, y, etot);
+            child(std::integral_constant<int, M1>{}, md1, a1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, w, e1);
 #pragma unroll
             for (int u = 0; u < U; u++) {
+                etot[u] += e1[u];
                 // z = y * w and the per-site max over the 20 real rows (4 lane groups x 8 values)
                 TX m = 0;
 #pragma unroll
@@ -353,20 +418,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                     for (int v = 0; v < 4; v++)
                         y[u][rb][v] = ok ? (TX)ldexp(y[u][rb][v], -ex) : nanv;
                 etot[u] += ok ? ex : 0;
-                if constexpr (!IS_ROOT) {
-                    if (valid[u]) {
-                        const int ns = nslot(n);
-                        TX *xo = Xp + ((size_t)ns * R + r) * 20 * S_pad + slot[u];
-#pragma unroll
-                        for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-                            for (int v = 0; v < 4; v++) {
-                                const int row = Mfma<TX>::crow(rb, g4, v);
-                                if (row < 20) xo[(size_t)row * S_pad] = y[u][rb][v];
-                            }
-                        if (g4 == 0) Ep[((size_t)ns * R + r) * S_pad + slot[u]] = etot[u];
-                    }
-                } else {
+                if constexpr (IS_ROOT) {
                     double dot = 0.0;
 #pragma unroll
                     for (int rb = 0; rb < 2; rb++)
@@ -385,6 +437,53 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                         if (k == sb0 + u) lse_push(lse_m[k], lse_s[k], l, mix_mode);
                 }
             }
+            if constexpr (!IS_ROOT) {
+                TX *xo = Xp + ((size_t)nslot(n) * R + r) * 20 * S_pad;
+                int *eo = Ep + ((size_t)nslot(n) * R + r) * S_pad;
+                if constexpr (PAIRED) {
+#pragma unroll
+                    for (int j = 0; j < U / 2; j++) {
+                        const int pb = tl.start + site_of(sb0, 2 * j, c16);
+                        const bool v0 = valid[2 * j], v1 = valid[2 * j + 1];
+#pragma unroll
+                        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int row = Mfma<TX>::crow(rb, g4, v);
+                                if (row >= 20) continue;
+                                TX *q = xo + (size_t)row * S_pad + pb;
+                                if (v0 && v1) {
+                                    *(T2 *)q = T2{y[2 * j][rb][v], y[2 * j + 1][rb][v]};
+                                } else if (v0) {
+                                    q[0] = y[2 * j][rb][v];
+                                } else if (v1) {
+                                    q[1] = y[2 * j + 1][rb][v];
+                                }
+                            }
+                        if (g4 == 0) {
+                            if (v0 && v1)
+                                *(int2v *)(eo + pb) = int2v{etot[2 * j], etot[2 * j + 1]};
+                            else if (v0)
+                                eo[pb] = etot[2 * j];
+                            else if (v1)
+                                eo[pb + 1] = etot[2 * j + 1];
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (!valid[u]) continue;
+#pragma unroll
+                        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int row = Mfma<TX>::crow(rb, g4, v);
+                                if (row < 20) xo[(size_t)row * S_pad + slot[u]] = y[u][rb][v];
+                            }
+                        if (g4 == 0) eo[slot[u]] = etot[u];
+                    }
+                }
+            }
         }
     }
     if constexpr (IS_ROOT) {
@@ -392,8 +491,8 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         const double pv = pinv[p];
 #pragma unroll
         for (int sb = 0; sb < 4; sb++) {
-            const int o = 16 * sb + c16;
-            if (g4 == 0 && o < tl.len) {
+            const int o = site_of(sb / U * U, sb % U, c16);
+            if (g4 == 0 && o >= tl.lo && o < tl.len) {
                 const int orig = perm[(size_t)p * S_pad + tl.start + o];
                 lf += site_log_likelihood(lse_m[sb], lse_s[sb], pv, pi[invchar[orig]], isinv[orig],
                                           mix_mode);
@@ -452,6 +551,18 @@ void launch_pair(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStre
         launch_one<TX, M0, M1, false, 1>(a, ka, grid, s);
 }
 
+// launches whose child modes are decided per item (roots, merged small buckets)
+template <typename TX, bool IS_ROOT>
+void launch_rt(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
+{
+    if (a.unroll >= 4)
+        launch_one<TX, MODE_RT, MODE_RT, IS_ROOT, 4>(a, ka, grid, s);
+    else if (a.unroll == 2)
+        launch_one<TX, MODE_RT, MODE_RT, IS_ROOT, 2>(a, ka, grid, s);
+    else
+        launch_one<TX, MODE_RT, MODE_RT, IS_ROOT, 1>(a, ka, grid, s);
+}
+
 template <typename TX>
 void launch_typed(const PruneArgs &a, hipStream_t s)
 {
@@ -466,7 +577,7 @@ void launch_typed(const PruneArgs &a, hipStream_t s)
     int64_t units = a.bucket_off ? INT64_MAX : (int64_t)(a.n_items + X - 1) / X * X * bpi;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
     if (a.roots) {
-        launch_one<TX, MODE_RT, MODE_RT, true, 1>(a, ka, grid, s);
+        launch_rt<TX, true>(a, ka, grid, s);
         return;
     }
     switch (a.kind) {
@@ -476,7 +587,7 @@ void launch_typed(const PruneArgs &a, hipStream_t s)
         case PRUNE_CC: launch_pair<TX, MODE_CHERRY, MODE_CHERRY>(a, ka, grid, s); break;
         case PRUNE_CS: launch_pair<TX, MODE_CHERRY, MODE_STORED>(a, ka, grid, s); break;
         case PRUNE_SS: launch_pair<TX, MODE_STORED, MODE_STORED>(a, ka, grid, s); break;
-        default: launch_one<TX, MODE_RT, MODE_RT, false, 1>(a, ka, grid, s); break;
+        default: launch_rt<TX, false>(a, ka, grid, s); break;
     }
 }
 

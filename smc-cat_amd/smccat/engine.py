@@ -272,10 +272,14 @@ class Engine:
     def tuning(self):
         t = _lib.Tuning()
         self._check(self.lib.smc_engine_get_tuning(self.h, ctypes.byref(t)), "smc_engine_get_tuning")
-        return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll), "xcd_groups": t.xcd_groups}
+        return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll), "xcd_groups": t.xcd_groups,
+                "unroll_runtime": t.unroll_runtime}
 
-    def set_tuning(self, blocks_per_cu, unroll, xcd_groups=8):
-        t = _lib.Tuning(int(blocks_per_cu), (ctypes.c_int32 * 3)(*unroll), 0.0, 0, int(xcd_groups))
+    def set_tuning(self, blocks_per_cu, unroll, xcd_groups=8, unroll_runtime=None):
+        if unroll_runtime is None:
+            unroll_runtime = self.tuning()["unroll_runtime"]
+        t = _lib.Tuning(int(blocks_per_cu), (ctypes.c_int32 * 3)(*unroll), 0.0, 0, int(xcd_groups),
+                        int(unroll_runtime))
         self._check(self.lib.smc_engine_set_tuning(self.h, ctypes.byref(t)), "smc_engine_set_tuning")
 
     def autotune(self, first=0, count=None, sweeps=3):
@@ -286,7 +290,8 @@ class Engine:
         self._check(self.lib.smc_engine_autotune(self.h, int(first), int(count), int(sweeps), ctypes.byref(t)),
                     "smc_engine_autotune")
         return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll), "xcd_groups": t.xcd_groups,
-                "ms_per_sweep": t.ms_per_sweep, "configs_tried": t.configs_tried}
+                "unroll_runtime": t.unroll_runtime, "ms_per_sweep": t.ms_per_sweep,
+                "configs_tried": t.configs_tried}
 
     def set_graphs(self, enable):
         self._check(self.lib.smc_engine_set_graphs(self.h, int(bool(enable))), "smc_engine_set_graphs")
