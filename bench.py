@@ -326,7 +326,8 @@ def main():
                          "learned-proposal benchmark")
     ap.add_argument("--no-ess", action="store_true", help="skip the SMC ESS/sec comparison")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--unroll", default="", help="fixed launch shape instead of autotuning: U0,U1,U2")
+    ap.add_argument("--unroll", default="",
+                    help="fixed launch shape instead of autotuning: U0,U1,U2[,U_runtime]")
     ap.add_argument("--blocks-per-cu", type=int, default=32)
     ap.add_argument("--xcd-groups", type=int, default=8)
     ap.add_argument("--no-graphs", action="store_true", help="time eager launches instead of graph replays")
@@ -394,7 +395,8 @@ def main():
     progress("sweep workload %s: %d particles uploaded" % (args.config, N))
     tune = None
     if args.variant == "mfma" and args.unroll:
-        eng.set_tuning(args.blocks_per_cu, [int(x) for x in args.unroll.split(",")], args.xcd_groups)
+        u = [int(x) for x in args.unroll.split(",")]
+        eng.set_tuning(args.blocks_per_cu, u[:3], args.xcd_groups, unroll_runtime=u[3] if len(u) > 3 else None)
     elif args.variant == "mfma" and not args.no_autotune:
         tune = eng.autotune(0, N, sweeps=2)
         progress("autotune: %s" % tune)

@@ -79,7 +79,7 @@ static void jacobi_sym(double S[400], double V[400])
                 tot += v;
                 if (i != j) off += v;
             }
-        if (off <= 1e-34 * tot) break;
+        if (off <= 1e-30 * tot) break;  /* the rounding floor is about 1e-32 of tot */
         for (int p = 0; p < NS - 1; p++) {
             for (int q = p + 1; q < NS; q++) {
                 double apq = S[p * NS + q];
