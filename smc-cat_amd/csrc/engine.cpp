@@ -291,10 +291,29 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
             if (depth < 0) return fail(e, SMC_ERR_ARG, "internal error: scoring program");
             if (n != root) maxd = std::max(maxd, ++depth);
         }
-        if ((int)prog.size() != Ti || maxd > kSiteStack)
+        if ((int)prog.size() != Ti || (maxd > kSiteStack && T > kSiteLevelMaxTaxa))
             return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) +
                                             ": tree too deep for single-site scoring");
         std::copy(prog.begin(), prog.end(), prog_out);
+    }
+    // the same nodes by height (counting sort, node order within a level), for the
+    // level-parallel scoring kernel: level h holds the nodes whose children all
+    // lie below it
+    {
+        int H = 0;
+        for (int n = T; n < nn; ++n) H = std::max(H, h[n]);
+        int *lv = prog_out + Ti, *off = prog_out + 2 * Ti + 1;
+        prog_out[2 * Ti] = H;
+        std::fill(off, off + H + 1, 0);
+        for (int n = T; n < nn; ++n) off[h[n]]++;  // off[h] counts height h (1..H)
+        int run = 0;
+        for (int l = 0; l <= H; ++l) {  // off[l] becomes the start of height l + 1
+            const int c = l + 1 <= H ? off[l + 1] : 0;
+            off[l] = run;
+            run += c;
+        }
+        std::vector<int> fill(off, off + H);
+        for (int n = T; n < nn; ++n) lv[fill[h[n] - 1]++] = n;
     }
     *root_out = root;
     for (int k = 0; k < Ti; ++k) heights[k] = h[T + k];
@@ -372,7 +391,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_nonfinite, P);
     A(&e->d_items, P * (T - 1));
     A(&e->d_pcls_off, P);
-    A(&e->d_prog, P * (T - 1));
+    A(&e->d_prog, P * site_prog_stride(T));
     A(&e->d_plist, 2 * P);
     A(&e->d_pnodes, P * nn);
     const size_t W = e->Wmax, C = (size_t)e->Cmax;
@@ -513,7 +532,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     std::vector<double> blen((size_t)count * nn), rates((size_t)count * R);
     std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
     std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
-    std::vector<int> progs((size_t)count * Ti);
+    std::vector<int> progs((size_t)count * site_prog_stride(T));
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const int32_t *ch = children + (size_t)i * Ti * 2;
@@ -521,7 +540,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
         std::vector<int> h(nn, 0);
         {
             std::vector<int> hh(Ti);
-            int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * Ti);
+            int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * site_prog_stride(T));
             if (rc != SMC_OK) return rc;
             for (int k = 0; k < Ti; ++k) h[T + k] = hh[k];
         }
@@ -607,7 +626,7 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_ntiles + P0, ntiles.data(), ntiles.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * Ti, progs.data(), progs.size() * sizeof(int),
+    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     e->class_version++;
@@ -902,14 +921,14 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
 {
     const int T = e->T, Ti = T - 1, nn = e->nn, R = e->R;
     std::vector<double> blen((size_t)count * nn), rates((size_t)count * R, 1.0);
-    std::vector<int> progs((size_t)count * Ti), roots(count);
+    std::vector<int> progs((size_t)count * site_prog_stride(T)), roots(count);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
         const int32_t *ch = children + (size_t)i * Ti * 2;
         int root = -1;
         std::vector<int> hh(Ti);
-        int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * Ti);
+        int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * site_prog_stride(T));
         if (rc != SMC_OK) return rc;
         if (R > 1) {
             if (!(shape[i] > 0.0)) return fail(e, SMC_ERR_ARG, "shape must be > 0");
@@ -946,7 +965,7 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_root + P0, roots.data(), roots.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * Ti, progs.data(), progs.size() * sizeof(int),
+    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipStreamSynchronize(s));
     e->topo_version++;

@@ -90,7 +90,7 @@ struct smc_engine {
     std::vector<float> h_sitefreq;        // [S][20] amino-acid frequencies of each column
     std::vector<double> h_rates;          // [P][R] discrete-gamma rates of h_shape
     int *d_pcls_off = nullptr;    // [P] compact class slot of (particle, class 0) in its wave
-    int *d_prog = nullptr;        // [P][T-1] Strahler-ordered post-order of internal nodes
+    int *d_prog = nullptr;        // [P][3T] scoring programs: Strahler post-order, height levels
     uint8_t *d_xflag = nullptr;   // [P][T-1] current workspace of each internal node
     int *d_plist = nullptr;       // [P] particle list scratch for partial recomputes
     // accept/reject state (Particle.cpp:343-478): committed flags, lnL and parameters

@@ -139,6 +139,11 @@ void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores
 
 // Single-site x candidate-profile scoring (Gibbs / learned allocation proposal).
 constexpr int kSiteStack = 32;  // LDS stack entries for live internal partials
+// Per-particle scoring program ([P][3T] ints): the Strahler-ordered post-order of
+// the internal nodes (T-1), the internal nodes sorted by height (T-1), the level
+// count H, then H+1 offsets of the height levels in the sorted list.
+__host__ __device__ inline int site_prog_stride(int T) { return 3 * T; }
+constexpr int kSiteLevelMaxTaxa = 320;  // the level kernel keeps every internal partial in LDS
 struct SiteClassArgs {
     const int *ent_particle;  // [count] particle of each entry
     const int *ent_site;      // [count] alignment column of each entry
@@ -146,7 +151,7 @@ struct SiteClassArgs {
     int n_cand;
     const double *cand_eig;   // [count][n_cand][840]
     const int *children;      // [P][T-1][2]
-    const int *prog;          // [P][T-1] internal nodes, Strahler-ordered post-order
+    const int *prog;          // [P][3T] scoring programs (site_prog_stride)
     const uint8_t *codes;     // [T][S_pad]
     const double *blen;       // [P][nn]
     const double *rates;      // [P][R]

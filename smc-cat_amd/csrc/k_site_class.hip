@@ -2,9 +2,11 @@
 // profiles: the per-site, per-class conditional that the Gibbs partition move
 // (Particle.cpp:857-941) and the learned allocation proposal both need.
 //
-// One wavefront scores one (entry, candidate, rate): entry = (particle, site).
-// The tree is walked in a Strahler-ordered post-order prepared on the host, so
-// only O(log T) internal partials are live; they sit on a small LDS stack.
+// k_site_class_lv (trees up to kSiteLevelMaxTaxa taxa) walks the tree by height
+// levels with twelve nodes in flight; k_site_class, for larger trees, is one
+// wavefront per (entry, candidate, rate): entry = (particle, site), walking a
+// Strahler-ordered post-order prepared on the host, so only O(log T) internal
+// partials are live; they sit on a small LDS stack.
 // P(t) is never formed: a child's contribution is A (e^{lam t r} (B x)),
 // 2 x 400 FMA per child instead of the 8,000 of a back-transform.  Lane i
 // (< 20) owns state i and keeps row i of A and B and lam_i in registers.
@@ -19,6 +21,14 @@ namespace smccat {
 namespace {
 
 constexpr double kLn2s = 0.69314718055994530942;
+
+// LDS written by lanes of one wavefront is visible to its other lanes after this
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __global__ __launch_bounds__(64) void k_site_class(
     const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
@@ -52,7 +62,7 @@ __global__ __launch_bounds__(64) void k_site_class(
     const double rate = rates[(size_t)p * R + r];
     const int nn = 2 * T - 1;
     const int *ch = children + (size_t)p * (T - 1) * 2;
-    const int *pg = prog + (size_t)p * (T - 1);
+    const int *pg = prog + (size_t)p * site_prog_stride(T);
     const double *bl = blen + (size_t)p * nn;
 
     int top = 0;  // stack size (wave-uniform)
@@ -133,6 +143,118 @@ __global__ __launch_bounds__(64) void k_site_class(
     if (lane == 0) rate_lnl[id] = log(d) + (double)ez * kLn2s;
 }
 
+// Level-parallel form: one 256-lane workgroup per (entry, candidate, rate).  Each
+// wavefront holds three 20-lane groups (lanes 60-63 idle), twelve per workgroup;
+// the groups share the nodes of one height level, so a level costs one node's
+// latency and a C4-sized tree (200 taxa, height about 17) takes about 30 steps
+// instead of 199.  Every internal partial stays in LDS; a group exchanges the
+// vector between its two matrix-vector products inside its own wavefront.  The
+// arithmetic of every node is the same as in k_site_class, in the same order.
+constexpr int kLvGroups = 12;
+
+__global__ __launch_bounds__(256) void k_site_class_lv(
+    const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
+    const int *__restrict__ ent_ncand, int n_cand, const double *__restrict__ cand_eig,
+    const int *__restrict__ children, const int *__restrict__ prog, const uint8_t *__restrict__ codes,
+    const double *__restrict__ blen, const double *__restrict__ rates,
+    double *__restrict__ rate_lnl, int T, int R, int S_pad)
+{
+    extern __shared__ double lv_smem[];
+    const int Ti = T - 1;
+    double *zs = lv_smem;                    // [T-1][20] scaled partials by node - T
+    double *tmp = zs + (size_t)Ti * 20;      // [12][20] per-group exchange
+    int *ezs = (int *)(tmp + kLvGroups * 20);  // [T-1] cumulative exponents
+
+    const int id = blockIdx.x;  // (entry * n_cand + cand) * R + r
+    const int r = id % R;
+    const int ec = id / R;
+    const int cand = ec % n_cand, entry = ec / n_cand;
+    if (cand >= ent_ncand[entry]) return;  // whole workgroup exits; no barrier crossed
+    const int p = ent_particle[entry], site = ent_site[entry];
+    const int tid = threadIdx.x, wl = tid & 63;
+    const int gw = wl / 20;                // group within the wavefront (3: idle lanes)
+    const bool act = gw < 3;
+    const int i = act ? wl - 20 * gw : 0;  // state owned by this lane
+    const int g = (tid >> 6) * 3 + gw;     // group within the workgroup
+    double *tg = tmp + (act ? g : 0) * 20;
+    const double *eg = cand_eig + (size_t)ec * kEigStride;
+    double arow[20], brow[20];
+#pragma unroll
+    for (int k = 0; k < 20; k++) {
+        arow[k] = eg[i * 20 + k];
+        brow[k] = eg[kEigB + i * 20 + k];
+    }
+    const double lam = eg[kEigLam + i];
+    const double rate = rates[(size_t)p * R + r];
+    const int nn = 2 * T - 1;
+    const int *ch = children + (size_t)p * Ti * 2;
+    const int *lvn = prog + (size_t)p * site_prog_stride(T) + Ti;  // nodes by height
+    const int H = lvn[Ti];
+    const int *off = lvn + Ti + 1;
+    const double *bl = blen + (size_t)p * nn;
+
+    for (int h = 0; h < H; h++) {
+        if (act) {
+            for (int k = off[h] + g; k < off[h + 1]; k += kLvGroups) {
+                const int n = lvn[k];  // uniform over the group's 20 lanes
+                const int c0 = ch[(n - T) * 2], c1 = ch[(n - T) * 2 + 1];
+                double y = 1.0;
+                int e = 0;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const int child = c ? c1 : c0;
+                    double u = 0.0;
+                    if (child < T) {
+                        const int code = codes[(size_t)child * S_pad + site];
+                        if (code < 20) {
+                            u = eg[kEigB + i * 20 + code];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 20; q++) u += brow[q];
+                        }
+                    } else {
+                        const double *zc = zs + (size_t)(child - T) * 20;
+#pragma unroll
+                        for (int q = 0; q < 20; q++) u = fma(brow[q], zc[q], u);
+                        e += ezs[child - T];
+                    }
+                    const double v = exp(lam * bl[child] * rate) * u;
+                    tg[i] = v;
+                    wave_sync_lds();
+                    double yc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 20; q++) yc = fma(arow[q], tg[q], yc);
+                    wave_sync_lds();  // the group's reads are done before tg is rewritten
+                    y *= yc;
+                }
+                // binary-exponent scaling of the 20 states
+                tg[i] = y;
+                wave_sync_lds();
+                double m = 0.0;
+#pragma unroll
+                for (int q = 0; q < 20; q++) m = fmax(m, tg[q]);
+                wave_sync_lds();
+                int ex;
+                (void)frexp(m, &ex);
+                const bool ok = m > 0.0 && isfinite(m);
+                zs[(size_t)(n - T) * 20 + i] = ok ? ldexp(y, -ex) : NAN;
+                if (i == 0) ezs[n - T] = e + (ok ? ex : 0);
+            }
+        }
+        __syncthreads();  // this level's partials are visible to the next
+    }
+    // root (the one node of the top level): l_r = log(pi . z) + e ln 2
+    if (tid < 64) {
+        const int root = lvn[Ti - 1];
+        const double d0 = tid < 20 ? eg[kEigPi + tid] * zs[(size_t)(root - T) * 20 + tid] : 0.0;
+        double d = d0;
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+        d += __shfl_xor(d, 32, 64);
+        if (tid == 0) rate_lnl[id] = log(d) + (double)ezs[root - T] * kLn2s;
+    }
+}
+
 __global__ void k_site_class_fold(const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
                                   const int *__restrict__ ent_ncand, int n_cand,
                                   const double *__restrict__ cand_eig,
@@ -161,9 +283,16 @@ void launch_site_class(const SiteClassArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return;
     const unsigned blocks = (unsigned)((size_t)a.count * a.n_cand * a.R);
-    hipLaunchKernelGGL(k_site_class, dim3(blocks), dim3(64), 0, s, a.ent_particle, a.ent_site,
-                       a.ent_ncand, a.n_cand, a.cand_eig, a.children, a.prog, a.codes, a.blen,
-                       a.rates, a.rate_lnl, a.T, a.R, a.S_pad);
+    if (a.T <= kSiteLevelMaxTaxa) {
+        const size_t lds = ((size_t)(a.T - 1) * 20 + kLvGroups * 20) * sizeof(double) + (size_t)(a.T - 1) * sizeof(int);
+        hipLaunchKernelGGL(k_site_class_lv, dim3(blocks), dim3(256), lds, s, a.ent_particle, a.ent_site,
+                           a.ent_ncand, a.n_cand, a.cand_eig, a.children, a.prog, a.codes, a.blen,
+                           a.rates, a.rate_lnl, a.T, a.R, a.S_pad);
+    } else {
+        hipLaunchKernelGGL(k_site_class, dim3(blocks), dim3(64), 0, s, a.ent_particle, a.ent_site,
+                           a.ent_ncand, a.n_cand, a.cand_eig, a.children, a.prog, a.codes, a.blen,
+                           a.rates, a.rate_lnl, a.T, a.R, a.S_pad);
+    }
     const int n = a.count * a.n_cand;
     hipLaunchKernelGGL(k_site_class_fold, dim3((n + 127) / 128), dim3(128), 0, s, a.ent_particle,
                        a.ent_site, a.ent_ncand, a.n_cand, a.cand_eig, a.rate_lnl, a.pinv, a.isinv,
