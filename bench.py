@@ -270,6 +270,8 @@ def rejuv_bench(args):
                                    "host_phase_seconds": phases, "mh_stats": stats.tolist(),
                                    "collect_seconds": t_col, "train_seconds": t_train,
                                    "final_train_loss": float(losses[-1]) if len(losses) else None}
+    if npl < 1:  # fast path only (profiling)
+        return out
     # plain path: FP64, one particle at a time; the Gibbs move is timed on its own
     # (one particle, a tenth of its S/2 iterations, scaled) and weighted by its
     # probability 0.1, so a small sample gives the expected cost
