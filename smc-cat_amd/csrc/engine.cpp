@@ -418,6 +418,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_children.assign(P * (T - 1) * 2, 0);
     e->h_root.assign(P, -1);
     e->h_height.assign(P * (T - 1), 0);
+    e->h_prog.assign(P * site_prog_stride(T), 0);
     e->h_valid.assign(P, 0);
     e->h_nclass.assign(P, 1);
     e->h_blen.assign(P * nn, 0.0);
@@ -549,6 +550,8 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
             e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
             e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
         }
+        std::copy_n(progs.data() + (size_t)i * site_prog_stride(T), site_prog_stride(T),
+                    e->h_prog.data() + (size_t)p * site_prog_stride(T));
         e->h_root[p] = root;
         roots[i] = root;
         for (int n = 0; n < nn; ++n) {
@@ -926,20 +929,27 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
         const int p = first + i;
         if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
         const int32_t *ch = children + (size_t)i * Ti * 2;
+        const int ps = site_prog_stride(T);
+        int *prog = progs.data() + (size_t)i * ps;
         int root = -1;
-        std::vector<int> hh(Ti);
-        int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * site_prog_stride(T));
-        if (rc != SMC_OK) return rc;
+        if (std::equal(ch, ch + Ti * 2, &e->h_children[(size_t)p * Ti * 2])) {
+            // same topology (branch-length, rate and invariant moves, restores of
+            // them): its root, heights and programs are the ones already derived
+            root = e->h_root[p];
+            std::copy_n(&e->h_prog[(size_t)p * ps], ps, prog);
+        } else {
+            std::vector<int> hh(Ti);
+            int rc = smc_topology(e, p, ch, &root, hh.data(), prog);
+            if (rc != SMC_OK) return rc;
+            std::copy(hh.begin(), hh.end(), &e->h_height[(size_t)p * Ti]);
+            std::copy_n(prog, ps, &e->h_prog[(size_t)p * ps]);
+            std::copy(ch, ch + Ti * 2, &e->h_children[(size_t)p * Ti * 2]);
+        }
         if (R > 1) {
             if (!(shape[i] > 0.0)) return fail(e, SMC_ERR_ARG, "shape must be > 0");
             if (shape[i] != e->h_shape[p])  // rates of an unchanged shape are kept
                 discretize_gamma(shape[i], R, &e->h_rates[(size_t)p * R]);
             std::copy_n(&e->h_rates[(size_t)p * R], R, &rates[(size_t)i * R]);
-        }
-        for (int k = 0; k < Ti; ++k) {
-            e->h_height[(size_t)p * Ti + k] = hh[k];
-            e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
-            e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
         }
         e->h_root[p] = root;
         roots[i] = root;

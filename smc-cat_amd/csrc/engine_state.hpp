@@ -79,6 +79,7 @@ struct smc_engine {
     std::vector<int> h_children;  // [P][T-1][2]
     std::vector<int> h_root;      // [P]
     std::vector<int> h_height;    // [P][T-1]
+    std::vector<int> h_prog;      // [P][3T] host copy of the scoring programs
     std::vector<int> h_valid;     // [P] particle uploaded
     std::vector<int> h_nclass;    // [P] classes per particle
     // host copies of the uploaded particle state (for moves that edit it)
