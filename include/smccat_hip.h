@@ -285,7 +285,7 @@ typedef struct {
     int32_t xcd_groups;      /* 1, 2, 4 or 8 (default 8): the workgroups of one item are
                                 kept on one XCD (workgroup i runs on XCD i mod 8), so the
                                 P matrices and codes it reads are fetched into one L2 */
-    int32_t unroll_runtime;  /* 1, 2 or 4 (default 4): unroll of the launches that decide the
+    int32_t unroll_runtime;  /* 1, 2 or 4 (default 2): unroll of the launches that decide the
                                 child modes per item (roots, and the merged small buckets of
                                 incremental refreshes) */
 } smc_tuning;

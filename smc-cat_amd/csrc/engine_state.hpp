@@ -135,7 +135,7 @@ struct smc_engine {
         int blocks_per_cu = 32;       // persistent grid of the pruning launches: CUs x this
         int unroll[3] = {2, 1, 1};    // 16-site column blocks per step, by stored children (0, 1, 2)
         int xcd_groups = 8;           // an item's workgroups share one of this many XCDs
-        int unroll_rt = 4;            // the same for launches that decide child modes per item
+        int unroll_rt = 2;            // the same for launches that decide child modes per item
     };
     Tune tune;
     int num_cus = 256;
