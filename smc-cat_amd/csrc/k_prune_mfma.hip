@@ -93,7 +93,8 @@ __device__ __forceinline__ int cherry_b(const TX *__restrict__ Pa, const TX *__r
 #pragma unroll
     for (int kb = 0; kb < 5; kb++) {
         const int row = 4 * kb + g4;
-        b[kb] = valid ? Pa[row * 21 + ca] * Pb[row * 21 + cb] : (TX)0;
+        const TX x = Pa[row * 21 + ca] * Pb[row * 21 + cb];
+        b[kb] = valid ? x : (TX)0;
     }
     return 0;
 }
@@ -113,24 +114,27 @@ using int2v = __attribute__((ext_vector_type(2))) int;
 // B operands of two adjacent slots (xpair: the even one) with one vector load per
 // row; a slot outside its tile contributes zeros
 template <typename TX>
-__device__ __forceinline__ void stored_b2(const TX *__restrict__ xpair, int S_pad, bool v0, bool v1, bool any,
-                                          int g4, TX (&b0)[5], TX (&b1)[5])
+__device__ __forceinline__ void stored_b2(const TX *__restrict__ xpair, int S_pad, bool v0, bool v1, int g4,
+                                          TX (&b0)[5], TX (&b1)[5])
 {
     using T2 = typename Vec2<TX>::t;
 #pragma unroll
     for (int kb = 0; kb < 5; kb++) {
-        T2 x = {(TX)0, (TX)0};
-        if (any) x = *(const T2 *)(xpair + (size_t)(4 * kb + g4) * S_pad);
+        const T2 x = *(const T2 *)(xpair + (size_t)(4 * kb + g4) * S_pad);
         b0[kb] = v0 ? x.x : (TX)0;
         b1[kb] = v1 ? x.y : (TX)0;
     }
 }
 
+// (xcol must be a readable slot even when `valid` is false: the load is unconditional)
 template <typename TX>
 __device__ __forceinline__ void stored_b(const TX *__restrict__ xcol, int S_pad, bool valid, int g4, TX (&b)[5])
 {
 #pragma unroll
-    for (int kb = 0; kb < 5; kb++) b[kb] = valid ? xcol[(size_t)(4 * kb + g4) * S_pad] : (TX)0;
+    for (int kb = 0; kb < 5; kb++) {
+        const TX x = xcol[(size_t)(4 * kb + g4) * S_pad];
+        b[kb] = valid ? x : (TX)0;
+    }
 }
 
 template <typename TX>
@@ -144,31 +148,6 @@ __device__ __forceinline__ void mma_block(const TX (&a)[2][5], const TX (&b)[5],
         for (int kb = 0; kb < 5; kb++) acc = Mfma<TX>::mma(a[rb][kb], b[kb], acc);
         y[rb] = acc;
     }
-}
-
-// One child over one column block: Y (C layout) and the exponent it adds.
-// `mode` is the child's mode (wave-uniform; MODE is it when known at compile time).
-template <typename TX, int MODE>
-__device__ __forceinline__ int child_block(int mode, const TX (&a)[2][5], const TX *__restrict__ Pc,
-                                           const TX *__restrict__ Pa, const TX *__restrict__ Pb,
-                                           const TX *__restrict__ xcol, const int *__restrict__ ecol,
-                                           int S_pad, int col, int ca, int cb, bool valid, int g4,
-                                           typename Mfma<TX>::v4 (&y)[2])
-{
-    if (MODE == MODE_TIP || (MODE == MODE_RT && mode == MODE_TIP)) {
-        tip_block<TX>(Pc, col, g4, y);
-        return 0;
-    }
-    TX b[5];
-    int e = 0;
-    if (MODE == MODE_CHERRY || (MODE == MODE_RT && mode == MODE_CHERRY)) {
-        e = cherry_b<TX>(Pa, Pb, ca, cb, valid, g4, b);
-    } else {
-        stored_b<TX>(xcol, S_pad, valid, g4, b);
-        e = *ecol;
-    }
-    mma_block<TX>(a, b, y);
-    return e;
 }
 
 // Kernel arguments (one struct, passed by value).
@@ -296,6 +275,11 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         lse_s[sb] = 0.0;
     }
 
+    // workspaces of the stored children and of this node (read once: the flags do
+    // not change during a launch)
+    const int cs0 = md0 == MODE_STORED ? nslot(c0) : 0, cs1 = md1 == MODE_STORED ? nslot(c1) : 0;
+    const int ns = IS_ROOT ? 0 : nslot(n);
+
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
         const TX *P1 = Pg + ((size_t)c1 * R + r) * kPStride;
@@ -334,7 +318,6 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 a0[rb][kb] = (md0 != MODE_TIP && row < 20) ? P0[row * 21 + 4 * kb + g4] : (TX)0;
                 a1[rb][kb] = (md1 != MODE_TIP && row < 20) ? P1[row * 21 + 4 * kb + g4] : (TX)0;
             }
-        const int cs0 = md0 == MODE_STORED ? nslot(c0) : 0, cs1 = md1 == MODE_STORED ? nslot(c1) : 0;
         const TX *X0 = Xp + ((size_t)cs0 * R + r) * 20 * S_pad;
         const TX *X1 = Xp + ((size_t)cs1 * R + r) * 20 * S_pad;
         const int *E0 = Ep + ((size_t)cs0 * R + r) * S_pad;
@@ -349,30 +332,35 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
             for (int u = 0; u < U; u++) {
                 const int o = site_of(sb0, u, c16);
                 valid[u] = o >= tl.lo && o < tl.len;
-                slot[u] = tl.start + (valid[u] ? o : tl.lo);
+                slot[u] = tl.start + (valid[u] ? o : tl.lo);  // a lane outside the tile reads a slot inside it
             }
-            // one child into acc[U][2]; stored children of paired launches read both
-            // slots of a lane with one vector load per row
-            auto child = [&](auto mode_c, int md, const TX(&a)[2][5], const TX *Pc, const TX *Pa, const TX *Pb,
-                             const TX *Xc, const int *Ec, int c, int ga, int gb, V4(&acc)[U][2], int(&ex)[U]) {
+            // Phase 1: every load of the step for both children -- stored partial rows
+            // and exponents, tip codes and their P columns, cherry products -- issued
+            // before any MFMA, so the two children's memory latencies overlap.  The
+            // loads are unconditional (every lane's slot is inside the tile's window)
+            // and lanes outside the tile are zeroed afterwards.
+            auto fetch = [&](auto mode_c, int md, const TX *Pc, const TX *Pa, const TX *Pb, const TX *Xc,
+                             const int *Ec, int c, int ga, int gb, TX(&bv)[U][5], int(&ex)[U], V4(&acc)[U][2]) {
                 constexpr int MC = decltype(mode_c)::value;
-                if constexpr (PAIRED) {
-                    if (MC == MODE_STORED || (MC == MODE_RT && md == MODE_STORED)) {
+                if (MC == MODE_STORED || (MC == MODE_RT && md == MODE_STORED)) {
+                    if constexpr (PAIRED) {
 #pragma unroll
                         for (int j = 0; j < U / 2; j++) {
                             const int pb = tl.start + site_of(sb0, 2 * j, c16);  // even, in the window
-                            TX b0[5], b1[5];
-                            const bool any = valid[2 * j] || valid[2 * j + 1];
-                            stored_b2<TX>(Xc + pb, S_pad, valid[2 * j], valid[2 * j + 1], any, g4, b0, b1);
-                            int2v ev = {0, 0};
-                            if (any) ev = *(const int2v *)(Ec + pb);
-                            mma_block<TX>(a, b0, acc[2 * j]);
-                            mma_block<TX>(a, b1, acc[2 * j + 1]);
+                            stored_b2<TX>(Xc + pb, S_pad, valid[2 * j], valid[2 * j + 1], g4, bv[2 * j],
+                                          bv[2 * j + 1]);
+                            const int2v ev = *(const int2v *)(Ec + pb);
                             ex[2 * j] = ev.x;
                             ex[2 * j + 1] = ev.y;
                         }
-                        return;
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            stored_b<TX>(Xc + slot[u], S_pad, valid[u], g4, bv[u]);
+                            ex[u] = Ec[slot[u]];
+                        }
                     }
+                    return;
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -381,19 +369,27 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                         const int code = codes[(size_t)tip * S_pad + orig];
                         return code < 20 ? code : 20;
                     };
-                    int col = 20, ca = 20, cb = 20;
-                    if (md == MODE_TIP) col = col_of(c);
-                    if (md == MODE_CHERRY) {
-                        ca = col_of(ga);
-                        cb = col_of(gb);
+                    ex[u] = 0;
+                    if (MC == MODE_TIP || (MC == MODE_RT && md == MODE_TIP)) {
+                        tip_block<TX>(Pc, col_of(c), g4, acc[u]);
+                    } else {
+                        cherry_b<TX>(Pa, Pb, col_of(ga), col_of(gb), valid[u], g4, bv[u]);
                     }
-                    ex[u] = child_block<TX, MC>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, col, ca, cb,
-                                                valid[u], g4, acc[u]);
                 }
             };
+            TX bv0[U][5], bv1[U][5];
             int e1[U];
-            child(std::integral_constant<int, M0>{}, md0, a0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, y, etot);
-            child(std::integral_constant<int, M1>{}, md1, a1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, w, e1);
+            fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, bv0, etot, y);
+            fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, bv1, e1, w);
+            // Phase 2: the products of the children that are not tips
+            if (M0 != MODE_TIP && md0 != MODE_TIP) {
+#pragma unroll
+                for (int u = 0; u < U; u++) mma_block<TX>(a0, bv0[u], y[u]);
+            }
+            if (M1 != MODE_TIP && md1 != MODE_TIP) {
+#pragma unroll
+                for (int u = 0; u < U; u++) mma_block<TX>(a1, bv1[u], w[u]);
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 etot[u] += e1[u];
@@ -438,8 +434,8 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 }
             }
             if constexpr (!IS_ROOT) {
-                TX *xo = Xp + ((size_t)nslot(n) * R + r) * 20 * S_pad;
-                int *eo = Ep + ((size_t)nslot(n) * R + r) * S_pad;
+                TX *xo = Xp + ((size_t)ns * R + r) * 20 * S_pad;
+                int *eo = Ep + ((size_t)ns * R + r) * S_pad;
                 if constexpr (PAIRED) {
 #pragma unroll
                     for (int j = 0; j < U / 2; j++) {
