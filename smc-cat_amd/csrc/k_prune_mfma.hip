@@ -381,7 +381,12 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
             int e1[U];
             fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, bv0, etot, y);
             fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, bv1, e1, w);
-            // Phase 2: the products of the children that are not tips
+            // Phase 2: the products of the children that are not tips.  The scheduling
+            // groups ask the compiler to issue the step's vector loads before its MFMAs
+            // (it otherwise interleaves them to shorten register live ranges, which
+            // serialises the two children's memory latencies).
+            __builtin_amdgcn_sched_group_barrier(0x020, 12 * U, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 20 * U, 0);
             if (M0 != MODE_TIP && md0 != MODE_TIP) {
 #pragma unroll
                 for (int u = 0; u < U; u++) mma_block<TX>(a0, bv0[u], y[u]);
