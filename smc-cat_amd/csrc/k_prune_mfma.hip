@@ -150,6 +150,31 @@ __device__ __forceinline__ void mma_block(const TX (&a)[2][5], const TX (&b)[5],
     }
 }
 
+// One child of a run-time-mode item over one column block: its loads, then its
+// products into Y (C layout); returns the exponent the child adds.
+template <typename TX>
+__device__ __forceinline__ int child_block_rt(int mode, const TX (&a)[2][5], const TX *__restrict__ Pc,
+                                              const TX *__restrict__ Pa, const TX *__restrict__ Pb,
+                                              const TX *__restrict__ xcol, const int *__restrict__ ecol,
+                                              int S_pad, int col, int ca, int cb, bool valid, int g4,
+                                              typename Mfma<TX>::v4 (&y)[2])
+{
+    if (mode == MODE_TIP) {
+        tip_block<TX>(Pc, col, g4, y);
+        return 0;
+    }
+    TX b[5];
+    int e = 0;
+    if (mode == MODE_CHERRY) {
+        e = cherry_b<TX>(Pa, Pb, ca, cb, valid, g4, b);
+    } else {
+        stored_b<TX>(xcol, S_pad, valid, g4, b);
+        e = *ecol;
+    }
+    mma_block<TX>(a, b, y);
+    return e;
+}
+
 // Kernel arguments (one struct, passed by value).
 template <typename TX>
 struct KArgs {
@@ -176,12 +201,25 @@ struct KArgs {
 // One unit of work: tile group `tile_group` (4 tiles, one per wavefront) of
 // item `item`.  U is the number of 16-site column blocks processed together
 // (independent MFMA chains and loads in flight); a tile has at most 4 blocks.
-// LDS per wavefront for the tips' P matrices of cherry children (2 per cherry).
+// LDS per wavefront for the tips' P matrices of cherry children (2 per cherry).  (A
+// tip child's own P gathered from LDS as well was measured: no gain in FP32, slower in
+// FP64.)
+__host__ __device__ constexpr int lds_mats(int mode) { return mode == MODE_CHERRY || mode == MODE_RT ? 2 : 0; }
 template <int M0, int M1>
 struct CherryLds {
-    static constexpr int mats = (M0 == MODE_RT || M1 == MODE_RT) ? 4 : 2 * ((M0 == MODE_CHERRY) + (M1 == MODE_CHERRY));
+    static constexpr int mats = lds_mats(M0) + lds_mats(M1);
     static constexpr int elems = mats > 0 ? mats * kPStride : 1;
 };
+
+// Copy of one P matrix (kPStride values, 16-byte aligned at both ends) into LDS.
+template <typename TX>
+__device__ __forceinline__ void stage_p(TX *__restrict__ dst, const TX *__restrict__ src, int lane)
+{
+    using Q = __attribute__((ext_vector_type(4))) unsigned int;
+    constexpr int n = kPStride * (int)sizeof(TX) / 16;
+    static_assert(kPStride * sizeof(TX) % 16 == 0, "P matrices must be whole 16-byte blocks");
+    for (int i = lane; i < n; i += 64) ((Q *)dst)[i] = ((const Q *)src)[i];
+}
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -283,28 +321,22 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
         const TX *P1 = Pg + ((size_t)c1 * R + r) * kPStride;
-        // the tips' P matrices of cherry children, staged in this wavefront's LDS
-        // (the gathers by tip code then stay out of the vector L1)
+        // the tips' P matrices of cherry children, staged in this wavefront's LDS by
+        // 16-byte loads (the gathers by tip code then stay out of the vector L1)
         TX *lw = lds[wave];
-        const TX *Pa0 = lw, *Pb0 = lw + kPStride;
-        const TX *Pa1 = lw + (md0 == MODE_CHERRY ? 2 : 0) * kPStride;
-        const TX *Pb1 = Pa1 + kPStride;
+        TX *l0 = lw, *l1 = lw + lds_mats(md0) * kPStride;
+        const TX *Pa0 = l0, *Pb0 = l0 + kPStride, *Pa1 = l1, *Pb1 = l1 + kPStride;
+
         if constexpr (CherryLds<M0, M1>::mats > 0) {
-            if (md0 == MODE_CHERRY || md1 == MODE_CHERRY) {
+            if (lds_mats(md0) + lds_mats(md1) > 0) {
                 wave_sync();  // the previous rate's reads are done
-                const TX *ga = Pg + ((size_t)(md0 == MODE_CHERRY ? ga0 : ga1) * R + r) * kPStride;
-                const TX *gb = Pg + ((size_t)(md0 == MODE_CHERRY ? gb0 : gb1) * R + r) * kPStride;
-                for (int i = lane; i < kPStride; i += 64) {
-                    lw[i] = ga[i];
-                    lw[kPStride + i] = gb[i];
+                if (md0 == MODE_CHERRY) {
+                    stage_p<TX>(l0, Pg + ((size_t)ga0 * R + r) * kPStride, lane);
+                    stage_p<TX>(l0 + kPStride, Pg + ((size_t)gb0 * R + r) * kPStride, lane);
                 }
-                if (md0 == MODE_CHERRY && md1 == MODE_CHERRY) {
-                    const TX *ga_ = Pg + ((size_t)ga1 * R + r) * kPStride;
-                    const TX *gb_ = Pg + ((size_t)gb1 * R + r) * kPStride;
-                    for (int i = lane; i < kPStride; i += 64) {
-                        lw[2 * kPStride + i] = ga_[i];
-                        lw[3 * kPStride + i] = gb_[i];
-                    }
+                if (md1 == MODE_CHERRY) {
+                    stage_p<TX>(l1, Pg + ((size_t)ga1 * R + r) * kPStride, lane);
+                    stage_p<TX>(l1 + kPStride, Pg + ((size_t)gb1 * R + r) * kPStride, lane);
                 }
                 wave_sync();
             }
@@ -377,23 +409,70 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                     }
                 }
             };
-            TX bv0[U][5], bv1[U][5];
             int e1[U];
-            fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, bv0, etot, y);
-            fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, bv1, e1, w);
-            // Phase 2: the products of the children that are not tips.  The scheduling
-            // groups ask the compiler to issue the step's vector loads before its MFMAs
-            // (it otherwise interleaves them to shorten register live ranges, which
-            // serialises the two children's memory latencies).
-            __builtin_amdgcn_sched_group_barrier(0x020, 12 * U, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 20 * U, 0);
-            if (M0 != MODE_TIP && md0 != MODE_TIP) {
+            if constexpr (M0 == MODE_RT) {
+                // child modes decided per item: per child, its loads then its products,
+                // which keeps this general kernel's register count (and occupancy)
+                auto child_rt = [&](int md, const TX(&a)[2][5], const TX *Pc, const TX *Pa, const TX *Pb,
+                                    const TX *Xc, const int *Ec, int c, int ga, int gb, V4(&acc)[U][2],
+                                    int(&ex)[U]) {
+                    if constexpr (PAIRED) {
+                        if (md == MODE_STORED) {
 #pragma unroll
-                for (int u = 0; u < U; u++) mma_block<TX>(a0, bv0[u], y[u]);
-            }
-            if (M1 != MODE_TIP && md1 != MODE_TIP) {
+                            for (int j = 0; j < U / 2; j++) {
+                                const int pb = tl.start + site_of(sb0, 2 * j, c16);
+                                TX b0[5], b1[5];
+                                stored_b2<TX>(Xc + pb, S_pad, valid[2 * j], valid[2 * j + 1], g4, b0, b1);
+                                const int2v ev = *(const int2v *)(Ec + pb);
+                                mma_block<TX>(a, b0, acc[2 * j]);
+                                mma_block<TX>(a, b1, acc[2 * j + 1]);
+                                ex[2 * j] = ev.x;
+                                ex[2 * j + 1] = ev.y;
+                            }
+                            return;
+                        }
+                    }
 #pragma unroll
-                for (int u = 0; u < U; u++) mma_block<TX>(a1, bv1[u], w[u]);
+                    for (int u = 0; u < U; u++) {
+                        const int orig = perm[(size_t)p * S_pad + slot[u]];
+                        auto col_of = [&](int tip) {
+                            const int code = codes[(size_t)tip * S_pad + orig];
+                            return code < 20 ? code : 20;
+                        };
+                        int col = 20, ca = 20, cb = 20;
+                        if (md == MODE_TIP) col = col_of(c);
+                        if (md == MODE_CHERRY) {
+                            ca = col_of(ga);
+                            cb = col_of(gb);
+                        }
+                        ex[u] = child_block_rt<TX>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, col, ca,
+                                                   cb, valid[u], g4, acc[u]);
+                    }
+                };
+                child_rt(md0, a0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, y, etot);
+                child_rt(md1, a1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, w, e1);
+            } else {
+                TX bv0[U][5], bv1[U][5];
+                fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, bv0, etot, y);
+                fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, bv1, e1, w);
+                // Phase 2: the products of the children that are not tips.  The
+                // scheduling groups ask the compiler to issue the step's vector loads
+                // before its MFMAs (it otherwise interleaves them to shorten register
+                // live ranges, which serialises the two children's memory latencies);
+                // only in kinds with a stored child: elsewhere the longer live ranges
+                // cost more occupancy than the overlap gains.
+                if constexpr (M0 == MODE_STORED || M1 == MODE_STORED) {
+                    __builtin_amdgcn_sched_group_barrier(0x020, 12 * U, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, 20 * U, 0);
+                }
+                if (M0 != MODE_TIP && md0 != MODE_TIP) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) mma_block<TX>(a0, bv0[u], y[u]);
+                }
+                if (M1 != MODE_TIP && md1 != MODE_TIP) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) mma_block<TX>(a1, bv1[u], w[u]);
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -520,7 +599,7 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
     // unit w runs on XCD w mod X (the grid is a multiple of X); XCD x takes the
     // items x, x + X, ..., each with all its tile groups, so the P matrices and
     // tip codes an item reads are fetched into one L2 only
-    __shared__ TX lds[4][CherryLds<M0, M1>::elems];
+    __shared__ __attribute__((aligned(16))) TX lds[4][CherryLds<M0, M1>::elems];
     const int bpi = ka.blocks_per_item, X = ka.xcds;
     const int n_work = (n + X - 1) / X * X * bpi;
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
