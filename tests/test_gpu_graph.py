@@ -71,6 +71,14 @@ def test_every_launch_shape_gives_identical_lnl(gpu):
                     eng.set_graphs(graphs)
                     got, _ = eng.refresh_likelihood()
                     assert np.array_equal(got, want), (g, u, x, graphs)
+        # the roots' unroll changes which sites a lane sums (and so the last bits of
+        # the per-particle sum), never the per-site values
+        for rt in (1, 2, 4):
+            eng.set_tuning(8, (1, 1, 1), 8, unroll_runtime=rt)
+            for graphs in (False, True):
+                eng.set_graphs(graphs)
+                got, _ = eng.refresh_likelihood()
+                np.testing.assert_allclose(got, want, rtol=1e-13, err_msg="unroll_runtime %d" % rt)
 
 
 def test_autotune_then_oracle(gpu):

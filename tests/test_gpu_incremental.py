@@ -106,8 +106,10 @@ def _oracle(codes, batch, R):
     return np.array([orc.particle_lnl(codes, batch.particle(i), R) for i in range(batch.n)])
 
 
-@pytest.mark.parametrize("variant", ["mfma", "valu"])
+@pytest.mark.parametrize("variant", ["mfma", "mfma-u1", "mfma-u4", "valu"])
 def test_incremental_moves_and_reject(gpu, variant):
+    """Incremental refreshes against the oracle; the mfma-uN cases run every launch
+    (per-kind, merged small buckets, roots) with N column blocks per step."""
     _say("case incremental: 7 move kinds, accept/reject, %s" % variant)
     rng = np.random.default_rng(5)
     T, S, R = 30, 150, 4
@@ -117,7 +119,10 @@ def test_incremental_moves_and_reject(gpu, variant):
     n = len(kinds) * 2
     batch = sim.prior_particles(rng, n, T, S, num_rates=R, invar=True, k_range=(1, 4))
     with Engine(T, S, R, n, 4, workspaces=2) as eng:
-        eng.set_prune_variant(variant)
+        eng.set_prune_variant(variant.split("-")[0])
+        if "-u" in variant:
+            u = int(variant.split("-u")[1])
+            eng.set_tuning(32, (u, u, u), 8, unroll_runtime=u)
         eng.set_alignment(codes)
         eng.set_particles(batch)
         l0, _ = eng.refresh_likelihood()
