@@ -317,6 +317,36 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     // not change during a launch)
     const int cs0 = md0 == MODE_STORED ? nslot(c0) : 0, cs1 = md1 == MODE_STORED ? nslot(c1) : 0;
     const int ns = IS_ROOT ? 0 : nslot(n);
+    // Tip codes of the tile's (at most 4) column positions, gathered once for all
+    // rates: byte 0 / 1 the tip of child 0 or its cherry's two tips, bytes 2 / 3 the
+    // same for child 1 (20 stands for a gap or unknown state).  Position k is the
+    // one a step's block u takes when k = sb0 + u.
+    uint32_t tcode[4] = {0, 0, 0, 0};
+    if (md0 != MODE_STORED || md1 != MODE_STORED) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int o = site_of(k / U * U, k % U, c16);
+            const int sl = tl.start + ((o >= tl.lo && o < tl.len) ? o : tl.lo);
+            const int orig = perm[(size_t)p * S_pad + sl];
+            auto cc = [&](int tip) {
+                const int code = codes[(size_t)tip * S_pad + orig];
+                return (uint32_t)(code < 20 ? code : 20);
+            };
+            uint32_t wd = 0;
+            if (md0 == MODE_TIP) wd |= cc(c0);
+            if (md0 == MODE_CHERRY) wd |= cc(ga0) | cc(gb0) << 8;
+            if (md1 == MODE_TIP) wd |= cc(c1) << 16;
+            if (md1 == MODE_CHERRY) wd |= cc(ga1) << 16 | cc(gb1) << 24;
+            tcode[k] = wd;
+        }
+    }
+    auto code_word = [&](int k) {  // k is wave-uniform: select without register indexing
+        uint32_t wd = tcode[0];
+#pragma unroll
+        for (int q = 1; q < 4; q++)
+            if (k == q) wd = tcode[q];
+        return wd;
+    };
 
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
@@ -372,7 +402,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
             // loads are unconditional (every lane's slot is inside the tile's window)
             // and lanes outside the tile are zeroed afterwards.
             auto fetch = [&](auto mode_c, int md, const TX *Pc, const TX *Pa, const TX *Pb, const TX *Xc,
-                             const int *Ec, int c, int ga, int gb, TX(&bv)[U][5], int(&ex)[U], V4(&acc)[U][2]) {
+                             const int *Ec, int shift, TX(&bv)[U][5], int(&ex)[U], V4(&acc)[U][2]) {
                 constexpr int MC = decltype(mode_c)::value;
                 if (MC == MODE_STORED || (MC == MODE_RT && md == MODE_STORED)) {
                     if constexpr (PAIRED) {
@@ -396,16 +426,13 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const int orig = perm[(size_t)p * S_pad + slot[u]];
-                    auto col_of = [&](int tip) {
-                        const int code = codes[(size_t)tip * S_pad + orig];
-                        return code < 20 ? code : 20;
-                    };
+                    const uint32_t wd = code_word(sb0 + u) >> shift;
+                    const int ca = (int)(wd & 0xff), cb = (int)((wd >> 8) & 0xff);
                     ex[u] = 0;
                     if (MC == MODE_TIP || (MC == MODE_RT && md == MODE_TIP)) {
-                        tip_block<TX>(Pc, col_of(c), g4, acc[u]);
+                        tip_block<TX>(Pc, ca, g4, acc[u]);
                     } else {
-                        cherry_b<TX>(Pa, Pb, col_of(ga), col_of(gb), valid[u], g4, bv[u]);
+                        cherry_b<TX>(Pa, Pb, ca, cb, valid[u], g4, bv[u]);
                     }
                 }
             };
@@ -414,8 +441,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 // child modes decided per item: per child, its loads then its products,
                 // which keeps this general kernel's register count (and occupancy)
                 auto child_rt = [&](int md, const TX(&a)[2][5], const TX *Pc, const TX *Pa, const TX *Pb,
-                                    const TX *Xc, const int *Ec, int c, int ga, int gb, V4(&acc)[U][2],
-                                    int(&ex)[U]) {
+                                    const TX *Xc, const int *Ec, int shift, V4(&acc)[U][2], int(&ex)[U]) {
                     if constexpr (PAIRED) {
                         if (md == MODE_STORED) {
 #pragma unroll
@@ -434,27 +460,18 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        const int orig = perm[(size_t)p * S_pad + slot[u]];
-                        auto col_of = [&](int tip) {
-                            const int code = codes[(size_t)tip * S_pad + orig];
-                            return code < 20 ? code : 20;
-                        };
-                        int col = 20, ca = 20, cb = 20;
-                        if (md == MODE_TIP) col = col_of(c);
-                        if (md == MODE_CHERRY) {
-                            ca = col_of(ga);
-                            cb = col_of(gb);
-                        }
-                        ex[u] = child_block_rt<TX>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, col, ca,
+                        const uint32_t wd = code_word(sb0 + u) >> shift;
+                        const int ca = (int)(wd & 0xff), cb = (int)((wd >> 8) & 0xff);
+                        ex[u] = child_block_rt<TX>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, ca, ca,
                                                    cb, valid[u], g4, acc[u]);
                     }
                 };
-                child_rt(md0, a0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, y, etot);
-                child_rt(md1, a1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, w, e1);
+                child_rt(md0, a0, P0, Pa0, Pb0, X0, E0, 0, y, etot);
+                child_rt(md1, a1, P1, Pa1, Pb1, X1, E1, 16, w, e1);
             } else {
                 TX bv0[U][5], bv1[U][5];
-                fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, c0, ga0, gb0, bv0, etot, y);
-                fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, c1, ga1, gb1, bv1, e1, w);
+                fetch(std::integral_constant<int, M0>{}, md0, P0, Pa0, Pb0, X0, E0, 0, bv0, etot, y);
+                fetch(std::integral_constant<int, M1>{}, md1, P1, Pa1, Pb1, X1, E1, 16, bv1, e1, w);
                 // Phase 2: the products of the children that are not tips.  The
                 // scheduling groups ask the compiler to issue the step's vector loads
                 // before its MFMAs (it otherwise interleaves them to shorten register
