@@ -321,31 +321,26 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     // rates: byte 0 / 1 the tip of child 0 or its cherry's two tips, bytes 2 / 3 the
     // same for child 1 (20 stands for a gap or unknown state).  Position k is the
     // one a step's block u takes when k = sb0 + u.
-    uint32_t tcode[4] = {0, 0, 0, 0};
-    if (md0 != MODE_STORED || md1 != MODE_STORED) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int o = site_of(k / U * U, k % U, c16);
-            const int sl = tl.start + ((o >= tl.lo && o < tl.len) ? o : tl.lo);
-            const int orig = perm[(size_t)p * S_pad + sl];
-            auto cc = [&](int tip) {
-                const int code = codes[(size_t)tip * S_pad + orig];
-                return (uint32_t)(code < 20 ? code : 20);
-            };
-            uint32_t wd = 0;
-            if (md0 == MODE_TIP) wd |= cc(c0);
-            if (md0 == MODE_CHERRY) wd |= cc(ga0) | cc(gb0) << 8;
-            if (md1 == MODE_TIP) wd |= cc(c1) << 16;
-            if (md1 == MODE_CHERRY) wd |= cc(ga1) << 16 | cc(gb1) << 24;
-            tcode[k] = wd;
-        }
-    }
-    auto code_word = [&](int k) {  // k is wave-uniform: select without register indexing
-        uint32_t wd = tcode[0];
-#pragma unroll
-        for (int q = 1; q < 4; q++)
-            if (k == q) wd = tcode[q];
+    auto tip_codes = [&](int k) -> uint32_t {
+        if (md0 == MODE_STORED && md1 == MODE_STORED) return 0;
+        const int o = site_of(k / U * U, k % U, c16);
+        const int sl = tl.start + ((o >= tl.lo && o < tl.len) ? o : tl.lo);
+        const int orig = perm[(size_t)p * S_pad + sl];
+        auto cc = [&](int tip) {
+            const int code = codes[(size_t)tip * S_pad + orig];
+            return (uint32_t)(code < 20 ? code : 20);
+        };
+        uint32_t wd = 0;
+        if (md0 == MODE_TIP) wd |= cc(c0);
+        if (md0 == MODE_CHERRY) wd |= cc(ga0) | cc(gb0) << 8;
+        if (md1 == MODE_TIP) wd |= cc(c1) << 16;
+        if (md1 == MODE_CHERRY) wd |= cc(ga1) << 16 | cc(gb1) << 24;
         return wd;
+    };
+    // four scalars, not an array: a run-time index into an array sends it to scratch
+    const uint32_t tc0 = tip_codes(0), tc1 = tip_codes(1), tc2 = tip_codes(2), tc3 = tip_codes(3);
+    auto code_word = [&](int k) {  // k is wave-uniform
+        return k == 0 ? tc0 : k == 1 ? tc1 : k == 2 ? tc2 : tc3;
     };
 
     for (int r = 0; r < R; r++) {
