@@ -16,10 +16,12 @@
 //   C/D:       at site 16*sb + (lane & 15), register v < 4 holds row
 //              16*rb + 4*(lane >> 4) + v for f32 and 16*rb + (lane >> 4) + 4*v for f64
 //              (the two builtins place result rows differently; A and B agree).
-// P stays in VGPRs for all column blocks of the tile; partial rows are read and written
-// as 16-lane contiguous runs of the state-major [state][site] layout.  Tip
-// children are not multiplied: their P x is a column (or the row sums) of P,
-// gathered straight into the C layout.
+// (With unroll >= 2 the sites of a lane are paired instead; see the column mapping in
+// prune_unit.)  P stays in VGPRs for all column blocks of the tile; partial rows are
+// read and written as 16-lane contiguous runs of the state-major [state][site] layout.
+// Tip children are not multiplied: their P x is a column (or the row sums) of P,
+// gathered straight into the C layout.  Each step issues every load of both children
+// before its MFMAs, and a tile's tip codes are gathered once for all rates.
 //
 // Cherries (internal nodes whose children are both tips, a third of the nodes
 // of a random tree) are never stored: the parent builds the cherry's partial
