@@ -264,12 +264,34 @@ def rejuv_bench(args):
     t_fast = time.perf_counter() - t0
     phases = {k: round(v[0] / 1e3, 3) for k, v in eng.phase_stats().items()}
     progress("rejuv fast: %d particles in %.1f s" % (nf, t_fast))
-    net.close()
-    eng.close()
     out["fp32_batched_learned"] = {"particles": nf, "seconds": t_fast, "seconds_per_particle": t_fast / nf,
                                    "host_phase_seconds": phases, "mh_stats": stats.tolist(),
                                    "collect_seconds": t_col, "train_seconds": t_train,
                                    "final_train_loss": float(losses[-1]) if len(losses) else None}
+    # The sampler's shape (smc_run_sampler): MCMC batch by batch, then one CAT lockstep
+    # over the chosen particles of the whole population (about 410 of c4's 4,096 at
+    # probability 0.1) and one refresh of every particle.  Here all resident particles
+    # move in one lockstep and a tenth of that move is charged per particle; the
+    # refresh is charged in full.
+    eng.set_particles(batch)
+    t0 = time.perf_counter()
+    eng.rejuvenate(tau, it, seed=5, invariant=True, gibbs_probability=0.0, dpp_alpha=1.0)
+    t_mcmc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net.move(range(nf), tau, seed=6, dpp_alpha=1.0)
+    t_move = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng.refresh_likelihood(0, nf)
+    t_ref = time.perf_counter() - t0
+    per_def = (t_mcmc + 0.1 * t_move + t_ref) / nf
+    progress("rejuv fast, sampler shape: %.1f ms MCMC + 0.1 x %.1f ms CAT lockstep of %d + %.1f ms refresh "
+             "per particle" % (t_mcmc / nf * 1e3, t_move / nf * 1e3, nf, t_ref / nf * 1e3))
+    out["fp32_batched_learned_sampler_shape"] = {
+        "particles": nf, "mcmc_seconds_per_particle": t_mcmc / nf,
+        "cat_lockstep_particles": nf, "cat_move_seconds_per_particle": t_move / nf,
+        "refresh_seconds_per_particle": t_ref / nf, "seconds_per_particle": per_def}
+    net.close()
+    eng.close()
     if npl < 1:  # fast path only (profiling)
         return out
     # plain path: FP64, one particle at a time; the Gibbs move is timed on its own
@@ -295,6 +317,7 @@ def rejuv_bench(args):
     out["plain_fp64"] = {"particles_sampled": npl, "mcmc_seconds_per_particle": t_mcmc,
                          "gibbs_seconds_per_move": t_gibbs, "seconds_per_particle": per}
     out["ratio_per_particle"] = per / out["fp32_batched_learned"]["seconds_per_particle"]
+    out["ratio_per_particle_sampler_shape"] = per / per_def
     return out
 
 

@@ -191,10 +191,17 @@ struct AllocationSink {
 int smc_upload_partition(smc_engine *e, int slot, int K, const double *logits, const int32_t *assign);
 
 // smc_rejuvenate that also adds the CAT move's site updates and site moves to
-// cat_stats[0..1] (nullable).
+// cat_stats[0..1] (nullable).  With deferred_cat (nullable) the slots chosen for
+// the CAT move are appended there instead of moved: the caller runs smc_cat_moves
+// over them later (the move of a slot depends on nothing but that slot and the
+// seed, so the result is the same) and refreshes their likelihoods.
 int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                       const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out,
-                      int64_t *cat_stats);
+                      int64_t *cat_stats, std::vector<int32_t> *deferred_cat = nullptr);
+// The CAT move of smc_rejuvenate (learned, recording Gibbs or Gibbs, as the options
+// say) over any list of slots, all in lockstep.
+int smc_cat_moves(smc_engine *e, const std::vector<int32_t> &slots, double tempering,
+                  const smc_rejuvenation_options &o, int64_t *cat_stats);
 
 // smc_site_class_lnl where cand_class[i][c] >= 0 (nullable) names a class of
 // the entry's particle whose eigensystem the engine holds (its logits are then

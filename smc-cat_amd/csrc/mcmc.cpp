@@ -195,9 +195,37 @@ extern "C" int smc_rejuvenate(smc_engine *e, int first, int count, double temper
     return smc_rejuvenate_ex(e, first, count, tempering, opt, lnl_out, stats_out, nullptr);
 }
 
+int smc_cat_moves(smc_engine *e, const std::vector<int32_t> &slots, double tempering,
+                  const smc_rejuvenation_options &o, int64_t *cat_stats)
+{
+    if (slots.empty()) return SMC_OK;
+    PhaseTimer cat_timer(e, SMC_PHASE_CAT_MOVE);
+    const int n = (int)slots.size();
+    const uint64_t seed = o.seed ^ 0x61626273ull;
+    int64_t st[5] = {};
+    int rc;
+    if (o.allocation_proposal && !o.record_allocations) {
+        rc = smc_learned_partition_move(o.allocation_proposal, n, slots.data(), tempering, seed, 0, o.dpp_alpha,
+                                        -1.0, st);
+        if (cat_stats) {
+            cat_stats[0] += st[0];
+            cat_stats[1] += st[2];
+        }
+    } else {
+        rc = o.allocation_proposal
+                 ? smc_mlp_collect_gibbs(o.allocation_proposal, n, slots.data(), tempering, seed, 0, o.dpp_alpha, st)
+                 : smc_gibbs_partition_move(e, n, slots.data(), tempering, seed, 0, o.dpp_alpha, st);
+        if (cat_stats) {
+            cat_stats[0] += (int64_t)n * (int64_t)(e->S / 2.0);
+            cat_stats[1] += st[0];
+        }
+    }
+    return rc;
+}
+
 int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                       const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out,
-                      int64_t *cat_stats)
+                      int64_t *cat_stats, std::vector<int32_t> *deferred_cat)
 {
     if (!e) return SMC_ERR_ARG;
     if (!opt || first < 0 || count < 1 || first + count > e->Pmax || opt->iterations < 0)
@@ -356,28 +384,10 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         std::vector<int32_t> slots;
         for (int i = 0; i < count; ++i)
             if (rng[i].uniform() < o.gibbs_probability) slots.push_back(first + i);
-        if (!slots.empty()) {
-            PhaseTimer cat_timer(e, SMC_PHASE_CAT_MOVE);
-            const int n = (int)slots.size();
-            const uint64_t seed = o.seed ^ 0x61626273ull;
-            int64_t st[5] = {};
-            if (o.allocation_proposal && !o.record_allocations) {
-                rc = smc_learned_partition_move(o.allocation_proposal, n, slots.data(), tempering, seed, 0,
-                                                o.dpp_alpha, -1.0, st);
-                if (cat_stats) {
-                    cat_stats[0] += st[0];
-                    cat_stats[1] += st[2];
-                }
-            } else {
-                rc = o.allocation_proposal
-                         ? smc_mlp_collect_gibbs(o.allocation_proposal, n, slots.data(), tempering, seed, 0,
-                                                 o.dpp_alpha, st)
-                         : smc_gibbs_partition_move(e, n, slots.data(), tempering, seed, 0, o.dpp_alpha, st);
-                if (cat_stats) {
-                    cat_stats[0] += (int64_t)n * (int64_t)(e->S / 2.0);
-                    cat_stats[1] += st[0];
-                }
-            }
+        if (deferred_cat) {
+            deferred_cat->insert(deferred_cat->end(), slots.begin(), slots.end());
+        } else if (!slots.empty()) {
+            rc = smc_cat_moves(e, slots, tempering, o, cat_stats);
             if (rc != SMC_OK) return rc;
             rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
             if (rc != SMC_OK) return rc;

@@ -172,12 +172,27 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
                                    asg.data(), pv.data(), sh.data());
             if (rc != SMC_OK) return rc;
             const auto r0 = clk::now();
+            // Batched (batch > 1): the CAT moves of every batch run together after the
+            // last batch, one lockstep move over all chosen particles of the population
+            // (its per-site steps are latency-bound, so many particles per step cost
+            // little more than a few).  A slot's move depends only on the slot and the
+            // seed, so the particles end as they would with the move inside each batch.
+            // batch = 1 keeps the reference's shape: each particle's move right after
+            // its MCMC.
+            std::vector<int32_t> cat_slots;
             for (int first = 0; first < N; first += batch) {
                 const int cnt = std::min(batch, N - first);
                 int64_t st[8];
-                rc = smc_rejuvenate_ex(e, first, cnt, cur, &ro, lnl.data() + first, st, res->cat_stats);
+                rc = smc_rejuvenate_ex(e, first, cnt, cur, &ro, lnl.data() + first, st, res->cat_stats,
+                                       batch > 1 ? &cat_slots : nullptr);
                 if (rc != SMC_OK) return rc;
                 for (int k = 0; k < 8; ++k) res->move_stats[k] += st[k];
+            }
+            if (!cat_slots.empty()) {
+                rc = smc_cat_moves(e, cat_slots, cur, ro, res->cat_stats);
+                if (rc != SMC_OK) return rc;
+                rc = smc_refresh_likelihood(e, 0, N, lnl.data(), nullptr);  // in waves when N exceeds them
+                if (rc != SMC_OK) return rc;
             }
             rejuv_s += std::chrono::duration<double>(clk::now() - r0).count();
             if (ro.record_allocations && res->resamples + 1 == o.proposal_warmup) {

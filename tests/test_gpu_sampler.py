@@ -104,3 +104,56 @@ def test_sampler_cat_learned_proposal(gpu):
             assert res["cat_stats"][0] > res["cat_stats"][2] and res["cat_stats"][1] > 0
             assert res["training_seconds"] > 0
         _check_final(eng, batch, codes, R, lnl, n_check=N)
+
+
+def test_cat_moves_do_not_depend_on_their_lockstep_companions(gpu):
+    """The sampler runs the CAT moves of all its lockstep batches together after the
+    last batch; that gives the same particles because a slot's move depends only on
+    the slot and the seed.  Checked here directly: rejuvenation with the CAT move of
+    all 12 slots in one lockstep equals four locksteps of 3 slots, for the Gibbs and the
+    learned move; and the sampler gives the same population with batch 1 (moves inline),
+    3 and 12 (moves deferred)."""
+    _say("case cat lockstep independence: 12 taxa x 60 sites LG+G4 CAT K<=4, 12 particles")
+    from smccat import ProposalNetwork
+    rng = np.random.default_rng(11)
+    T, S, R, N = 12, 60, 4, 12
+    codes = sim.simulate_config("c1")[0][:T, :S].copy()
+    batch = sim.prior_particles(rng, N, T, S, num_rates=R, lg=True, lg_table=lg_base_matrix(),
+                                k_range=(1, 4))
+
+    def state(eng):
+        out = []
+        for i in range(N):
+            k, lg, a = eng.particle_state(i)
+            out.append((k, lg[:k].copy(), a.copy()))
+        return out
+
+    with Engine(T, S, R, N, 8, workspaces=2) as eng:
+        eng.set_alignment(codes)
+        net = ProposalNetwork(eng, seed=4)
+        for learned in (False, True):
+            runs = []
+            for groups in ((0, N),), ((0, 3), (3, 3), (6, 3), (9, 3)):
+                eng.set_particles(batch)
+                lnl = np.concatenate([eng.rejuvenate(0.7, 2, seed=9, first=f, count=c, using_lg=True,
+                                                     invariant=False, gibbs_probability=0.6, dpp_alpha=1.0,
+                                                     proposal=net if learned else None)[0]
+                                      for f, c in groups])
+                runs.append((lnl, state(eng)))
+            np.testing.assert_array_equal(runs[0][0], runs[1][0])
+            for (k0, l0, a0), (k1, l1, a1) in zip(runs[0][1], runs[1][1]):
+                assert k0 == k1
+                np.testing.assert_array_equal(l0, l1)
+                np.testing.assert_array_equal(a0, a1)
+        net.close()
+        outs = []
+        for b in (1, 3, N):  # 1: each move inline after its particle's MCMC; 3, N: deferred
+            eng.set_particles(batch)
+            res, lnl, w = eng.run_sampler(N, 2, seed=3, using_lg=True, invariant=False, cat=True,
+                                          alg8_probability=0.5, dpp_alpha=1.0, batch=b, max_iterations=6)
+            outs.append((lnl, state(eng)))
+        for other in outs[1:]:
+            np.testing.assert_array_equal(outs[0][0], other[0])
+            for (k0, l0, a0), (k1, l1, a1) in zip(outs[0][1], other[1]):
+                assert k0 == k1
+                np.testing.assert_array_equal(a0, a1)
