@@ -175,7 +175,9 @@ int smc_rejuvenate(smc_engine *e, int first, int count, double tempering,
  * resampling, proposal tuning from the resampled population (main.cpp:197-289),
  * the copy of each parent into its new slot (on the engine, exact values), and
  * the rejuvenation of every slot (smc_rejuvenate) in lockstep batches of
- * `batch` particles.  Differences: copies are exact (the reference rounds
+ * `batch` particles.  With batch > 1 the CAT moves of all batches run in one
+ * lockstep after the last batch, followed by one refresh of the population (the
+ * particles end the same: each slot has its own random streams).  Differences: copies are exact (the reference rounds
  * branch lengths to 6 decimals through Newick text, Tree.cpp:429,433); split
  * posteriors and consensus (main.cpp:203-207,338-425) are not computed. */
 typedef struct {
