@@ -221,80 +221,106 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 
 // P(t r) = A diag(e^{lam t r}) B of one eigensystem (A, lam, B at eg), written
 // row-major with a 21st column of row sums.
-template <typename TP>
-__device__ __forceinline__ void pmat_one(const double *__restrict__ eg, double t, double rate, TP *__restrict__ out)
+// P(t) = A diag(e^{lam t r}) B of one class for a run of (node, rate) pairs.  A
+// 256-lane workgroup keeps, per lane, the outer-product columns
+// M[q][k] = A[i][k] Bx[k][j] of its (at most two) entries q = 21 i + j, where Bx is
+// B with a 21st column of row sums (P 1 = A diag(e) (B 1)); then for each pair
+// P[q] = sum_k M[q][k] e_k.  The exponentials of all pairs of the run are computed
+// once, spread over the workgroup, and the 420 values of each P are written by
+// consecutive lanes.
+constexpr int kPmLanes = 256;
+constexpr int kPmPairs = 32;  // (node, rate) pairs per workgroup
+
+template <typename TP, typename PairNode>
+__device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const double *__restrict__ bl,
+                                           const double *__restrict__ rt, int R, int q0, int q1,
+                                           PairNode node_of, TP *__restrict__ Pcls)
 {
-    const double *A = eg;
-    const double *lam = eg + kEigLam;
-    const double *B = eg + kEigB;
-    double e[20];
+    __shared__ double bsum[20];
+    __shared__ double ex[kPmPairs][20];
+    const int tid = threadIdx.x;
+    if (tid < 20) {
+        double a = 0.0;
+        for (int j = 0; j < 20; j++) a += eg[kEigB + tid * 20 + j];
+        bsum[tid] = a;
+    }
+    // the exponentials of every pair of the run, spread over the workgroup
+    for (int x = tid; x < (q1 - q0) * 20; x += kPmLanes) {
+        const int q = q0 + x / 20, k = x % 20;
+        ex[x / 20][k] = exp(eg[kEigLam + k] * bl[node_of(q / R)] * rt[q % R]);
+    }
+    __syncthreads();
+    const int qa = tid, qb = tid + kPmLanes;  // entries of this lane (qb < 420 for 164 lanes)
+    const bool hb = qb < kPStride;
+    double ma[20], mb[20];
+    {
+        const int ia = qa / 21, ja = qa % 21, ib = hb ? qb / 21 : 0, jb = hb ? qb % 21 : 0;
 #pragma unroll
-    for (int kk = 0; kk < 20; kk++) e[kk] = exp(lam[kk] * t * rate);
-    for (int i = 0; i < 20; i++) {
-        double a[20];
-#pragma unroll
-        for (int kk = 0; kk < 20; kk++) a[kk] = A[i * 20 + kk] * e[kk];
-        double rowsum = 0.0;
-#pragma unroll
-        for (int j = 0; j < 20; j++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int kk = 0; kk < 20; kk++) acc = fma(a[kk], B[kk * 20 + j], acc);
-            out[i * 21 + j] = (TP)acc;
-            rowsum += acc;
+        for (int k = 0; k < 20; k++) {
+            ma[k] = eg[ia * 20 + k] * (ja < 20 ? eg[kEigB + k * 20 + ja] : bsum[k]);
+            mb[k] = eg[ib * 20 + k] * (jb < 20 ? eg[kEigB + k * 20 + jb] : bsum[k]);
         }
-        out[i * 21 + 20] = (TP)rowsum;
+    }
+    for (int q = q0; q < q1; q++) {
+        const int n = node_of(q / R), r = q % R;
+        double pa = 0.0, pb = 0.0;
+#pragma unroll
+        for (int k = 0; k < 20; k++) {
+            const double e = ex[q - q0][k];
+            pa = fma(ma[k], e, pa);
+            pb = fma(mb[k], e, pb);
+        }
+        TP *out = Pcls + ((size_t)n * R + r) * kPStride;
+        out[qa] = (TP)pa;
+        if (hb) out[qb] = (TP)pb;
     }
 }
 
-// One thread = one whole P matrix (particle, class, node, rate).  Every lane of
-// a block shares the class, so A, lam and B are wave-uniform scalar loads.
+// Every (node, rate) of every class of particles first.. (or plist[..]):
+// workgroup (particle * max_classes + class, chunk of kPmPairs pairs).
 template <typename TP>
-__global__ __launch_bounds__(64) void k_pmat(const double *__restrict__ eig,
-                                              const double *__restrict__ blen,
-                                              const double *__restrict__ rates,
-                                              const int *__restrict__ num_classes,
-                                              const int *__restrict__ pcls_off,
-                                              TP *__restrict__ Pout, int nn, int R,
-                                              int max_classes, int first,
-                                              const int *__restrict__ plist)
-{
-    const int p = plist ? plist[blockIdx.x / max_classes] : first + blockIdx.x / max_classes;
-    const int k = blockIdx.x % max_classes;
-    if (k >= num_classes[p]) return;
-    const int idx = blockIdx.y * 64 + threadIdx.x;
-    if (idx >= nn * R) return;
-    const int n = idx / R, r = idx % R;
-    const size_t g = (size_t)p * max_classes + k;
-    pmat_one<TP>(eig + g * kEigStride, blen[(size_t)p * nn + n], rates[(size_t)p * R + r],
-                 Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
-}
-
-// P of listed nodes only: block = (listed particle, class, chunk of 64 of its
-// (node, rate) pairs), so A, lam and B stay wave-uniform (scalar loads) as in
-// k_pmat.
-template <typename TP>
-__global__ __launch_bounds__(64) void k_pmat_nodes(const double *__restrict__ eig,
+__global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ eig,
                                                     const double *__restrict__ blen,
                                                     const double *__restrict__ rates,
                                                     const int *__restrict__ num_classes,
                                                     const int *__restrict__ pcls_off,
-                                                    TP *__restrict__ Pout, int nn, int R, int max_classes,
-                                                    const int *__restrict__ plist,
-                                                    const int *__restrict__ noff,
-                                                    const int *__restrict__ nodes)
+                                                    TP *__restrict__ Pout, int nn, int R,
+                                                    int max_classes, int first,
+                                                    const int *__restrict__ plist)
+{
+    const int p = plist ? plist[blockIdx.x / max_classes] : first + blockIdx.x / max_classes;
+    const int k = blockIdx.x % max_classes;
+    if (k >= num_classes[p]) return;  // whole workgroup
+    const int q0 = blockIdx.y * kPmPairs, q1 = min(q0 + kPmPairs, nn * R);
+    if (q0 >= q1) return;
+    const size_t g = (size_t)p * max_classes + k;
+    pmat_pairs<TP>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+                   [](int n) { return n; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
+}
+
+// P of listed nodes only: workgroup (listed particle * max_classes + class, chunk
+// of kPmPairs of its (node, rate) pairs).
+template <typename TP>
+__global__ __launch_bounds__(kPmLanes) void k_pmat_nodes(const double *__restrict__ eig,
+                                                          const double *__restrict__ blen,
+                                                          const double *__restrict__ rates,
+                                                          const int *__restrict__ num_classes,
+                                                          const int *__restrict__ pcls_off,
+                                                          TP *__restrict__ Pout, int nn, int R, int max_classes,
+                                                          const int *__restrict__ plist,
+                                                          const int *__restrict__ noff,
+                                                          const int *__restrict__ nodes)
 {
     const int pi = blockIdx.x / max_classes, k = blockIdx.x % max_classes;
     const int p = plist[pi];
     if (k >= num_classes[p]) return;
     const int n0 = noff[pi], cnt = noff[pi + 1] - n0;
+    const int q0 = blockIdx.y * kPmPairs, q1 = min(q0 + kPmPairs, cnt * R);
+    if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
-    const int q = blockIdx.y * 64 + threadIdx.x;  // (node, rate) pair of this lane
-    if (q < cnt * R) {
-        const int n = nodes[n0 + q / R], r = q % R;
-        pmat_one<TP>(eig + g * kEigStride, blen[(size_t)p * nn + n], rates[(size_t)p * R + r],
-                     Pout + ((((size_t)pcls_off[p] + k) * nn + n) * R + r) * kPStride);
-    }
+    const int *nl = nodes + n0;
+    pmat_pairs<TP>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+                   [nl](int i) { return nl[i]; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
 }  // namespace
@@ -318,13 +344,13 @@ void launch_eigen_cand(const double *base, const int *ent_particle, const int *e
 void launch_pmat(const PmatArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return;
-    dim3 grid(a.count * a.max_classes, (a.nn * a.R + 63) / 64);
+    dim3 grid(a.count * a.max_classes, (a.nn * a.R + kPmPairs - 1) / kPmPairs);
     if (a.fp32)
-        hipLaunchKernelGGL(k_pmat<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL(k_pmat<float>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
     else
-        hipLaunchKernelGGL(k_pmat<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL(k_pmat<double>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
 }
@@ -337,13 +363,13 @@ void launch_pmat_nodes(const PmatArgs &a, const int *plist, const int *noff, con
                        int max_nodes, hipStream_t s)
 {
     if (n_particles <= 0 || max_nodes <= 0) return;
-    const dim3 grid((unsigned)n_particles * a.max_classes, (unsigned)((max_nodes * a.R + 63) / 64));
+    const dim3 grid((unsigned)n_particles * a.max_classes, (unsigned)((max_nodes * a.R + kPmPairs - 1) / kPmPairs));
     if (a.fp32)
-        hipLaunchKernelGGL(k_pmat_nodes<float>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates, a.num_classes,
-                           a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
+        hipLaunchKernelGGL(k_pmat_nodes<float>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
     else
-        hipLaunchKernelGGL(k_pmat_nodes<double>, grid, dim3(64), 0, s, a.eig, a.blen, a.rates, a.num_classes,
-                           a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
+        hipLaunchKernelGGL(k_pmat_nodes<double>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+                           a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes, plist, noff, nodes);
 }
 
 }  // namespace smccat
