@@ -478,6 +478,11 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 if constexpr (M0 == MODE_STORED || M1 == MODE_STORED) {
                     __builtin_amdgcn_sched_group_barrier(0x020, 12 * U, 0);
                     __builtin_amdgcn_sched_group_barrier(0x008, 20 * U, 0);
+                } else if constexpr (M0 == MODE_CHERRY || M1 == MODE_CHERRY) {
+                    // cherry products: their LDS gathers first, then the MFMAs
+                    constexpr int nc = (M0 == MODE_CHERRY) + (M1 == MODE_CHERRY);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 10 * U * nc, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, 10 * U * nc, 0);
                 }
                 if (M0 != MODE_TIP && md0 != MODE_TIP) {
 #pragma unroll
