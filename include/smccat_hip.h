@@ -281,7 +281,7 @@ int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, doub
  * internal nodes with two tip children, are built in registers). */
 typedef struct {
     int32_t blocks_per_cu;   /* 1..64 (default 32)         */
-    int32_t unroll[3];       /* 1, 2 or 4 (default 2, 1, 1) */
+    int32_t unroll[3];       /* 1, 2 or 4 (default 1, 1, 2 for FP64, 2, 2, 2 for FP32) */
     double ms_per_sweep;     /* autotune: best measured     */
     int32_t configs_tried;   /* autotune: candidates timed  */
     int32_t xcd_groups;      /* 1, 2, 4 or 8 (default 8): the workgroups of one item are

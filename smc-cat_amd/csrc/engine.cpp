@@ -348,6 +348,12 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     if (e->Cmax < cfg->max_classes) e->Cmax = cfg->max_classes;
     e->tile_stride = (cfg->sites + kTile - 1) / kTile + cfg->max_classes;
     e->fp32 = cfg->precision == SMC_PREC_FP32;
+    if (e->fp32) {  // measured defaults (the autotuner refines them for a workload)
+        e->tune.unroll[0] = e->tune.unroll[1] = e->tune.unroll[2] = 2;
+    } else {
+        e->tune.unroll[0] = e->tune.unroll[1] = 1;
+        e->tune.unroll[2] = 2;
+    }
     e->tx_size = e->fp32 ? 4 : 8;
     if (e->tile_stride > 65535) {
         int rc = fail(e, SMC_ERR_ARG, "too many tiles per particle");

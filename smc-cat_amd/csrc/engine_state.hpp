@@ -133,7 +133,8 @@ struct smc_engine {
     // launch tuning (smc_engine_autotune) and the captured full sweep
     struct Tune {
         int blocks_per_cu = 32;       // persistent grid of the pruning launches: CUs x this
-        int unroll[3] = {2, 1, 1};    // 16-site column blocks per step, by stored children (0, 1, 2)
+        int unroll[3] = {2, 1, 1};    // 16-site column blocks per step, by stored children (0, 1, 2);
+                                      // set per precision at creation
         int xcd_groups = 8;           // an item's workgroups share one of this many XCDs
         int unroll_rt = 2;            // the same for launches that decide child modes per item
     };

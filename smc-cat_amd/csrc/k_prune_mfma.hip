@@ -339,11 +339,13 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         if (md1 == MODE_CHERRY) wd |= cc(ga1) << 16 | cc(gb1) << 24;
         return wd;
     };
-    // four scalars, not an array: a run-time index into an array sends it to scratch
+    // Four scalars, not an array (a run-time index into an array sends it to scratch).
+    // Each rate's column loop walks a working copy that is shifted by U positions
+    // per step, so step sb0's block u always reads working register u: indices known
+    // at compile time, no per-step selects or branches.
     const uint32_t tc0 = tip_codes(0), tc1 = tip_codes(1), tc2 = tip_codes(2), tc3 = tip_codes(3);
-    auto code_word = [&](int k) {  // k is wave-uniform
-        return k == 0 ? tc0 : k == 1 ? tc1 : k == 2 ? tc2 : tc3;
-    };
+    uint32_t w0, w1, w2, w3;
+    auto code_word = [&](int u) { return u == 0 ? w0 : u == 1 ? w1 : u == 2 ? w2 : w3; };
 
     for (int r = 0; r < R; r++) {
         const TX *P0 = Pg + ((size_t)c0 * R + r) * kPStride;
@@ -382,6 +384,15 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         const int *E0 = Ep + ((size_t)cs0 * R + r) * S_pad;
         const int *E1 = Ep + ((size_t)cs1 * R + r) * S_pad;
 
+        w0 = tc0;
+        w1 = tc1;
+        w2 = tc2;
+        w3 = tc3;
+        for (int q = 0; q < sb_begin; q++) {  // the first step's positions (wave-uniform)
+            w0 = w1;
+            w1 = w2;
+            w2 = w3;
+        }
 #pragma unroll 1
         for (int sb0 = sb_begin; sb0 < nsb; sb0 += U) {
             V4 y[U][2], w[U][2];
@@ -423,7 +434,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const uint32_t wd = code_word(sb0 + u) >> shift;
+                    const uint32_t wd = code_word(u) >> shift;
                     const int ca = (int)(wd & 0xff), cb = (int)((wd >> 8) & 0xff);
                     ex[u] = 0;
                     if (MC == MODE_TIP || (MC == MODE_RT && md == MODE_TIP)) {
@@ -457,7 +468,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        const uint32_t wd = code_word(sb0 + u) >> shift;
+                        const uint32_t wd = code_word(u) >> shift;
                         const int ca = (int)(wd & 0xff), cb = (int)((wd >> 8) & 0xff);
                         ex[u] = child_block_rt<TX>(md, a, Pc, Pa, Pb, Xc + slot[u], Ec + slot[u], S_pad, ca, ca,
                                                    cb, valid[u], g4, acc[u]);
@@ -475,14 +486,20 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 // live ranges, which serialises the two children's memory latencies);
                 // only in kinds with a stored child: elsewhere the longer live ranges
                 // cost more occupancy than the overlap gains.
-                if constexpr (M0 == MODE_STORED || M1 == MODE_STORED) {
-                    __builtin_amdgcn_sched_group_barrier(0x020, 12 * U, 0);
+                // vector-memory loads of a step: per stored child 5 rows + the exponent
+                // per block (half as many instructions when paired), per tip child 8
+                // gathers per block; cherry products gather from LDS
+                constexpr int n_st = (M0 == MODE_STORED) + (M1 == MODE_STORED);
+                constexpr int n_tip = (M0 == MODE_TIP) + (M1 == MODE_TIP);
+                constexpr int n_ch = (M0 == MODE_CHERRY) + (M1 == MODE_CHERRY);
+                constexpr int n_vmem = n_st * (PAIRED ? 3 * U : 6 * U) + n_tip * 8 * U;
+                if constexpr (n_st > 0) {
+                    __builtin_amdgcn_sched_group_barrier(0x020, n_vmem, 0);
                     __builtin_amdgcn_sched_group_barrier(0x008, 20 * U, 0);
-                } else if constexpr (M0 == MODE_CHERRY || M1 == MODE_CHERRY) {
+                } else if constexpr (n_ch > 0) {
                     // cherry products: their LDS gathers first, then the MFMAs
-                    constexpr int nc = (M0 == MODE_CHERRY) + (M1 == MODE_CHERRY);
-                    __builtin_amdgcn_sched_group_barrier(0x100, 10 * U * nc, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x008, 10 * U * nc, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 10 * U * n_ch, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, 10 * U * n_ch, 0);
                 }
                 if (M0 != MODE_TIP && md0 != MODE_TIP) {
 #pragma unroll
@@ -492,6 +509,12 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
 #pragma unroll
                     for (int u = 0; u < U; u++) mma_block<TX>(a1, bv1[u], w[u]);
                 }
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {  // the next step's positions
+                w0 = w1;
+                w1 = w2;
+                w2 = w3;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
