@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_binding as orc
-from smccat import Engine
+from smccat import Engine, lg_base_matrix
 from smccat import simulate as sim
 
 pytestmark = pytest.mark.gpu
@@ -205,6 +205,29 @@ def test_c2_full_size_properties(gpu):
     for i in [0, 511, 1023]:
         want = orc.particle_lnl(codes, batch.particle(i), 4)
         assert lnl[i] == pytest.approx(want, rel=RTOL_FP64)
+
+
+@pytest.mark.parametrize("config,n,check", [("c3", 512, (0, 255, 511)), ("c4", 12, (0, 11))])
+def test_cat_full_size_fp32_against_fp64_oracle(gpu, config, n, check):
+    """The FP32 fast path at the CAT configurations' full alignment size (c3: 100 taxa x
+    5,000 sites, 512 particles, K 5-60; c4: 200 taxa x 10,000 sites, K 5-120, a dozen
+    particles of the 4,096), log-space mixture: every lnL finite, and the checked
+    particles within the fast path's bound of the FP64 oracle."""
+    _say("case %s full size fp32: %d particles" % (config, n))
+    codes, cfg = sim.simulate_config(config)
+    T, S, R, kr = cfg["taxa"], cfg["sites"], cfg["rates"], cfg["k_range"]
+    batch = sim.prior_particles(np.random.default_rng(303), n, T, S, num_rates=R, invar=cfg["invar"],
+                                lg=cfg["lg"], lg_table=lg_base_matrix() if cfg["lg"] else None, k_range=kr)
+    with Engine(T, S, R, n, kr[1], "fp32", partial_slots=min(n, 64)) as eng:
+        eng.set_mixture("logspace")
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        lnl, nf = eng.refresh_likelihood()
+    assert not nf.any() and np.all(np.isfinite(lnl))
+    for i in check:
+        want = orc.particle_lnl_mode(codes, batch.particle(i), R, 1)[0]
+        _say("  particle %d: fp32 %.6f oracle %.6f" % (i, lnl[i], want))
+        assert lnl[i] == pytest.approx(want, rel=RTOL_FP32)
 
 
 def test_next_step_vs_oracle(gpu):
