@@ -292,6 +292,41 @@ def rejuv_bench(args):
         "refresh_seconds_per_particle": t_ref / nf, "seconds_per_particle": per_def}
     net.close()
     eng.close()
+    if args.rj_batches > 1:
+        # The sampler's lockstep batches over a larger population: a batch runs as many
+        # rounds as its longest chain has MH steps, so the sampler keys the slots'
+        # random streams in order of their chains' step counts.  MCMC only, nf
+        # particles per batch, keys in slot order against keys sorted by steps.
+        nb = args.rj_batches
+        popn = nf * nb
+        pop = sim.prior_particles(np.random.default_rng(9), popn, T, S, num_rates=R, invar=True, k_range=kr,
+                                  max_classes=kmax)
+        eng = Engine(T, S, R, popn, kmax, "fp32", workspaces=2, partial_slots=nf,
+                     pmat_classes=int(nf * (kr[0] + kr[1]) / 2 * 1.4))
+        eng.set_mixture("logspace")
+        eng.set_alignment(codes)
+        slot_keys = np.arange(popn, dtype=np.int32)
+        steps = eng.rejuvenation_steps(it, 5, slot_keys, invariant=True)
+        sorted_keys = slot_keys[np.argsort(-steps, kind="stable")]
+        res = {"particles": popn, "batch": nf, "mean_steps": float(steps.mean())}
+        for name, keys in (("slot_order", slot_keys), ("sorted_by_steps", sorted_keys)):
+            eng.set_particles(pop)
+            t0 = time.perf_counter()
+            for b in range(nb):
+                eng.rejuvenate(tau, it, seed=5, first=b * nf, count=nf, invariant=True, gibbs_probability=0.0,
+                               stream_keys=keys[b * nf:(b + 1) * nf])
+            dt = time.perf_counter() - t0
+            rounds = [int(steps[keys[b * nf:(b + 1) * nf]].max()) for b in range(nb)]
+            res[name] = {"mcmc_seconds_per_particle": dt / popn, "rounds_per_batch": rounds}
+            progress("rejuv population of %d in batches of %d, %s keys: %.1f ms MCMC per particle, rounds %s"
+                     % (popn, nf, name, dt / popn * 1e3, rounds))
+        eng.close()
+        out["fp32_batched_population"] = res
+        sh = out["fp32_batched_learned_sampler_shape"]
+        sh["population_mcmc_seconds_per_particle"] = res["sorted_by_steps"]["mcmc_seconds_per_particle"]
+        sh["seconds_per_particle_population"] = (res["sorted_by_steps"]["mcmc_seconds_per_particle"] +
+                                                 0.1 * sh["cat_move_seconds_per_particle"] +
+                                                 sh["refresh_seconds_per_particle"])
     if npl < 1:  # fast path only (profiling)
         return out
     # plain path: FP64, one particle at a time; the Gibbs move is timed on its own
@@ -318,6 +353,9 @@ def rejuv_bench(args):
                          "gibbs_seconds_per_move": t_gibbs, "seconds_per_particle": per}
     out["ratio_per_particle"] = per / out["fp32_batched_learned"]["seconds_per_particle"]
     out["ratio_per_particle_sampler_shape"] = per / per_def
+    if "seconds_per_particle_population" in out["fp32_batched_learned_sampler_shape"]:
+        out["ratio_per_particle_sampler_population"] = (
+            per / out["fp32_batched_learned_sampler_shape"]["seconds_per_particle_population"])
     return out
 
 
@@ -346,6 +384,8 @@ def main():
     ap.add_argument("--rj-temp", type=float, default=0.5)
     ap.add_argument("--rj-train-iters", type=int, default=1000)
     ap.add_argument("--rj-epochs", type=int, default=6)
+    ap.add_argument("--rj-batches", type=int, default=4,
+                    help="lockstep batches of the population measurement (0: skip it)")
     ap.add_argument("--mode", default="sweep", choices=["sweep", "c5", "rejuv"],
                     help="sweep: the updates/s line (with the SMC ESS/sec comparison); c5: the "
                          "learned-proposal benchmark")

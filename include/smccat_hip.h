@@ -153,6 +153,10 @@ typedef struct {
                                     the network's examples; NULL: the Gibbs move                 */
     int32_t record_allocations;
     int32_t reserved2;           /* must be 0                                                     */
+    const int32_t *stream_keys;  /* [count] (nullable): the random streams of slot first+i are keyed
+                                    by stream_keys[i] instead of by the slot; the move types of an
+                                    iteration come from a stream of their own, so a key fixes its
+                                    chain's number of MH steps (smc_rejuvenation_steps)           */
 } smc_rejuvenation_options;
 
 /* Rejuvenation of slots [first, first+count) at tempering T, replacing the
@@ -168,6 +172,13 @@ typedef struct {
  * acceptances per move kind (tree, shape, invariant, rate matrix). */
 int smc_rejuvenate(smc_engine *e, int first, int count, double tempering,
                    const smc_rejuvenation_options *opt, double *lnl_out, int64_t *stats_out);
+
+/* The number of MH steps (proposals) smc_rejuvenate will make for each of `count`
+ * stream keys under these options: the sum over its iterations of the drawn move's
+ * repetitions.  No counterpart in the reference: lockstep batches of chains with
+ * similar step counts finish together (smc_run_sampler assigns keys to slots so). */
+int smc_rejuvenation_steps(smc_engine *e, const smc_rejuvenation_options *opt, int count,
+                           const int32_t *keys, int32_t *steps_out);
 
 /* The SMC sampler loop of main.cpp:143-336 over slots [0, num_particles),
  * which must hold the initial population (main.cpp:129-139): per iteration

@@ -43,6 +43,42 @@ double log_std_normal(double z) { return -0.5 * z * z - 0.9189385332046727; }
 
 enum Move { TREE = 0, SHAPE = 1, INVAR = 2, RATES = 3, N_MOVES = 4 };
 
+// The move table of main.cpp:54-105: weights, repetitions per iteration and the
+// weighted draw (stationary moves are no-ops and skipped: -1).  The draws come from
+// a stream of their own (seed ^ kMoveStream), so a chain's step count is known
+// before it runs.
+constexpr uint64_t kMoveStream = 0x6d6f766573ull;
+struct MoveTable {
+    double w[N_MOVES];
+    double wtot;
+    int reps[N_MOVES];
+    template <typename G>
+    int draw(G &g) const
+    {
+        const double u = g.uniform() * wtot;
+        double c = 0.0;
+        for (int k = 0; k < N_MOVES; ++k) {
+            c += w[k];
+            if (u < c) return k;
+        }
+        return -1;
+    }
+};
+
+MoveTable move_table(const smc_engine *e, const smc_rejuvenation_options &o)
+{
+    MoveTable m{};
+    const double w[N_MOVES] = {3.0, e->R > 1 ? 1.0 : 0.0, o.invariant ? 1.0 : 0.0, o.using_lg ? 0.0 : 1.0};
+    const int reps[N_MOVES] = {std::max(1, (int)(e->T * 0.25)), 2, 2, 5};
+    m.wtot = 1.0;  // the stationary move's weight
+    for (int k = 0; k < N_MOVES; ++k) {
+        m.w[k] = w[k];
+        m.wtot += w[k];
+        m.reps[k] = reps[k];
+    }
+    return m;
+}
+
 struct Params {
     std::vector<int32_t> ch;
     std::vector<double> bl, base;
@@ -195,6 +231,25 @@ extern "C" int smc_rejuvenate(smc_engine *e, int first, int count, double temper
     return smc_rejuvenate_ex(e, first, count, tempering, opt, lnl_out, stats_out, nullptr);
 }
 
+extern "C" int smc_rejuvenation_steps(smc_engine *e, const smc_rejuvenation_options *opt, int count,
+                                      const int32_t *keys, int32_t *steps_out)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (!opt || count < 0 || (count > 0 && (!keys || !steps_out)) || opt->iterations < 0)
+        return smc_fail(e, SMC_ERR_ARG, "invalid rejuvenation-steps arguments");
+    const MoveTable mt = move_table(e, *opt);
+    for (int i = 0; i < count; ++i) {
+        Rng g(opt->seed ^ kMoveStream, (uint64_t)(uint32_t)keys[i]);
+        int32_t n = 0;
+        for (int it = 0; it < opt->iterations; ++it) {
+            const int m = mt.draw(g);
+            n += m >= 0 ? mt.reps[m] : 0;
+        }
+        steps_out[i] = n;
+    }
+    return SMC_OK;
+}
+
 int smc_cat_moves(smc_engine *e, const std::vector<int32_t> &slots, double tempering,
                   const smc_rejuvenation_options &o, int64_t *cat_stats)
 {
@@ -237,16 +292,17 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     int rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
     if (rc != SMC_OK) return rc;
 
-    // the move table of main.cpp:54-105 (stationary moves are no-ops and skipped)
-    double w[N_MOVES] = {3.0, e->R > 1 ? 1.0 : 0.0, o.invariant ? 1.0 : 0.0, o.using_lg ? 0.0 : 1.0};
-    const double w_stat = 1.0;
-    int reps[N_MOVES] = {std::max(1, (int)(T * 0.25)), 2, 2, 5};
-    double wtot = w_stat;
-    for (double x : w) wtot += x;
+    const MoveTable mt = move_table(e, o);
+    const int *reps = mt.reps;
 
-    std::vector<Rng> rng;
+    std::vector<Rng> rng, mrng;
     rng.reserve(count);
-    for (int i = 0; i < count; ++i) rng.emplace_back(o.seed, (uint64_t)(first + i));
+    mrng.reserve(count);
+    for (int i = 0; i < count; ++i) {
+        const uint64_t key = o.stream_keys ? (uint64_t)(uint32_t)o.stream_keys[i] : (uint64_t)(first + i);
+        rng.emplace_back(o.seed, key);
+        mrng.emplace_back(o.seed ^ kMoveStream, key);
+    }
     std::vector<double> lnl(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count);
     std::vector<double> prior(count);
     auto load = [&](int p) {
@@ -273,15 +329,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     std::vector<int> started(count, 0);
     auto next_move = [&](int i) {
         while (left[i] <= 0 && started[i] < o.iterations) {
-            double u = rng[i].uniform() * wtot, c = 0.0;
-            move[i] = -1;  // stationary (no-op)
-            for (int k = 0; k < N_MOVES; ++k) {
-                c += w[k];
-                if (u < c) {
-                    move[i] = k;
-                    break;
-                }
-            }
+            move[i] = mt.draw(mrng[i]);
             left[i] = move[i] >= 0 ? reps[move[i]] : 0;
             started[i]++;
         }

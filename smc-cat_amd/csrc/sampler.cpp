@@ -180,9 +180,18 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
             // batch = 1 keeps the reference's shape: each particle's move right after
             // its MCMC.
             std::vector<int32_t> cat_slots;
+            // Stream keys sorted by their chains' MH step counts: a lockstep batch runs
+            // as many rounds as its longest chain has steps, so chains of similar length
+            // share a batch.  The keys of a slot do not depend on the batch size.
+            std::vector<int32_t> keys(N), steps(N);
+            for (int n = 0; n < N; ++n) keys[n] = n;
+            rc = smc_rejuvenation_steps(e, &ro, N, keys.data(), steps.data());
+            if (rc != SMC_OK) return rc;
+            std::stable_sort(keys.begin(), keys.end(), [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
             for (int first = 0; first < N; first += batch) {
                 const int cnt = std::min(batch, N - first);
                 int64_t st[8];
+                ro.stream_keys = keys.data() + first;
                 rc = smc_rejuvenate_ex(e, first, cnt, cur, &ro, lnl.data() + first, st, res->cat_stats,
                                        batch > 1 ? &cat_slots : nullptr);
                 if (rc != SMC_OK) return rc;

@@ -45,7 +45,7 @@ class RejuvenationOptions(ctypes.Structure):
                 ("invar_alpha", ctypes.c_double), ("gibbs_probability", ctypes.c_double),
                 ("dpp_alpha", ctypes.c_double), ("rate_cholesky", ctypes.c_void_p),
                 ("allocation_proposal", ctypes.c_void_p), ("record_allocations", ctypes.c_int32),
-                ("reserved2", ctypes.c_int32)]
+                ("reserved2", ctypes.c_int32), ("stream_keys", ctypes.c_void_p)]
 
 
 class Tuning(ctypes.Structure):
@@ -125,6 +125,7 @@ SIGNATURES = {
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
     "smc_accept_reject": (_I, [_P, _I, _I, _BP]),
+    "smc_rejuvenation_steps": (_I, [_P, ctypes.POINTER(RejuvenationOptions), _I, _IP, _IP]),
     "smc_rejuvenate": (_I, [_P, _I, _I, _D, ctypes.POINTER(RejuvenationOptions), _DP,
                             ctypes.POINTER(ctypes.c_int64)]),
 }

@@ -224,25 +224,45 @@ class Engine:
                                                acc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
                     "smc_accept_reject")
 
+    def _rejuv_options(self, iterations, seed, using_lg, invariant, gibbs_probability, dpp_alpha,
+                       proposal, stream_keys, deltas):
+        return _lib.RejuvenationOptions(int(iterations), int(using_lg), int(invariant), 0, int(seed),
+                                        deltas.get("scale_delta", 1.0),
+                                        deltas.get("subtree_scale_delta", 1.0),
+                                        deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
+                                        deltas.get("rate_full_scale", 0.5),
+                                        deltas.get("invar_alpha", 100.0), float(gibbs_probability),
+                                        float(dpp_alpha), None, proposal.h if proposal is not None else None,
+                                        0, 0, stream_keys.ctypes.data if stream_keys is not None else None)
+
     def rejuvenate(self, tempering, iterations, seed, first=0, count=None, using_lg=False,
-                   invariant=True, gibbs_probability=0.0, dpp_alpha=0.01, proposal=None, **deltas):
+                   invariant=True, gibbs_probability=0.0, dpp_alpha=0.01, proposal=None,
+                   stream_keys=None, **deltas):
         """MCMC rejuvenation of slots [first, first+count); returns (lnl, stats[4, 2]).
-        proposal: a ProposalNetwork makes the CAT move the learned one."""
+        proposal: a ProposalNetwork makes the CAT move the learned one; stream_keys
+        [count] (optional) keys the slots' random streams instead of the slot ids."""
         if count is None:
             count = self.max_particles - first
-        o = _lib.RejuvenationOptions(int(iterations), int(using_lg), int(invariant), 0, int(seed),
-                                     deltas.get("scale_delta", 1.0),
-                                     deltas.get("subtree_scale_delta", 1.0),
-                                     deltas.get("shape_delta", 1.0), deltas.get("rate_delta", 0.25),
-                                     deltas.get("rate_full_scale", 0.5),
-                                     deltas.get("invar_alpha", 100.0), float(gibbs_probability),
-                                     float(dpp_alpha), None, proposal.h if proposal is not None else None,
-                                     0, 0)
+        keys = None
+        if stream_keys is not None:
+            keys = np.ascontiguousarray(stream_keys, dtype=np.int32)
+            assert keys.shape == (count,)
+        o = self._rejuv_options(iterations, seed, using_lg, invariant, gibbs_probability, dpp_alpha, proposal,
+                                keys, deltas)
         lnl = np.empty(count)
         stats = (ctypes.c_int64 * 8)()
         self._check(self.lib.smc_rejuvenate(self.h, first, count, float(tempering), ctypes.byref(o),
                                             _dp(lnl), stats), "smc_rejuvenate")
         return lnl, np.array(list(stats)).reshape(4, 2)
+
+    def rejuvenation_steps(self, iterations, seed, keys, using_lg=False, invariant=True):
+        """The number of MH steps rejuvenate() makes for each stream key."""
+        keys = np.ascontiguousarray(keys, dtype=np.int32)
+        out = np.empty(len(keys), dtype=np.int32)
+        o = self._rejuv_options(iterations, seed, using_lg, invariant, 0.0, 0.01, None, None, {})
+        self._check(self.lib.smc_rejuvenation_steps(self.h, ctypes.byref(o), len(keys), _ip(keys), _ip(out)),
+                    "smc_rejuvenation_steps")
+        return out
 
     def run_sampler(self, num_particles, iterations, seed, using_lg=False, invariant=True, cat=False,
                     threshold=0.6, alg8_probability=0.1, batch=0, max_iterations=0, dpp_alpha=0.01,

@@ -64,3 +64,29 @@ def test_rejuvenation_climbs_at_full_tempering(gpu):
         l0, _ = eng.refresh_likelihood()
         l1, stats = eng.rejuvenate(1.0, 20, seed=4, using_lg=True, invariant=False)
     assert np.median(l1) > np.median(l0) + 10.0
+
+
+def test_stream_keys_fix_the_chains_and_their_step_counts(gpu):
+    """A chain depends on its particle and its stream key, not on its slot: four
+    particles moved with keys 10..13 end as the same particles in reversed slots with
+    the keys reversed.  smc_rejuvenation_steps predicts every chain's proposals."""
+    _say("case stream keys: 14 taxa x 90 sites GTR+G4+I, 4 particles")
+    rng = np.random.default_rng(41)
+    T, S, R, n = 14, 90, 4, 4
+    codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
+    batch = sim.prior_particles(rng, n, T, S, num_rates=R, invar=True)
+    rev = batch.take(np.arange(n)[::-1])
+    keys = np.arange(10, 10 + n, dtype=np.int32)
+    with Engine(T, S, R, n, 1, workspaces=2) as eng:
+        eng.set_alignment(codes)
+        steps = eng.rejuvenation_steps(6, 8, keys)
+        eng.set_particles(batch)
+        l_a, st_a = eng.rejuvenate(0.6, 6, seed=8, stream_keys=keys)
+        a = eng.snapshot(range(n), batch)
+        eng.set_particles(rev)
+        l_b, st_b = eng.rejuvenate(0.6, 6, seed=8, stream_keys=keys[::-1])
+        b = eng.snapshot(range(n), batch)
+    np.testing.assert_array_equal(l_a, l_b[::-1])
+    for f in ("children", "branch_len", "base", "pinv", "shape"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f)[::-1])
+    assert st_a[:, 0].sum() == steps.sum() == st_b[:, 0].sum()
