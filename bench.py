@@ -277,6 +277,7 @@ def rejuv_bench(args):
     t0 = time.perf_counter()
     eng.rejuvenate(tau, it, seed=5, invariant=True, gibbs_probability=0.0, dpp_alpha=1.0)
     t_mcmc = time.perf_counter() - t0
+    steps_fast = eng.rejuvenation_steps(it, 5, np.arange(nf), invariant=True)
     t0 = time.perf_counter()
     net.move(range(nf), tau, seed=6, dpp_alpha=1.0)
     t_move = time.perf_counter() - t0
@@ -288,6 +289,7 @@ def rejuv_bench(args):
              "per particle" % (t_mcmc / nf * 1e3, t_move / nf * 1e3, nf, t_ref / nf * 1e3))
     out["fp32_batched_learned_sampler_shape"] = {
         "particles": nf, "mcmc_seconds_per_particle": t_mcmc / nf,
+        "mcmc_seconds_per_step": t_mcmc / float(steps_fast.sum()),
         "cat_lockstep_particles": nf, "cat_move_seconds_per_particle": t_move / nf,
         "refresh_seconds_per_particle": t_ref / nf, "seconds_per_particle": per_def}
     net.close()
@@ -317,16 +319,12 @@ def rejuv_bench(args):
                                stream_keys=keys[b * nf:(b + 1) * nf])
             dt = time.perf_counter() - t0
             rounds = [int(steps[keys[b * nf:(b + 1) * nf]].max()) for b in range(nb)]
-            res[name] = {"mcmc_seconds_per_particle": dt / popn, "rounds_per_batch": rounds}
+            res[name] = {"mcmc_seconds_per_particle": dt / popn, "mcmc_seconds_per_step": dt / float(steps.sum()),
+                         "rounds_per_batch": rounds}
             progress("rejuv population of %d in batches of %d, %s keys: %.1f ms MCMC per particle, rounds %s"
                      % (popn, nf, name, dt / popn * 1e3, rounds))
         eng.close()
         out["fp32_batched_population"] = res
-        sh = out["fp32_batched_learned_sampler_shape"]
-        sh["population_mcmc_seconds_per_particle"] = res["sorted_by_steps"]["mcmc_seconds_per_particle"]
-        sh["seconds_per_particle_population"] = (res["sorted_by_steps"]["mcmc_seconds_per_particle"] +
-                                                 0.1 * sh["cat_move_seconds_per_particle"] +
-                                                 sh["refresh_seconds_per_particle"])
     if npl < 1:  # fast path only (profiling)
         return out
     # plain path: FP64, one particle at a time; the Gibbs move is timed on its own
@@ -341,6 +339,7 @@ def rejuv_bench(args):
     for i in range(npl):
         eng.rejuvenate(tau, it, seed=5, first=i, count=1, invariant=True, gibbs_probability=0.0)
     t_mcmc = (time.perf_counter() - t0) / npl
+    steps_plain = eng.rejuvenation_steps(it, 5, np.arange(npl), invariant=True)
     gi = max(1, S // 20)
     t0 = time.perf_counter()
     eng.gibbs_partition_move([0], tau, seed=7, num_iter=gi, dpp_alpha=1.0)
@@ -349,13 +348,27 @@ def rejuv_bench(args):
     eng.close()
     per = t_mcmc + 0.1 * t_gibbs
     progress("rejuv plain: %.2f s MCMC + 0.1 x %.1f s Gibbs per particle" % (t_mcmc, t_gibbs))
+    s_plain = t_mcmc * npl / float(steps_plain.sum())
     out["plain_fp64"] = {"particles_sampled": npl, "mcmc_seconds_per_particle": t_mcmc,
+                         "mcmc_seconds_per_step": s_plain, "mean_steps": float(steps_plain.mean()),
                          "gibbs_seconds_per_move": t_gibbs, "seconds_per_particle": per}
     out["ratio_per_particle"] = per / out["fp32_batched_learned"]["seconds_per_particle"]
     out["ratio_per_particle_sampler_shape"] = per / per_def
-    if "seconds_per_particle_population" in out["fp32_batched_learned_sampler_shape"]:
-        out["ratio_per_particle_sampler_population"] = (
-            per / out["fp32_batched_learned_sampler_shape"]["seconds_per_particle_population"])
+    # the same comparison per MH step: a few sampled chains differ in length from the
+    # batch's average, so both MCMC parts are priced at the mean step count of the
+    # batch's chains (or of the population's, when measured)
+    sh = out["fp32_batched_learned_sampler_shape"]
+    m = float(steps_fast.mean())
+    s_fast = sh["mcmc_seconds_per_step"]
+    if "fp32_batched_population" in out:
+        m = out["fp32_batched_population"]["mean_steps"]
+        s_fast = out["fp32_batched_population"]["sorted_by_steps"]["mcmc_seconds_per_step"]
+    plain_m = s_plain * m + 0.1 * t_gibbs
+    fast_m = s_fast * m + 0.1 * sh["cat_move_seconds_per_particle"] + sh["refresh_seconds_per_particle"]
+    out["per_step_normalised"] = {"mean_steps": m, "plain_seconds_per_particle": plain_m,
+                                  "fast_seconds_per_particle": fast_m, "ratio": plain_m / fast_m}
+    progress("rejuv per-step normalised at %.0f steps: plain %.3f s, fast %.4f s per particle: %.2fx"
+             % (m, plain_m, fast_m, plain_m / fast_m))
     return out
 
 
@@ -379,7 +392,7 @@ def main():
     ap.add_argument("--rj-taxa", type=int, default=0)
     ap.add_argument("--rj-sites", type=int, default=0)
     ap.add_argument("--rj-particles", type=int, default=100)
-    ap.add_argument("--rj-plain", type=int, default=4)
+    ap.add_argument("--rj-plain", type=int, default=8)
     ap.add_argument("--rj-iterations", type=int, default=16)
     ap.add_argument("--rj-temp", type=float, default=0.5)
     ap.add_argument("--rj-train-iters", type=int, default=1000)
