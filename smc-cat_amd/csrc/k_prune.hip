@@ -203,19 +203,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
 }
 
-__global__ void k_reduce_lnl(const double *__restrict__ tile_lnl, const int *__restrict__ ntiles,
-                             int tile_stride, int first, int count, double *__restrict__ lnl,
-                             uint8_t *__restrict__ nonfinite)
+// One wavefront per particle: lane l sums tiles l, l + 64, ... in order, then a
+// butterfly over the lanes; the order is fixed, so the sum does not depend on timing.
+__global__ __launch_bounds__(256) void k_reduce_lnl(const double *__restrict__ tile_lnl,
+                                                     const int *__restrict__ ntiles, int tile_stride, int first,
+                                                     int count, double *__restrict__ lnl,
+                                                     uint8_t *__restrict__ nonfinite)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= count) return;  // whole wavefront
     const int p = first + i;
     const double *t = tile_lnl + (size_t)p * tile_stride;
-    double acc = 0.0;
     const int nt = ntiles[p];
-    for (int k = 0; k < nt; k++) acc += t[k];
-    lnl[p] = acc;
-    nonfinite[p] = isfinite(acc) ? 0 : 1;
+    double acc = 0.0;
+    for (int k = lane; k < nt; k += 64) acc += t[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) {
+        lnl[p] = acc;
+        nonfinite[p] = isfinite(acc) ? 0 : 1;
+    }
 }
 
 }  // namespace
@@ -249,8 +257,8 @@ void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_strid
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s)
 {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_reduce_lnl, dim3((count + 255) / 256), dim3(256), 0, s, tile_lnl,
-                       ntiles, tile_stride, first, count, lnl, nonfinite);
+    hipLaunchKernelGGL(k_reduce_lnl, dim3((count + 3) / 4), dim3(256), 0, s, tile_lnl, ntiles, tile_stride,
+                       first, count, lnl, nonfinite);
 }
 
 }  // namespace smccat
