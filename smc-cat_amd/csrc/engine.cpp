@@ -33,6 +33,32 @@ int smc_fail(smc_engine *e, int code, const std::string &msg)
     return code;
 }
 
+int sync_stream(smc_engine *e)
+{
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->stage_off = 0;  // every staged copy has completed
+    return SMC_OK;
+}
+
+int h2d(smc_engine *e, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return SMC_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > e->stage_cap) {  // larger than the staging buffer: copy and wait
+        HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream));
+        return sync_stream(e);
+    }
+    if (e->stage_off + need > e->stage_cap) {  // full: wait for the queued copies
+        int rc = sync_stream(e);
+        if (rc != SMC_OK) return rc;
+    }
+    uint8_t *st = e->h_stage + e->stage_off;
+    std::memcpy(st, src, bytes);
+    e->stage_off += need;
+    HIP_TRY(e, hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, e->stream));
+    return SMC_OK;
+}
+
 namespace {
 
 int fail(smc_engine *e, int code, const std::string &msg) { return smc_fail(e, code, msg); }
@@ -421,6 +447,17 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     (void)hipMemsetAsync(e->d_nclass, 0, P * sizeof(int), e->stream);
     (void)hipMemsetAsync(e->d_xflag, 0, P * (T - 1), e->stream);
     (void)hipStreamSynchronize(e->stream);
+    {
+        constexpr size_t kStageBytes = size_t(16) << 20;
+        void *hs = nullptr;
+        if (hipHostMalloc(&hs, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+            smc_engine_destroy(e);
+            fprintf(stderr, "smc_engine_create: pinned staging buffer of %zu bytes failed\n", kStageBytes);
+            return SMC_ERR_HIP;
+        }
+        e->h_stage = (uint8_t *)hs;
+        e->stage_cap = kStageBytes;
+    }
     e->h_children.assign(P * (T - 1) * 2, 0);
     e->h_root.assign(P, -1);
     e->h_height.assign(P * (T - 1), 0);
@@ -457,6 +494,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
@@ -858,17 +896,15 @@ int set_single_wave(smc_engine *e, int first, int count)
         ncls += e->h_nclass[first + i];
     }
     if (ncls > e->Cmax) return fail(e, SMC_ERR_ARG, "range exceeds pmat_classes");
-    HIP_TRY(e, hipMemcpyAsync(e->d_pcls_off + first, pcls.data(), count * sizeof(int),
-                              hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipStreamSynchronize(e->stream));  // pcls is a local buffer
+    int rc = h2d(e, e->d_pcls_off + first, pcls.data(), count * sizeof(int));
+    if (rc != SMC_OK) return rc;
     e->pcls_first = first;
     e->pcls_count = count;
     e->pcls_version = e->class_version;
     return SMC_OK;
 }
 
-// Eigen-decompositions and/or P matrices of the particles in the lists (the
-// lists must stay alive until the caller synchronises the stream).
+// Eigen-decompositions and/or P matrices of the particles in the lists.
 int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
                      const std::vector<int> &p_list, const std::vector<Item> &p_nodes = {})
 {
@@ -878,8 +914,7 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
             return fail(e, SMC_ERR_STATE, "too many P nodes");
         // grouped by particle (the lists are built particle by particle):
         // [particles | offsets | nodes] in one upload
-        std::vector<int> &buf = e->h_pnode_buf;
-        buf.clear();
+        std::vector<int> buf;
         std::vector<int> plist, noff{0}, nodes;
         for (size_t i = 0; i < p_nodes.size(); ++i) {
             if (i == 0 || p_nodes[i].particle != p_nodes[i - 1].particle) {
@@ -896,15 +931,16 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
         buf.insert(buf.end(), noff.begin(), noff.end());
         buf.insert(buf.end(), nodes.begin(), nodes.end());
         int *d = reinterpret_cast<int *>(e->d_pnodes);  // capacity: 2 ints per (particle, node)
-        HIP_TRY(e, hipMemcpyAsync(d, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        int rc = h2d(e, d, buf.data(), buf.size() * sizeof(int));
+        if (rc != SMC_OK) return rc;
         TimedLaunch tl(e, SMC_K_PMAT);
         PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
                     e->R, e->Kmax, first, 0, e->fp32, nullptr};
         launch_pmat_nodes(pa, d, d + np, d + 2 * np + 1, np, max_nodes, s);
     }
     if (!eig_list.empty()) {
-        HIP_TRY(e, hipMemcpyAsync(e->d_plist, eig_list.data(), eig_list.size() * sizeof(int),
-                                  hipMemcpyHostToDevice, s));
+        int rc = h2d(e, e->d_plist, eig_list.data(), eig_list.size() * sizeof(int));
+        if (rc != SMC_OK) return rc;
         TimedLaunch tl(e, SMC_K_EIGEN);
         EigenArgs ea{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first,
                      (int)eig_list.size(), e->d_plist};
@@ -912,8 +948,8 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
     }
     if (!p_list.empty()) {
         int *dl = e->d_plist + e->Pmax;  // second half: the eigen launch may still read the first
-        HIP_TRY(e, hipMemcpyAsync(dl, p_list.data(), p_list.size() * sizeof(int),
-                                  hipMemcpyHostToDevice, s));
+        int rc = h2d(e, dl, p_list.data(), p_list.size() * sizeof(int));
+        if (rc != SMC_OK) return rc;
         TimedLaunch tl(e, SMC_K_PMAT);
         PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
                     e->R, e->Kmax, first, (int)p_list.size(), e->fp32, dl};
@@ -967,23 +1003,18 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
         e->h_pinv[p] = pinv[i];
         e->h_shape[p] = shape[i];
     }
-    hipStream_t s = e->stream;
     const size_t P0 = first;
-    HIP_TRY(e, hipMemcpyAsync(e->d_children + P0 * Ti * 2, e->h_children.data() + P0 * Ti * 2,
-                              (size_t)count * Ti * 2 * sizeof(int), hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_blen + P0 * nn, blen.data(), blen.size() * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_base + P0 * 190, e->h_base.data() + P0 * 190,
-                              (size_t)count * 190 * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_pinv + P0, e->h_pinv.data() + P0, (size_t)count * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_rates + P0 * R, rates.data(), rates.size() * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_root + P0, roots.data(), roots.size() * sizeof(int),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipStreamSynchronize(s));
+    int rc = h2d(e, e->d_children + P0 * Ti * 2, e->h_children.data() + P0 * Ti * 2,
+                 (size_t)count * Ti * 2 * sizeof(int));
+    if (rc == SMC_OK) rc = h2d(e, e->d_blen + P0 * nn, blen.data(), blen.size() * sizeof(double));
+    if (rc == SMC_OK)
+        rc = h2d(e, e->d_base + P0 * 190, e->h_base.data() + P0 * 190, (size_t)count * 190 * sizeof(double));
+    if (rc == SMC_OK) rc = h2d(e, e->d_pinv + P0, e->h_pinv.data() + P0, (size_t)count * sizeof(double));
+    if (rc == SMC_OK) rc = h2d(e, e->d_rates + P0 * R, rates.data(), rates.size() * sizeof(double));
+    if (rc == SMC_OK) rc = h2d(e, e->d_root + P0, roots.data(), roots.size() * sizeof(int));
+    if (rc == SMC_OK)
+        rc = h2d(e, e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int));
+    if (rc != SMC_OK) return rc;
     e->topo_version++;
     return SMC_OK;
 }
@@ -1109,8 +1140,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     rc = recompute_models(e, first, eig_list, p_list, p_nodes);
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
-    HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
-                              (size_t)count * Ti, hipMemcpyHostToDevice, s));
+    rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
+    if (rc != SMC_OK) return rc;
     std::vector<Item> items;
     std::vector<int> off;
     for (auto &b : buckets) {
@@ -1120,8 +1151,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     off.push_back((int)items.size());
     const int root_off = (int)items.size();
     items.insert(items.end(), roots.begin(), roots.end());
-    HIP_TRY(e, hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(Item),
-                              hipMemcpyHostToDevice, s));
+    rc = h2d(e, e->d_items, items.data(), items.size() * sizeof(Item));
+    if (rc != SMC_OK) return rc;
     e->sched_first = -1;  // the item buffer no longer holds the full schedule
     PruneArgs a = prune_args(e, first);
     a.xflag = e->d_xflag;
@@ -1172,7 +1203,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     if (nonfinite_out)
         HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
                                   hipMemcpyDeviceToHost, s));
-    HIP_TRY(e, hipStreamSynchronize(s));
+    rc = sync_stream(e);
+    if (rc != SMC_OK) return rc;
     collect_timings(e);
     for (size_t k = 0; k < keep.size(); ++k) {
         e->h_lnl[keep[k]] = keep_lnl[k];
@@ -1236,12 +1268,17 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
         e->h_dirty[p] = 0;
         e->h_pnodes[p].clear();
     }
-    HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
-                              (size_t)count * Ti, hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_lnl + first, e->h_lnl.data() + first, (size_t)count * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipStreamSynchronize(s));
-    collect_timings(e);
+    int rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
+    if (rc == SMC_OK)
+        rc = h2d(e, e->d_lnl + first, e->h_lnl.data() + first, (size_t)count * sizeof(double));
+    if (rc != SMC_OK) return rc;
+    // the restores run on while the caller draws its next proposals (the stream
+    // orders them before any later work); timed launches need their events done
+    if (e->timing) {
+        rc = sync_stream(e);
+        if (rc != SMC_OK) return rc;
+        collect_timings(e);
+    }
     return SMC_OK;
 }
 

@@ -101,7 +101,6 @@ struct smc_engine {
     std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
     std::vector<uint8_t> h_touched;             // [P] pInvar changed by the last parameter upload
     Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
-    std::vector<int> h_pnode_buf;               // host staging of those lists (kept until the next call)
     struct Committed {
         std::vector<int32_t> children;
         std::vector<double> blen, base;
@@ -155,6 +154,11 @@ struct smc_engine {
     double phase_ms[SMC_PHASE_COUNT] = {};
     int64_t phase_calls[SMC_PHASE_COUNT] = {};
 
+    // pinned staging of the uploads of the per-step calls (h2d): bump-allocated,
+    // recycled at sync_stream
+    uint8_t *h_stage = nullptr;
+    size_t stage_cap = 0, stage_off = 0;
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -164,6 +168,14 @@ struct smc_engine {
 };
 
 int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+// Queues a copy of `bytes` host bytes to device `dst` on the engine's stream
+// through the pinned staging buffer: it returns without waiting for the device,
+// and `src` may be changed at once.  (Copies from pageable memory wait for the
+// stream to drain first, which leaves the GPU idle while the host catches up.)
+int h2d(smc_engine *e, void *dst, const void *src, size_t bytes);
+// Waits for the engine's stream; the staging buffer is free again afterwards.
+int sync_stream(smc_engine *e);
 
 // Adds the wall time of its scope to e->phase_ms[phase].
 struct PhaseTimer {
