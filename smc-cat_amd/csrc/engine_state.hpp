@@ -122,8 +122,8 @@ struct smc_engine {
         int max_level;
         int graph_levels;             // max_level rounded up: the levels a captured sweep walks
         int boff;                     // first entry of this wave's offsets in d_bucket_off
-        // items of (level L, kind k): [off[3L+k], off[3L+k+1]) for L <= graph_levels, then
-        // the roots: [off[nb], off[nb+1]) with nb = 3 (graph_levels + 1)
+        // items of (level L, kind k): [off[K L + k], off[K L + k + 1]) for L <= graph_levels
+        // (K = PRUNE_KINDS), then the roots: [off[nb], off[nb + 1]) with nb = K (graph_levels + 1)
         std::vector<int> bucket_off;
     };
     int *d_bucket_off = nullptr;      // the waves' bucket offsets on the device
