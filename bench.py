@@ -245,6 +245,7 @@ def rejuv_bench(args):
                        "CAT move probability 0.1, DP alpha 1.0, tempering %.2f" % (T, S, kr[0], kr[1], it, tau)}
     # fast path: FP32, lockstep batch, learned CAT move (trained beforehand on Gibbs draws)
     eng = Engine(T, S, R, nf, kmax, "fp32", workspaces=2)
+    eng.set_graphs(not args.no_graphs)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
     eng.set_particles(batch)
@@ -305,6 +306,7 @@ def rejuv_bench(args):
                                   max_classes=kmax)
         eng = Engine(T, S, R, popn, kmax, "fp32", workspaces=2, partial_slots=nf,
                      pmat_classes=int(nf * (kr[0] + kr[1]) / 2 * 1.4))
+        eng.set_graphs(not args.no_graphs)
         eng.set_mixture("logspace")
         eng.set_alignment(codes)
         slot_keys = np.arange(popn, dtype=np.int32)
@@ -332,6 +334,7 @@ def rejuv_bench(args):
     # probability 0.1, so a small sample gives the expected cost
     sub = batch.take(np.arange(npl))
     eng = Engine(T, S, R, npl, kmax, "fp64", workspaces=2)
+    eng.set_graphs(not args.no_graphs)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
     eng.set_particles(sub)

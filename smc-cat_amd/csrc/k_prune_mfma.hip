@@ -307,13 +307,12 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     const int nsb = (tl.len + 15) >> 4;               // 16-slot column blocks of the window in use
     const int sb_begin = ((tl.lo >> 4) / U) * U;      // the first step holding a valid slot
 
-    // running log-sum-exp over rates of the root's 4 column blocks
-    double lse_m[4], lse_s[4];
-#pragma unroll
-    for (int sb = 0; sb < 4; sb++) {
-        lse_m[sb] = -INFINITY;
-        lse_s[sb] = 0.0;
-    }
+    // Root mixture: after the column blocks of a rate, lane group g4 holds the
+    // column (site c16) of block g4, so each (site, rate) log and log-sum-exp step is
+    // evaluated once per wavefront rather than once per lane group (the FP64 log
+    // and exp dominate the root's instruction count otherwise).  The running
+    // log-sum-exp over rates of that column:
+    double lse_m = -INFINITY, lse_s = 0.0;
 
     // workspaces of the stored children and of this node (read once: the flags do
     // not change during a launch)
@@ -393,6 +392,8 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
             w1 = w2;
             w2 = w3;
         }
+        double rdot = 0.0;  // root: pi . x and exponent of this lane group's column
+        int rexp = 0;
 #pragma unroll 1
         for (int sb0 = sb_begin; sb0 < nsb; sb0 += U) {
             V4 y[U][2], w[U][2];
@@ -551,12 +552,12 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                         }
                     dot += __shfl_xor(dot, 16, 64);
                     dot += __shfl_xor(dot, 32, 64);
-                    const double l = log(dot) + (double)etot[u] * kLn2m;
-                    // the column block is wave-uniform and < 4: select its accumulator
-                    // without dynamic register indexing
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if (k == sb0 + u) lse_push(lse_m[k], lse_s[k], l, mix_mode);
+                    // every lane group now holds the column's pi . x; group g4 keeps
+                    // block g4's (the block index sb0 + u is wave-uniform and < 4)
+                    if (g4 == sb0 + u) {
+                        rdot = dot;
+                        rexp = etot[u];
+                    }
                 }
             }
             if constexpr (!IS_ROOT) {
@@ -607,18 +608,15 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
                 }
             }
         }
+        // (a block outside the tile leaves rdot = 0: its lanes hold no valid site)
+        if constexpr (IS_ROOT) lse_push(lse_m, lse_s, log(rdot) + (double)rexp * kLn2m, mix_mode);
     }
     if constexpr (IS_ROOT) {
         double lf = 0.0;
-        const double pv = pinv[p];
-#pragma unroll
-        for (int sb = 0; sb < 4; sb++) {
-            const int o = site_of(sb / U * U, sb % U, c16);
-            if (g4 == 0 && o >= tl.lo && o < tl.len) {
-                const int orig = perm[(size_t)p * S_pad + tl.start + o];
-                lf += site_log_likelihood(lse_m[sb], lse_s[sb], pv, pi[invchar[orig]], isinv[orig],
-                                          mix_mode);
-            }
+        const int o = site_of(g4 / U * U, g4 % U, c16);  // this lane's column: block g4
+        if (o >= tl.lo && o < tl.len) {
+            const int orig = perm[(size_t)p * S_pad + tl.start + o];
+            lf = site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig], mix_mode);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) lf += __shfl_xor(lf, off, 64);
