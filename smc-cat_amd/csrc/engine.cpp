@@ -37,6 +37,29 @@ int sync_stream(smc_engine *e)
 {
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     e->stage_off = 0;  // every staged copy has completed
+    for (const smc_engine::Readback &r : e->readbacks) std::memcpy(r.dst, e->h_read + r.off, r.bytes);
+    e->readbacks.clear();
+    e->read_off = 0;
+    return SMC_OK;
+}
+
+int d2h(smc_engine *e, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return SMC_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > e->read_cap) {  // larger than the readback buffer: copy and wait
+        int rc = sync_stream(e);
+        if (rc != SMC_OK) return rc;
+        HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->stream));
+        return sync_stream(e);
+    }
+    if (e->read_off + need > e->read_cap) {  // full: wait and copy the queued ones out
+        int rc = sync_stream(e);
+        if (rc != SMC_OK) return rc;
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->h_read + e->read_off, src, bytes, hipMemcpyDeviceToHost, e->stream));
+    e->readbacks.push_back(smc_engine::Readback{dst, e->read_off, bytes});
+    e->read_off += need;
     return SMC_OK;
 }
 
@@ -199,11 +222,9 @@ int build_schedule(smc_engine *e, int first, int count)
         const int64_t cap = std::max<int64_t>((int64_t)boffs.size(), 1024);
         HIP_TRY(e, hipMalloc((void **)&e->d_bucket_off, (size_t)cap * sizeof(int)));
         e->bucket_cap = cap;
-        if (e->graph_exec) {  // the captured sweep reads the old buffer
-            (void)hipGraphExecDestroy(e->graph_exec);
-            e->graph_exec = nullptr;
-            e->graph_key.clear();
-        }
+        for (smc_engine::CachedGraph &c : e->graphs)  // captured sweeps read the old buffer
+            (void)hipGraphExecDestroy(c.exec);
+        e->graphs.clear();
     }
     HIP_TRY(e, hipMemcpyAsync(e->d_bucket_off, boffs.data(), boffs.size() * sizeof(int),
                               hipMemcpyHostToDevice, e->stream));
@@ -457,6 +478,15 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
         }
         e->h_stage = (uint8_t *)hs;
         e->stage_cap = kStageBytes;
+        constexpr size_t kReadBytes = size_t(4) << 20;
+        void *hr = nullptr;
+        if (hipHostMalloc(&hr, kReadBytes, hipHostMallocDefault) != hipSuccess) {
+            smc_engine_destroy(e);
+            fprintf(stderr, "smc_engine_create: pinned readback buffer of %zu bytes failed\n", kReadBytes);
+            return SMC_ERR_HIP;
+        }
+        e->h_read = (uint8_t *)hr;
+        e->read_cap = kReadBytes;
     }
     e->h_children.assign(P * (T - 1) * 2, 0);
     e->h_root.assign(P, -1);
@@ -495,7 +525,8 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
-    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->h_read) (void)hipHostFree(e->h_read);
+    for (smc_engine::CachedGraph &c : e->graphs) (void)hipGraphExecDestroy(c.exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
     for (void *b : {(void *)e->pool.d_eig, (void *)e->pool.d_logits, (void *)e->pool.d_ent})
@@ -816,26 +847,36 @@ int run_sweep(smc_engine *e, int first, int count)
         key.push_back(w.graph_levels);
         key.push_back(w.boff);
     }
-    if (!e->graph_exec || key != e->graph_key) {
-        if (e->graph_exec) {
-            HIP_TRY(e, hipGraphExecDestroy(e->graph_exec));
-            e->graph_exec = nullptr;
+    smc_engine::CachedGraph *hit = nullptr;
+    for (smc_engine::CachedGraph &c : e->graphs)
+        if (c.key == key) {
+            hit = &c;
+            break;
         }
-        e->graph_key.clear();
+    if (!hit) {
+        if (e->graphs.size() >= smc_engine::kMaxGraphs) {  // drop the least recently used
+            auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
+                                        [](const smc_engine::CachedGraph &a, const smc_engine::CachedGraph &b) {
+                                            return a.used < b.used;
+                                        });
+            HIP_TRY(e, hipGraphExecDestroy(lru->exec));
+            e->graphs.erase(lru);
+        }
         hipGraph_t g = nullptr;
         HIP_TRY(e, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         issue_sweep(e, first, count, true);
         HIP_TRY(e, hipStreamEndCapture(s, &g));
-        hipError_t st = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+        hipGraphExec_t exec = nullptr;
+        hipError_t st = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (st != hipSuccess) {
-            e->graph_exec = nullptr;
+        if (st != hipSuccess)
             return smc_fail(e, SMC_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(st));
-        }
-        e->graph_key = key;
+        e->graphs.push_back(smc_engine::CachedGraph{key, exec, 0});
+        hit = &e->graphs.back();
         e->graph_captures++;
     }
-    HIP_TRY(e, hipGraphLaunch(e->graph_exec, s));
+    hit->used = ++e->graph_clock;
+    HIP_TRY(e, hipGraphLaunch(hit->exec, s));
     e->graph_launches++;
     return SMC_OK;
 }
@@ -861,18 +902,16 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
         // forced refresh: every internal node writes its other workspace
         // (Particle.cpp:566-571), so the committed partials stay intact until commit
         for (size_t k = (size_t)first * Ti; k < (size_t)(first + count) * Ti; ++k) e->h_xflag[k] ^= 1;
-        HIP_TRY(e, hipMemcpyAsync(e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti,
-                                  (size_t)count * Ti, hipMemcpyHostToDevice, s));
+        rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
+        if (rc != SMC_OK) return rc;
     }
     rc = run_sweep(e, first, count);
     if (rc != SMC_OK) return rc;
     HIP_TRY(e, hipGetLastError());
-    HIP_TRY(e, hipMemcpyAsync(e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-    if (nonfinite_out)
-        HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
-                                  hipMemcpyDeviceToHost, s));
-    HIP_TRY(e, hipStreamSynchronize(s));
+    rc = d2h(e, e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double));
+    if (rc == SMC_OK && nonfinite_out) rc = d2h(e, nonfinite_out, e->d_nonfinite + first, (size_t)count);
+    if (rc == SMC_OK) rc = sync_stream(e);
+    if (rc != SMC_OK) return rc;
     collect_timings(e);
     if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
     smc_commit_range(e, first, count);
@@ -1198,12 +1237,9 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                           e->d_nonfinite, s);
     }
     HIP_TRY(e, hipGetLastError());
-    HIP_TRY(e, hipMemcpyAsync(e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-    if (nonfinite_out)
-        HIP_TRY(e, hipMemcpyAsync(nonfinite_out, e->d_nonfinite + first, (size_t)count,
-                                  hipMemcpyDeviceToHost, s));
-    rc = sync_stream(e);
+    rc = d2h(e, e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double));
+    if (rc == SMC_OK && nonfinite_out) rc = d2h(e, nonfinite_out, e->d_nonfinite + first, (size_t)count);
+    if (rc == SMC_OK) rc = sync_stream(e);
     if (rc != SMC_OK) return rc;
     collect_timings(e);
     for (size_t k = 0; k < keep.size(); ++k) {
@@ -1403,12 +1439,10 @@ int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, co
         ent[count + i] = sites[i];
         ent[2 * count + i] = ncand[i];
     }
-    HIP_TRY(e, hipMemcpyAsync(e->d_sc_ent, ent.data(), ent.size() * sizeof(int),
-                              hipMemcpyHostToDevice, s));
-    HIP_TRY(e, hipMemcpyAsync(e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-    if (cand_class)
-        HIP_TRY(e, hipMemcpyAsync(e->d_sc_cls, cand_class, (size_t)need * sizeof(int), hipMemcpyHostToDevice, s));
+    int rc = h2d(e, e->d_sc_ent, ent.data(), ent.size() * sizeof(int));
+    if (rc == SMC_OK) rc = h2d(e, e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double));
+    if (rc == SMC_OK && cand_class) rc = h2d(e, e->d_sc_cls, cand_class, (size_t)need * sizeof(int));
+    if (rc != SMC_OK) return rc;
     {
         TimedLaunch tl(e, SMC_K_EIGEN);
         launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count,
@@ -1442,9 +1476,9 @@ int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, co
         launch_site_class(a, s);
     }
     HIP_TRY(e, hipGetLastError());
-    HIP_TRY(e, hipMemcpyAsync(out, e->d_sc_out, (size_t)need * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-    HIP_TRY(e, hipStreamSynchronize(s));
+    rc = d2h(e, out, e->d_sc_out, (size_t)need * sizeof(double));
+    if (rc == SMC_OK) rc = sync_stream(e);
+    if (rc != SMC_OK) return rc;
     collect_timings(e);
     return SMC_OK;
 }

@@ -140,8 +140,18 @@ struct smc_engine {
     Tune tune;
     int num_cus = 256;
     bool use_graph = true;
-    std::vector<int64_t> graph_key;
-    hipGraphExec_t graph_exec = nullptr;
+    // captured sweeps keyed by their launch sequence (particle range, waves, level
+    // counts, settings); the least recently used one is dropped beyond kMaxGraphs,
+    // so a one-particle-at-a-time caller replays one graph per slot instead of
+    // capturing anew on every call
+    struct CachedGraph {
+        std::vector<int64_t> key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    static constexpr size_t kMaxGraphs = 256;
+    std::vector<CachedGraph> graphs;
+    uint64_t graph_clock = 0;
     int64_t graph_launches = 0, graph_captures = 0;
     int sched_first = -1, sched_count = -1;
     // single-wave class layout last written to d_pcls_off (incremental path)
@@ -158,6 +168,15 @@ struct smc_engine {
     // recycled at sync_stream
     uint8_t *h_stage = nullptr;
     size_t stage_cap = 0, stage_off = 0;
+    // pinned readback of the per-step calls' results (d2h): bump-allocated, copied
+    // out to their destinations and recycled at sync_stream
+    struct Readback {
+        void *dst;
+        size_t off, bytes;
+    };
+    uint8_t *h_read = nullptr;
+    size_t read_cap = 0, read_off = 0;
+    std::vector<Readback> readbacks;
 
     // timing
     bool timing = false;
@@ -174,7 +193,13 @@ int smc_fail(smc_engine *e, int code, const std::string &msg);
 // and `src` may be changed at once.  (Copies from pageable memory wait for the
 // stream to drain first, which leaves the GPU idle while the host catches up.)
 int h2d(smc_engine *e, void *dst, const void *src, size_t bytes);
-// Waits for the engine's stream; the staging buffer is free again afterwards.
+// Queues a copy of `bytes` device bytes at `src` to host `dst` on the engine's
+// stream through the pinned readback buffer; `dst` holds them after the next
+// sync_stream.  (A copy to pageable memory is staged by the runtime with a wait of
+// its own.)
+int d2h(smc_engine *e, void *dst, const void *src, size_t bytes);
+// Waits for the engine's stream and copies the queued readbacks out; the staging
+// and readback buffers are free again afterwards.
 int sync_stream(smc_engine *e);
 
 // Adds the wall time of its scope to e->phase_ms[phase].

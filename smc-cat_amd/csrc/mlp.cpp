@@ -600,11 +600,13 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 c[21] = c[22] = c[23] = 0.f;
             }
         }
-        HIP_TRY(e, hipMemcpyAsync(m->d_mv_hdr, hdr.data(), hdr.size() * 4, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(e, hipMemcpyAsync(m->d_mv_K, Kv.data(), (size_t)count * 4, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(e, hipMemcpyAsync(m->d_mv_coff, coff.data(), (size_t)count * 8, hipMemcpyHostToDevice, e->stream));
+        // per-step copies go through the engine's pinned staging and readback buffers
         const size_t n_ent = (size_t)(coff[count - 1] + Kv[count - 1]);
-        HIP_TRY(e, hipMemcpyAsync(m->d_mv_cls, cls.data(), n_ent * kCls * 4, hipMemcpyHostToDevice, e->stream));
+        rc = h2d(e, m->d_mv_hdr, hdr.data(), hdr.size() * 4);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_K, Kv.data(), (size_t)count * 4);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_coff, coff.data(), (size_t)count * 8);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_cls, cls.data(), n_ent * kCls * 4);
+        if (rc != SMC_OK) return rc;
         MlpArgs a = batch_args(m, count, nullptr);
         a.hdr = m->d_mv_hdr;
         a.K = m->d_mv_K;
@@ -615,8 +617,9 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         launch_mlp_trunk(a, false, e->stream);
         launch_mlp_head(a, false, e->stream);
         HIP_TRY(e, hipGetLastError());
-        HIP_TRY(e, hipMemcpyAsync(logq.data(), m->d_logq, logq.size() * 4, hipMemcpyDeviceToHost, e->stream));
-        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        rc = d2h(e, logq.data(), m->d_logq, logq.size() * 4);
+        if (rc == SMC_OK) rc = sync_stream(e);
+        if (rc != SMC_OK) return rc;
 
         // draw the proposals; a different slot needs the exact site likelihoods
         ent_p.clear();

@@ -58,6 +58,24 @@ def test_graph_replays_equal_eager(gpu, precision):
         np.testing.assert_allclose(got[:4], ref, rtol=1e-10)
 
 
+def test_graph_cache_one_slot_at_a_time(gpu):
+    """One-particle refreshes of alternating slots (the plain path's call shape)
+    replay one cached graph per slot and equal the eager sweep on every call."""
+    _say("case graph cache: 6 particles refreshed one at a time, 24 taxa x 300 sites")
+    eng, codes, batch = _setup(N=6)
+    with eng:
+        eng.set_graphs(False)
+        want, _ = eng.refresh_likelihood()
+        eng.set_graphs(True)
+        c0 = eng.graph_stats()["captures"]
+        for rep in range(3):
+            for p in range(6):
+                got, _ = eng.refresh_likelihood(p, 1)
+                assert np.array_equal(got, want[p:p + 1]), (rep, p)
+        st = eng.graph_stats()
+        assert st["captures"] - c0 == 6 and st["launches"] >= 18
+
+
 def test_every_launch_shape_gives_identical_lnl(gpu):
     _say("case launch shapes: CAT K<=6, 12 particles")
     eng, codes, batch = _setup(N=12, kmax=6)
