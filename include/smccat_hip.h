@@ -399,8 +399,12 @@ int smc_engine_reset_stats(smc_engine *e);
 
 /* Kernel variant of a family (tuning; no reference counterpart).  For
  * SMC_K_PRUNE: SMC_PRUNE_MFMA (matrix cores, the default) or SMC_PRUNE_VALU
- * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding. */
-enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1 };
+ * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding.
+ * SMC_PRUNE_MFMA runs the short dirty paths of an incremental refresh as chains
+ * (one launch walks each path bottom-up with the partial in registers);
+ * SMC_PRUNE_MFMA_LEVELS launches them node level by node level instead, with the
+ * same partials bit for bit. */
+enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1, SMC_PRUNE_MFMA_LEVELS = 2 };
 int smc_engine_set_variant(smc_engine *e, int family, int variant);
 
 /* Site mixture at the root.  SMC_MIX_REFERENCE (default) is the reference's

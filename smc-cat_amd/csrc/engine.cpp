@@ -86,6 +86,10 @@ namespace {
 
 int fail(smc_engine *e, int code, const std::string &msg) { return smc_fail(e, code, msg); }
 
+// An incremental refresh runs a particle's dirty nodes as chains when at most this
+// many of them are stored partials (tree moves touch one or two paths to the root).
+constexpr int kMaxChainNodes = 48;
+
 template <typename T>
 int dev_alloc(smc_engine *e, T **ptr, size_t count)
 {
@@ -443,6 +447,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_lnl, P);
     A(&e->d_nonfinite, P);
     A(&e->d_items, P * (T - 1));
+    A(&e->d_chain_nodes, P * (T - 1));
+    A(&e->d_chain_len, P * (T - 1));
     A(&e->d_pcls_off, P);
     A(&e->d_prog, P * site_prog_stride(T));
     A(&e->d_plist, 2 * P);
@@ -522,7 +528,8 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
-                    e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
+                    e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls,
+                    e->d_chain_nodes, e->d_chain_len};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->h_read) (void)hipHostFree(e->h_read);
@@ -1089,7 +1096,10 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<std::vector<Item>> buckets;  // (level * PRUNE_KINDS + kind)
     std::vector<Item> roots;
     std::vector<uint8_t> dirty(nn);
-    std::vector<int> parent(nn), dlevel(nn, 0), dl_order;
+    std::vector<int> parent(nn), dlevel(nn, 0), dl_order, clevel(nn, 0);
+    std::vector<std::vector<Item>> chains;              // per chain level: {particle, first node}
+    std::vector<std::vector<int>> chain_lens_by_level;  // their lengths
+    std::vector<int> chain_nodes;
     std::vector<Item> p_nodes;
     std::vector<int> keep;
     std::vector<double> keep_lnl;
@@ -1158,13 +1168,54 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             }
             dlevel[v] = L;
         }
+        // Short dirty paths run as chains (see launch_prune_chain): a chain starts at
+        // every dirty stored node that has no dirty stored child or two of them, and
+        // climbs while the parent's only dirty stored child is the chain's top; a
+        // chain runs after the chains that end at its first node's children.  Larger
+        // dirty sets (rate-matrix and shape moves, big subtree scalings) keep the
+        // per-level bucket launches and their kind-specialised kernels.
+        int n_stored = 0;
+        for (int n = T; n < nn; ++n)
+            if (dirty[n] && n != root && prune_kind(ch, T, n) != PRUNE_TT) n_stored++;
+        const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && n_stored <= kMaxChainNodes;
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
+            if (use_chains) continue;
             const int L = dlevel[n];
             const int kind = prune_kind(ch, T, n);
             if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
             buckets[PRUNE_KINDS * L + kind].push_back(Item{p, n});
+        }
+        if (use_chains) {
+            auto stored_dirty = [&](int c) { return c >= T && dirty[c] && prune_kind(ch, T, c) != PRUNE_TT; };
+            auto dsc = [&](int v) {
+                return (int)stored_dirty(ch[2 * (v - T)]) + (int)stored_dirty(ch[2 * (v - T) + 1]);
+            };
+            // dl_order is a pre-order of the dirty subtree: walked backwards, a node's
+            // dirty children come before it
+            for (size_t k = dl_order.size(); k-- > 0;) {
+                const int v = dl_order[k];
+                if (v == root || !stored_dirty(v)) continue;
+                const int d = dsc(v);
+                if (d == 1) continue;  // continues the chain of its dirty stored child
+                int L = 1;
+                if (d == 2)
+                    L = 1 + std::max(clevel[ch[2 * (v - T)]], clevel[ch[2 * (v - T) + 1]]);
+                if ((int)chains.size() < L + 1) chains.resize(L + 1);
+                chains[L].push_back(Item{p, (int)chain_nodes.size()});
+                int q = v, len = 0;
+                for (;;) {
+                    chain_nodes.push_back(q);
+                    clevel[q] = L;
+                    len++;
+                    const int up = parent[q];
+                    if (up == root || dsc(up) != 1) break;
+                    q = up;
+                }
+                chain_lens_by_level.resize(chains.size());
+                chain_lens_by_level[L].push_back(len);
+            }
         }
         bool changed = eig || e->h_touched[p];  // pInvar enters at the root only
         e->h_touched[p] = 0;
@@ -1188,9 +1239,21 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         items.insert(items.end(), b.begin(), b.end());
     }
     off.push_back((int)items.size());
+    std::vector<int> chain_off;  // chain level L: items [chain_off[L], chain_off[L + 1])
+    std::vector<int> lens(items.size(), 0);
+    for (size_t L = 0; L < chains.size(); ++L) {
+        chain_off.push_back((int)items.size());
+        items.insert(items.end(), chains[L].begin(), chains[L].end());
+        lens.insert(lens.end(), chain_lens_by_level[L].begin(), chain_lens_by_level[L].end());
+    }
+    chain_off.push_back((int)items.size());
     const int root_off = (int)items.size();
     items.insert(items.end(), roots.begin(), roots.end());
     rc = h2d(e, e->d_items, items.data(), items.size() * sizeof(Item));
+    if (rc == SMC_OK && !chain_nodes.empty()) {
+        rc = h2d(e, e->d_chain_nodes, chain_nodes.data(), chain_nodes.size() * sizeof(int));
+        if (rc == SMC_OK) rc = h2d(e, e->d_chain_len, lens.data(), lens.size() * sizeof(int));
+    }
     if (rc != SMC_OK) return rc;
     e->sched_first = -1;  // the item buffer no longer holds the full schedule
     PruneArgs a = prune_args(e, first);
@@ -1229,6 +1292,15 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         } else if (kind != PRUNE_TT) {  // cherries are built by their parents
             run(off[b], off[b + 1] - off[b], kind, false);
         }
+    }
+    for (size_t L = 0; L + 1 < chain_off.size(); ++L) {
+        const int n = chain_off[L + 1] - chain_off[L];
+        if (n <= 0) continue;
+        TimedLaunch tl(e, SMC_K_PRUNE);
+        a.items = e->d_items + chain_off[L];
+        a.n_items = n;
+        a.roots = false;
+        launch_prune_chain(a, e->d_chain_nodes, e->d_chain_len + chain_off[L], s);
     }
     run(root_off, (int)roots.size(), PRUNE_ANY, true);
     {
@@ -1498,7 +1570,8 @@ void smc_lg_base_matrix(double out[190])
 int smc_engine_set_variant(smc_engine *e, int family, int variant)
 {
     if (!e) return SMC_ERR_ARG;
-    if (family != SMC_K_PRUNE || (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU))
+    if (family != SMC_K_PRUNE ||
+        (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU && variant != SMC_PRUNE_MFMA_LEVELS))
         return fail(e, SMC_ERR_ARG, "unknown kernel variant");
     e->prune_variant = variant;
     return SMC_OK;

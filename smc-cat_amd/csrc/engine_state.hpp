@@ -101,6 +101,8 @@ struct smc_engine {
     std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
     std::vector<uint8_t> h_touched;             // [P] pInvar changed by the last parameter upload
     Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
+    int *d_chain_nodes = nullptr;               // [P * (T-1)] dirty-path chains' nodes, bottom up
+    int *d_chain_len = nullptr;                 // [P * (T-1)] chain lengths, aligned with d_items
     struct Committed {
         std::vector<int32_t> children;
         std::vector<double> blen, base;

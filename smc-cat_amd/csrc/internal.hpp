@@ -136,6 +136,11 @@ struct PruneArgs {
 };
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores
+// Dirty-path chains on the matrix cores: items [0, n_items) of a.items are chains,
+// item i = {particle, index of its first node in chain_nodes}, chain_len[i] nodes
+// from the bottom of the path upwards (every node after the first has the node
+// before it as its only dirty stored child).
+void launch_prune_chain(const PruneArgs &a, const int *chain_nodes, const int *chain_len, hipStream_t s);
 
 // Single-site x candidate-profile scoring (Gibbs / learned allocation proposal).
 constexpr int kSiteStack = 32;  // LDS stack entries for live internal partials

@@ -10,12 +10,13 @@
 // class inside a 64-slot aligned window of the class-sorted slot order, so that
 // every column block starts on a cache-line boundary -- and walks it in 16-site
 // column blocks:
-//   A operand: P rows 16*rb + (lane & 15), columns 4*kb + (lane >> 4)   (rb < 2, kb < 5)
+//   A operand: P row Mfma::arow(rb, lane & 15), columns 4*kb + (lane >> 4)   (rb < 2, kb < 5)
 //              rows 20..31 are zero padding, so 62.5% of the MFMA work is useful;
 //   B operand: partial rows 4*kb + (lane >> 4) at site 16*sb + (lane & 15);
 //   C/D:       at site 16*sb + (lane & 15), register v < 4 holds row
-//              16*rb + 4*(lane >> 4) + v for f32 and 16*rb + (lane >> 4) + 4*v for f64
-//              (the two builtins place result rows differently; A and B agree).
+//              16*rb + (lane >> 4) + 4*v (for f32 through a row permutation of A: the
+//              two builtins place result rows differently; see Mfma<float>), so the
+//              rb = 0 registers and register 0 of rb = 1 are the B operand of a parent.
 // (With unroll >= 2 the sites of a lane are paired instead; see the column mapping in
 // prune_unit.)  P stays in VGPRs for all column blocks of the tile; partial rows are
 // read and written as 16-lane contiguous runs of the state-major [state][site] layout.
@@ -54,6 +55,8 @@ struct Mfma<double> {
     }
     // result row held by lane group g4, register v, of row block rb
     static __device__ __forceinline__ int crow(int rb, int g4, int v) { return 16 * rb + g4 + 4 * v; }
+    // P row that lane c16 supplies as A-operand row c16 of row block rb
+    static __device__ __forceinline__ int arow(int rb, int c16) { return 16 * rb + c16; }
 };
 
 template <>
@@ -63,7 +66,16 @@ struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int crow(int rb, int g4, int v) { return 16 * rb + 4 * g4 + v; }
+    // The f32 builtin leaves result row i of a block in lane group i / 4, register
+    // i % 4.  Its A operand is row-permuted (lane c16 of block rb supplies P row
+    // 16 rb + c16 / 4 + 4 (c16 % 4)) so that results land as they do for f64: lane
+    // group g4, register v holds row 16 rb + g4 + 4 v, which for rb = 0 and v = kb
+    // (and rb = 1, v = 0 for kb = 4) is B-operand row 4 kb + g4 of the same lane.
+    // A node's result is then the B operand of its parent without any lane
+    // exchange (the chain kernel carries it in registers).  Each result element
+    // is the same dot product in the same order, so only its position changes.
+    static __device__ __forceinline__ int crow(int rb, int g4, int v) { return 16 * rb + g4 + 4 * v; }
+    static __device__ __forceinline__ int arow(int rb, int c16) { return 16 * rb + (c16 >> 2) + 4 * (c16 & 3); }
 };
 
 // Child modes; MODE_RT decides per item at run time (roots).
@@ -374,7 +386,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         for (int rb = 0; rb < 2; rb++)
 #pragma unroll
             for (int kb = 0; kb < 5; kb++) {
-                const int row = 16 * rb + c16;
+                const int row = Mfma<TX>::arow(rb, c16);
                 a0[rb][kb] = (md0 != MODE_TIP && row < 20) ? P0[row * 21 + 4 * kb + g4] : (TX)0;
                 a1[rb][kb] = (md1 != MODE_TIP && row < 20) ? P1[row * 21 + 4 * kb + g4] : (TX)0;
             }
@@ -624,6 +636,223 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     }
 }
 
+// Dirty-path chains (incremental refreshes).  A chain is a run of internal nodes
+// n_0, n_1 = parent(n_0), ..., in which every node after the first has exactly
+// one child that is dirty and stored: the node below it.  One wavefront walks one
+// tile of a chain from the bottom up, rate by rate, and keeps each node's scaled
+// partial in registers (rows 4 kb + g4 of the lane's two slots, which is the
+// parent's B operand, see Mfma) together with its exponent, so a node after the
+// first reads only its other child -- a tip, a cherry built in registers or a
+// stored partial -- and writes its own partial and exponent.  The first node reads
+// both children.  Every node runs the products, scaling and exponent sums of
+// prune_unit in the same order, so a chain gives the per-node launches' partials
+// bit for bit.  Column mapping: paired (unroll 2) -- block k of the 64-slot window
+// holds, at lane c16, window offset 32 (k / 2) + 2 c16 + (k % 2).  A workgroup
+// takes one tile and its wavefronts take its rates (wave w: rates w, w + 4, ...):
+// a chain node depends on the one below, so splitting the rates shortens each
+// wavefront's dependent sequence fourfold where the per-node kernels split tiles.
+template <typename TX>
+__device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__restrict__ chain_nodes,
+                                           const int *__restrict__ chain_len, int item, int tile_group,
+                                           TX (&lds)[4][4 * kPStride])
+{
+    using V4 = typename Mfma<TX>::v4;
+    const int *__restrict__ ntiles = ka.ntiles;
+    const int *__restrict__ children = ka.children;
+    const int *__restrict__ perm = ka.perm;
+    const uint8_t *__restrict__ codes = ka.codes;
+    const TX *__restrict__ Pm = ka.Pm;
+    const int T = ka.T, R = ka.R, S_pad = ka.S_pad;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile = tile_group;
+    const Item it = ka.items[item];
+    const int p = it.particle, start = it.node, len = chain_len[item];
+    if (tile >= ntiles[p]) return;  // no block barrier follows
+    const Tile tl = ka.tiles[(size_t)p * ka.tile_stride + tile];
+    const int lane = threadIdx.x & 63;
+    const int c16 = lane & 15, g4 = lane >> 4;
+    const int nn = 2 * T - 1;
+    const int *__restrict__ chp = children + (size_t)p * (T - 1) * 2;
+    auto mode_of = [&](int c) {
+        if (c < T) return MODE_TIP;
+        return (chp[(c - T) * 2] < T && chp[(c - T) * 2 + 1] < T) ? MODE_CHERRY : MODE_STORED;
+    };
+    const size_t node_elems = (size_t)R * 20 * S_pad;
+    const size_t lp = (size_t)(p - ka.p_base);
+    TX *Xp = ka.X + lp * ka.WS * (T - 1) * node_elems;
+    int *Ep = ka.E + lp * ka.WS * (T - 1) * R * S_pad;
+    const uint8_t *xf = ka.xflag ? ka.xflag + (size_t)p * (T - 1) : nullptr;
+    auto nslot = [&](int node) { return (xf ? (int)xf[node - T] * (T - 1) : 0) + node - T; };
+    const TX *Pg = Pm + ((size_t)ka.pcls_off[p] + tl.cls) * nn * R * kPStride;
+    using T2 = typename Vec2<TX>::t;
+    const int nsb = (tl.len + 15) >> 4;
+    const int sb_begin = ((tl.lo >> 4) / 2) * 2;
+    // window offsets and alignment columns of the lane's 4 slots (block k)
+    int off[4], orig[4];
+    bool valid[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        off[k] = 32 * (k >> 1) + 2 * c16 + (k & 1);
+        valid[k] = off[k] >= tl.lo && off[k] < tl.len;
+        const int sl = tl.start + (valid[k] ? off[k] : tl.lo);  // a lane outside the tile reads a slot inside it
+        orig[k] = perm[(size_t)p * S_pad + sl];
+    }
+    auto code_at = [&](int tip, int k) {
+        const int code = codes[(size_t)tip * S_pad + orig[k]];
+        return code < 20 ? code : 20;
+    };
+
+    for (int r = wave; r < R; r += 4) {
+        TX cy[4][5];  // the previous chain node's partial (B layout) and exponent, per block
+        int ce[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            ce[k] = 0;
+#pragma unroll
+            for (int kb = 0; kb < 5; kb++) cy[k][kb] = (TX)0;
+        }
+        for (int j = 0; j < len; j++) {
+            const int n = chain_nodes[start + j];
+            const int c0 = chp[(n - T) * 2], c1 = chp[(n - T) * 2 + 1];
+            // child A: the chain node below (carried), or for the first node its
+            // first child; child B: the other child
+            const int below = j > 0 ? chain_nodes[start + j - 1] : -1;
+            const int cA = j > 0 ? below : c0;
+            const int cB = j > 0 ? (c0 == below ? c1 : c0) : c1;
+            const int mA = j > 0 ? MODE_RT : mode_of(cA);  // MODE_RT here: carried in registers
+            const int mB = mode_of(cB);
+            const TX *PA = Pg + ((size_t)cA * R + r) * kPStride;
+            const TX *PB = Pg + ((size_t)cB * R + r) * kPStride;
+            TX *lA = lds[wave], *lB = lds[wave] + 2 * kPStride;
+            if (mA == MODE_CHERRY || mB == MODE_CHERRY) {
+                wave_sync();  // the previous node's reads are done
+                if (mA == MODE_CHERRY) {
+                    stage_p<TX>(lA, Pg + ((size_t)chp[(cA - T) * 2] * R + r) * kPStride, lane);
+                    stage_p<TX>(lA + kPStride, Pg + ((size_t)chp[(cA - T) * 2 + 1] * R + r) * kPStride, lane);
+                }
+                if (mB == MODE_CHERRY) {
+                    stage_p<TX>(lB, Pg + ((size_t)chp[(cB - T) * 2] * R + r) * kPStride, lane);
+                    stage_p<TX>(lB + kPStride, Pg + ((size_t)chp[(cB - T) * 2 + 1] * R + r) * kPStride, lane);
+                }
+                wave_sync();
+            }
+            TX aA[2][5], aB[2][5];
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int kb = 0; kb < 5; kb++) {
+                    const int row = Mfma<TX>::arow(rb, c16);
+                    aA[rb][kb] = (mA != MODE_TIP && row < 20) ? PA[row * 21 + 4 * kb + g4] : (TX)0;
+                    aB[rb][kb] = (mB != MODE_TIP && row < 20) ? PB[row * 21 + 4 * kb + g4] : (TX)0;
+                }
+            const int sA = mA == MODE_STORED ? nslot(cA) : 0, sB = mB == MODE_STORED ? nslot(cB) : 0;
+            const TX *XA = Xp + ((size_t)sA * R + r) * 20 * S_pad;
+            const TX *XB = Xp + ((size_t)sB * R + r) * 20 * S_pad;
+            const int *EA = Ep + ((size_t)sA * R + r) * S_pad;
+            const int *EB = Ep + ((size_t)sB * R + r) * S_pad;
+            TX *xo = Xp + ((size_t)nslot(n) * R + r) * 20 * S_pad;
+            int *eo = Ep + ((size_t)nslot(n) * R + r) * S_pad;
+            // one child over the two blocks k0, k0 + 1 of a step: its products into
+            // acc (C layout) and its exponents
+            auto child = [&](int mode, int c, const TX(&a)[2][5], const TX *Pc, const TX *lc, const TX *Xc,
+                             const int *Ec, int k0, V4(&acc)[2][2], int(&ex)[2]) {
+                const int pb = tl.start + off[k0];  // the even slot of the lane's pair, in the window
+                if (mode == MODE_STORED) {
+                    TX b0[5], b1[5];
+                    stored_b2<TX>(Xc + pb, S_pad, valid[k0], valid[k0 + 1], g4, b0, b1);
+                    const int2v ev = *(const int2v *)(Ec + pb);
+                    mma_block<TX>(a, b0, acc[0]);
+                    mma_block<TX>(a, b1, acc[1]);
+                    ex[0] = ev.x;
+                    ex[1] = ev.y;
+                    return;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int k = k0 + u;
+                    ex[u] = 0;
+                    if (mode == MODE_TIP) {
+                        tip_block<TX>(Pc, code_at(c, k), g4, acc[u]);
+                    } else if (mode == MODE_CHERRY) {
+                        TX b[5];
+                        cherry_b<TX>(lc, lc + kPStride, code_at(chp[(c - T) * 2], k),
+                                     code_at(chp[(c - T) * 2 + 1], k), valid[k], g4, b);
+                        mma_block<TX>(a, b, acc[u]);
+                    } else {  // carried from the chain node below
+                        TX b[5];
+#pragma unroll
+                        for (int kb = 0; kb < 5; kb++) b[kb] = cy[k][kb];
+                        mma_block<TX>(a, b, acc[u]);
+                        ex[u] = ce[k];
+                    }
+                }
+            };
+#pragma unroll
+            for (int k0 = 0; k0 < 4; k0 += 2) {
+                if (k0 < sb_begin || k0 >= nsb) continue;  // no valid slot in this step (wave-uniform)
+                V4 y[2][2], w[2][2];
+                int ey[2], ew[2];
+                child(mA, cA, aA, PA, lA, XA, EA, k0, y, ey);
+                child(mB, cB, aB, PB, lB, XB, EB, k0, w, ew);
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int k = k0 + u;
+                    int etot = ey[u] + ew[u];
+                    TX m = 0;
+#pragma unroll
+                    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            y[u][rb][v] *= w[u][rb][v];
+                            m = fmax(m, y[u][rb][v]);
+                        }
+                    m = fmax(m, __shfl_xor(m, 16, 64));
+                    m = fmax(m, __shfl_xor(m, 32, 64));
+                    int e;
+                    (void)frexp(m, &e);
+                    const bool ok = (m > (TX)0) && isfinite(m);
+                    const TX nanv = (TX)NAN;
+#pragma unroll
+                    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) y[u][rb][v] = ok ? (TX)ldexp(y[u][rb][v], -e) : nanv;
+                    etot += ok ? e : 0;
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++) cy[k][kb] = y[u][0][kb];
+                    cy[k][4] = y[u][1][0];
+                    ce[k] = etot;
+                }
+                // store the step's two slots: one paired access per row where both are valid
+                const int pb = tl.start + off[k0];
+                const bool v0 = valid[k0], v1 = valid[k0 + 1];
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int row = Mfma<TX>::crow(rb, g4, v);
+                        if (row >= 20) continue;
+                        TX *q = xo + (size_t)row * S_pad + pb;
+                        if (v0 && v1) {
+                            *(T2 *)q = T2{y[0][rb][v], y[1][rb][v]};
+                        } else if (v0) {
+                            q[0] = y[0][rb][v];
+                        } else if (v1) {
+                            q[1] = y[1][rb][v];
+                        }
+                    }
+                if (g4 == 0) {
+                    if (v0 && v1)
+                        *(int2v *)(eo + pb) = int2v{ce[k0], ce[k0 + 1]};
+                    else if (v0)
+                        eo[pb] = ce[k0];
+                    else if (v1)
+                        eo[pb + 1] = ce[k0 + 1];
+                }
+            }
+        }
+    }
+}
+
 // Persistent grid-stride form: the launch walks the units of items
 // [base, base + n), read from the device (bucket_off[bucket], [bucket + 1]) when
 // bucket_off is given, so one launch configuration serves every topology and a
@@ -646,6 +875,23 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
         const int l = w / X;
         const int q = (l / bpi) * X + (w - l * X);
         if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, l % bpi, lds);
+    }
+}
+
+// Chains, persistent grid-stride like k_prune_mfma with one tile per workgroup
+// (ka.blocks_per_item = tile stride): items [0, n) of `items`, chain i's nodes at
+// chain_nodes[items[i].node ...], chain_len[i] of them.
+template <typename TX>
+__global__ __launch_bounds__(256) void k_prune_chain(KArgs<TX> ka, const int *__restrict__ chain_nodes,
+                                                     const int *__restrict__ chain_len, int n)
+{
+    __shared__ __attribute__((aligned(16))) TX lds[4][4 * kPStride];
+    const int bpi = ka.blocks_per_item, X = ka.xcds;
+    const int n_work = (n + X - 1) / X * X * bpi;
+    for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const int l = w / X;
+        const int q = (l / bpi) * X + (w - l * X);
+        if (q < n) chain_unit<TX>(ka, chain_nodes, chain_len, q, l % bpi, lds);
     }
 }
 
@@ -684,13 +930,31 @@ void launch_rt(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream
 }
 
 template <typename TX>
+KArgs<TX> kargs(const PruneArgs &a)
+{
+    const int bpi = (a.tile_stride + 3) / 4;
+    return KArgs<TX>{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes,
+                     (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
+                     a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS,
+                     std::max(1, a.xcds)};
+}
+
+template <typename TX>
+void launch_chain_typed(const PruneArgs &a, const int *chain_nodes, const int *chain_len, hipStream_t s)
+{
+    KArgs<TX> ka = kargs<TX>(a);
+    ka.blocks_per_item = a.tile_stride;  // one workgroup per tile, its wavefronts per rate
+    const int X = ka.xcds, bpi = ka.blocks_per_item;
+    const int64_t units = (int64_t)(a.n_items + X - 1) / X * X * bpi;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
+    hipLaunchKernelGGL((k_prune_chain<TX>), dim3(grid), dim3(256), 0, s, ka, chain_nodes, chain_len, a.n_items);
+}
+
+template <typename TX>
 void launch_typed(const PruneArgs &a, hipStream_t s)
 {
     const int bpi = (a.tile_stride + 3) / 4;
-    KArgs<TX> ka{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes,
-                 (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
-                 a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS,
-                 std::max(1, a.xcds)};
+    KArgs<TX> ka = kargs<TX>(a);
     // without device bucket offsets the work is known: launch exactly what it needs
     // (items rounded up to whole XCD groups), capped at the persistent grid
     const int X = ka.xcds;
@@ -712,6 +976,15 @@ void launch_typed(const PruneArgs &a, hipStream_t s)
 }
 
 }  // namespace
+
+void launch_prune_chain(const PruneArgs &a, const int *chain_nodes, const int *chain_len, hipStream_t s)
+{
+    if (a.n_items <= 0) return;
+    if (a.fp32)
+        launch_chain_typed<float>(a, chain_nodes, chain_len, s);
+    else
+        launch_chain_typed<double>(a, chain_nodes, chain_len, s);
+}
 
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s)
 {

@@ -369,8 +369,10 @@ class Engine:
         return out
 
     def set_prune_variant(self, variant):
-        """'mfma' (matrix cores, default) or 'valu'."""
-        v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU}[variant]
+        """'mfma' (matrix cores, default: short dirty paths run as chains), 'mfma_levels'
+        (dirty nodes launched level by level) or 'valu'."""
+        v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU,
+             "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
 
     def set_mixture(self, mode):

@@ -106,10 +106,12 @@ def _oracle(codes, batch, R):
     return np.array([orc.particle_lnl(codes, batch.particle(i), R) for i in range(batch.n)])
 
 
-@pytest.mark.parametrize("variant", ["mfma", "mfma-u1", "mfma-u4", "valu"])
+@pytest.mark.parametrize("variant", ["mfma", "mfma_levels", "mfma-u1", "mfma-u4", "valu"])
 def test_incremental_moves_and_reject(gpu, variant):
     """Incremental refreshes against the oracle; the mfma-uN cases run every launch
-    (per-kind, merged small buckets, roots) with N column blocks per step."""
+    (per-kind, merged small buckets, roots) with N column blocks per step.  With 30
+    taxa every dirty set is small enough to run as chains ("mfma"); "mfma_levels"
+    launches the same nodes level by level."""
     _say("case incremental: 7 move kinds, accept/reject, %s" % variant)
     rng = np.random.default_rng(5)
     T, S, R = 30, 150, 4
@@ -154,3 +156,58 @@ def _replace(dst, src, i):
     for f in ("children", "branch_len", "base", "pinv", "shape"):
         getattr(out, f)[i] = getattr(src, f)[i]
     return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_chains_equal_level_launches(gpu, precision):
+    """Dirty paths run as chains (one launch, the partial carried in registers from
+    node to node) give the per-level launches' lnL bit for bit, round after round
+    of accepted and rejected moves; 120 taxa, so tree moves run as chains (one or
+    two of them per particle, SPR joining two paths) and whole-tree moves keep the
+    bucket launches; CAT classes split the tiles."""
+    _say("case chains vs level launches %s: 120 taxa x 160 sites, 14 particles" % precision)
+    rng = np.random.default_rng(11)
+    T, S, R = 120, 160, 4
+    codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
+    codes[rng.random(codes.shape) < 0.04] = 20
+    kinds = ["branch", "subtree", "nni", "spr", "spr", "shape", "rates"]
+    n = len(kinds) * 2
+    batch = sim.prior_particles(rng, n, T, S, num_rates=R, invar=True, k_range=(1, 6))
+    engs = [Engine(T, S, R, n, 6, precision, workspaces=2) for _ in range(2)]
+    try:
+        engs[1].set_prune_variant("mfma_levels")
+        for eng in engs:
+            eng.set_alignment(codes)
+            eng.set_particles(batch)
+        l0 = [eng.refresh_likelihood()[0] for eng in engs]
+        assert np.array_equal(l0[0], l0[1])
+        committed = batch.take(np.arange(n))
+        for round_ in range(4):
+            prop = committed.take(np.arange(n))
+            flags = np.zeros((n, 2 * T - 1), dtype=np.uint8)
+            eig = np.zeros(n, dtype=np.uint8)
+            for i in range(n):
+                flags[i], eig[i] = propose(rng, prop, i, T, kinds[(i + round_) % len(kinds)])
+            got = []
+            for eng in engs:
+                eng.update_params(prop)
+                l1, nf = eng.refresh_incremental(flags, eig)
+                assert not nf.any()
+                got.append(l1)
+            assert np.array_equal(got[0], got[1]), round_
+            if precision == "fp64" and round_ == 0:
+                np.testing.assert_allclose(got[0], _oracle(codes, prop, R), rtol=1e-10)
+            accept = (rng.random(n) < 0.5).astype(np.uint8)
+            for eng in engs:
+                eng.accept_reject(accept)
+            for i in range(n):
+                if accept[i]:
+                    committed = _replace(committed, prop, i)
+        # nothing dirty: the roots recombine each engine's committed partials
+        l3 = [eng.refresh_incremental(np.zeros((n, 2 * T - 1), dtype=np.uint8))[0] for eng in engs]
+        assert np.array_equal(l3[0], l3[1])
+        if precision == "fp64":
+            np.testing.assert_allclose(l3[0], _oracle(codes, committed, R), rtol=1e-10)
+    finally:
+        for eng in engs:
+            eng.close()
