@@ -246,6 +246,8 @@ def rejuv_bench(args):
     # fast path: FP32, lockstep batch, learned CAT move (trained beforehand on Gibbs draws)
     eng = Engine(T, S, R, nf, kmax, "fp32", workspaces=2)
     eng.set_graphs(not args.no_graphs)
+    if args.chain_max >= 0:
+        eng.set_chain_max_nodes(args.chain_max)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
     eng.set_particles(batch)
@@ -307,6 +309,8 @@ def rejuv_bench(args):
         eng = Engine(T, S, R, popn, kmax, "fp32", workspaces=2, partial_slots=nf,
                      pmat_classes=int(nf * (kr[0] + kr[1]) / 2 * 1.4))
         eng.set_graphs(not args.no_graphs)
+        if args.chain_max >= 0:
+            eng.set_chain_max_nodes(args.chain_max)
         eng.set_mixture("logspace")
         eng.set_alignment(codes)
         slot_keys = np.arange(popn, dtype=np.int32)
@@ -335,6 +339,8 @@ def rejuv_bench(args):
     sub = batch.take(np.arange(npl))
     eng = Engine(T, S, R, npl, kmax, "fp64", workspaces=2)
     eng.set_graphs(not args.no_graphs)
+    if args.chain_max >= 0:
+        eng.set_chain_max_nodes(args.chain_max)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
     eng.set_particles(sub)
@@ -412,6 +418,9 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=32)
     ap.add_argument("--xcd-groups", type=int, default=8)
     ap.add_argument("--no-graphs", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--chain-max", type=int, default=-1,
+                    help="incremental refreshes: dirty sets of at most this many stored nodes run as chains "
+                         "(-1: the engine default)")
     ap.add_argument("--ess-taxa", type=int, default=16)
     ap.add_argument("--ess-sites", type=int, default=200)
     ap.add_argument("--ess-particles", type=int, default=64)
@@ -485,6 +494,8 @@ def main():
         step()
     # timed steps: each sweep is one replay of its captured graph
     eng.set_graphs(not args.no_graphs)
+    if args.chain_max >= 0:
+        eng.set_chain_max_nodes(args.chain_max)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         lnl, nf, ess, temp = step()

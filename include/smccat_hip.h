@@ -301,6 +301,12 @@ typedef struct {
     int32_t unroll_runtime;  /* 1, 2 or 4 (default 2): unroll of the launches that decide the
                                 child modes per item (roots, and the merged small buckets of
                                 incremental refreshes) */
+    int32_t chain_max_nodes; /* >= 0 (default 48): an incremental refresh runs a particle's
+                                dirty nodes as chains (SMC_PRUNE_MFMA) when at most this many
+                                of them are stored partials and the call's moving particles
+                                are too few to fill per-level launches (8 workgroups per CU:
+                                one-particle rounds, the tail of a lockstep batch); else
+                                level by level */
 } smc_tuning;
 
 int smc_engine_get_tuning(smc_engine *e, smc_tuning *t);
@@ -401,7 +407,8 @@ int smc_engine_reset_stats(smc_engine *e);
  * SMC_K_PRUNE: SMC_PRUNE_MFMA (matrix cores, the default) or SMC_PRUNE_VALU
  * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding.
  * SMC_PRUNE_MFMA runs the short dirty paths of an incremental refresh as chains
- * (one launch walks each path bottom-up with the partial in registers);
+ * (one launch walks each path bottom-up with the partial in registers) when few
+ * particles move in the call (see smc_tuning.chain_max_nodes);
  * SMC_PRUNE_MFMA_LEVELS launches them node level by node level instead, with the
  * same partials bit for bit. */
 enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1, SMC_PRUNE_MFMA_LEVELS = 2 };

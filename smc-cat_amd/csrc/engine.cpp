@@ -86,9 +86,6 @@ namespace {
 
 int fail(smc_engine *e, int code, const std::string &msg) { return smc_fail(e, code, msg); }
 
-// An incremental refresh runs a particle's dirty nodes as chains when at most this
-// many of them are stored partials (tree moves touch one or two paths to the root).
-constexpr int kMaxChainNodes = 48;
 
 template <typename T>
 int dev_alloc(smc_engine *e, T **ptr, size_t count)
@@ -1103,6 +1100,20 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<Item> p_nodes;
     std::vector<int> keep;
     std::vector<double> keep_lnl;
+    // Chains pay off where the per-level launches would be small: a one-particle-at-a-
+    // time caller, or the last rounds of a lockstep batch.  With many particles moving,
+    // the per-level launches fill the GPU and their kind-specialised kernels stream
+    // faster than the chain kernel (whose wavefronts run ~20 dependent steps each):
+    // chains are used when the particles that move in this call give per-level
+    // launches of fewer units than the merged-small-launch bound below.
+    int moving = 0;
+    for (int i = 0; i < count; ++i) {
+        bool any = eig_flags && eig_flags[i];
+        const uint8_t *nf = node_flags + (size_t)i * nn;
+        for (int n = 0; n < nn && !any; ++n) any = nf[n] != 0;
+        moving += any ? 1 : 0;
+    }
+    const bool chains_fit = (int64_t)moving * ((e->tile_stride + 3) / 4) <= (int64_t)e->num_cus * 8;
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
@@ -1177,7 +1188,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         int n_stored = 0;
         for (int n = T; n < nn; ++n)
             if (dirty[n] && n != root && prune_kind(ch, T, n) != PRUNE_TT) n_stored++;
-        const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && n_stored <= kMaxChainNodes;
+        const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && chains_fit && n_stored <= e->tune.chain_max;
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
@@ -1673,6 +1684,7 @@ int smc_engine_get_tuning(smc_engine *e, smc_tuning *t)
     for (int k = 0; k < 3; ++k) t->unroll[k] = e->tune.unroll[k];
     t->xcd_groups = e->tune.xcd_groups;
     t->unroll_runtime = e->tune.unroll_rt;
+    t->chain_max_nodes = e->tune.chain_max;
     t->ms_per_sweep = 0.0;
     t->configs_tried = 0;
     return SMC_OK;
@@ -1688,10 +1700,12 @@ int smc_engine_set_tuning(smc_engine *e, const smc_tuning *t)
     }
     if (t->xcd_groups != 1 && t->xcd_groups != 2 && t->xcd_groups != 4 && t->xcd_groups != 8)
         return fail(e, SMC_ERR_ARG, "xcd_groups must be 1, 2, 4 or 8");
+    if (t->chain_max_nodes < 0) return fail(e, SMC_ERR_ARG, "chain_max_nodes must be >= 0");
     e->tune.blocks_per_cu = t->blocks_per_cu;
     for (int k = 0; k < 3; ++k) e->tune.unroll[k] = t->unroll[k];
     e->tune.xcd_groups = t->xcd_groups;
     e->tune.unroll_rt = t->unroll_runtime;
+    e->tune.chain_max = t->chain_max_nodes;
     return SMC_OK;
 }
 
@@ -1788,6 +1802,7 @@ int smc_engine_autotune(smc_engine *e, int first, int count, int sweeps, smc_tun
         for (int k = 0; k < 3; ++k) out->unroll[k] = best.unroll[k];
         out->xcd_groups = best.xcd_groups;
         out->unroll_runtime = best.unroll_rt;
+        out->chain_max_nodes = best.chain_max;
         out->ms_per_sweep = best_ms;
         out->configs_tried = tried;
     }

@@ -138,6 +138,9 @@ struct smc_engine {
                                       // set per precision at creation
         int xcd_groups = 8;           // an item's workgroups share one of this many XCDs
         int unroll_rt = 2;            // the same for launches that decide child modes per item
+        int chain_max = 48;           // incremental refreshes: dirty sets of at most this many
+                                      // stored nodes run as chains (tree moves touch one or two
+                                      // paths to the root)
     };
     Tune tune;
     int num_cus = 256;

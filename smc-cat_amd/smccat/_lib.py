@@ -51,7 +51,8 @@ class RejuvenationOptions(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_int32), ("unroll", ctypes.c_int32 * 3),
                 ("ms_per_sweep", ctypes.c_double), ("configs_tried", ctypes.c_int32),
-                ("xcd_groups", ctypes.c_int32), ("unroll_runtime", ctypes.c_int32)]
+                ("xcd_groups", ctypes.c_int32), ("unroll_runtime", ctypes.c_int32),
+                ("chain_max_nodes", ctypes.c_int32)]
 
 
 class SamplerOptions(ctypes.Structure):

@@ -293,14 +293,22 @@ class Engine:
         t = _lib.Tuning()
         self._check(self.lib.smc_engine_get_tuning(self.h, ctypes.byref(t)), "smc_engine_get_tuning")
         return {"blocks_per_cu": t.blocks_per_cu, "unroll": list(t.unroll), "xcd_groups": t.xcd_groups,
-                "unroll_runtime": t.unroll_runtime}
+                "unroll_runtime": t.unroll_runtime, "chain_max_nodes": t.chain_max_nodes}
 
-    def set_tuning(self, blocks_per_cu, unroll, xcd_groups=8, unroll_runtime=None):
+    def set_tuning(self, blocks_per_cu, unroll, xcd_groups=8, unroll_runtime=None, chain_max_nodes=None):
+        cur = self.tuning()
         if unroll_runtime is None:
-            unroll_runtime = self.tuning()["unroll_runtime"]
+            unroll_runtime = cur["unroll_runtime"]
+        if chain_max_nodes is None:
+            chain_max_nodes = cur["chain_max_nodes"]
         t = _lib.Tuning(int(blocks_per_cu), (ctypes.c_int32 * 3)(*unroll), 0.0, 0, int(xcd_groups),
-                        int(unroll_runtime))
+                        int(unroll_runtime), int(chain_max_nodes))
         self._check(self.lib.smc_engine_set_tuning(self.h, ctypes.byref(t)), "smc_engine_set_tuning")
+
+    def set_chain_max_nodes(self, n):
+        """Incremental refreshes run a particle's dirty set as chains up to n stored nodes."""
+        t = self.tuning()
+        self.set_tuning(t["blocks_per_cu"], t["unroll"], t["xcd_groups"], t["unroll_runtime"], n)
 
     def autotune(self, first=0, count=None, sweeps=3):
         """Measured search over the sweep's launch shape; returns the chosen shape."""

@@ -80,6 +80,9 @@ def test_every_launch_shape_gives_identical_lnl(gpu):
     _say("case launch shapes: CAT K<=6, 12 particles")
     eng, codes, batch = _setup(N=12, kmax=6)
     with eng:
+        eng.set_tuning(3, (2, 4, 2), 2, unroll_runtime=4, chain_max_nodes=7)
+        assert eng.tuning() == {"blocks_per_cu": 3, "unroll": [2, 4, 2], "xcd_groups": 2, "unroll_runtime": 4,
+                                "chain_max_nodes": 7}
         eng.set_graphs(False)
         want, _ = eng.refresh_likelihood()
         for g, x in ((1, 1), (3, 8), (8, 2)):
