@@ -368,6 +368,69 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
     return SMC_OK;
 }
 
+int ensure_site_class_buffers(smc_engine *e, int count, int64_t need)
+{
+    if (need <= e->sc_cap && count <= e->sc_ent_cap) return SMC_OK;
+    void *bufs[] = {e->d_sc_ent, e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    e->d_sc_ent = e->d_sc_cls = nullptr;
+    e->d_sc_logits = e->d_sc_eig = e->d_sc_rate = e->d_sc_out = nullptr;
+    // grow geometrically: the moves call this with varying batch sizes
+    const int64_t cap = std::max<int64_t>(need, e->sc_cap * 3 / 2);
+    const int ecap = std::max(count, e->sc_ent_cap * 3 / 2);
+    e->sc_cap = 0;
+    e->sc_ent_cap = 0;
+    int rc = dev_alloc(e, &e->d_sc_ent, (size_t)3 * ecap);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_logits, (size_t)cap * 20);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_eig, (size_t)cap * kEigStride);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_rate, (size_t)cap * e->R);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_out, (size_t)cap);
+    if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_cls, (size_t)cap);
+    if (rc != SMC_OK) return rc;
+    e->sc_cap = cap;
+    e->sc_ent_cap = ecap;
+    return SMC_OK;
+}
+
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, double tempering)
+{
+    hipStream_t s = e->stream;
+    {
+        TimedLaunch tl(e, SMC_K_EIGEN);
+        launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count, n_cand,
+                          e->d_sc_eig, with_cls ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax, e->pool.d_eig, s);
+    }
+    {
+        TimedLaunch tl(e, SMC_K_SITE);
+        SiteClassArgs a{};
+        a.ent_particle = e->d_sc_ent;
+        a.ent_site = e->d_sc_ent + count;
+        a.ent_ncand = e->d_sc_ent + 2 * count;
+        a.n_cand = n_cand;
+        a.cand_eig = e->d_sc_eig;
+        a.children = e->d_children;
+        a.prog = e->d_prog;
+        a.codes = e->d_codes;
+        a.blen = e->d_blen;
+        a.rates = e->d_rates;
+        a.pinv = e->d_pinv;
+        a.isinv = e->d_isinv;
+        a.invchar = e->d_invchar;
+        a.rate_lnl = e->d_sc_rate;
+        a.out = e->d_sc_out;
+        a.count = count;
+        a.T = e->T;
+        a.R = e->R;
+        a.S_pad = e->S_pad;
+        a.mix_mode = e->mix_mode;
+        a.tempering = tempering;
+        launch_site_class(a, s);
+    }
+    HIP_TRY(e, hipGetLastError());
+    return SMC_OK;
+}
+
 extern "C" {
 
 int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
@@ -1494,72 +1557,19 @@ int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, co
             return fail(e, SMC_ERR_ARG, "entry " + std::to_string(i) + ": candidate count");
     }
     const int64_t need = (int64_t)count * n_cand;
-    if (need > e->sc_cap || count > e->sc_ent_cap) {
-        void *bufs[] = {e->d_sc_ent, e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls};
-        for (void *b : bufs)
-            if (b) (void)hipFree(b);
-        e->d_sc_ent = e->d_sc_cls = nullptr;
-        e->d_sc_logits = e->d_sc_eig = e->d_sc_rate = e->d_sc_out = nullptr;
-        // grow geometrically: the moves call this with varying batch sizes
-        const int64_t cap = std::max<int64_t>(need, e->sc_cap * 3 / 2);
-        const int ecap = std::max(count, e->sc_ent_cap * 3 / 2);
-        e->sc_cap = 0;
-        e->sc_ent_cap = 0;
-        int rc = dev_alloc(e, &e->d_sc_ent, (size_t)3 * ecap);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_logits, (size_t)cap * 20);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_eig, (size_t)cap * kEigStride);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_rate, (size_t)cap * e->R);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_out, (size_t)cap);
-        if (rc == SMC_OK) rc = dev_alloc(e, &e->d_sc_cls, (size_t)cap);
-        if (rc != SMC_OK) return rc;
-        e->sc_cap = cap;
-        e->sc_ent_cap = ecap;
-    }
-    hipStream_t s = e->stream;
+    int rc = ensure_site_class_buffers(e, count, need);
+    if (rc != SMC_OK) return rc;
     std::vector<int> ent((size_t)3 * count);
     for (int i = 0; i < count; ++i) {
         ent[i] = particles[i];
         ent[count + i] = sites[i];
         ent[2 * count + i] = ncand[i];
     }
-    int rc = h2d(e, e->d_sc_ent, ent.data(), ent.size() * sizeof(int));
+    rc = h2d(e, e->d_sc_ent, ent.data(), ent.size() * sizeof(int));
     if (rc == SMC_OK) rc = h2d(e, e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double));
     if (rc == SMC_OK && cand_class) rc = h2d(e, e->d_sc_cls, cand_class, (size_t)need * sizeof(int));
-    if (rc != SMC_OK) return rc;
-    {
-        TimedLaunch tl(e, SMC_K_EIGEN);
-        launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count,
-                          n_cand, e->d_sc_eig, cand_class ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax,
-                          e->pool.d_eig, s);
-    }
-    {
-        TimedLaunch tl(e, SMC_K_SITE);
-        SiteClassArgs a{};
-        a.ent_particle = e->d_sc_ent;
-        a.ent_site = e->d_sc_ent + count;
-        a.ent_ncand = e->d_sc_ent + 2 * count;
-        a.n_cand = n_cand;
-        a.cand_eig = e->d_sc_eig;
-        a.children = e->d_children;
-        a.prog = e->d_prog;
-        a.codes = e->d_codes;
-        a.blen = e->d_blen;
-        a.rates = e->d_rates;
-        a.pinv = e->d_pinv;
-        a.isinv = e->d_isinv;
-        a.invchar = e->d_invchar;
-        a.rate_lnl = e->d_sc_rate;
-        a.out = e->d_sc_out;
-        a.count = count;
-        a.T = e->T;
-        a.R = e->R;
-        a.S_pad = e->S_pad;
-        a.mix_mode = e->mix_mode;
-        a.tempering = tempering;
-        launch_site_class(a, s);
-    }
-    HIP_TRY(e, hipGetLastError());
-    rc = d2h(e, out, e->d_sc_out, (size_t)need * sizeof(double));
+    if (rc == SMC_OK) rc = issue_site_class(e, count, n_cand, cand_class != nullptr, tempering);
+    if (rc == SMC_OK) rc = d2h(e, out, e->d_sc_out, (size_t)need * sizeof(double));
     if (rc == SMC_OK) rc = sync_stream(e);
     if (rc != SMC_OK) return rc;
     collect_timings(e);

@@ -206,6 +206,13 @@ int d2h(smc_engine *e, void *dst, const void *src, size_t bytes);
 // Waits for the engine's stream and copies the queued readbacks out; the staging
 // and readback buffers are free again afterwards.
 int sync_stream(smc_engine *e);
+// Site x candidate scoring buffers (d_sc_*) for `count` entries and `need` (entry,
+// candidate) pairs, grown geometrically.
+int ensure_site_class_buffers(smc_engine *e, int count, int64_t need);
+// Queues the candidates' eigensystems and the site-class scoring of the entries in
+// d_sc_ent ([particle | site | candidate count] x count) and, with with_cls, the
+// class codes in d_sc_cls; the scores land in d_sc_out ([count][n_cand]).
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, double tempering);
 
 // Adds the wall time of its scope to e->phase_ms[phase].
 struct PhaseTimer {

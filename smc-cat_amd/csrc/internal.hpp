@@ -205,6 +205,21 @@ struct MlpArgs {
     float inv_B = 0.f;
 };
 
+// The learned allocation move's proposal draw and the site-class entries of its
+// MH step (see k_mlp_draw).
+struct MoveDrawArgs {
+    int count = 0, stride = 0, kmax = 0;
+    const float *logq = nullptr;   // [count][stride] proposal log-weights
+    const int *K = nullptr;        // [count] classes without the site
+    const int *aux = nullptr;      // [count][4]: particle slot, site, current class, its class code
+    const int *codes = nullptr;    // [count][kmax + 1]: class code of each slot (engine class,
+                                   // or -2 - pool index for a fresh profile)
+    const double *u = nullptr;     // [count] uniforms of the draws
+    int *pick = nullptr;           // [count] drawn slot
+    int *ent = nullptr;            // [3][count] entries: particle, site, candidate count (2 or 0)
+    int *cand_cls = nullptr;       // [count][2] candidate class codes: current, drawn
+};
+void launch_mlp_draw(const MoveDrawArgs &a, hipStream_t s);
 void launch_mlp_trunk(const MlpArgs &a, bool train, hipStream_t s);
 void launch_mlp_head(const MlpArgs &a, bool train, hipStream_t s);
 void launch_mlp_backward(const MlpArgs &a, hipStream_t s);

@@ -356,7 +356,45 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ theta, flo
     theta[p] -= lr * (mp * c1) / (sqrtf(vp * c2) + eps);
 }
 
+// The learned move's proposal draw, one lane per particle: class c of the nslot
+// slots (the K classes, plus the new-class slot while a class slot is free) with
+// probability exp(logq[c]) / sum, by the walk of cumulative weights against
+// u * sum, in double; then the site-class entry of the MH step -- the current and
+// the drawn class's codes, or no candidates when the draw is the current class.
+__global__ __launch_bounds__(256) void k_mlp_draw(MoveDrawArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const float *lq = a.logq + (size_t)i * a.stride;
+    const int K = a.K[i], nslot = K < a.kmax ? K + 1 : K;
+    double tot = 0.0;
+    for (int c = 0; c < nslot; ++c) tot += exp((double)lq[c]);
+    const double x = a.u[i] * tot;
+    int pick = nslot - 1;
+    double acc = 0.0;
+    for (int c = 0; c < nslot; ++c) {
+        acc += exp((double)lq[c]);
+        if (acc > x) {
+            pick = c;
+            break;
+        }
+    }
+    const int *ax = a.aux + 4 * (size_t)i;
+    a.pick[i] = pick;
+    a.ent[i] = ax[0];
+    a.ent[a.count + i] = ax[1];
+    a.ent[2 * a.count + i] = pick != ax[2] ? 2 : 0;
+    a.cand_cls[2 * i] = ax[3];
+    a.cand_cls[2 * i + 1] = a.codes[(size_t)i * (a.kmax + 1) + pick];
+}
+
 }  // namespace
+
+void launch_mlp_draw(const MoveDrawArgs &a, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL(k_mlp_draw, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+}
 
 void launch_mlp_trunk(const MlpArgs &a, bool train, hipStream_t s)
 {

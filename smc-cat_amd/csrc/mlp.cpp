@@ -55,6 +55,10 @@ struct smc_mlp {
     int *d_mv_K = nullptr;
     int64_t *d_mv_coff = nullptr;
     int64_t cap_mv_hdr = 0, cap_mv_cls = 0, cap_mv_K = 0, cap_mv_coff = 0;
+    // the draw of the move's proposals on the device
+    int *d_mv_aux = nullptr, *d_mv_codes = nullptr, *d_mv_pick = nullptr;
+    double *d_mv_u = nullptr;
+    int64_t cap_mv_aux = 0, cap_mv_codes = 0, cap_mv_pick = 0, cap_mv_u = 0;
 };
 
 namespace {
@@ -261,7 +265,7 @@ int smc_mlp_destroy(smc_mlp *m)
     void *bufs[] = {m->d_theta, m->d_m, m->d_v, m->d_grad, m->d_hdr, m->d_cls, m->d_K, m->d_lab,
                     m->d_idx, m->d_coff, m->d_Z1, m->d_H1, m->d_Z2, m->d_H2, m->d_dH2, m->d_loss,
                     m->d_part, m->d_logq, m->d_lacc, m->d_mv_hdr, m->d_mv_cls, m->d_mv_K,
-                    m->d_mv_coff};
+                    m->d_mv_coff, m->d_mv_aux, m->d_mv_codes, m->d_mv_pick, m->d_mv_u};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete m;
@@ -538,6 +542,12 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     HIP_TRY(e, grow(m->d_mv_coff, m->cap_mv_coff, count));
     HIP_TRY(e, grow(m->d_mv_cls, m->cap_mv_cls, (int64_t)count * Kmax * kCls));
     HIP_TRY(e, grow(m->d_logq, m->cap_logq, (int64_t)count * stride));
+    HIP_TRY(e, grow(m->d_mv_aux, m->cap_mv_aux, (int64_t)count * 4));
+    HIP_TRY(e, grow(m->d_mv_codes, m->cap_mv_codes, (int64_t)count * (Kmax + 1)));
+    HIP_TRY(e, grow(m->d_mv_pick, m->cap_mv_pick, count));
+    HIP_TRY(e, grow(m->d_mv_u, m->cap_mv_u, count));
+    rc = ensure_site_class_buffers(e, count, (int64_t)count * 2);
+    if (rc != SMC_OK) return rc;
 
     std::vector<float> hdr((size_t)count * kHdr), cls((size_t)count * Kmax * kCls), logq((size_t)count * stride);
     std::vector<float> logn(S + 1, 0.f);  // log of class sizes
@@ -546,10 +556,10 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     std::vector<int64_t> coff(count);
     std::vector<std::array<double, 20>> phi(count), phi_new(count);
     std::vector<int> phi_orig(count, -1);
-    std::vector<int32_t> cand_cls;
     std::vector<uint8_t> was_single(count);
-    std::vector<int32_t> prop(count), ent_p, ent_s, ent_n, ent_i;
-    std::vector<double> cand, ll;
+    std::vector<int32_t> prop(count), aux((size_t)count * 4), codes((size_t)count * (Kmax + 1));
+    std::vector<int64_t> fresh_pool(count);
+    std::vector<double> uu(count), ll((size_t)count * 2);
     int64_t updates = 0, proposed = 0, accepted = 0, created = 0, erased = 0;
 
     for (int it = 0; it < iters; ++it) {
@@ -584,6 +594,20 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
             ps.assign[s] = -1;
             cur[i] = was_single[i] ? K : k0;
             Kv[i] = K;
+            // the proposal's uniform (the device draws with it; the stream order is
+            // site, proposal, acceptance, as in a host draw)
+            uu[i] = g.uniform();
+            int32_t *ax = &aux[(size_t)i * 4];
+            ax[0] = slots[i];
+            ax[1] = s;
+            ax[2] = cur[i];
+            ax[3] = was_single[i] ? phi_orig[i] : ps.orig[cur[i]];
+            int32_t *cd = &codes[(size_t)i * (Kmax + 1)];
+            for (int k = 0; k < K; ++k) cd[k] = ps.orig[k];
+            // slot K: the site's own singleton profile (then it is the current class), or
+            // the next fresh profile of the slot's pool
+            fresh_pool[i] = (int64_t)i * pool_q + e->pool.next[i];
+            cd[K] = was_single[i] ? phi_orig[i] : (int32_t)(-2 - fresh_pool[i]);
             coff[i] = i ? coff[i - 1] + Kv[i - 1] : 0;
             float *h = &hdr[(size_t)i * kHdr];
             for (int a = 0; a < 20; ++a) {
@@ -600,12 +624,17 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 c[21] = c[22] = c[23] = 0.f;
             }
         }
-        // per-step copies go through the engine's pinned staging and readback buffers
+        // per-step copies go through the engine's pinned staging and readback buffers;
+        // the network, the proposal draw and the exact site likelihoods of the drawn
+        // and current classes run back to back, with one wait per site step
         const size_t n_ent = (size_t)(coff[count - 1] + Kv[count - 1]);
         rc = h2d(e, m->d_mv_hdr, hdr.data(), hdr.size() * 4);
         if (rc == SMC_OK) rc = h2d(e, m->d_mv_K, Kv.data(), (size_t)count * 4);
         if (rc == SMC_OK) rc = h2d(e, m->d_mv_coff, coff.data(), (size_t)count * 8);
         if (rc == SMC_OK) rc = h2d(e, m->d_mv_cls, cls.data(), n_ent * kCls * 4);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_aux, aux.data(), aux.size() * 4);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_codes, codes.data(), codes.size() * 4);
+        if (rc == SMC_OK) rc = h2d(e, m->d_mv_u, uu.data(), uu.size() * 8);
         if (rc != SMC_OK) return rc;
         MlpArgs a = batch_args(m, count, nullptr);
         a.hdr = m->d_mv_hdr;
@@ -616,67 +645,37 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         a.out_stride = stride;
         launch_mlp_trunk(a, false, e->stream);
         launch_mlp_head(a, false, e->stream);
+        MoveDrawArgs da;
+        da.count = count;
+        da.stride = stride;
+        da.kmax = Kmax;
+        da.logq = m->d_logq;
+        da.K = m->d_mv_K;
+        da.aux = m->d_mv_aux;
+        da.codes = m->d_mv_codes;
+        da.u = m->d_mv_u;
+        da.pick = m->d_mv_pick;
+        da.ent = e->d_sc_ent;
+        da.cand_cls = e->d_sc_cls;
+        launch_mlp_draw(da, e->stream);
         HIP_TRY(e, hipGetLastError());
-        rc = d2h(e, logq.data(), m->d_logq, logq.size() * 4);
+        rc = issue_site_class(e, count, 2, true, tempering);
+        if (rc == SMC_OK) rc = d2h(e, logq.data(), m->d_logq, logq.size() * 4);
+        if (rc == SMC_OK) rc = d2h(e, prop.data(), m->d_mv_pick, (size_t)count * 4);
+        if (rc == SMC_OK) rc = d2h(e, ll.data(), e->d_sc_out, ll.size() * 8);
         if (rc == SMC_OK) rc = sync_stream(e);
         if (rc != SMC_OK) return rc;
-
-        // draw the proposals; a different slot needs the exact site likelihoods
-        ent_p.clear();
-        ent_s.clear();
-        ent_n.clear();
-        ent_i.clear();
-        cand.clear();
-        cand_cls.clear();
         for (int i = 0; i < count; ++i) {
-            MvState &ps = st[i];
-            smccat::Rng &g = rng[i];
-            const int K = Kv[i];
-            const float *lq = &logq[(size_t)i * stride];
-            // a new class needs a free class slot (the Gibbs move's cap, Kmax)
-            const int nslot = K < Kmax ? K + 1 : K;
-            double w[kMaxCand], tot = 0.0;
-            for (int c = 0; c < nslot; ++c) tot += (w[c] = std::exp((double)lq[c]));
-            const double u = g.uniform() * tot;
-            int pick = nslot - 1;
-            double acc = 0.0;
-            for (int c = 0; c < nslot; ++c) {
-                acc += w[c];
-                if (acc > u) {
-                    pick = c;
-                    break;
-                }
-            }
-            prop[i] = pick;
             updates++;
-            if (pick == cur[i]) continue;
+            if (prop[i] == cur[i]) continue;
             proposed++;
-            ent_i.push_back(i);
-            ent_p.push_back(slots[i]);
-            ent_s.push_back(site[i]);
-            ent_n.push_back(2);
-            const std::array<double, 20> &pc = was_single[i] ? phi[i] : ps.logits[cur[i]];
-            cand.insert(cand.end(), pc.begin(), pc.end());
-            cand_cls.push_back(was_single[i] ? phi_orig[i] : ps.orig[cur[i]]);
-            if (pick == K) {  // a fresh profile from the logit prior (the pool)
+            if (prop[i] == Kv[i]) {  // a fresh profile from the logit prior (the pool) is used up
                 const int slot = e->pool.next[i]++;
                 std::copy_n(&e->pool.logits[((size_t)i * pool_q + slot) * 20], 20, phi_new[i].begin());
-                cand.insert(cand.end(), phi_new[i].begin(), phi_new[i].end());
-                cand_cls.push_back(-2 - (i * pool_q + slot));
-            } else {
-                cand.insert(cand.end(), ps.logits[pick].begin(), ps.logits[pick].end());
-                cand_cls.push_back(ps.orig[pick]);
             }
         }
-        const int ne = (int)ent_i.size();
-        if (ne > 0) {
-            ll.resize((size_t)ne * 2);
-            rc = smc_site_class_lnl_ex(e, ne, ent_p.data(), ent_s.data(), 2, ent_n.data(), cand.data(),
-                                       cand_cls.data(), tempering, ll.data());
-            if (rc != SMC_OK) return rc;
-        }
         // accept or reject, then put the site back
-        for (int q = 0, i = 0; i < count; ++i) {
+        for (int i = 0; i < count; ++i) {
             MvState &ps = st[i];
             smccat::Rng &g = rng[i];
             const int K = Kv[i], s = site[i];
@@ -686,9 +685,8 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 const int c0 = cur[i], c1 = prop[i];
                 const float *lq = &logq[(size_t)i * stride];
                 auto logw = [&](int c) { return c == K ? log_alpha : std::log((double)ps.size[c]); };
-                const double ratio = ll[2 * q + 1] - ll[2 * q] + logw(c1) - logw(c0) + (double)lq[c0] -
+                const double ratio = ll[2 * i + 1] - ll[2 * i] + logw(c1) - logw(c0) + (double)lq[c0] -
                                      (double)lq[c1];
-                q++;
                 if (std::log(g.uniform()) <= ratio) {
                     accepted++;
                     dest = c1;
@@ -700,7 +698,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
                 if (fresh) {  // a pool profile: its eigensystem moves into a free class slot
                     slot_k = ps.free.back();
                     ps.free.pop_back();
-                    rc = smc_pool_to_class(e, slots[i], -2 - cand_cls[2 * q - 1], slot_k);
+                    rc = smc_pool_to_class(e, slots[i], (int)fresh_pool[i], slot_k);
                     if (rc != SMC_OK) return rc;
                     created++;
                 }
