@@ -393,13 +393,14 @@ int ensure_site_class_buffers(smc_engine *e, int count, int64_t need)
     return SMC_OK;
 }
 
-int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, double tempering)
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering)
 {
     hipStream_t s = e->stream;
-    {
+    if (decompose) {  // candidates coded -1 (or every candidate without codes)
         TimedLaunch tl(e, SMC_K_EIGEN);
         launch_eigen_cand(e->d_base, e->d_sc_ent, e->d_sc_ent + 2 * count, e->d_sc_logits, count, n_cand,
-                          e->d_sc_eig, with_cls ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax, e->pool.d_eig, s);
+                          e->d_sc_eig, with_cls ? e->d_sc_cls : nullptr, e->d_eig, e->Kmax, e->pool.d_eig, s,
+                          false);
     }
     {
         TimedLaunch tl(e, SMC_K_SITE);
@@ -409,6 +410,10 @@ int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, double
         a.ent_ncand = e->d_sc_ent + 2 * count;
         a.n_cand = n_cand;
         a.cand_eig = e->d_sc_eig;
+        a.cand_cls = with_cls ? e->d_sc_cls : nullptr;  // existing classes and pool profiles read in place
+        a.eig_cache = e->d_eig;
+        a.max_classes = e->Kmax;
+        a.pool = e->pool.d_eig;
         a.children = e->d_children;
         a.prog = e->d_prog;
         a.codes = e->d_codes;
@@ -1568,7 +1573,7 @@ int smc_site_class_lnl_ex(smc_engine *e, int count, const int32_t *particles, co
     rc = h2d(e, e->d_sc_ent, ent.data(), ent.size() * sizeof(int));
     if (rc == SMC_OK) rc = h2d(e, e->d_sc_logits, cand_logits, (size_t)need * 20 * sizeof(double));
     if (rc == SMC_OK && cand_class) rc = h2d(e, e->d_sc_cls, cand_class, (size_t)need * sizeof(int));
-    if (rc == SMC_OK) rc = issue_site_class(e, count, n_cand, cand_class != nullptr, tempering);
+    if (rc == SMC_OK) rc = issue_site_class(e, count, n_cand, cand_class != nullptr, true, tempering);
     if (rc == SMC_OK) rc = d2h(e, out, e->d_sc_out, (size_t)need * sizeof(double));
     if (rc == SMC_OK) rc = sync_stream(e);
     if (rc != SMC_OK) return rc;

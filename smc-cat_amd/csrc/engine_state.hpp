@@ -209,10 +209,12 @@ int sync_stream(smc_engine *e);
 // Site x candidate scoring buffers (d_sc_*) for `count` entries and `need` (entry,
 // candidate) pairs, grown geometrically.
 int ensure_site_class_buffers(smc_engine *e, int count, int64_t need);
-// Queues the candidates' eigensystems and the site-class scoring of the entries in
-// d_sc_ent ([particle | site | candidate count] x count) and, with with_cls, the
-// class codes in d_sc_cls; the scores land in d_sc_out ([count][n_cand]).
-int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, double tempering);
+// Queues the site-class scoring of the entries in d_sc_ent ([particle | site |
+// candidate count] x count) with, when with_cls, the class codes in d_sc_cls
+// (existing classes and pool profiles are read where they lie); `decompose` first
+// decomposes the candidates coded -1 (every candidate without codes) from
+// d_sc_logits.  The scores land in d_sc_out ([count][n_cand]).
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering);
 
 // Adds the wall time of its scope to e->phase_ms[phase].
 struct PhaseTimer {

@@ -155,6 +155,13 @@ struct SiteClassArgs {
     const int *ent_ncand;     // [count] candidates of each entry (<= n_cand)
     int n_cand;
     const double *cand_eig;   // [count][n_cand][840]
+    // Optional [count][n_cand] class codes: a candidate coded >= 0 (a class of the
+    // entry's particle, in eig_cache [particles][max_classes]) or <= -2 (pool[-2 - v])
+    // is read where it lies; -1 (or no codes) reads cand_eig.
+    const int *cand_cls = nullptr;
+    const double *eig_cache = nullptr;
+    int max_classes = 0;
+    const double *pool = nullptr;
     const int *children;      // [P][T-1][2]
     const int *prog;          // [P][3T] scoring programs (site_prog_stride)
     const uint8_t *codes;     // [T][S_pad]
@@ -173,11 +180,13 @@ void launch_site_class(const SiteClassArgs &a, hipStream_t s);
 // Eigen-decomposition of candidate profiles: entry e uses the base matrix of
 // particle ent_particle[e] and logits[e][c][20] for c < ent_ncand[e].
 // cand_cls (nullable) [count][n_cand]: >= 0 copies the class's eigensystem from
-// eig_cache ([particles][max_classes] layout), <= -2 copies pool[-2 - v],
-// -1 decomposes the logits.
+// eig_cache ([particles][max_classes] layout), <= -2 copies pool[-2 - v] (neither is
+// copied with copy_known false: the site-class kernels then read them where they
+// lie), -1 decomposes the logits.
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
                        const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
-                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s);
+                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s,
+                       bool copy_known = true);
 
 void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_stride, int first,
                        int count, double *lnl, uint8_t *nonfinite, hipStream_t s);

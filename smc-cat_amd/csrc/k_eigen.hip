@@ -203,12 +203,13 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
                                                     double *__restrict__ eig,
                                                     const int *__restrict__ cand_cls,
                                                     const double *__restrict__ eig_cache, int max_classes,
-                                                    const double *__restrict__ pool)
+                                                    const double *__restrict__ pool, int copy_known)
 {
     const int e = blockIdx.x / n_cand, c = blockIdx.x % n_cand;
     if (c >= ent_ncand[e]) return;
     const int k = cand_cls ? cand_cls[blockIdx.x] : -1;
     if (k >= 0 || k <= -2) {  // block-uniform: a copy of a decomposition made earlier
+        if (!copy_known) return;  // read where it lies
         const double *src = k >= 0 ? eig_cache + ((size_t)ent_particle[e] * max_classes + k) * kEigStride
                                    : pool + (size_t)(-2 - k) * kEigStride;
         double *dst = eig + (size_t)blockIdx.x * kEigStride;
@@ -334,11 +335,13 @@ void launch_eigen(const EigenArgs &a, hipStream_t s)
 
 void launch_eigen_cand(const double *base, const int *ent_particle, const int *ent_ncand,
                        const double *logits, int count, int n_cand, double *eig, const int *cand_cls,
-                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s)
+                       const double *eig_cache, int max_classes, const double *pool, hipStream_t s,
+                       bool copy_known)
 {
     if (count <= 0 || n_cand <= 0) return;
     hipLaunchKernelGGL(k_eigen_cand, dim3(count * n_cand), dim3(64), 0, s, base, ent_particle,
-                       ent_ncand, logits, n_cand, eig, cand_cls, eig_cache, max_classes, pool);
+                       ent_ncand, logits, n_cand, eig, cand_cls, eig_cache, max_classes, pool,
+                       copy_known ? 1 : 0);
 }
 
 void launch_pmat(const PmatArgs &a, hipStream_t s)

@@ -30,9 +30,25 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Where candidate ec of an entry of particle p keeps its eigensystem (see SiteClassArgs).
+struct CandEig {
+    const double *cand_eig;
+    const int *cls;
+    const double *cache;
+    int max_classes;
+    const double *pool;
+};
+__device__ __forceinline__ const double *cand_eigensystem(const CandEig &c, int ec, int p)
+{
+    const int k = c.cls ? c.cls[ec] : -1;
+    if (k >= 0) return c.cache + ((size_t)p * c.max_classes + k) * kEigStride;
+    if (k <= -2) return c.pool + (size_t)(-2 - k) * kEigStride;
+    return c.cand_eig + (size_t)ec * kEigStride;
+}
+
 __global__ __launch_bounds__(64) void k_site_class(
     const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
-    const int *__restrict__ ent_ncand, int n_cand, const double *__restrict__ cand_eig,
+    const int *__restrict__ ent_ncand, int n_cand, CandEig ce,
     const int *__restrict__ children, const int *__restrict__ prog, const uint8_t *__restrict__ codes,
     const double *__restrict__ blen, const double *__restrict__ rates,
     double *__restrict__ rate_lnl, int T, int R, int S_pad)
@@ -51,7 +67,7 @@ __global__ __launch_bounds__(64) void k_site_class(
     const int lane = threadIdx.x;
     const bool act = lane < 20;
     const int i = act ? lane : 0;
-    const double *eg = cand_eig + (size_t)ec * kEigStride;
+    const double *eg = cand_eigensystem(ce, ec, p);
     double arow[20], brow[20];
 #pragma unroll
     for (int k = 0; k < 20; k++) {
@@ -154,7 +170,7 @@ constexpr int kLvGroups = 12;
 
 __global__ __launch_bounds__(256) void k_site_class_lv(
     const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
-    const int *__restrict__ ent_ncand, int n_cand, const double *__restrict__ cand_eig,
+    const int *__restrict__ ent_ncand, int n_cand, CandEig ce,
     const int *__restrict__ children, const int *__restrict__ prog, const uint8_t *__restrict__ codes,
     const double *__restrict__ blen, const double *__restrict__ rates,
     double *__restrict__ rate_lnl, int T, int R, int S_pad)
@@ -177,7 +193,7 @@ __global__ __launch_bounds__(256) void k_site_class_lv(
     const int i = act ? wl - 20 * gw : 0;  // state owned by this lane
     const int g = (tid >> 6) * 3 + gw;     // group within the workgroup
     double *tg = tmp + (act ? g : 0) * 20;
-    const double *eg = cand_eig + (size_t)ec * kEigStride;
+    const double *eg = cand_eigensystem(ce, ec, p);
     double arow[20], brow[20];
 #pragma unroll
     for (int k = 0; k < 20; k++) {
@@ -256,8 +272,7 @@ __global__ __launch_bounds__(256) void k_site_class_lv(
 }
 
 __global__ void k_site_class_fold(const int *__restrict__ ent_particle, const int *__restrict__ ent_site,
-                                  const int *__restrict__ ent_ncand, int n_cand,
-                                  const double *__restrict__ cand_eig,
+                                  const int *__restrict__ ent_ncand, int n_cand, CandEig ce,
                                   const double *__restrict__ rate_lnl,
                                   const double *__restrict__ pinv, const double *__restrict__ isinv,
                                   const int *__restrict__ invchar, int R, double tempering,
@@ -273,7 +288,7 @@ __global__ void k_site_class_fold(const int *__restrict__ ent_particle, const in
     const int p = ent_particle[entry], site = ent_site[entry];
     double m = -INFINITY, s = 0.0;
     for (int r = 0; r < R; r++) lse_push(m, s, rate_lnl[(size_t)ec * R + r], mix_mode);
-    const double *pi = cand_eig + (size_t)ec * kEigStride + kEigPi;
+    const double *pi = cand_eigensystem(ce, ec, p) + kEigPi;
     out[ec] = tempering * site_log_likelihood(m, s, pinv[p], pi[invchar[site]], isinv[site], mix_mode);
 }
 
@@ -283,19 +298,20 @@ void launch_site_class(const SiteClassArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return;
     const unsigned blocks = (unsigned)((size_t)a.count * a.n_cand * a.R);
+    const CandEig ce{a.cand_eig, a.cand_cls, a.eig_cache, a.max_classes, a.pool};
     if (a.T <= kSiteLevelMaxTaxa) {
         const size_t lds = ((size_t)(a.T - 1) * 20 + kLvGroups * 20) * sizeof(double) + (size_t)(a.T - 1) * sizeof(int);
         hipLaunchKernelGGL(k_site_class_lv, dim3(blocks), dim3(256), lds, s, a.ent_particle, a.ent_site,
-                           a.ent_ncand, a.n_cand, a.cand_eig, a.children, a.prog, a.codes, a.blen,
+                           a.ent_ncand, a.n_cand, ce, a.children, a.prog, a.codes, a.blen,
                            a.rates, a.rate_lnl, a.T, a.R, a.S_pad);
     } else {
         hipLaunchKernelGGL(k_site_class, dim3(blocks), dim3(64), 0, s, a.ent_particle, a.ent_site,
-                           a.ent_ncand, a.n_cand, a.cand_eig, a.children, a.prog, a.codes, a.blen,
+                           a.ent_ncand, a.n_cand, ce, a.children, a.prog, a.codes, a.blen,
                            a.rates, a.rate_lnl, a.T, a.R, a.S_pad);
     }
     const int n = a.count * a.n_cand;
     hipLaunchKernelGGL(k_site_class_fold, dim3((n + 127) / 128), dim3(128), 0, s, a.ent_particle,
-                       a.ent_site, a.ent_ncand, a.n_cand, a.cand_eig, a.rate_lnl, a.pinv, a.isinv,
+                       a.ent_site, a.ent_ncand, a.n_cand, ce, a.rate_lnl, a.pinv, a.isinv,
                        a.invchar, a.R, a.tempering, a.mix_mode, a.count, a.out);
 }
 

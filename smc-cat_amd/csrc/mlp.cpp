@@ -659,7 +659,7 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         da.cand_cls = e->d_sc_cls;
         launch_mlp_draw(da, e->stream);
         HIP_TRY(e, hipGetLastError());
-        rc = issue_site_class(e, count, 2, true, tempering);
+        rc = issue_site_class(e, count, 2, true, false, tempering);  // every candidate is coded
         if (rc == SMC_OK) rc = d2h(e, logq.data(), m->d_logq, logq.size() * 4);
         if (rc == SMC_OK) rc = d2h(e, prop.data(), m->d_mv_pick, (size_t)count * 4);
         if (rc == SMC_OK) rc = d2h(e, ll.data(), e->d_sc_out, ll.size() * 8);
