@@ -393,7 +393,8 @@ int ensure_site_class_buffers(smc_engine *e, int count, int64_t need)
     return SMC_OK;
 }
 
-int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering)
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering,
+                     double *out)
 {
     hipStream_t s = e->stream;
     if (decompose) {  // candidates coded -1 (or every candidate without codes)
@@ -423,7 +424,7 @@ int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool d
         a.isinv = e->d_isinv;
         a.invchar = e->d_invchar;
         a.rate_lnl = e->d_sc_rate;
-        a.out = e->d_sc_out;
+        a.out = out ? out : e->d_sc_out;
         a.count = count;
         a.T = e->T;
         a.R = e->R;

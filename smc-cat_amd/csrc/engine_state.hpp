@@ -213,8 +213,9 @@ int ensure_site_class_buffers(smc_engine *e, int count, int64_t need);
 // candidate count] x count) with, when with_cls, the class codes in d_sc_cls
 // (existing classes and pool profiles are read where they lie); `decompose` first
 // decomposes the candidates coded -1 (every candidate without codes) from
-// d_sc_logits.  The scores land in d_sc_out ([count][n_cand]).
-int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering);
+// d_sc_logits.  The scores land in `out` ([count][n_cand]; null: d_sc_out).
+int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering,
+                     double *out = nullptr);
 
 // Adds the wall time of its scope to e->phase_ms[phase].
 struct PhaseTimer {

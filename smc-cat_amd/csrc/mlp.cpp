@@ -51,14 +51,10 @@ struct smc_mlp {
     int64_t cap_logq = 0;
     double *d_lacc = nullptr;
     // allocation-move batch (one example per particle and iteration)
-    float *d_mv_hdr = nullptr, *d_mv_cls = nullptr;
-    int *d_mv_K = nullptr;
-    int64_t *d_mv_coff = nullptr;
-    int64_t cap_mv_hdr = 0, cap_mv_cls = 0, cap_mv_K = 0, cap_mv_coff = 0;
-    // the draw of the move's proposals on the device
-    int *d_mv_aux = nullptr, *d_mv_codes = nullptr, *d_mv_pick = nullptr;
-    double *d_mv_u = nullptr;
-    int64_t cap_mv_aux = 0, cap_mv_codes = 0, cap_mv_pick = 0, cap_mv_u = 0;
+    // per site step, one packed upload (examples, class tables, draw inputs) and one
+    // packed readback (proposal log-weights, draws, site scores); see MoveStep
+    uint8_t *d_mv_step = nullptr;
+    int64_t cap_mv_step = 0;
 };
 
 namespace {
@@ -264,8 +260,7 @@ int smc_mlp_destroy(smc_mlp *m)
     if (!m) return SMC_OK;
     void *bufs[] = {m->d_theta, m->d_m, m->d_v, m->d_grad, m->d_hdr, m->d_cls, m->d_K, m->d_lab,
                     m->d_idx, m->d_coff, m->d_Z1, m->d_H1, m->d_Z2, m->d_H2, m->d_dH2, m->d_loss,
-                    m->d_part, m->d_logq, m->d_lacc, m->d_mv_hdr, m->d_mv_cls, m->d_mv_K,
-                    m->d_mv_coff, m->d_mv_aux, m->d_mv_codes, m->d_mv_pick, m->d_mv_u};
+                    m->d_part, m->d_logq, m->d_lacc, m->d_mv_step};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete m;
@@ -479,6 +474,31 @@ void add_class(MvState &ps, const std::array<double, 20> &lg, int orig)
 
 }  // namespace
 
+// Byte layout of one site step's device region: inputs [hdr | K | coff | aux | codes |
+// u | class table], then outputs [log q | draws | site scores], each part 256-byte
+// aligned.
+struct MoveStep {
+    size_t o_hdr, o_K, o_coff, o_aux, o_codes, o_u, o_cls, in_bytes;
+    size_t o_logq, o_pick, o_ll, out_bytes;
+    MoveStep(int count, int kmax, int stride)
+    {
+        auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+        const size_t n = (size_t)count;
+        o_hdr = 0;
+        o_K = up(o_hdr + n * kHdr * 4);
+        o_coff = up(o_K + n * 4);
+        o_aux = up(o_coff + n * 8);
+        o_codes = up(o_aux + n * 16);
+        o_u = up(o_codes + n * (kmax + 1) * 4);
+        o_cls = up(o_u + n * 8);
+        in_bytes = up(o_cls + n * kmax * kCls * 4);
+        o_logq = 0;
+        o_pick = up(n * stride * 4);
+        o_ll = up(o_pick + n * 4);
+        out_bytes = up(o_ll + n * 2 * 8);
+    }
+};
+
 extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *slots, double tempering,
                                           uint64_t seed, int num_iter, double dpp_alpha,
                                           double proposal_tempering, int64_t *stats_out)
@@ -537,15 +557,12 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
     rc = ensure_batch(m, count);
     if (rc != SMC_OK) return rc;
     const int stride = Kmax + 1;
-    HIP_TRY(e, grow(m->d_mv_hdr, m->cap_mv_hdr, (int64_t)count * kHdr));
-    HIP_TRY(e, grow(m->d_mv_K, m->cap_mv_K, count));
-    HIP_TRY(e, grow(m->d_mv_coff, m->cap_mv_coff, count));
-    HIP_TRY(e, grow(m->d_mv_cls, m->cap_mv_cls, (int64_t)count * Kmax * kCls));
-    HIP_TRY(e, grow(m->d_logq, m->cap_logq, (int64_t)count * stride));
-    HIP_TRY(e, grow(m->d_mv_aux, m->cap_mv_aux, (int64_t)count * 4));
-    HIP_TRY(e, grow(m->d_mv_codes, m->cap_mv_codes, (int64_t)count * (Kmax + 1)));
-    HIP_TRY(e, grow(m->d_mv_pick, m->cap_mv_pick, count));
-    HIP_TRY(e, grow(m->d_mv_u, m->cap_mv_u, count));
+    // one device region per site step: the inputs (uploaded as one copy; the class
+    // table, whose used length varies, last) and then the outputs (read back as one)
+    const MoveStep ms(count, Kmax, stride);
+    HIP_TRY(e, grow(m->d_mv_step, m->cap_mv_step, (int64_t)(ms.in_bytes + ms.out_bytes)));
+    uint8_t *const dst = m->d_mv_step, *const dout = m->d_mv_step + ms.in_bytes;
+    std::vector<uint8_t> step_in(ms.in_bytes), step_out(ms.out_bytes);
     rc = ensure_site_class_buffers(e, count, (int64_t)count * 2);
     if (rc != SMC_OK) return rc;
 
@@ -628,20 +645,21 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         // the network, the proposal draw and the exact site likelihoods of the drawn
         // and current classes run back to back, with one wait per site step
         const size_t n_ent = (size_t)(coff[count - 1] + Kv[count - 1]);
-        rc = h2d(e, m->d_mv_hdr, hdr.data(), hdr.size() * 4);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_K, Kv.data(), (size_t)count * 4);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_coff, coff.data(), (size_t)count * 8);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_cls, cls.data(), n_ent * kCls * 4);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_aux, aux.data(), aux.size() * 4);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_codes, codes.data(), codes.size() * 4);
-        if (rc == SMC_OK) rc = h2d(e, m->d_mv_u, uu.data(), uu.size() * 8);
+        std::memcpy(step_in.data() + ms.o_hdr, hdr.data(), hdr.size() * 4);
+        std::memcpy(step_in.data() + ms.o_K, Kv.data(), (size_t)count * 4);
+        std::memcpy(step_in.data() + ms.o_coff, coff.data(), (size_t)count * 8);
+        std::memcpy(step_in.data() + ms.o_aux, aux.data(), aux.size() * 4);
+        std::memcpy(step_in.data() + ms.o_codes, codes.data(), codes.size() * 4);
+        std::memcpy(step_in.data() + ms.o_u, uu.data(), uu.size() * 8);
+        std::memcpy(step_in.data() + ms.o_cls, cls.data(), n_ent * kCls * 4);
+        rc = h2d(e, dst, step_in.data(), ms.o_cls + n_ent * kCls * 4);
         if (rc != SMC_OK) return rc;
         MlpArgs a = batch_args(m, count, nullptr);
-        a.hdr = m->d_mv_hdr;
-        a.K = m->d_mv_K;
-        a.coff = m->d_mv_coff;
-        a.cls = m->d_mv_cls;
-        a.logq = m->d_logq;
+        a.hdr = (const float *)(dst + ms.o_hdr);
+        a.K = (const int *)(dst + ms.o_K);
+        a.coff = (const int64_t *)(dst + ms.o_coff);
+        a.cls = (const float *)(dst + ms.o_cls);
+        a.logq = (float *)(dout + ms.o_logq);
         a.out_stride = stride;
         launch_mlp_trunk(a, false, e->stream);
         launch_mlp_head(a, false, e->stream);
@@ -649,22 +667,24 @@ extern "C" int smc_learned_partition_move(smc_mlp *m, int count, const int32_t *
         da.count = count;
         da.stride = stride;
         da.kmax = Kmax;
-        da.logq = m->d_logq;
-        da.K = m->d_mv_K;
-        da.aux = m->d_mv_aux;
-        da.codes = m->d_mv_codes;
-        da.u = m->d_mv_u;
-        da.pick = m->d_mv_pick;
+        da.logq = a.logq;
+        da.K = a.K;
+        da.aux = (const int *)(dst + ms.o_aux);
+        da.codes = (const int *)(dst + ms.o_codes);
+        da.u = (const double *)(dst + ms.o_u);
+        da.pick = (int *)(dout + ms.o_pick);
         da.ent = e->d_sc_ent;
         da.cand_cls = e->d_sc_cls;
         launch_mlp_draw(da, e->stream);
         HIP_TRY(e, hipGetLastError());
-        rc = issue_site_class(e, count, 2, true, false, tempering);  // every candidate is coded
-        if (rc == SMC_OK) rc = d2h(e, logq.data(), m->d_logq, logq.size() * 4);
-        if (rc == SMC_OK) rc = d2h(e, prop.data(), m->d_mv_pick, (size_t)count * 4);
-        if (rc == SMC_OK) rc = d2h(e, ll.data(), e->d_sc_out, ll.size() * 8);
+        rc = issue_site_class(e, count, 2, true, false, tempering,  // every candidate is coded
+                              (double *)(dout + ms.o_ll));
+        if (rc == SMC_OK) rc = d2h(e, step_out.data(), dout, ms.out_bytes);
         if (rc == SMC_OK) rc = sync_stream(e);
         if (rc != SMC_OK) return rc;
+        std::memcpy(logq.data(), step_out.data() + ms.o_logq, logq.size() * 4);
+        std::memcpy(prop.data(), step_out.data() + ms.o_pick, (size_t)count * 4);
+        std::memcpy(ll.data(), step_out.data() + ms.o_ll, ll.size() * 8);
         for (int i = 0; i < count; ++i) {
             updates++;
             if (prop[i] == cur[i]) continue;
