@@ -400,7 +400,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--rj-taxa", type=int, default=0)
     ap.add_argument("--rj-sites", type=int, default=0)
-    ap.add_argument("--rj-particles", type=int, default=100)
+    ap.add_argument("--rj-particles", type=int, default=150)
     ap.add_argument("--rj-plain", type=int, default=8)
     ap.add_argument("--rj-iterations", type=int, default=16)
     ap.add_argument("--rj-temp", type=float, default=0.5)
@@ -425,7 +425,7 @@ def main():
     ap.add_argument("--ess-sites", type=int, default=200)
     ap.add_argument("--ess-particles", type=int, default=64)
     ap.add_argument("--ess-iterations", type=int, default=4)
-    ap.add_argument("--ess-seeds", type=int, default=1)
+    ap.add_argument("--ess-seeds", type=int, default=3)
     ap.add_argument("--ess-attribution", action="store_true",
                     help="also run FP64 batched with the Gibbs move")
     ap.add_argument("--c5-taxa", type=int, default=0)
