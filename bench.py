@@ -34,9 +34,11 @@ def algorithmic_bytes_per_column(T, word):
     return (T - 2) * 20 * word + T * 1 + (T - 1) * (20 * word + 8)
 
 
-def profiled_traffic(config, precision, variant, particles, default_particles):
-    """HBM bytes per pruning launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
-    (calibrated against a plain copy) for this exact workload, or None."""
+def profiled_value(name, config, precision, variant, particles, default_particles):
+    """A figure measured with rocprofv3 for this exact workload (machine-readable
+    lines of the committed profile summary), or None: traffic_bytes_per_launch (HBM
+    bytes per pruning launch, FETCH_SIZE/WRITE_SIZE calibrated against a plain copy),
+    mfma_busy (MfmaUtil over the pruning dispatches), mfma_tflops (MfmaFlops / time)."""
     if config != "c2" or variant != "mfma" or particles != default_particles:
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -44,12 +46,16 @@ def profiled_traffic(config, precision, variant, particles, default_particles):
     try:
         with open(path) as f:
             for line in f:
-                m = re.match(r"traffic_bytes_per_launch\[(\w+)\] = ([0-9.e+]+)", line.strip())
-                if m and m.group(1) == precision:
-                    return float(m.group(2))
+                m = re.match(r"(\w+)\[(\w+)\] = ([0-9.e+]+)", line.strip())
+                if m and m.group(1) == name and m.group(2) == precision:
+                    return float(m.group(3))
     except OSError:
         return None
     return None
+
+
+def profiled_traffic(config, precision, variant, particles, default_particles):
+    return profiled_value("traffic_bytes_per_launch", config, precision, variant, particles, default_particles)
 
 
 def cpu_baseline(codes, batch, R, threads, budget_s=10.0):
@@ -544,6 +550,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.config, args.precision, args.variant, N,
                                                  cfg["particles"]),
+                     "mfma": {"busy": profiled_value("mfma_busy", args.config, args.precision, args.variant, N,
+                                                     cfg["particles"]),
+                              "achieved": profiled_value("mfma_tflops", args.config, args.precision, args.variant,
+                                                         N, cfg["particles"]),
+                              "peak": 78.6 if word == 8 else 157.3, "unit": "TFLOP/s",
+                              "source": "rocprofv3 --pmc MfmaUtil MfmaFlops (committed profile)"},
                      "kernel": "%s<%s>" % ("k_prune_mfma" if args.variant == "mfma" else "k_prune",
                                            "double" if word == 8 else "float"),
                      "launches_per_step": launches_per_sweep,
