@@ -230,15 +230,17 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // once, spread over the workgroup, and the 420 values of each P are written by
 // consecutive lanes.
 constexpr int kPmLanes = 256;
-constexpr int kPmPairs = 32;  // (node, rate) pairs per workgroup
+constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
+constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
+constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 128 pairs from which launches use them
 
-template <typename TP, typename PairNode>
+template <typename TP, int PAIRS, typename PairNode>
 __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const double *__restrict__ bl,
                                            const double *__restrict__ rt, int R, int q0, int q1,
                                            PairNode node_of, TP *__restrict__ Pcls)
 {
     __shared__ double bsum[20];
-    __shared__ double ex[kPmPairs][20];
+    __shared__ double ex[PAIRS][20];
     const int tid = threadIdx.x;
     if (tid < 20) {
         double a = 0.0;
@@ -278,8 +280,10 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
 }
 
 // Every (node, rate) of every class of particles first.. (or plist[..]):
-// workgroup (particle * max_classes + class, chunk of kPmPairs pairs).
-template <typename TP>
+// workgroup (particle * max_classes + class, chunk of PAIRS pairs).  The per-class
+// setup of a workgroup (row sums, per-entry products) is shared by its PAIRS pairs:
+// launches with many workgroups take long runs, small ones short runs for parallelism.
+template <typename TP, int PAIRS>
 __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ eig,
                                                     const double *__restrict__ blen,
                                                     const double *__restrict__ rates,
@@ -292,11 +296,11 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ ei
     const int p = plist ? plist[blockIdx.x / max_classes] : first + blockIdx.x / max_classes;
     const int k = blockIdx.x % max_classes;
     if (k >= num_classes[p]) return;  // whole workgroup
-    const int q0 = blockIdx.y * kPmPairs, q1 = min(q0 + kPmPairs, nn * R);
+    const int q0 = blockIdx.y * PAIRS, q1 = min(q0 + PAIRS, nn * R);
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
-    pmat_pairs<TP>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
-                   [](int n) { return n; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
+    pmat_pairs<TP, PAIRS>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+                          [](int n) { return n; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
 // P of listed nodes only: workgroup (listed particle * max_classes + class, chunk
@@ -320,8 +324,8 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat_nodes(const double *__restric
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
     const int *nl = nodes + n0;
-    pmat_pairs<TP>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
-                   [nl](int i) { return nl[i]; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
+    pmat_pairs<TP, kPmPairs>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+                             [nl](int i) { return nl[i]; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
 }  // namespace
@@ -344,18 +348,28 @@ void launch_eigen_cand(const double *base, const int *ent_particle, const int *e
                        copy_known ? 1 : 0);
 }
 
-void launch_pmat(const PmatArgs &a, hipStream_t s)
+template <int PAIRS>
+void launch_pmat_pairs(const PmatArgs &a, hipStream_t s)
 {
-    if (a.count <= 0) return;
-    dim3 grid(a.count * a.max_classes, (a.nn * a.R + kPmPairs - 1) / kPmPairs);
+    dim3 grid(a.count * a.max_classes, (a.nn * a.R + PAIRS - 1) / PAIRS);
     if (a.fp32)
-        hipLaunchKernelGGL(k_pmat<float>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL((k_pmat<float, PAIRS>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
     else
-        hipLaunchKernelGGL(k_pmat<double>, grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL((k_pmat<double, PAIRS>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
+}
+
+void launch_pmat(const PmatArgs &a, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    const int64_t large = (int64_t)a.count * a.max_classes * ((a.nn * a.R + kPmPairsLarge - 1) / kPmPairsLarge);
+    if (large >= kPmLargeGrid)
+        launch_pmat_pairs<kPmPairsLarge>(a, s);
+    else
+        launch_pmat_pairs<kPmPairs>(a, s);
 }
 
 }  // namespace smccat
