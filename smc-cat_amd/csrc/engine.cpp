@@ -217,6 +217,8 @@ int build_schedule(smc_engine *e, int first, int count)
         boffs.insert(boffs.end(), w.bucket_off.begin(), w.bucket_off.end());
     }
     if ((int64_t)boffs.size() > e->bucket_cap) {
+        // captured sweeps still in flight read the old buffer
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
         if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
         e->d_bucket_off = nullptr;
         e->bucket_cap = 0;
@@ -913,7 +915,7 @@ int run_sweep(smc_engine *e, int first, int count)
     }
     std::vector<int64_t> key = {first, count, e->fp32, e->mix_mode, e->WS, e->tune.blocks_per_cu,
                                 e->tune.unroll[0], e->tune.unroll[1], e->tune.unroll[2], e->tune.xcd_groups,
-                                (int64_t)(intptr_t)e->d_bucket_off};
+                                e->tune.unroll_rt, (int64_t)(intptr_t)e->d_bucket_off};
     for (const smc_engine::Wave &w : e->waves) {
         key.push_back(w.first);
         key.push_back(w.count);
@@ -928,6 +930,8 @@ int run_sweep(smc_engine *e, int first, int count)
         }
     if (!hit) {
         if (e->graphs.size() >= smc_engine::kMaxGraphs) {  // drop the least recently used
+            // the evicted graph may still be running on the stream: let it finish first
+            HIP_TRY(e, hipStreamSynchronize(s));
             auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
                                         [](const smc_engine::CachedGraph &a, const smc_engine::CachedGraph &b) {
                                             return a.used < b.used;

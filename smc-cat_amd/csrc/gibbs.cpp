@@ -12,6 +12,16 @@
 // (Q15) are the corrected ones.  Random numbers come from a counter-based
 // Philox4x32-10 stream keyed by (seed, particle slot), so results do not depend
 // on batching or thread schedules.
+//
+// Two differences make each update an exact Algorithm-8 step:
+//  * a site that was alone in its class keeps that class's profile as the first
+//    auxiliary (Neal 2000, Algorithm 8); the reference draws all five afresh;
+//  * classes are capped at max_classes, so the target is the DP posterior
+//    restricted to K <= max_classes: with K classes left after the removal the
+//    update uses m' = min(5, max_classes - K) auxiliaries, each weighted
+//    log(alpha / m') (a valid Algorithm 8 with m' auxiliaries), and with m' = 0
+//    the site can only join an existing class (the exact conditional of the
+//    restricted target).  The learned move targets the same restriction.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -132,11 +142,14 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
     };
     rc0 = smc_pool_refill(e, count, slots, pool_q, draw, 1);
     if (rc0 != SMC_OK) return rc0;
-    const double log_aux = std::log(dpp_alpha / m);
     const int n_cand = Kmax + m;
     std::vector<int32_t> ent_p(count), ent_s(count), ent_n(count);
     std::vector<double> cand((size_t)count * n_cand * 20, 0.0), ll((size_t)count * n_cand);
     std::vector<int32_t> cand_cls((size_t)count * n_cand, -1);
+    // per particle of the current iteration: auxiliaries, and the engine class slot and
+    // profile of a class the removal emptied (-1: none), which is auxiliary 0
+    std::vector<int> naux(count, 0), single_slot(count, -1);
+    std::vector<std::array<double, 20>> single_logits(count);
     int64_t moved = 0, created = 0, erased = 0;
 
     for (int it = 0; it < iters; ++it) {
@@ -154,8 +167,11 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             ps.size[k0]--;
             ps.assign[site] = -1;
             int was = k0;
+            single_slot[i] = -1;
             if (ps.size[k0] == 0) {  // erase the emptied class (Particle.cpp:819-821)
                 was = -1;
+                single_slot[i] = ps.orig[k0];
+                single_logits[i] = ps.logits[k0];
                 ps.logits.erase(ps.logits.begin() + k0);
                 ps.free.push_back(ps.orig[k0]);
                 ps.orig.erase(ps.orig.begin() + k0);
@@ -163,10 +179,11 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
                 for (int &a : ps.assign)
                     if (a > k0) a--;
                 ps.K--;
-                erased++;
             }
-            // auxiliary classes cannot exceed the engine's class capacity
+            // auxiliary classes cannot exceed the engine's class capacity (an emptied
+            // class leaves at least one slot)
             const int n_aux = std::min(m, Kmax - ps.K);
+            naux[i] = n_aux;
             ent_p[i] = slots[i];
             ent_s[i] = site;
             ent_n[i] = ps.K + n_aux;
@@ -176,7 +193,13 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
                 std::copy(ps.logits[k].begin(), ps.logits[k].end(), c + 20 * k);
                 cc[k] = ps.orig[k];
             }
-            for (int a = 0; a < n_aux; ++a) {
+            int a0 = 0;
+            if (single_slot[i] >= 0) {  // the emptied class's profile, scored from its eigensystem
+                std::copy(single_logits[i].begin(), single_logits[i].end(), c + 20 * ps.K);
+                cc[ps.K] = single_slot[i];
+                a0 = 1;
+            }
+            for (int a = a0; a < n_aux; ++a) {
                 const int slot = e->pool.next[i]++;
                 cc[ps.K + a] = -2 - (i * pool_q + slot);
                 std::copy_n(&e->pool.logits[((size_t)i * pool_q + slot) * 20], 20, c + 20 * (ps.K + a));
@@ -194,6 +217,7 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             const double *l = &ll[(size_t)i * n_cand];
             std::vector<double> w(nc);
             double mx = -INFINITY;
+            const double log_aux = naux[i] > 0 ? std::log(dpp_alpha / naux[i]) : -INFINITY;
             for (int k = 0; k < nc; ++k) {
                 w[k] = l[k] + (k < ps.K ? std::log((double)ps.size[k]) : log_aux);
                 if (w[k] > mx) mx = w[k];
@@ -215,24 +239,33 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
             }
             const int was = -2 - ps.assign[site];
             if (sink) sink->add(site, ps.logits, ps.size, std::min(pick, ps.K), tempering, dpp_alpha);
+            bool stayed = false;
             if (pick >= ps.K) {  // a new class from the chosen auxiliary
                 std::array<double, 20> lg;
                 std::copy(&cand[((size_t)i * n_cand + pick) * 20],
                           &cand[((size_t)i * n_cand + pick) * 20] + 20, lg.begin());
                 ps.logits.push_back(lg);
-                // its eigensystem (a pool profile) moves into a free engine class slot
+                // the free slot pushed last is the emptied class's, if any: the site's
+                // own profile keeps its eigensystem there, a pool profile's moves in
                 const int f = ps.free.back();
                 ps.free.pop_back();
-                int rc = smc_pool_to_class(e, slots[i], -2 - cand_cls[(size_t)i * n_cand + pick], f);
-                if (rc != SMC_OK) return rc;
+                const int32_t code = cand_cls[(size_t)i * n_cand + pick];
+                if (code >= 0) {
+                    if (code != f) return smc_fail(e, SMC_ERR_STATE, "Gibbs move: class slot bookkeeping");
+                    stayed = true;
+                } else {
+                    int rc = smc_pool_to_class(e, slots[i], -2 - code, f);
+                    if (rc != SMC_OK) return rc;
+                    created++;
+                }
                 ps.orig.push_back(f);
                 ps.size.push_back(0);
                 pick = ps.K++;
-                created++;
             }
+            if (single_slot[i] >= 0 && !stayed) erased++;
             ps.assign[site] = pick;
             ps.size[pick]++;
-            if (pick != was) moved++;
+            if (pick != was && !stayed) moved++;
         }
     }
     // write the new allocations back and re-upload the particles

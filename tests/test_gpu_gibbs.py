@@ -69,3 +69,82 @@ def test_gibbs_conditional_is_exact(gpu):
         sd = math.sqrt(p * (1 - p) / n)
         assert abs(obs - p) < 4.5 * sd + 1e-12, (obs, p)
     assert np.all((n0 >= a - 1) & (n0 <= a + 1))
+
+
+def _crp_outcomes(groups, kmax, alpha, m=5):
+    """Exact distribution of one Algorithm-8 update on uninformative data.
+
+    groups: site labels of the starting partition.  A uniformly drawn site leaves
+    its class; with K classes left it joins class k with weight n_k or one of
+    m' = min(m, kmax - K) auxiliaries with weight alpha / m' each.  Auxiliary 0 is
+    the site's own profile when it was alone in its class (label kept), the others
+    are fresh prior profiles (label 'F').  Returns {final labels per site: prob}."""
+    S = len(groups)
+    out = {}
+    for s in range(S):
+        rest = [g for i, g in enumerate(groups) if i != s]
+        labels = sorted(set(rest))
+        sizes = {g: rest.count(g) for g in labels}
+        K = len(labels)
+        naux = min(m, kmax - K)
+        singleton = groups[s] not in sizes
+        opts = [(g, float(sizes[g])) for g in labels]
+        for a in range(naux):
+            opts.append((groups[s] if (singleton and a == 0) else "F", alpha / naux))
+        tot = sum(w for _, w in opts)
+        for g, w in opts:
+            key = tuple(groups[:s]) + (g,) + tuple(groups[s + 1:])
+            out[key] = out.get(key, 0.0) + w / (S * tot)
+    return out
+
+
+@pytest.mark.parametrize("kmax", [4, 6])
+def test_gibbs_update_at_class_capacity(gpu, kmax):
+    """Near max_classes the update is Algorithm 8 with m' = min(5, max_classes - K)
+    auxiliaries of weight alpha / m' (new-class mass alpha whatever m'), and none at
+    K = max_classes; a class emptied by the removal offers its own profile as the
+    first auxiliary.  All-gap columns with pInvar 0 make every site likelihood equal
+    (L = K_r under any profile), so the draw follows the CRP weights alone: the
+    final partitions of 6,000 one-step moves are checked by chi-square against the
+    exact enumeration."""
+    from scipy import stats
+
+    _say("case gibbs capacity: max_classes %d, K 4, 6000 one-step draws" % kmax)
+    rng = np.random.default_rng(31 + kmax)
+    T, S, R, alpha = 8, 6, 2, 1.5
+    codes = np.full((T, S), 20, dtype=np.int8)               # every column all '-'
+    base = sim.prior_particles(rng, 1, T, S, num_rates=R, invar=True, k_range=(4, 4), max_classes=kmax)
+    groups = ["A", "A", "B", "B", "C", "D"]
+    base.assignment[0] = np.array([0, 0, 1, 1, 2, 3], dtype=np.int32)
+    base.pinv[:] = 0.0
+    n = 6000
+    batch = base.take(np.zeros(n, dtype=int))
+    orig = {tuple(base.logits[0, k]): "ABCD"[k] for k in range(4)}
+    with Engine(T, S, R, n, kmax) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        eng.gibbs_partition_move(np.arange(n), 1.0, seed=77, num_iter=1, dpp_alpha=alpha)
+        seen = {}
+        for p in range(n):
+            K, logits, asg = eng.particle_state(p)
+            lab = [orig.get(tuple(logits[k]), "F") for k in range(K)]
+            key = tuple(lab[a] for a in asg)
+            seen[key] = seen.get(key, 0) + 1
+    want = _crp_outcomes(groups, kmax, alpha)
+    assert set(seen) <= set(want), set(seen) - set(want)
+    if kmax == 4:                                            # no room for a fresh profile
+        assert all("F" not in k for k in seen)
+    keys = sorted(want, key=lambda k: -want[k])
+    big = [k for k in keys if want[k] * n >= 5]
+    obs = [seen.get(k, 0) for k in big]
+    exp = [want[k] * n for k in big]
+    rest_o, rest_e = n - sum(obs), n - sum(exp)
+    if rest_e >= 5:
+        obs.append(rest_o)
+        exp.append(rest_e)
+    else:
+        exp[-1] += rest_e
+        obs[-1] += rest_o
+    chi2, pval = stats.chisquare(obs, exp)
+    _say("  %d cells, chi2 %.1f, p %.3g" % (len(obs), chi2, pval))
+    assert pval > 1e-3, (chi2, pval)
