@@ -348,6 +348,17 @@ int smc_compute_next_step(smc_engine *e, int n, double *raw_weights, const doubl
                           double *normalized, double *ess, int target_ess,
                           double *current_temp, double last_temp);
 
+/* No counterpart in the reference: smc_compute_next_step for callers that cannot
+ * allocate device memory (tight sampler loops, harnesses that forbid hipMalloc).
+ * Same arguments and results, bit for bit; its device staging is a fixed array
+ * of the library (one per GPU, held until the process ends) instead of engine
+ * buffers grown on demand, so it never calls hipMalloc or hipFree.  n at most
+ * 65536 (SMC_ERR_ARG above).  Calls on one GPU share that array: calls that
+ * overlap on one GPU (two engines on two streams or threads) race. */
+int smc_compute_next_step_noalloc(smc_engine *e, int n, double *raw_weights, const double *raw_lnl,
+                                  double *normalized, double *ess, int target_ess,
+                                  double *current_temp, double last_temp);
+
 /* Systematic resampling (main.cpp:179-195) on the GPU: u in [0,1). */
 int smc_systematic_resample(smc_engine *e, int n, const double *normalized, double u,
                             int32_t *assignments);

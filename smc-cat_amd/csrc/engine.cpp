@@ -538,6 +538,13 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
         fprintf(stderr, "smc_engine_create: %s\n", msg.c_str());
         return rc;
     }
+    rc = smc_noalloc_init(e);  // this device's address of the no-allocation scratch
+    if (rc != SMC_OK) {
+        std::string msg = e->err;
+        smc_engine_destroy(e);
+        fprintf(stderr, "smc_engine_create: %s\n", msg.c_str());
+        return rc;
+    }
     (void)hipMemsetAsync(e->d_ntiles, 0, P * sizeof(int), e->stream);
     (void)hipMemsetAsync(e->d_nclass, 0, P * sizeof(int), e->stream);
     (void)hipMemsetAsync(e->d_xflag, 0, P * (T - 1), e->stream);

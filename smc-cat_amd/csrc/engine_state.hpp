@@ -70,6 +70,9 @@ struct smc_engine {
     double *d_lnl = nullptr;
     uint8_t *d_nonfinite = nullptr;
     Item *d_items = nullptr;
+    // staging of smc_compute_next_step_noalloc: this device's copy of a file-scope
+    // device array (never allocated or freed by the engine)
+    double *d_noalloc_scratch = nullptr;
     // SMC scratch (grown on demand)
     int smc_cap = 0;
     double *d_w = nullptr, *d_l = nullptr, *d_norm = nullptr, *d_out2 = nullptr;
@@ -192,6 +195,10 @@ struct smc_engine {
 };
 
 int smc_fail(smc_engine *e, int code, const std::string &msg);
+
+// Looks up this device's address of the no-allocation scratch array (once per
+// device, with the engine's device current) into e->d_noalloc_scratch.
+int smc_noalloc_init(smc_engine *e);
 
 // Queues a copy of `bytes` host bytes to device `dst` on the engine's stream
 // through the pinned staging buffer: it returns without waiting for the device,

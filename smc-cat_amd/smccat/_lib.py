@@ -86,6 +86,7 @@ SIGNATURES = {
     "smc_set_particles": (_I, [_P, _I, _I, _IP, _DP, _DP, _IP, _DP, _IP, _DP, _DP]),
     "smc_refresh_likelihood": (_I, [_P, _I, _I, _DP, _BP]),
     "smc_compute_next_step": (_I, [_P, _I, _DP, _DP, _DP, _DP, _I, _DP, _D]),
+    "smc_compute_next_step_noalloc": (_I, [_P, _I, _DP, _DP, _DP, _DP, _I, _DP, _D]),
     "smc_systematic_resample": (_I, [_P, _I, _DP, _D, _IP]),
     "smc_discretize_gamma": (_I, [_D, _I, _DP]),
     "smc_lg_base_matrix": (None, [_DP]),
