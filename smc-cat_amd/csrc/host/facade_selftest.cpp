@@ -10,12 +10,19 @@
 //   logits <20 values> (K lines) / assign <S values> / end
 // --cpu: Newick parse/write only (no GPU calls).  --moves: also runs the Gibbs
 // move and rejuvenation and writes the resulting records to out.txt.
+// --mcmc <iterations>: runs the loop of Mcmc::operator() (Mcmc.cpp:25-73), in its
+// own shape, on every record and on one particle built by the seeded constructor,
+// through the facade's moves, refreshLikelihood(), accept() and reject(); prints
+// each final state with its incremental and forced likelihoods.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
+#include <map>
 #include <sstream>
+#include <tuple>
 
 #include "particle.hpp"
 
@@ -62,6 +69,105 @@ static std::vector<SerializedParticle> read_particles(const std::string &path)
     return out;
 }
 
+// Mcmc::operator() (Mcmc.cpp:25-73) with the move table of main.cpp:54-105,
+// written as the reference writes it, on a facade Particle.
+struct FacadeMcmc {
+    std::vector<std::tuple<double, std::function<int(Particle &)>, std::function<double(Particle &)>>> moves;
+    std::vector<double> moveProbs;
+    long accepted = 0, rejected = 0;
+
+    void initMoveProbs()
+    {
+        double tot = 0.0, cum = 0.0;
+        for (auto &m : moves) tot += std::get<0>(m);
+        for (auto &m : moves) {
+            cum += std::get<0>(m);
+            moveProbs.push_back(cum / tot);
+        }
+    }
+    void operator()(Particle &model, int iterations, double tempering)
+    {
+        Rng &rng = model.getRng();
+        model.refreshLikelihood();
+        double currentLikelihood = model.lnLikelihood() * tempering;
+        double currentPrior = model.lnPrior();
+        double currentPosterior = currentLikelihood + currentPrior;
+        for (int i = 0; i < iterations; i++) {
+            const double moveChoice = rng.uniform();
+            std::function<double(Particle &)> proposal;
+            int numGibbs = 3;
+            for (size_t j = 0; j < moveProbs.size(); j++) {
+                if (moveChoice < moveProbs[j]) {
+                    proposal = std::get<2>(moves[j]);
+                    numGibbs = std::get<1>(moves[j])(model);
+                    break;
+                }
+            }
+            for (int j = 0; j < numGibbs; j++) {
+                const double hastings = proposal(model);
+                model.refreshLikelihood();
+                const double newPrior = model.lnPrior();
+                const double newLikelihood = model.lnLikelihood() * tempering;
+                const double newPosterior = newPrior + newLikelihood;
+                const double posteriorRatio = newPosterior - currentPosterior + hastings;
+                if (std::log(rng.uniform()) <= posteriorRatio) {
+                    model.accept();
+                    accepted++;
+                    currentPosterior = newPosterior;
+                    currentPrior = newPrior;
+                    currentLikelihood = newLikelihood;
+                } else {
+                    model.reject();
+                    rejected++;
+                }
+            }
+        }
+    }
+};
+
+static int run_mcmc(const Alignment &aln, std::vector<SerializedParticle> &recs, int R, int K, bool lg,
+                    int iterations)
+{
+    const int T = aln.getNumTaxa();
+    const int n = (int)recs.size() + 1;
+    ParticleEngine eng(aln, R, n, K, lg, SMC_PREC_FP64, 0, 2);
+    const std::map<int, double> splitPosteriorProbabilities;  // adaptive NNI's input (unused)
+    FacadeMcmc mcmc;
+    mcmc.moves.emplace_back(3.0, [T](Particle &) { return (int)(T * 0.25); },
+                            [&](Particle &m) { return m.treeMove(splitPosteriorProbabilities); });
+    mcmc.moves.emplace_back(1.0, [](Particle &m) { return m.getNumCategories() * 2; },
+                            [](Particle &m) { return m.stationaryMove(); });
+    if (R > 1) mcmc.moves.emplace_back(1.0, [](Particle &) { return 2; }, [](Particle &m) { return m.shapeMove(); });
+    mcmc.moves.emplace_back(1.0, [](Particle &) { return 2; }, [](Particle &m) { return m.invarMove(); });
+    if (!lg) mcmc.moves.emplace_back(1.0, [](Particle &) { return 5; }, [](Particle &m) { return m.rateMatrixMove(); });
+    mcmc.initMoveProbs();
+    for (int i = 0; i < n; ++i) {
+        const bool built = i == (int)recs.size();
+        Particle p = built ? Particle(42, eng, i, true, true) : Particle(eng, i);
+        if (!built) p.copyFromSerialized(recs[i]);
+        const long a0 = mcmc.accepted, r0 = mcmc.rejected;
+        mcmc(p, iterations, 0.5);
+        const double inc = p.lnLikelihood(), prior = p.lnPrior();
+        p.refreshLikelihood(true);
+        printf("mcmc_lnl %d %.17g %.17g %.17g %ld %ld\n", i, inc, p.lnLikelihood(), prior, mcmc.accepted - a0,
+               mcmc.rejected - r0);
+        const ParticleEngine::Slot &s = eng.slots.at(i);
+        printf("mcmc_ch %d", i);
+        for (int32_t c : s.cur.ch) printf(" %d", c);
+        printf("\nmcmc_len %d", i);
+        for (double b : s.cur.bl) printf(" %.17g", b);
+        printf("\nmcmc_par %d %.17g %.17g\nmcmc_base %d", i, s.cur.pinv, s.cur.shape, i);
+        for (double b : s.cur.base) printf(" %.17g", b);
+        printf("\nmcmc_cls %d %d", i, p.getNumCategories());
+        for (auto &st : p.getCategories())
+            for (double v : st) printf(" %.17g", v);
+        printf("\nmcmc_asg %d", i);
+        for (int a : p.getAssignments()) printf(" %d", a);
+        printf("\n");
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 6) {
@@ -74,6 +180,7 @@ int main(int argc, char **argv)
         auto recs = read_particles(argv[2]);
         const int R = std::atoi(argv[3]), K = std::atoi(argv[4]);
         const bool lg = std::atoi(argv[5]) != 0;
+        if (argc > 7 && std::strcmp(argv[6], "--mcmc") == 0) return run_mcmc(aln, recs, R, K, lg, std::atoi(argv[7]));
         if (cpu) {
             for (size_t i = 0; i < recs.size(); ++i) {
                 TreeTopology t = parseNewick(recs[i].newick, aln.getTaxaNames());

@@ -218,10 +218,51 @@ ParticleEngine::~ParticleEngine()
     if (eng) smc_engine_destroy(eng);
 }
 
+namespace {
+
+// Sets the slot's current and committed parameters from the engine's copy.
+void params_from_engine(smc_engine *eng, int slot, int T, ParticleEngine::Slot &sl)
+{
+    MoveParams q;
+    q.ch.assign((size_t)(T - 1) * 2, 0);
+    q.bl.assign(2 * T - 1, 0.0);
+    q.base.assign(190, 0.0);
+    check(eng, smc_get_particle_params(eng, slot, q.ch.data(), q.bl.data(), q.base.data(), &q.pinv, &q.shape),
+          "smc_get_particle_params");
+    sl.cur = q;
+    sl.committed = q;
+    sl.flags.assign(2 * T - 1, 0);
+    sl.eig = 0;
+    sl.unsent = sl.pending = false;
+}
+
+// The tree and record fields of a slot from its current parameters.
+void sync_state(ParticleEngine::Slot &sl, int T)
+{
+    TreeTopology &t = sl.tree;
+    t.children = sl.cur.ch;
+    t.branchLen = sl.cur.bl;
+    std::vector<char> is_child(2 * T - 1, 0);
+    for (int32_t c : t.children) is_child[c] = 1;
+    for (int n = T; n < 2 * T - 1; ++n)
+        if (!is_child[n]) t.root = n;
+    t.branchLen[t.root] = 0.0;
+    std::copy(sl.cur.base.begin(), sl.cur.base.end(), sl.sp.baseMatrix.begin());
+    sl.sp.pInvar = sl.cur.pinv;
+    sl.sp.shape = sl.cur.shape;
+    sl.sp.newick = writeNewick(t);
+}
+
+}  // namespace
+
 void ParticleEngine::upload(int slot, const SerializedParticle &sp)
 {
+    uploadState(slot, parseNewick(sp.newick, aln.getTaxaNames()), sp);
+}
+
+void ParticleEngine::uploadState(int slot, TreeTopology t, const SerializedParticle &sp)
+{
     const int T = aln.getNumTaxa(), S = aln.getNumChar();
-    TreeTopology t = parseNewick(sp.newick, aln.getTaxaNames());
     const int K = (int)sp.stationaries.size();
     if (K < 1 || K > kmax) throw std::runtime_error("number of classes out of range");
     if ((int)sp.assignments.size() != S) throw std::runtime_error("assignments size != sites");
@@ -235,16 +276,28 @@ void ParticleEngine::upload(int slot, const SerializedParticle &sp)
                                  sp.baseMatrix.data(), &nc, logits.data(), sp.assignments.data(),
                                  &pinv, &shape),
           "smc_set_particles");
-    slots.at(slot).sp = sp;
-    slots.at(slot).tree = std::move(t);
-    slots.at(slot).valid = true;
+    Slot &sl = slots.at(slot);
+    sl.sp = sp;
+    sl.tree = std::move(t);
+    sl.valid = true;
+    sl.haveLnl = false;
+    params_from_engine(eng, slot, T, sl);
 }
 
 void ParticleEngine::refreshAll(int n, std::vector<double> &out)
 {
     out.resize(n);
+    for (int i = 0; i < n; ++i)
+        if (slots.at(i).unsent || slots.at(i).pending)
+            throw std::runtime_error("refreshAll: slot " + std::to_string(i) + " holds an unrefreshed proposal");
     check(eng, smc_refresh_likelihood(eng, 0, n, out.data(), nullptr), "smc_refresh_likelihood");
-    for (int i = 0; i < n; ++i) lnls[i] = out[i];
+    for (int i = 0; i < n; ++i) {
+        lnls[i] = out[i];
+        Slot &sl = slots[i];
+        sl.committed = sl.cur;  // a forced refresh commits
+        sl.lnlCommitted = out[i];
+        sl.haveLnl = true;
+    }
 }
 
 void ParticleEngine::pull(int slot)
@@ -279,6 +332,9 @@ void ParticleEngine::pull(int slot)
     sl.sp.pInvar = pinv;
     sl.sp.shape = shape;
     sl.sp.newick = writeNewick(t);
+    params_from_engine(eng, slot, T, sl);
+    sl.haveLnl = true;  // the batched calls leave every moved slot refreshed and committed
+    sl.lnlCommitted = lnls.at(slot);
 }
 
 void ParticleEngine::rejuvenate(int first, int count, double tempering, const smc_rejuvenation_options &opt,
@@ -310,6 +366,7 @@ smc_sampler_result ParticleEngine::runSampler(const smc_sampler_options &opt, st
 void Particle::copyFromSerialized(SerializedParticle &sp)
 {
     e.upload(slot, sp);
+    e.slots.at(slot).rng = Rng(e.nextSeed(), (uint64_t)slot);
     refreshLikelihood(true);  // Particle.cpp:276
 }
 
@@ -333,30 +390,222 @@ void Particle::writeToSerialized(SerializedParticle &sp)
     for (int a : s.sp.assignments) sp.categorySize[a]++;
 }
 
-void Particle::refreshLikelihood(bool)
+void Particle::sendParams()
 {
-    // The engine keeps no stale state between uploads: every refresh is the
-    // forced form (incremental dirty-node updates are not exposed here yet).
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    if (!s.unsent) return;
+    check(e.handle(),
+          smc_update_params(e.handle(), slot, 1, s.cur.ch.data(), s.cur.bl.data(), s.cur.base.data(), &s.cur.pinv,
+                            &s.cur.shape),
+          "smc_update_params");
+    s.unsent = false;
+}
+
+void Particle::refreshLikelihood(bool forceUpdate)
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    if (!s.valid) throw std::runtime_error("refreshLikelihood: slot not uploaded");
+    sendParams();
     double l = 0.0;
     uint8_t nf = 0;
-    check(e.handle(), smc_refresh_likelihood(e.handle(), slot, 1, &l, &nf), "smc_refresh_likelihood");
-    currentLnLikelihood = l;
+    if (forceUpdate || !s.haveLnl) {
+        // every node, into the other workspace, and committed (Particle.cpp:566-571)
+        check(e.handle(), smc_refresh_likelihood(e.handle(), slot, 1, &l, &nf), "smc_refresh_likelihood");
+        s.committed = s.cur;
+        s.lnlCommitted = l;
+        s.haveLnl = true;
+    } else if (s.pending) {
+        // the dirty nodes of the proposals since the last refresh (Particle.cpp:577-612)
+        check(e.handle(),
+              smc_refresh_incremental(e.handle(), slot, 1, s.flags.data(), &s.eig, &l, &nf),
+              "smc_refresh_incremental");
+    } else {
+        l = e.lnls.at(slot);  // nothing changed
+    }
+    std::fill(s.flags.begin(), s.flags.end(), 0);
+    s.eig = 0;
+    s.pending = false;
     e.lnls[slot] = l;
+}
+
+void Particle::accept()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    if (s.pending || s.unsent) refreshLikelihood(false);  // an accepted proposal is evaluated first
+    const uint8_t yes = 1;
+    check(e.handle(), smc_accept_reject(e.handle(), slot, 1, &yes), "smc_accept_reject");
+    s.committed = s.cur;
+    s.lnlCommitted = e.lnls.at(slot);
+}
+
+void Particle::reject()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    const uint8_t no = 0;
+    // the engine returns to its committed flags, parameters, eigensystems and P
+    check(e.handle(), smc_accept_reject(e.handle(), slot, 1, &no), "smc_accept_reject");
+    s.cur = s.committed;
+    std::fill(s.flags.begin(), s.flags.end(), 0);
+    s.eig = 0;
+    s.unsent = s.pending = false;
+    e.lnls[slot] = s.lnlCommitted;
+    sync_state(s, e.alignment().getNumTaxa());
+}
+
+smc_rejuvenation_options Particle::moveOptions() const
+{
+    smc_rejuvenation_options o{};
+    o.scale_delta = scaleDelta;
+    o.subtree_scale_delta = subtreeScaleDelta;
+    o.shape_delta = shapeDelta;
+    o.rate_delta = rateMatrixDelta;
+    o.rate_full_scale = 0.5;  // Particle.hpp:92: Identity * 0.5
+    o.invar_alpha = invarAlpha;
+    o.rate_cholesky = rateMatrixCholesky.size() == 190 * 190 ? rateMatrixCholesky.data() : nullptr;
+    return o;
+}
+
+void Particle::proposed(ParticleEngine::Slot &s)
+{
+    s.unsent = true;
+    s.pending = true;
+    sync_state(s, e.alignment().getNumTaxa());
+}
+
+double Particle::treeMove()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    const smc_rejuvenation_options o = moveOptions();
+    const double h = tree_move(s.rng, s.cur, e.alignment().getNumTaxa(), s.flags, o);
+    proposed(s);
+    return h;
+}
+
+double Particle::shapeMove()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    if (e.numRates() < 2) return 0.0;  // no rate heterogeneity to move
+    const double h = shape_move(s.rng, s.cur, s.flags, shapeDelta);
+    proposed(s);
+    return h;
+}
+
+double Particle::invarMove()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    const double h = invar_move(s.rng, s.cur, invarAlpha);
+    proposed(s);
+    return h;
+}
+
+double Particle::rateMatrixMove()
+{
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    const smc_rejuvenation_options o = moveOptions();
+    const double h = rate_matrix_move(s.rng, s.cur, o);
+    s.eig = 1;  // every class is re-decomposed and every P recomputed (Tree::updateAll)
+    proposed(s);
+    return h;
 }
 
 double Particle::lnPrior()
 {
+    sendParams();  // the prior of the current, possibly proposed, state
     double lp = 0.0;
     check(e.handle(), smc_ln_prior(e.handle(), slot, e.usingLG() ? 1 : 0, dppAlpha, &lp), "smc_ln_prior");
     return lp;
 }
 
+Particle::Particle(int seed, ParticleEngine &engine, int sl_index, bool initInvar, bool cat)
+    : e(engine), slot(sl_index)
+{
+    const Alignment &aln = e.alignment();
+    const int T = aln.getNumTaxa(), S = aln.getNumChar(), R = e.numRates();
+    Rng g((uint64_t)(uint32_t)seed, (uint64_t)slot);
+    // random tree (Tree.cpp:10-77): a root over tips 0 and 1, then tip i splits the
+    // pendant edge of a uniformly chosen earlier tip; internal node T + i - 1 takes
+    // children (i, r) in r's old child slot; every length ~ Exp(rate 10), the root's 0
+    TreeTopology t;
+    t.names = aln.getTaxaNames();
+    t.children.assign((size_t)(T - 1) * 2, -1);
+    t.branchLen.assign(2 * T - 1, 0.0);
+    std::vector<int> parent(2 * T - 1, -1), pslot(2 * T - 1, 0);
+    t.children[0] = 0;
+    t.children[1] = 1;
+    parent[0] = parent[1] = T;
+    pslot[1] = 1;
+    for (int i = 2; i < T; ++i) {
+        const int r = std::min(i - 1, (int)(g.uniform() * i));
+        const int nw = T + i - 1, a = parent[r], sl = pslot[r];
+        t.children[(size_t)(a - T) * 2 + sl] = nw;
+        parent[nw] = a;
+        pslot[nw] = sl;
+        t.children[(size_t)(nw - T) * 2] = i;
+        t.children[(size_t)(nw - T) * 2 + 1] = r;
+        parent[i] = nw;
+        pslot[i] = 0;
+        parent[r] = nw;
+        pslot[r] = 1;
+    }
+    for (int n = 0; n < 2 * T - 1; ++n) t.branchLen[n] = -std::log1p(-g.uniform()) / 10.0;
+    t.root = T;
+    t.branchLen[T] = 0.0;
+    SerializedParticle sp;
+    sp.pInvar = initInvar ? g.uniform() : 0.0;                 // Particle.cpp:117-120
+    sp.shape = R > 1 ? 2.0 * g.gamma(3.0) : 1.0;              // Particle.cpp:122-127
+    if (e.usingLG()) {                                          // Particle.cpp:84-107
+        smc_lg_base_matrix(sp.baseMatrix.data());
+    } else {
+        for (double &b : sp.baseMatrix) b = g.normal();
+    }
+    auto profile = [&]() {                                      // TransitionProbabilityClass.cpp:26-35
+        std::array<double, 20> lg{};
+        for (int j = 0; j < 19; ++j) lg[j] = g.normal();
+        return lg;
+    };
+    sp.assignments.assign(S, 0);
+    if (cat) {  // Chinese restaurant process (Particle.cpp:129-158)
+        std::vector<int> size;
+        for (int i = 0; i < S; ++i) {
+            const double u = g.uniform();
+            double tot = dppAlpha / (i + dppAlpha);
+            if (tot > u) {
+                if ((int)size.size() >= e.maxClasses())
+                    throw std::runtime_error("Particle: initial classes exceed the engine's max_classes");
+                sp.stationaries.push_back(profile());
+                sp.assignments[i] = (int)size.size();
+                size.push_back(1);
+                continue;
+            }
+            int k = 0;
+            for (; k < (int)size.size(); ++k) {
+                tot += size[k] / (i + dppAlpha);
+                if (tot > u) break;
+            }
+            k = std::min(k, (int)size.size() - 1);
+            sp.assignments[i] = k;
+            size[k]++;
+        }
+    } else {
+        sp.stationaries.push_back(profile());
+    }
+    e.uploadState(slot, std::move(t), sp);
+    e.slots.at(slot).rng = g;
+    sync_state(e.slots.at(slot), T);
+}
+
 double Particle::gibbsPartitionMove(double tempering)
 {
     const int32_t sl = slot;
+    ParticleEngine::Slot &s = e.slots.at(slot);
+    if (s.unsent || s.pending)
+        throw std::runtime_error("gibbsPartitionMove: accept or reject the pending proposal first");
     check(e.handle(), smc_gibbs_partition_move(e.handle(), 1, &sl, tempering, e.nextSeed(), 0, dppAlpha, nullptr),
           "smc_gibbs_partition_move");
+    const double l = e.lnls.at(slot);
     e.pull(slot);
+    e.lnls[slot] = l;
+    s.haveLnl = false;  // new classes: the next refresh recomputes every node
     return INFINITY;  // Particle.cpp:994
 }
 

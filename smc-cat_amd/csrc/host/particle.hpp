@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "moves.hpp"
 #include "smccat_hip.h"
 
 namespace smccat {
@@ -82,6 +83,8 @@ public:
 
     // Uploads slot `slot` from a serialized record (Particle::copyFromSerialized).
     void upload(int slot, const SerializedParticle &sp);
+    // Uploads a parsed state: tree (ids as Tree.cpp), classes/assignments/pInvar/shape/base of sp.
+    void uploadState(int slot, TreeTopology t, const SerializedParticle &sp);
     // Full likelihood of slots [0, n) in one batched sweep; lnl resized to n.
     void refreshAll(int n, std::vector<double> &lnl);
     // mcmc(p, r, T) + optional gibbsPartitionMove for slots [first, first+count)
@@ -102,11 +105,21 @@ public:
     int maxClasses() const { return kmax; }
     bool usingLG() const { return lg; }
 
-    // slot state kept on the host for writeToSerialized / lnPrior
+    // slot state kept on the host for writeToSerialized / lnPrior and the MCMC moves
     struct Slot {
         SerializedParticle sp;
         TreeTopology tree;
         bool valid = false;
+        // Particle::accept / reject (Particle.cpp:343-478): the current (possibly
+        // proposed) parameters and the last committed ones
+        MoveParams cur, committed;
+        std::vector<uint8_t> flags;  // [2T-1] updateCL | updateTP << 1 since the last refresh
+        uint8_t eig = 0;             // the exchangeabilities changed since the last refresh
+        bool unsent = false;         // cur is not yet the engine's
+        bool pending = false;        // a proposal awaits refreshLikelihood
+        bool haveLnl = false;        // the engine holds this slot's likelihood
+        double lnlCommitted = 0.0;
+        Rng rng{1, 0};               // Particle::getRng (Particle.hpp:64)
     };
     std::vector<Slot> slots;
     std::vector<double> lnls;
@@ -121,12 +134,40 @@ private:
 
 class Particle {
 public:
+    // A view of slot `slot` (its state comes from copyFromSerialized or an earlier view).
     Particle(ParticleEngine &engine, int slot) : e(engine), slot(slot) {}
+    // Particle::Particle(seed, aln, nR, initInvar, lg, cat) (Particle.cpp:21-158) on an engine
+    // slot: the alignment, K_r and lg are the engine's; draws a random tree (Tree.cpp:10-77,
+    // Exp(10) lengths), pInvar ~ U(0,1) if initInvar, shape ~ Gamma(3, 2) if K_r > 1, the LG
+    // table or 190 N(0,1) log-exchangeabilities, and with cat a CRP(0.01) partition (one class
+    // otherwise) of N(0,1)-logit profiles; random numbers from a Philox stream keyed by
+    // (seed, slot), where the reference uses Boost's mt19937(seed).
+    Particle(int seed, ParticleEngine &engine, int slot, bool initInvar = false, bool cat = true);
     void copyFromSerialized(SerializedParticle &sp);   // Particle.cpp:218-300
     void writeToSerialized(SerializedParticle &sp);    // Particle.cpp:302-320
-    void refreshLikelihood(bool forceUpdate = false);  // Particle.cpp:530-675
-    double lnLikelihood() { return currentLnLikelihood; }  // Particle.cpp:480-482
+    // Particle.cpp:530-675: forceUpdate recomputes every node and commits; otherwise only
+    // the nodes the proposals since the last refresh made dirty (their other workspace),
+    // to be kept by accept() or dropped by reject()
+    void refreshLikelihood(bool forceUpdate = false);
+    double lnLikelihood() { return e.lnls.at(slot); }  // Particle.cpp:480-482
     double lnPrior();                                  // Particle.cpp:484-517
+    void accept();                                     // Particle.cpp:343-403
+    void reject();                                     // Particle.cpp:405-478
+    // Moves (Particle.cpp:677-783): edit the current state, mark the dirty nodes and
+    // return the log Hastings ratio.  treeMove's split posteriors (adaptive NNI) are
+    // not used: it proposes a plain NNI in their place.
+    double treeMove();
+    template <typename SplitPosterior>
+    double treeMove(const SplitPosterior &) { return treeMove(); }
+    double shapeMove();
+    double invarMove();
+    double rateMatrixMove();
+    double stationaryMove() { return 0.0; }  // a no-op in the reference (Q14)
+    Rng &getRng() { return e.slots.at(slot).rng; }
+    // tuning fields (Particle.hpp:85-94)
+    double shapeDelta = 1.0, scaleDelta = 1.0, subtreeScaleDelta = 1.0, rateMatrixDelta = 0.25;
+    double invarAlpha = 100.0;
+    std::vector<double> rateMatrixCholesky;  // [190 * 190] row-major lower; empty: 0.5 * identity
     // Particle::gibbsPartitionMove (Particle.cpp:785-995) on this slot; returns
     // +infinity as the reference does (the move is always accepted).
     double gibbsPartitionMove(double tempering);
@@ -143,8 +184,10 @@ public:
 private:
     ParticleEngine &e;
     int slot;
-    double currentLnLikelihood = 0.0;
     double dppAlpha = 0.01;  // Particle.hpp:106
+    smc_rejuvenation_options moveOptions() const;
+    void sendParams();       // uploads the current parameters if the engine lacks them
+    void proposed(ParticleEngine::Slot &s);
 };
 
 // Miscellaneous.hpp:42-44, same argument meaning; runs on the GPU of `engine`.

@@ -27,17 +27,13 @@
 
 #include "engine_state.hpp"
 #include "host_rng.hpp"
+#include "moves.hpp"
 
 namespace {
 
 double log_gamma_pdf(double k, double theta, double x)
 {
     return (k - 1.0) * std::log(x) - x / theta - std::lgamma(k) - k * std::log(theta);
-}
-double log_beta_pdf(double a, double b, double x)
-{
-    return (a - 1.0) * std::log(x) + (b - 1.0) * std::log1p(-x) + std::lgamma(a + b) - std::lgamma(a) -
-           std::lgamma(b);
 }
 double log_std_normal(double z) { return -0.5 * z * z - 0.9189385332046727; }
 
@@ -79,11 +75,7 @@ MoveTable move_table(const smc_engine *e, const smc_rejuvenation_options &o)
     return m;
 }
 
-struct Params {
-    std::vector<int32_t> ch;
-    std::vector<double> bl, base;
-    double pinv, shape;
-};
+using Params = MoveParams;
 
 // Particle::lnPrior (Particle.cpp:484-517), split in two: the terms of the
 // tree, shape and base matrix, which the moves change, and the CRP and class
@@ -114,112 +106,6 @@ double ln_prior_classes(const smc_engine *e, int p, double dpp_alpha)
     }
     lp += std::lgamma(dpp_alpha) - std::lgamma(dpp_alpha + S);
     return lp;
-}
-
-void parents(const std::vector<int32_t> &ch, int T, std::vector<int> &par)
-{
-    par.assign(2 * T - 1, -1);
-    for (int k = 0; k < T - 1; ++k) {
-        par[ch[2 * k]] = T + k;
-        par[ch[2 * k + 1]] = T + k;
-    }
-}
-
-void mark_path(std::vector<uint8_t> &f, const std::vector<int> &par, int n)
-{
-    for (; n >= 0; n = par[n]) f[n] |= 1;
-}
-
-// One tree move (Particle::treeMove mixture, Particle.cpp:680-701).  Returns the
-// log Hastings ratio; fills updateCL / updateTP flags.
-double tree_move(Rng &g, Params &q, int T, std::vector<uint8_t> &f, const smc_rejuvenation_options &o)
-{
-    const int nn = 2 * T - 1;
-    std::vector<int> par;
-    parents(q.ch, T, par);
-    int root = -1;
-    for (int n = T; n < nn; ++n)
-        if (par[n] < 0) root = n;
-    auto pick = [&](bool internal) {
-        for (;;) {
-            const int n = std::min(nn - 1, (int)(g.uniform() * nn));
-            if (n != root && (!internal || n >= T)) return n;
-        }
-    };
-    const double u = g.uniform();
-    if (u < 0.4) {  // NNI 0.1 and adaptive NNI 0.3 (Tree.cpp:625-666)
-        const int p = pick(true), a = par[p];
-        const int n1 = q.ch[2 * (p - T) + (g.uniform() < 0.5 ? 0 : 1)];
-        const int n2 = q.ch[2 * (a - T)] == p ? q.ch[2 * (a - T) + 1] : q.ch[2 * (a - T)];
-        for (int c = 0; c < 2; ++c) {
-            if (q.ch[2 * (p - T) + c] == n1) q.ch[2 * (p - T) + c] = n2;
-            if (q.ch[2 * (a - T) + c] == n2) q.ch[2 * (a - T) + c] = n1;
-        }
-        parents(q.ch, T, par);
-        mark_path(f, par, p);
-        return 0.0;
-    }
-    if (u < 0.5) {  // SPR (Tree.cpp:753-811)
-        const int p = pick(true);
-        std::vector<char> in_sub(nn, 0);
-        std::vector<int> st{p};
-        while (!st.empty()) {
-            const int x = st.back();
-            st.pop_back();
-            in_sub[x] = 1;
-            if (x >= T) {
-                st.push_back(q.ch[2 * (x - T)]);
-                st.push_back(q.ch[2 * (x - T) + 1]);
-            }
-        }
-        int tgt = -1;
-        for (int tries = 0; tries < 64 && tgt < 0; ++tries) {
-            const int a = std::min(nn - 1, (int)(g.uniform() * nn));
-            if (!in_sub[a] && a != root) tgt = a;
-        }
-        if (tgt < 0) return 0.0;
-        const int s = q.ch[2 * (p - T) + (g.uniform() < 0.5 ? 0 : 1)];
-        const int sib = q.ch[2 * (p - T)] == s ? q.ch[2 * (p - T) + 1] : q.ch[2 * (p - T)];
-        const int old = par[p];
-        if (tgt == sib) return 0.0;  // regrafting onto its own sibling changes nothing
-        for (int c = 0; c < 2; ++c)
-            if (q.ch[2 * (old - T) + c] == p) q.ch[2 * (old - T) + c] = sib;
-        const int tpar = par[tgt] == p ? old : par[tgt];
-        q.ch[2 * (p - T)] = s;
-        q.ch[2 * (p - T) + 1] = tgt;
-        for (int c = 0; c < 2; ++c)
-            if (q.ch[2 * (tpar - T) + c] == tgt) q.ch[2 * (tpar - T) + c] = p;
-        parents(q.ch, T, par);
-        mark_path(f, par, p);
-        mark_path(f, par, old);
-        return 0.0;
-    }
-    if (u < 0.9) {  // branch-length scale (Tree.cpp:553-577)
-        const int n = pick(false);
-        const double sc = std::exp(o.scale_delta * (g.uniform() - 0.5));
-        q.bl[n] *= sc;
-        f[n] |= 2;
-        mark_path(f, par, par[n]);
-        return std::log(sc);
-    }
-    // subtree scale (Tree.cpp:579-612)
-    const int n = pick(false);
-    const double sc = std::exp(o.subtree_scale_delta * (g.uniform() - 0.5));
-    int nb = 0;
-    std::vector<int> st{n};
-    while (!st.empty()) {
-        const int x = st.back();
-        st.pop_back();
-        q.bl[x] *= sc;
-        f[x] |= 3;
-        nb++;
-        if (x >= T) {
-            st.push_back(q.ch[2 * (x - T)]);
-            st.push_back(q.ch[2 * (x - T) + 1]);
-        }
-    }
-    mark_path(f, par, par[n]);
-    return nb * std::log(sc);
 }
 
 }  // namespace
@@ -356,38 +242,16 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                         case TREE:
                             hastings[i] = tree_move(g, q, T, f, o);
                             break;
-                        case SHAPE: {  // Particle.cpp:768-783
-                            const double sc = std::exp(o.shape_delta * (g.uniform() - 0.5));
-                            q.shape *= sc;
-                            std::fill(f.begin(), f.end(), 3);
-                            hastings[i] = std::log(sc);
-                        } break;
-                        case INVAR: {  // Particle.cpp:744-766
-                            const double a = o.invar_alpha + 1.0, b = o.invar_alpha / q.pinv - a + 2.0;
-                            const double np = g.beta(a, b);
-                            const double a2 = o.invar_alpha + 1.0, b2 = o.invar_alpha / np - a2 + 2.0;
-                            hastings[i] = log_beta_pdf(a2, b2, q.pinv) - log_beta_pdf(a, b, np);
-                            q.pinv = np;
-                        } break;
-                        case RATES: {  // Particle.cpp:706-732, eigensystems recomputed (Q13 fixed)
-                            if (g.uniform() < 0.25) {
-                                double z[190];
-                                for (double &v : z) v = g.normal();
-                                if (o.rate_cholesky) {  // base += L z
-                                    for (int a = 0; a < 190; ++a) {
-                                        double acc = 0.0;
-                                        for (int b = 0; b <= a; ++b) acc += o.rate_cholesky[a * 190 + b] * z[b];
-                                        q.base[a] += acc;
-                                    }
-                                } else {
-                                    for (int a = 0; a < 190; ++a) q.base[a] += o.rate_full_scale * z[a];
-                                }
-                            } else {
-                                const int idx = std::min(189, (int)(190 * g.uniform()));
-                                q.base[idx] += g.normal() * o.rate_delta;
-                            }
+                        case SHAPE:
+                            hastings[i] = shape_move(g, q, f, o.shape_delta);
+                            break;
+                        case INVAR:
+                            hastings[i] = invar_move(g, q, o.invar_alpha);
+                            break;
+                        case RATES:  // eigensystems recomputed (Q13 fixed)
+                            hastings[i] = rate_matrix_move(g, q, o);
                             eig[i] = 1;
-                        } break;
+                            break;
                     }
                 }
                 std::copy(q.ch.begin(), q.ch.end(), ch.begin() + (size_t)i * Ti * 2);

@@ -59,7 +59,7 @@ def build_facade_test(verbose=False):
     """C++ facade (reference Particle API) + its self-test executable."""
     os.makedirs(BUILD, exist_ok=True)
     exe = FACADE_EXE
-    cmd = ["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-I", os.path.join(CSRC, "host"),
+    cmd = ["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-I", CSRC, "-I", os.path.join(CSRC, "host"),
            os.path.join(CSRC, "host", "particle.cpp"), os.path.join(CSRC, "host", "facade_selftest.cpp"),
            "-L", PKG, "-lsmccat_hip", "-Wl,-rpath,$ORIGIN", "-o", exe]
     _run(cmd)

@@ -197,3 +197,38 @@ def test_facade_moves_roundtrip_oracle(gpu, tmp_path):
     np.testing.assert_allclose(rj[:, 2], want_prior, rtol=1e-10)
     # and the engine's own lnL before the text round trip is within its rounding
     np.testing.assert_allclose(rj[:, 0], want, rtol=1e-4)
+
+
+@pytest.mark.gpu
+def test_facade_mcmc_loop_incremental_matches_oracle(gpu, tmp_path):
+    """The loop of Mcmc::operator() (Mcmc.cpp:25-73), in its own shape, on facade
+    Particles: moves, refreshLikelihood() (incremental), lnPrior, accept() and
+    reject().  The incremental lnL of every final state equals a forced refresh
+    and the oracle's full recompute (1e-10), on copied records and on one particle
+    built by the seeded constructor (Particle.cpp:21-158)."""
+    print("case facade mcmc: 9 taxa x 57 sites, K_r=4, 5 records + 1 constructed, 6 iterations", flush=True)
+    codes, batch, names = _case(4)
+    fa, pf, _ = write_inputs(tmp_path, codes, batch, names)
+    out = run_facade([fa, pf, "4", "3", "0", "--mcmc", "6"])
+    T, S = codes.shape
+    rows = {k: {int(r[0]): r[1:] for r in out[k]} for k in
+            ("mcmc_lnl", "mcmc_ch", "mcmc_len", "mcmc_par", "mcmc_base", "mcmc_cls", "mcmc_asg")}
+    assert len(rows["mcmc_lnl"]) == batch.n + 1
+    acc = rej = 0
+    for i in sorted(rows["mcmc_lnl"]):
+        inc, full, prior = (float(v) for v in rows["mcmc_lnl"][i][:3])
+        acc += int(rows["mcmc_lnl"][i][3])
+        rej += int(rows["mcmc_lnl"][i][4])
+        cls = [float(v) for v in rows["mcmc_cls"][i]]
+        K = int(cls[0])
+        q = dict(children=np.array([int(v) for v in rows["mcmc_ch"][i]], dtype=np.int32).reshape(T - 1, 2),
+                 branch_len=np.array([float(v) for v in rows["mcmc_len"][i]]),
+                 base=np.array([float(v) for v in rows["mcmc_base"][i]]), num_classes=K,
+                 logits=np.array(cls[1:]).reshape(K, 20),
+                 assignment=np.array([int(v) for v in rows["mcmc_asg"][i]], dtype=np.int32),
+                 pinv=float(rows["mcmc_par"][i][0]), shape=float(rows["mcmc_par"][i][1]))
+        want = orc.particle_lnl(codes, q, 4)
+        assert inc == pytest.approx(full, rel=1e-12), i
+        assert inc == pytest.approx(want, rel=1e-10), i
+        assert prior == pytest.approx(orc.ln_prior(codes, q, 4, using_lg=False), rel=1e-10), i
+    assert acc > 0 and rej > 0, (acc, rej)
