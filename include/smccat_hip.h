@@ -127,6 +127,16 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                             const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out);
 int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept);
 
+/* The resampled copies of main.cpp:300,309,313 (each slot deserialises its parent's
+ * record, Particle::copyFromSerialized, Particle.cpp:218-276) on the device: slot i of
+ * [0, n) takes the state of slot parents[i] -- tree, branch lengths, exchangeabilities,
+ * classes, allocation, pInvar, shape, eigensystems and committed lnL -- by gathers of
+ * device rows, without an upload (0 <= parents[i] < n, any order, repeats allowed).
+ * Copies are exact (the reference rounds branch lengths to 6 decimals through Newick
+ * text, Q11).  A slot that keeps its own state (parents[i] == i) keeps its resident
+ * partial likelihoods, so rejuvenation recomputes only the copied slots. */
+int smc_copy_particles(smc_engine *e, int n, const int32_t *parents);
+
 typedef struct smc_mlp smc_mlp;  /* learned CAT-allocation proposal (below) */
 
 /* Rejuvenation settings (main.cpp:54-105, Particle.hpp:87-94, Settings.hpp:19-29). */

@@ -497,6 +497,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
         if (rc == SMC_OK) rc = dev_alloc(e, ptr, n);
     };
     A(&e->d_codes, T * Sp);
+    A(&e->d_pcodes, P * T * Sp);
     A(&e->d_isinv, Sp);
     A(&e->d_invchar, Sp);
     A(&e->d_children, P * (T - 1) * 2);
@@ -590,6 +591,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_touched.assign(P, 0);
     e->h_pnodes.assign(P, {});
     e->committed.resize(P);
+    e->h_owner.assign(e->Wmax, -1);
     *out = e;
     return SMC_OK;
 }
@@ -597,7 +599,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
 int smc_engine_destroy(smc_engine *e)
 {
     if (!e) return SMC_ERR_ARG;
-    void *bufs[] = {e->d_codes,  e->d_isinv, e->d_invchar, e->d_children, e->d_blen,
+    void *bufs[] = {e->d_codes,  e->d_pcodes, e->d_isinv, e->d_invchar, e->d_children, e->d_blen,
                     e->d_base,   e->d_logits, e->d_nclass, e->d_pinv,     e->d_rates,
                     e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
@@ -611,6 +613,7 @@ int smc_engine_destroy(smc_engine *e)
     for (smc_engine::CachedGraph &c : e->graphs) (void)hipGraphExecDestroy(c.exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
+    if (e->d_gather) (void)hipFree(e->d_gather);
     for (void *b : {(void *)e->pool.d_eig, (void *)e->pool.d_logits, (void *)e->pool.d_ent})
         if (b) (void)hipFree(b);
     for (void *b : bufs)
@@ -668,6 +671,18 @@ int smc_set_alignment(smc_engine *e, const int8_t *codes)
                               e->stream));
     HIP_TRY(e, hipMemcpyAsync(e->d_invchar, ich.data(), Sp * sizeof(int), hipMemcpyHostToDevice,
                               e->stream));
+    // tip codes in the slot order of the particles already uploaded (runs of valid slots)
+    for (int p = 0; p < e->Pmax;) {
+        if (!e->h_valid[p]) {
+            ++p;
+            continue;
+        }
+        int q = p;
+        while (q < e->Pmax && e->h_valid[q]) ++q;
+        launch_permute_codes(e->d_codes, e->d_perm, e->d_pcodes, T, Sp, p, q - p, e->stream);
+        p = q;
+    }
+    HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     e->have_alignment = true;
     return SMC_OK;
@@ -687,6 +702,8 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
         return fail(e, SMC_ERR_ARG, "null particle array");
     const int T = e->T, Ti = T - 1, nn = e->nn, S = e->S, Sp = e->S_pad, R = e->R, K = e->Kmax;
     const int ts = e->tile_stride;
+    for (int &o : e->h_owner)  // new states: resident partials of these slots are stale
+        if (o >= first && o < first + count) o = -1;
     std::vector<double> blen((size_t)count * nn), rates((size_t)count * R);
     std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
     std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
@@ -788,6 +805,8 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
+    if (e->have_alignment) launch_permute_codes(e->d_codes, e->d_perm, e->d_pcodes, T, Sp, first, count, s);
+    HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipStreamSynchronize(s));
     e->class_version++;
     e->topo_version++;
@@ -806,6 +825,7 @@ PruneArgs prune_args(smc_engine *e, int p_base)
     a.root_of = e->d_root;
     a.perm = e->d_perm;
     a.codes = e->d_codes;
+    a.pcodes = e->d_pcodes;
     a.P = e->d_P;
     a.pcls_off = e->d_pcls_off;
     a.eig = e->d_eig;
@@ -997,6 +1017,8 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
     if (rc == SMC_OK) rc = sync_stream(e);
     if (rc != SMC_OK) return rc;
     collect_timings(e);
+    for (const smc_engine::Wave &w : e->waves)  // the last wave of each slot stays resident
+        for (int k = 0; k < w.count; ++k) e->h_owner[k] = w.first + k;
     if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
     smc_commit_range(e, first, count);
     return SMC_OK;
@@ -1311,6 +1333,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         bool changed = eig || e->h_touched[p];  // pInvar enters at the root only
         e->h_touched[p] = 0;
         for (int n = 0; n < nn && !changed; ++n) changed = nf[n] != 0;
+        if (changed && e->h_owner[i] != p) e->h_owner[i] = -1;  // wave slot i held another slot's partials
         if (changed) {
             roots.push_back(Item{p, root});
         } else {  // nothing to recompute: its lnL stays (its tile sums may be stale)
@@ -1480,6 +1503,160 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
     }
     return SMC_OK;
 }
+
+extern "C++" {
+namespace {
+
+// Gathers rows of a host mirror: row dst[k] <- old row src[k] (all sources read first).
+template <typename V>
+void gather_host(V &v, size_t row, const std::vector<int> &dst, const std::vector<int> &src)
+{
+    V tmp(src.size() * row);
+    for (size_t k = 0; k < src.size(); ++k)
+        std::copy_n(v.begin() + (size_t)src[k] * row, row, tmp.begin() + k * row);
+    for (size_t k = 0; k < dst.size(); ++k) std::copy_n(tmp.begin() + k * row, row, v.begin() + (size_t)dst[k] * row);
+}
+
+}  // namespace
+}
+
+int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (n < 1 || n > e->Pmax || !parents) return fail(e, SMC_ERR_ARG, "invalid particle copy arguments");
+    for (int i = 0; i < n; ++i) {
+        if (parents[i] < 0 || parents[i] >= n)
+            return fail(e, SMC_ERR_ARG, "copy source " + std::to_string(parents[i]) + " out of [0, n)");
+        if (!e->h_valid[parents[i]]) return fail(e, SMC_ERR_STATE, "copy source slot not uploaded");
+    }
+    std::vector<int> dst, src;
+    for (int i = 0; i < n; ++i)
+        if (parents[i] != i) {
+            dst.push_back(i);
+            src.push_back(parents[i]);
+        }
+    if (dst.empty()) return SMC_OK;
+    const int T = e->T, Ti = T - 1, nn = e->nn, S = e->S, Sp = e->S_pad, R = e->R, K = e->Kmax;
+    const int m = (int)dst.size();
+    // host mirrors
+    gather_host(e->h_children, (size_t)Ti * 2, dst, src);
+    gather_host(e->h_root, 1, dst, src);
+    gather_host(e->h_height, (size_t)Ti, dst, src);
+    gather_host(e->h_prog, (size_t)site_prog_stride(T), dst, src);
+    gather_host(e->h_valid, 1, dst, src);
+    gather_host(e->h_nclass, 1, dst, src);
+    gather_host(e->h_blen, (size_t)nn, dst, src);
+    gather_host(e->h_base, 190, dst, src);
+    gather_host(e->h_logits, (size_t)K * 20, dst, src);
+    gather_host(e->h_assign, (size_t)S, dst, src);
+    gather_host(e->h_pinv, 1, dst, src);
+    gather_host(e->h_shape, 1, dst, src);
+    gather_host(e->h_rates, (size_t)R, dst, src);
+    gather_host(e->h_xflag_old, (size_t)Ti, dst, src);
+    gather_host(e->h_lnl_old, 1, dst, src);
+    {
+        std::vector<smc_engine::Committed> tmp(m);
+        for (int k = 0; k < m; ++k) tmp[k] = e->committed[src[k]];
+        for (int k = 0; k < m; ++k) e->committed[dst[k]] = tmp[k];
+    }
+    for (int d : dst) {  // a copy is a committed state (copyFromSerialized ends in a refresh)
+        std::copy_n(e->h_xflag_old.begin() + (size_t)d * Ti, Ti, e->h_xflag.begin() + (size_t)d * Ti);
+        e->h_lnl[d] = e->h_lnl_old[d];
+        e->h_dirty[d] = 0;
+        e->h_pnodes[d].clear();
+        e->h_touched[d] = 0;
+    }
+    for (int &o : e->h_owner)  // copied slots' resident partials are not their parents'
+        if (o >= 0 && o < n && parents[o] != o) o = -1;
+    // device rows: gather the sources into staging, then scatter to the destinations
+    struct Buf {
+        void *ptr;
+        size_t row;
+    };
+    const Buf bufs[] = {{e->d_children, sizeof(int) * Ti * 2}, {e->d_blen, sizeof(double) * nn},
+                        {e->d_base, sizeof(double) * 190},    {e->d_logits, sizeof(double) * K * 20},
+                        {e->d_nclass, sizeof(int)},           {e->d_pinv, sizeof(double)},
+                        {e->d_rates, sizeof(double) * R},     {e->d_root, sizeof(int)},
+                        {e->d_perm, sizeof(int) * Sp},        {e->d_tiles, sizeof(Tile) * e->tile_stride},
+                        {e->d_ntiles, sizeof(int)},           {e->d_prog, sizeof(int) * site_prog_stride(T)},
+                        {e->d_eig, sizeof(double) * K * kEigStride}, {e->d_pcodes, (size_t)T * Sp}};
+    size_t max_row = 0;
+    for (const Buf &b : bufs) max_row = std::max(max_row, b.row);
+    const int64_t need = (int64_t)m * (int64_t)max_row;
+    if (need > e->gather_cap) {
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        if (e->d_gather) (void)hipFree(e->d_gather);
+        e->d_gather = nullptr;
+        e->gather_cap = 0;
+        const int64_t cap = std::max<int64_t>(need, (int64_t)1 << 20);
+        HIP_TRY(e, hipMalloc((void **)&e->d_gather, (size_t)cap));
+        e->gather_cap = cap;
+    }
+    // the row lists go through d_plist ([2 * Pmax]): sources, then destinations
+    int *d_src = e->d_plist, *d_dst = e->d_plist + e->Pmax;
+    int rc = h2d(e, d_src, src.data(), src.size() * sizeof(int));
+    if (rc == SMC_OK) rc = h2d(e, d_dst, dst.data(), dst.size() * sizeof(int));
+    if (rc != SMC_OK) return rc;
+    for (const Buf &b : bufs) {
+        launch_gather_rows(b.ptr, e->d_gather, b.row, d_src, nullptr, m, e->stream);
+        launch_gather_rows(e->d_gather, b.ptr, b.row, nullptr, d_dst, m, e->stream);
+    }
+    HIP_TRY(e, hipGetLastError());
+    // the committed workspace flags and lnL of the range
+    rc = h2d(e, e->d_xflag, e->h_xflag.data(), (size_t)n * Ti);
+    if (rc == SMC_OK) rc = h2d(e, e->d_lnl, e->h_lnl.data(), (size_t)n * sizeof(double));
+    if (rc == SMC_OK) rc = sync_stream(e);
+    if (rc != SMC_OK) return rc;
+    e->class_version++;
+    e->topo_version++;
+    e->sched_first = -1;
+    return SMC_OK;
+}
+
+extern "C++" {
+int smc_refresh_stale(smc_engine *e, int first, int count)
+{
+    if (e->WS != 2 || count > e->Wmax) return smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+    const int T = e->T, nn = e->nn;
+    std::vector<uint8_t> flags((size_t)count * nn, 0);
+    int stale = 0;
+    for (int i = 0; i < count; ++i)
+        if (e->h_owner[i] != first + i) {
+            stale++;
+            std::fill(flags.begin() + (size_t)i * nn + T, flags.begin() + (size_t)(i + 1) * nn, 1);
+        }
+    if (stale == 0) return SMC_OK;
+    if (stale == count) return smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+    for (int p = first; p < first + count; ++p)
+        if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
+    int rc = set_single_wave(e, first, count);
+    if (rc != SMC_OK) return rc;
+    // eigensystems and P of every slot, in this range's class layout; partials only of the stale
+    std::vector<int> all(count);
+    for (int i = 0; i < count; ++i) all[i] = first + i;
+    rc = recompute_models(e, first, all, all);
+    if (rc != SMC_OK) return rc;
+    rc = smc_refresh_incremental(e, first, count, flags.data(), nullptr, nullptr, nullptr);
+    if (rc != SMC_OK) return rc;
+    smc_commit_range(e, first, count);
+    for (int i = 0; i < count; ++i) e->h_owner[i] = first + i;
+    return SMC_OK;
+}
+
+int smc_refresh_slots(smc_engine *e, int first, int count, const uint8_t *which)
+{
+    if (e->WS != 2 || count > e->Wmax) return fail(e, SMC_ERR_STATE, "slot refresh needs one resident wave");
+    std::vector<uint8_t> flags((size_t)count * e->nn, 0);
+    int rc = smc_refresh_incremental(e, first, count, flags.data(), which, nullptr, nullptr);
+    if (rc != SMC_OK) return rc;
+    for (int i = 0; i < count; ++i)
+        if (which[i]) {
+            smc_commit_range(e, first + i, 1);
+            e->h_owner[i] = first + i;
+        }
+    return SMC_OK;
+}
+}  // extern "C++"
 
 static int ensure_smc(smc_engine *e, int n)
 {

@@ -49,6 +49,7 @@ struct smc_engine {
 
     // device buffers
     uint8_t *d_codes = nullptr;
+    uint8_t *d_pcodes = nullptr;  // [P][T][S_pad] tip codes in each particle's class-sorted slot order
     double *d_isinv = nullptr;
     int *d_invchar = nullptr;
     int *d_children = nullptr;
@@ -103,6 +104,11 @@ struct smc_engine {
     std::vector<uint8_t> h_dirty;               // [P] bit0 P changed, bit1 eigen changed
     std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
     std::vector<uint8_t> h_touched;             // [P] pInvar changed by the last parameter upload
+    // [Wmax] the particle whose committed partials fill wave slot w (-1: none): a
+    // particle p refreshed with p_base b is resident while h_owner[p - b] == p
+    std::vector<int> h_owner;
+    uint8_t *d_gather = nullptr;                 // row-gather staging of smc_copy_particles
+    int64_t gather_cap = 0;
     Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
     int *d_chain_nodes = nullptr;               // [P * (T-1)] dirty-path chains' nodes, bottom up
     int *d_chain_len = nullptr;                 // [P * (T-1)] chain lengths, aligned with d_items
@@ -294,6 +300,17 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
 // Commits the current state of slots [first, first+count): workspace flags,
 // lnL and parameters become the ones a rejected proposal returns to.
 void smc_commit_range(smc_engine *e, int first, int count);
+
+// refreshLikelihood(true) of slots [first, first+count) (one resident wave, two
+// workspaces) that recomputes the partials only of slots whose committed partials
+// are not resident (h_owner); the others keep theirs and their lnL.  Eigensystems
+// and P are recomputed for the whole range.  Commits the range.
+int smc_refresh_stale(smc_engine *e, int first, int count);
+
+// refreshLikelihood(true) of the slots of [first, first+count) with which[i] != 0
+// (one resident wave, two workspaces): eigensystems, P and every node of those
+// slots, committed; the other slots keep their lnL and are not touched.
+int smc_refresh_slots(smc_engine *e, int first, int count, const uint8_t *which);
 
 // Validates the children array of particle p ([T-1][2]) and derives its root,
 // the heights of its internal nodes and the Strahler-ordered scoring program.

@@ -107,6 +107,7 @@ struct PruneArgs {
     const int *root_of;      // [P]
     const int *perm;         // [P][S_pad] slot -> alignment column
     const uint8_t *codes;    // [T][S_pad]
+    const uint8_t *pcodes;   // [P][T][S_pad] tip codes in each particle's slot order (20: gap or unknown)
     const void *P;           // [wave classes][nn][R][420]
     const int *pcls_off;     // [P] compact class slot of (particle, class 0) in P
     const double *eig;       // [P*Kmax][840]
@@ -194,6 +195,13 @@ void launch_reduce_lnl(const double *tile_lnl, const int *ntiles, int tile_strid
 void launch_next_step(int n, double *w, const double *l, double *norm, double *out2,
                       int target_ess, double last_temp, hipStream_t s);
 void launch_resample(int n, const double *norm, double u, int *assign, hipStream_t s);
+// pcodes[p][t][slot] = min(codes[t][perm[p][slot]], 20) for particles [first, first + count).
+void launch_permute_codes(const uint8_t *codes, const int *perm, uint8_t *pcodes, int T, int S_pad, int first,
+                          int count, hipStream_t s);
+// dst row dst_rows[i] (i when null) <- src row src_rows[i] (i when null), rows of
+// row_bytes bytes, i < n (buffers aligned to the largest of 16, 4, 1 dividing row_bytes).
+void launch_gather_rows(const void *src, void *dst, size_t row_bytes, const int *src_rows, const int *dst_rows,
+                        int n, hipStream_t s);
 
 // Proposal-network batch (mlp_layout.hpp): B examples, row r uses example
 // idx[r] (idx may be null: row r is example r).  Activations are per row.

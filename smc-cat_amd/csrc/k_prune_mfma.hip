@@ -198,6 +198,7 @@ struct KArgs {
     int tile_stride;
     const int *children, *perm;
     const uint8_t *codes;
+    const uint8_t *pcodes;  // [P][T][S_pad] tip codes in the particle's slot order, clamped to 20
     const TX *Pm;
     const double *eig, *pinv, *isinv;
     const int *invchar;
@@ -253,7 +254,6 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     const int tile_stride = ka.tile_stride;
     const int *__restrict__ children = ka.children;
     const int *__restrict__ perm = ka.perm;
-    const uint8_t *__restrict__ codes = ka.codes;
     const TX *__restrict__ Pm = ka.Pm;
     const double *__restrict__ eig = ka.eig;
     const double *__restrict__ pinv = ka.pinv;
@@ -334,15 +334,13 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     // rates: byte 0 / 1 the tip of child 0 or its cherry's two tips, bytes 2 / 3 the
     // same for child 1 (20 stands for a gap or unknown state).  Position k is the
     // one a step's block u takes when k = sb0 + u.
+    // (read in the particle's slot order: contiguous across the lanes of a block)
+    const uint8_t *__restrict__ pcp = ka.pcodes + (size_t)p * T * S_pad;
     auto tip_codes = [&](int k) -> uint32_t {
         if (md0 == MODE_STORED && md1 == MODE_STORED) return 0;
         const int o = site_of(k / U * U, k % U, c16);
         const int sl = tl.start + ((o >= tl.lo && o < tl.len) ? o : tl.lo);
-        const int orig = perm[(size_t)p * S_pad + sl];
-        auto cc = [&](int tip) {
-            const int code = codes[(size_t)tip * S_pad + orig];
-            return (uint32_t)(code < 20 ? code : 20);
-        };
+        auto cc = [&](int tip) { return (uint32_t)pcp[(size_t)tip * S_pad + sl]; };
         uint32_t wd = 0;
         if (md0 == MODE_TIP) wd |= cc(c0);
         if (md0 == MODE_CHERRY) wd |= cc(ga0) | cc(gb0) << 8;
@@ -659,8 +657,7 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
     using V4 = typename Mfma<TX>::v4;
     const int *__restrict__ ntiles = ka.ntiles;
     const int *__restrict__ children = ka.children;
-    const int *__restrict__ perm = ka.perm;
-    const uint8_t *__restrict__ codes = ka.codes;
+
     const TX *__restrict__ Pm = ka.Pm;
     const int T = ka.T, R = ka.R, S_pad = ka.S_pad;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -687,20 +684,17 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
     using T2 = typename Vec2<TX>::t;
     const int nsb = (tl.len + 15) >> 4;
     const int sb_begin = ((tl.lo >> 4) / 2) * 2;
-    // window offsets and alignment columns of the lane's 4 slots (block k)
-    int off[4], orig[4];
+    // window offsets and slots of the lane's 4 positions (block k)
+    int off[4], slk[4];
     bool valid[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         off[k] = 32 * (k >> 1) + 2 * c16 + (k & 1);
         valid[k] = off[k] >= tl.lo && off[k] < tl.len;
-        const int sl = tl.start + (valid[k] ? off[k] : tl.lo);  // a lane outside the tile reads a slot inside it
-        orig[k] = perm[(size_t)p * S_pad + sl];
+        slk[k] = tl.start + (valid[k] ? off[k] : tl.lo);  // a lane outside the tile reads a slot inside it
     }
-    auto code_at = [&](int tip, int k) {
-        const int code = codes[(size_t)tip * S_pad + orig[k]];
-        return code < 20 ? code : 20;
-    };
+    const uint8_t *__restrict__ pcp = ka.pcodes + (size_t)p * T * S_pad;
+    auto code_at = [&](int tip, int k) { return (int)pcp[(size_t)tip * S_pad + slk[k]]; };
 
     for (int r = wave; r < R; r += 4) {
         TX cy[4][5];  // the previous chain node's partial (B layout) and exponent, per block
@@ -933,7 +927,7 @@ template <typename TX>
 KArgs<TX> kargs(const PruneArgs &a)
 {
     const int bpi = (a.tile_stride + 3) / 4;
-    return KArgs<TX>{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes,
+    return KArgs<TX>{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, a.pcodes,
                      (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
                      a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS,
                      std::max(1, a.xcds)};

@@ -136,7 +136,57 @@ __global__ void k_resample(int n, const double *__restrict__ norm, double u, int
     while (cnt < n) assign[cnt++] = n - 1;
 }
 
+// Row gather: row dst_rows[r] (r when null) of dst is row src_rows[r] (r when null) of
+// src, rows of row_elems elements of W (16-, 4- or 1-byte words).  One workgroup per row.
+template <typename W>
+__global__ __launch_bounds__(256) void k_gather_rows(const W *__restrict__ src, W *__restrict__ dst, size_t row_elems,
+                                                     const int *__restrict__ src_rows, const int *__restrict__ dst_rows)
+{
+    const size_t r = blockIdx.x;
+    const W *s = src + (size_t)(src_rows ? src_rows[r] : (int)r) * row_elems;
+    W *d = dst + (size_t)(dst_rows ? dst_rows[r] : (int)r) * row_elems;
+    for (size_t i = threadIdx.x; i < row_elems; i += blockDim.x) d[i] = s[i];
+}
+
+// Tip codes in each particle's class-sorted slot order: pcodes[p][t][slot] =
+// min(codes[t][perm[p][slot]], 20) for particles [first, first + count).  One
+// workgroup per (particle, tip) row.
+__global__ __launch_bounds__(256) void k_permute_codes(const uint8_t *__restrict__ codes, const int *__restrict__ perm,
+                                                       uint8_t *__restrict__ pcodes, int T, int S_pad, int first)
+{
+    const int p = first + (int)blockIdx.x, t = (int)blockIdx.y;
+    const int *pm = perm + (size_t)p * S_pad;
+    const uint8_t *src = codes + (size_t)t * S_pad;
+    uint8_t *dst = pcodes + ((size_t)p * T + t) * S_pad;
+    for (int sl = threadIdx.x; sl < S_pad; sl += blockDim.x) {
+        const int c = src[pm[sl]];
+        dst[sl] = (uint8_t)(c < 20 ? c : 20);
+    }
+}
+
 }  // namespace
+
+void launch_permute_codes(const uint8_t *codes, const int *perm, uint8_t *pcodes, int T, int S_pad, int first,
+                          int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_permute_codes, dim3(count, T), dim3(256), 0, s, codes, perm, pcodes, T, S_pad, first);
+}
+
+void launch_gather_rows(const void *src, void *dst, size_t row_bytes, const int *src_rows, const int *dst_rows,
+                        int n, hipStream_t s)
+{
+    if (n <= 0 || row_bytes == 0) return;
+    if (row_bytes % 16 == 0)
+        hipLaunchKernelGGL(k_gather_rows<uint4>, dim3(n), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst,
+                           row_bytes / 16, src_rows, dst_rows);
+    else if (row_bytes % 4 == 0)
+        hipLaunchKernelGGL(k_gather_rows<uint32_t>, dim3(n), dim3(256), 0, s, (const uint32_t *)src,
+                           (uint32_t *)dst, row_bytes / 4, src_rows, dst_rows);
+    else
+        hipLaunchKernelGGL(k_gather_rows<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t *)src, (uint8_t *)dst,
+                           row_bytes, src_rows, dst_rows);
+}
 
 void launch_next_step(int n, double *w, const double *l, double *norm, double *out2,
                       int target_ess, double last_temp, hipStream_t s)

@@ -174,8 +174,9 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     if (e->WS != 2) return smc_fail(e, SMC_ERR_STATE, "rejuvenation needs workspaces = 2");
     const smc_rejuvenation_options &o = *opt;
     const int T = e->T, Ti = T - 1, nn = e->nn;
-    // full refresh of the freshly copied particles (copyFromSerialized, Particle.cpp:276)
-    int rc = smc_refresh_likelihood(e, first, count, nullptr, nullptr);
+    // full refresh of the freshly copied particles (copyFromSerialized, Particle.cpp:276);
+    // slots whose committed partials are still resident keep them
+    int rc = smc_refresh_stale(e, first, count);
     if (rc != SMC_OK) return rc;
 
     const MoveTable mt = move_table(e, o);

@@ -5,8 +5,9 @@
 // tuning, rejuvenation), computeNextStep (Miscellaneous.hpp:42-109).  The
 // per-thread Particle objects of the reference become slot ranges of one
 // engine; the serialised copy of a parent into its child slot
-// (copyFromSerialized, Particle.cpp:218-276) becomes a gather of the parent's
-// parameters followed by one upload of the population.
+// (copyFromSerialized, Particle.cpp:218-276) becomes a device-side gather of the
+// parents' states (smc_copy_particles), and only slots whose partials are not
+// resident are recomputed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -73,7 +74,7 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
     if (e->WS != 2) return smc_fail(e, SMC_ERR_STATE, "the sampler needs workspaces = 2");
     for (int p = 0; p < N; ++p)
         if (!e->h_valid[p]) return smc_fail(e, SMC_ERR_STATE, "slot without an initial particle");
-    const int T = e->T, Ti = T - 1, nn = e->nn, S = e->S, Kmax = e->Kmax;
+    const int nn = e->nn;
     const int batch = std::min(o.batch > 0 ? o.batch : e->Wmax, e->Wmax);
     *res = smc_sampler_result{};
 
@@ -86,9 +87,7 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
     int rc = smc_refresh_likelihood(e, 0, N, lnl.data(), nullptr);
     if (rc != SMC_OK) return rc;
 
-    // host copies of the resampled population, uploaded in one call
-    std::vector<int32_t> ch((size_t)N * Ti * 2), asg((size_t)N * S), nk(N), assign(N);
-    std::vector<double> bl((size_t)N * nn), base((size_t)N * 190), lg((size_t)N * Kmax * 20), pv(N), sh(N);
+    std::vector<int32_t> assign(N);
     std::vector<double> chol;
     const double rej_threshold = o.rejuvenation_threshold * N;
     const int target = (int)rej_threshold;  // bound to const int& in the reference (Q17)
@@ -156,20 +155,8 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
                 ro.rate_cholesky = chol.data();
                 ro.rate_delta = std::max(diag, 0.01);
             }
-            // copy every parent into its new slot
-            for (int n = 0; n < N; ++n) {
-                const int p = assign[n];
-                std::copy_n(&e->h_children[(size_t)p * Ti * 2], Ti * 2, &ch[(size_t)n * Ti * 2]);
-                std::copy_n(&e->h_blen[(size_t)p * nn], nn, &bl[(size_t)n * nn]);
-                std::copy_n(&e->h_base[(size_t)p * 190], 190, &base[(size_t)n * 190]);
-                std::copy_n(&e->h_logits[(size_t)p * Kmax * 20], Kmax * 20, &lg[(size_t)n * Kmax * 20]);
-                std::copy_n(&e->h_assign[(size_t)p * S], S, &asg[(size_t)n * S]);
-                nk[n] = e->h_nclass[p];
-                pv[n] = e->h_pinv[p];
-                sh[n] = e->h_shape[p];
-            }
-            rc = smc_set_particles(e, 0, N, ch.data(), bl.data(), base.data(), nk.data(), lg.data(),
-                                   asg.data(), pv.data(), sh.data());
+            // copy every parent into its new slot, on the device
+            rc = smc_copy_particles(e, N, assign.data());
             if (rc != SMC_OK) return rc;
             const auto r0 = clk::now();
             // Batched (batch > 1): the CAT moves of every batch run together after the
@@ -200,8 +187,17 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
             if (!cat_slots.empty()) {
                 rc = smc_cat_moves(e, cat_slots, cur, ro, res->cat_stats);
                 if (rc != SMC_OK) return rc;
-                rc = smc_refresh_likelihood(e, 0, N, lnl.data(), nullptr);  // in waves when N exceeds them
-                if (rc != SMC_OK) return rc;
+                // the moved slots' likelihoods, batch by batch (each batch resident in turn)
+                std::vector<uint8_t> moved(N, 0);
+                for (int32_t sl : cat_slots) moved[sl] = 1;
+                for (int first = 0; first < N; first += batch) {
+                    const int cnt = std::min(batch, N - first);
+                    if (std::none_of(moved.begin() + first, moved.begin() + first + cnt, [](uint8_t v) { return v; }))
+                        continue;
+                    rc = smc_refresh_slots(e, first, cnt, moved.data() + first);
+                    if (rc != SMC_OK) return rc;
+                }
+                std::copy(e->h_lnl.begin(), e->h_lnl.begin() + N, lnl.begin());
             }
             rejuv_s += std::chrono::duration<double>(clk::now() - r0).count();
             if (ro.record_allocations && res->resamples + 1 == o.proposal_warmup) {

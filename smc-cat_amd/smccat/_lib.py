@@ -127,6 +127,7 @@ SIGNATURES = {
     "smc_update_params": (_I, [_P, _I, _I, _IP, _DP, _DP, _DP, _DP]),
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
     "smc_accept_reject": (_I, [_P, _I, _I, _BP]),
+    "smc_copy_particles": (_I, [_P, _I, _IP]),
     "smc_rejuvenation_steps": (_I, [_P, ctypes.POINTER(RejuvenationOptions), _I, _IP, _IP]),
     "smc_rejuvenate": (_I, [_P, _I, _I, _D, ctypes.POINTER(RejuvenationOptions), _DP,
                             ctypes.POINTER(ctypes.c_int64)]),

@@ -218,6 +218,11 @@ class Engine:
                     "smc_refresh_incremental")
         return lnl, bad
 
+    def copy_particles(self, parents):
+        """Slot i of [0, len(parents)) takes the state of slot parents[i] (on the device)."""
+        par = np.ascontiguousarray(parents, dtype=np.int32)
+        self._check(self.lib.smc_copy_particles(self.h, len(par), _ip(par)), "smc_copy_particles")
+
     def accept_reject(self, accept, first=0):
         acc = np.ascontiguousarray(accept, dtype=np.uint8)
         self._check(self.lib.smc_accept_reject(self.h, first, len(acc),
