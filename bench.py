@@ -195,10 +195,11 @@ def sweep_bench(args, config, precision, steps, warmup, autotune, fp32_error=0, 
     eng.set_timing(False)
     shape = tune or eng.tuning()
     graphs = eng.graph_stats()
+    patterns = eng.pattern_count(0, N)  # site patterns within classes: the columns a sweep prunes
     eng.close()
 
-    updates = N * (T - 1) * S * R
-    bytes_sweep = N * S * R * algorithmic_bytes_per_column(T, word)
+    updates = patterns * (T - 1) * R
+    bytes_sweep = patterns * R * algorithmic_bytes_per_column(T, word)
     n_prune, ms_prune = stats["prune"]
     prune_ms = ms_prune / n_timed
     achieved = bytes_sweep / (prune_ms * 1e-3) / 1e9
@@ -211,7 +212,8 @@ def sweep_bench(args, config, precision, steps, warmup, autotune, fp32_error=0, 
                     (config, T, S, "LG" if cfg["lg"] else "GTR", R, "+I" if cfg["invar"] else "",
                      " CAT K~U{%d..%d}" % kr if kr[1] > 1 else "", N),
         "config": {"taxa": T, "sites": S, "rates": R, "particles": N, "partial_slots": slots, "precision": precision,
-                   "prune_kernel": args.variant, "mixture": mixture, "updates_per_step": updates},
+                   "prune_kernel": args.variant, "mixture": mixture, "updates_per_step": updates,
+                   "patterns_per_step": patterns, "columns_per_step": N * S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "mfma": {"busy": profiled_value("mfma_busy", config, precision, N, cfg["particles"]),

@@ -444,6 +444,17 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant);
 enum { SMC_MIX_REFERENCE = 0, SMC_MIX_LOGSPACE = 1 };
 int smc_engine_set_mixture(smc_engine *e, int mode);
 
+/* Site patterns (no reference counterpart: the reference evaluates one pattern per
+ * column, Alignment.cpp:83, Particle.cpp:22).  With compress = 1 (the default) the
+ * identical columns of one CAT class share one slot whose log-likelihood is counted
+ * once per column; 0 keeps one slot per column.  Applies to particles uploaded
+ * afterwards.  Columns are identical when every taxon has the same code, gaps and
+ * unknowns included, so the invariant-site terms (Q5) agree as well. */
+int smc_engine_set_patterns(smc_engine *e, int compress);
+/* Slots (site patterns within classes) of slots [first, first+count): the columns a
+ * sweep prunes; -1 on invalid arguments. */
+int64_t smc_pattern_count(const smc_engine *e, int first, int count);
+
 /* Bytes of device memory held by the engine. */
 int64_t smc_engine_device_bytes(const smc_engine *e);
 

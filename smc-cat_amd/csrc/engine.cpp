@@ -498,6 +498,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     };
     A(&e->d_codes, T * Sp);
     A(&e->d_pcodes, P * T * Sp);
+    A(&e->d_pweight, P * Sp);
     A(&e->d_isinv, Sp);
     A(&e->d_invchar, Sp);
     A(&e->d_children, P * (T - 1) * 2);
@@ -592,6 +593,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_pnodes.assign(P, {});
     e->committed.resize(P);
     e->h_owner.assign(e->Wmax, -1);
+    e->h_nslots.assign(P, 0);
     *out = e;
     return SMC_OK;
 }
@@ -599,7 +601,7 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
 int smc_engine_destroy(smc_engine *e)
 {
     if (!e) return SMC_ERR_ARG;
-    void *bufs[] = {e->d_codes,  e->d_pcodes, e->d_isinv, e->d_invchar, e->d_children, e->d_blen,
+    void *bufs[] = {e->d_codes,  e->d_pcodes, e->d_pweight, e->d_isinv, e->d_invchar, e->d_children, e->d_blen,
                     e->d_base,   e->d_logits, e->d_nclass, e->d_pinv,     e->d_rates,
                     e->d_root,   e->d_perm,  e->d_tiles,   e->d_ntiles,   e->d_eig,
                     e->d_P,      e->d_X,     e->d_E,       e->d_tile_lnl, e->d_lnl,
@@ -656,6 +658,27 @@ int smc_set_alignment(smc_engine *e, const int8_t *codes)
         inv[s] = same ? 1.0 : 0.0;
         ich[s] = (same && c1 < 20) ? c1 : 0;
     }
+    // site patterns: identical columns (raw codes, so the invariant terms agree too)
+    // share one id
+    {
+        std::vector<int> order(S);
+        for (int s = 0; s < S; ++s) order[s] = s;
+        auto col_less = [&](int a, int b) {
+            for (int t = 0; t < T; ++t) {
+                const uint8_t x = hc[(size_t)t * Sp + a], y = hc[(size_t)t * Sp + b];
+                if (x != y) return x < y;
+            }
+            return false;
+        };
+        std::stable_sort(order.begin(), order.end(), col_less);
+        e->h_pattern.assign(S, 0);
+        int id = -1;
+        for (int k = 0; k < S; ++k) {
+            if (k == 0 || col_less(order[k - 1], order[k])) ++id;
+            e->h_pattern[order[k]] = id;
+        }
+        e->n_patterns = id + 1;
+    }
     // amino-acid frequencies of each column (codes >= 20 carry no state)
     e->h_sitefreq.assign((size_t)S * 20, 0.f);
     for (int s = 0; s < S; ++s) {
@@ -671,7 +694,12 @@ int smc_set_alignment(smc_engine *e, const int8_t *codes)
                               e->stream));
     HIP_TRY(e, hipMemcpyAsync(e->d_invchar, ich.data(), Sp * sizeof(int), hipMemcpyHostToDevice,
                               e->stream));
-    // tip codes in the slot order of the particles already uploaded (runs of valid slots)
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->have_alignment = true;
+    // particles already uploaded: their slots (patterns) and tip codes follow the new
+    // alignment (runs of valid slots, re-uploaded from the host copies)
+    const int Ti = T - 1, nn = e->nn, K = e->Kmax, R = e->R;
+    (void)R;
     for (int p = 0; p < e->Pmax;) {
         if (!e->h_valid[p]) {
             ++p;
@@ -679,12 +707,21 @@ int smc_set_alignment(smc_engine *e, const int8_t *codes)
         }
         int q = p;
         while (q < e->Pmax && e->h_valid[q]) ++q;
-        launch_permute_codes(e->d_codes, e->d_perm, e->d_pcodes, T, Sp, p, q - p, e->stream);
+        const size_t c = (size_t)(q - p);
+        std::vector<int32_t> ch(e->h_children.begin() + (size_t)p * Ti * 2, e->h_children.begin() + (size_t)q * Ti * 2);
+        std::vector<double> bl(e->h_blen.begin() + (size_t)p * nn, e->h_blen.begin() + (size_t)q * nn);
+        std::vector<double> base(e->h_base.begin() + (size_t)p * 190, e->h_base.begin() + (size_t)q * 190);
+        std::vector<int32_t> nk(e->h_nclass.begin() + p, e->h_nclass.begin() + q);
+        std::vector<double> lg(e->h_logits.begin() + (size_t)p * K * 20, e->h_logits.begin() + (size_t)q * K * 20);
+        std::vector<int32_t> asg(e->h_assign.begin() + (size_t)p * S, e->h_assign.begin() + (size_t)q * S);
+        std::vector<double> pv(e->h_pinv.begin() + p, e->h_pinv.begin() + q);
+        std::vector<double> sh(e->h_shape.begin() + p, e->h_shape.begin() + q);
+        (void)c;
+        const int rc = smc_set_particles(e, p, q - p, ch.data(), bl.data(), base.data(), nk.data(), lg.data(),
+                                         asg.data(), pv.data(), sh.data());
+        if (rc != SMC_OK) return rc;
         p = q;
     }
-    HIP_TRY(e, hipGetLastError());
-    HIP_TRY(e, hipStreamSynchronize(e->stream));
-    e->have_alignment = true;
     return SMC_OK;
 }
 
@@ -704,8 +741,11 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     const int ts = e->tile_stride;
     for (int &o : e->h_owner)  // new states: resident partials of these slots are stale
         if (o >= first && o < first + count) o = -1;
-    std::vector<double> blen((size_t)count * nn), rates((size_t)count * R);
+    std::vector<double> blen((size_t)count * nn), rates((size_t)count * R), weight((size_t)count * Sp, 0.0);
     std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
+    // a column's pattern: its id (uploaded alignment), else the column itself
+    auto pattern_of = [&](int s) { return e->h_pattern.empty() ? s : e->h_pattern[s]; };
+    std::vector<int> stamp(std::max(e->n_patterns, S), -1), pslot(std::max(e->n_patterns, S), 0);
     std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
     std::vector<int> progs((size_t)count * site_prog_stride(T));
     for (int i = 0; i < count; ++i) {
@@ -753,8 +793,43 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
             cnt[as[s] + 1]++;
         }
         for (int k = 0; k < Kp; ++k) cnt[k + 1] += cnt[k];
-        std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
-        for (int s = 0; s < S; ++s) perm[(size_t)i * Sp + fillp[as[s]]++] = s;
+        // class-sorted slots: one per column, or (compressed) one per distinct pattern of
+        // a class, weighted by its columns; the slot holds its first column
+        std::vector<int> fillp(cnt.begin(), cnt.end() - 1), bycls(S);
+        for (int s = 0; s < S; ++s) bycls[fillp[as[s]]++] = s;  // sites grouped by class, in site order
+        std::vector<int> used(Kp, 0);
+        for (int k = 0; k < Kp; ++k)  // one class at a time: its patterns' stamps stay its own
+            for (int j = cnt[k]; j < cnt[k + 1]; ++j) {
+                const int s = bycls[j];
+                const int pid = pattern_of(s);
+                if (e->compress && stamp[pid] == i * K + k) {
+                    weight[(size_t)i * Sp + pslot[pid]] += 1.0;
+                    continue;
+                }
+                stamp[pid] = i * K + k;
+                const int sl = cnt[k] + used[k]++;
+                pslot[pid] = sl;
+                perm[(size_t)i * Sp + sl] = s;
+                weight[(size_t)i * Sp + sl] = 1.0;
+            }
+        // slots of class k: [cnt[k], cnt[k] + used[k]); pack them to the front
+        {
+            std::vector<int> pk(Sp, 0);
+            std::vector<double> pw(Sp, 0.0);
+            int o = 0;
+            for (int k = 0; k < Kp; ++k) {
+                const int c0 = cnt[k];
+                cnt[k] = o;
+                for (int j = 0; j < used[k]; ++j, ++o) {
+                    pk[o] = perm[(size_t)i * Sp + c0 + j];
+                    pw[o] = weight[(size_t)i * Sp + c0 + j];
+                }
+            }
+            cnt[Kp] = o;
+            std::copy(pk.begin(), pk.end(), perm.begin() + (size_t)i * Sp);
+            std::copy(pw.begin(), pw.end(), weight.begin() + (size_t)i * Sp);
+            e->h_nslots[p] = o;
+        }
         // tiles: each class's slots cut at the 64-slot window boundaries (at most
         // ceil(S / 64) windows plus one split window per class boundary)
         int nt = 0;
@@ -799,6 +874,8 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_perm + P0 * Sp, perm.data(), perm.size() * sizeof(int),
                               hipMemcpyHostToDevice, s));
+    HIP_TRY(e, hipMemcpyAsync(e->d_pweight + P0 * Sp, weight.data(), weight.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_tiles + P0 * ts, tiles.data(), tiles.size() * sizeof(Tile),
                               hipMemcpyHostToDevice, s));
     HIP_TRY(e, hipMemcpyAsync(e->d_ntiles + P0, ntiles.data(), ntiles.size() * sizeof(int),
@@ -826,6 +903,7 @@ PruneArgs prune_args(smc_engine *e, int p_base)
     a.perm = e->d_perm;
     a.codes = e->d_codes;
     a.pcodes = e->d_pcodes;
+    a.pweight = e->d_pweight;
     a.P = e->d_P;
     a.pcls_off = e->d_pcls_off;
     a.eig = e->d_eig;
@@ -1552,6 +1630,7 @@ int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
     gather_host(e->h_pinv, 1, dst, src);
     gather_host(e->h_shape, 1, dst, src);
     gather_host(e->h_rates, (size_t)R, dst, src);
+    gather_host(e->h_nslots, 1, dst, src);
     gather_host(e->h_xflag_old, (size_t)Ti, dst, src);
     gather_host(e->h_lnl_old, 1, dst, src);
     {
@@ -1579,7 +1658,8 @@ int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
                         {e->d_rates, sizeof(double) * R},     {e->d_root, sizeof(int)},
                         {e->d_perm, sizeof(int) * Sp},        {e->d_tiles, sizeof(Tile) * e->tile_stride},
                         {e->d_ntiles, sizeof(int)},           {e->d_prog, sizeof(int) * site_prog_stride(T)},
-                        {e->d_eig, sizeof(double) * K * kEigStride}, {e->d_pcodes, (size_t)T * Sp}};
+                        {e->d_eig, sizeof(double) * K * kEigStride}, {e->d_pcodes, (size_t)T * Sp},
+                        {e->d_pweight, sizeof(double) * Sp}};
     size_t max_row = 0;
     for (const Buf &b : bufs) max_row = std::max(max_row, b.row);
     const int64_t need = (int64_t)m * (int64_t)max_row;
@@ -1790,6 +1870,22 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant)
         return fail(e, SMC_ERR_ARG, "unknown kernel variant");
     e->prune_variant = variant;
     return SMC_OK;
+}
+
+int smc_engine_set_patterns(smc_engine *e, int compress)
+{
+    if (!e) return SMC_ERR_ARG;
+    if (compress != 0 && compress != 1) return fail(e, SMC_ERR_ARG, "compress must be 0 or 1");
+    e->compress = compress != 0;
+    return SMC_OK;
+}
+
+int64_t smc_pattern_count(const smc_engine *e, int first, int count)
+{
+    if (!e || first < 0 || count < 0 || first + count > e->Pmax) return -1;
+    int64_t n = 0;
+    for (int p = first; p < first + count; ++p) n += e->h_nslots[p];
+    return n;
 }
 
 int smc_engine_set_mixture(smc_engine *e, int mode)

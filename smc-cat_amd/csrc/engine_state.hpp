@@ -50,6 +50,11 @@ struct smc_engine {
     // device buffers
     uint8_t *d_codes = nullptr;
     uint8_t *d_pcodes = nullptr;  // [P][T][S_pad] tip codes in each particle's class-sorted slot order
+    double *d_pweight = nullptr;  // [P][S_pad] columns each slot stands for (site patterns within a class)
+    bool compress = true;         // merge identical columns of a class into one weighted slot
+    std::vector<int> h_pattern;   // [S] pattern id of each column (identical columns share one)
+    int n_patterns = 0;
+    std::vector<int> h_nslots;    // [P] slots (patterns) in use per particle
     double *d_isinv = nullptr;
     int *d_invchar = nullptr;
     int *d_children = nullptr;

@@ -108,6 +108,7 @@ struct PruneArgs {
     const int *perm;         // [P][S_pad] slot -> alignment column
     const uint8_t *codes;    // [T][S_pad]
     const uint8_t *pcodes;   // [P][T][S_pad] tip codes in each particle's slot order (20: gap or unknown)
+    const double *pweight;   // [P][S_pad] columns each slot stands for (roots weight its log-likelihood)
     const void *P;           // [wave classes][nn][R][420]
     const int *pcls_off;     // [P] compact class slot of (particle, class 0) in P
     const double *eig;       // [P*Kmax][840]

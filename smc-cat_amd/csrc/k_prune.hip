@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                double *__restrict__ tile_lnl, int T, int R,
                                                int S_pad, int max_classes, int blocks_per_item,
                                                const int *__restrict__ pcls_off, int p_base, int mix_mode,
-                                               const uint8_t *__restrict__ xflag, int WS)
+                                               const uint8_t *__restrict__ xflag, int WS,
+                                               const double *__restrict__ pweight)
 {
     // 1-D grid: consecutive blocks walk the tiles of one item, 4 tiles (one per
     // wavefront) per block, so concurrently running waves stream contiguous
@@ -194,8 +195,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if constexpr (is_root) {
         double lf = 0.0;
         if (active) {
-            lf = site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig],
-                                     mix_mode);
+            lf = pweight[(size_t)p * S_pad + s] *
+                 site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig], mix_mode);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) lf += __shfl_xor(lf, off, 64);
@@ -236,7 +237,7 @@ void launch_prune(const PruneArgs &a, hipStream_t s)
 #define SMC_PRUNE_ARGS                                                                          \
     a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.root_of, a.perm, a.codes,         \
         (const TXa *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TXa *)a.X, a.E, a.tile_lnl, a.T, \
-        a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS
+        a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS, a.pweight
     if (a.fp32) {
         using TXa = float;
         if (a.roots)

@@ -199,6 +199,7 @@ struct KArgs {
     const int *children, *perm;
     const uint8_t *codes;
     const uint8_t *pcodes;  // [P][T][S_pad] tip codes in the particle's slot order, clamped to 20
+    const double *pweight;  // [P][S_pad] columns each slot stands for
     const TX *Pm;
     const double *eig, *pinv, *isinv;
     const int *invchar;
@@ -625,8 +626,11 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
         double lf = 0.0;
         const int o = site_of(g4 / U * U, g4 % U, c16);  // this lane's column: block g4
         if (o >= tl.lo && o < tl.len) {
-            const int orig = perm[(size_t)p * S_pad + tl.start + o];
-            lf = site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig], mix_mode);
+            const size_t sl = (size_t)p * S_pad + tl.start + o;
+            const int orig = perm[sl];
+            // a slot stands for the identical columns of its class (weight 1 uncompressed)
+            lf = ka.pweight[sl] *
+                 site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig], mix_mode);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) lf += __shfl_xor(lf, off, 64);
@@ -927,7 +931,7 @@ template <typename TX>
 KArgs<TX> kargs(const PruneArgs &a)
 {
     const int bpi = (a.tile_stride + 3) / 4;
-    return KArgs<TX>{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, a.pcodes,
+    return KArgs<TX>{a.items, a.tiles, a.ntiles, a.tile_stride, a.children, a.perm, a.codes, a.pcodes, a.pweight,
                      (const TX *)a.P, a.eig, a.pinv, a.isinv, a.invchar, (TX *)a.X, a.E, a.tile_lnl,
                      a.T, a.R, a.S_pad, a.max_classes, bpi, a.pcls_off, a.p_base, a.mix_mode, a.xflag, a.WS,
                      std::max(1, a.xcds)};

@@ -128,6 +128,8 @@ SIGNATURES = {
     "smc_refresh_incremental": (_I, [_P, _I, _I, _BP, _BP, _DP, _BP]),
     "smc_accept_reject": (_I, [_P, _I, _I, _BP]),
     "smc_copy_particles": (_I, [_P, _I, _IP]),
+    "smc_engine_set_patterns": (_I, [_P, _I]),
+    "smc_pattern_count": (ctypes.c_int64, [_P, _I, _I]),
     "smc_rejuvenation_steps": (_I, [_P, ctypes.POINTER(RejuvenationOptions), _I, _IP, _IP]),
     "smc_rejuvenate": (_I, [_P, _I, _I, _D, ctypes.POINTER(RejuvenationOptions), _DP,
                             ctypes.POINTER(ctypes.c_int64)]),

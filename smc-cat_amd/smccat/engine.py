@@ -388,6 +388,17 @@ class Engine:
              "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
 
+    def set_patterns(self, compress):
+        """Merge identical columns of a class into one weighted slot (default on) for
+        particles uploaded afterwards."""
+        self._check(self.lib.smc_engine_set_patterns(self.h, 1 if compress else 0), "smc_engine_set_patterns")
+
+    def pattern_count(self, first=0, count=None):
+        """Slots (site patterns within classes) the sweep of slots [first, first+count) prunes."""
+        if count is None:
+            count = self.max_particles - first
+        return int(self.lib.smc_pattern_count(self.h, int(first), int(count)))
+
     def set_mixture(self, mode):
         """'reference' (Particle.cpp:653-670 exactly, default) or 'logspace'."""
         m = {"reference": _lib.SMC_MIX_REFERENCE, "logspace": _lib.SMC_MIX_LOGSPACE}[mode]
