@@ -444,6 +444,28 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant);
 enum { SMC_MIX_REFERENCE = 0, SMC_MIX_LOGSPACE = 1 };
 int smc_engine_set_mixture(smc_engine *e, int mode);
 
+/* Split posteriors, computeSplitPosteriors (Miscellaneous.hpp:115-150), on the GPU:
+ * the split sets of slots [0, n) (Tree::getSplitSet, Tree.cpp:485-535: the tips below
+ * every non-root node, flipped when they do not hold taxon 0) weighted by weights[n]
+ * (normalized weights, main.cpp:338-342, or 1/n per resampled copy, main.cpp:203-207).
+ * A split is W = ceil(taxa / 64) 64-bit words, bit t = taxon t.  Writes the number of
+ * distinct splits to *n_splits and, when it is at most max_splits, their words
+ * split_words[n_splits][W], posterior[n_splits] (the sum of the weights of the
+ * particles holding the split) and mean_length[n_splits] (the weighted mean of its
+ * branch length over those particles, Tree::getSplitBranchMap, main.cpp:367-383),
+ * ordered by a 64-bit hash of the words; any output may be NULL.  Two different
+ * splits sharing a hash fail with SMC_ERR_STATE. */
+int smc_split_posteriors(smc_engine *e, int n, const double *weights, int64_t max_splits,
+                         int64_t *n_splits, uint64_t *split_words, double *posterior,
+                         double *mean_length);
+/* The greedy consensus (main.cpp:344-365) of the same splits: in decreasing posterior,
+ * each split kept when it is compatible (four-point condition) with every split kept
+ * before, until the tree is resolved.  Writes *n_selected (at most 2 taxa - 3, the
+ * trivial splits included) and the kept splits' words, posteriors and mean lengths in
+ * the order they were kept (arrays of 2 taxa - 3 entries; any may be NULL). */
+int smc_consensus(smc_engine *e, int n, const double *weights, int32_t *n_selected,
+                  uint64_t *split_words, double *posterior, double *mean_length);
+
 /* Site patterns (no reference counterpart: the reference evaluates one pattern per
  * column, Alignment.cpp:83, Particle.cpp:22).  With compress = 1 (the default) the
  * identical columns of one CAT class share one slot whose log-likelihood is counted

@@ -616,6 +616,7 @@ int smc_engine_destroy(smc_engine *e)
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
     if (e->d_gather) (void)hipFree(e->d_gather);
+    if (e->d_split_ws) (void)hipFree(e->d_split_ws);
     for (void *b : {(void *)e->pool.d_eig, (void *)e->pool.d_logits, (void *)e->pool.d_ent})
         if (b) (void)hipFree(b);
     for (void *b : bufs)
@@ -1666,6 +1667,7 @@ int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
     if (need > e->gather_cap) {
         HIP_TRY(e, hipStreamSynchronize(e->stream));
         if (e->d_gather) (void)hipFree(e->d_gather);
+    if (e->d_split_ws) (void)hipFree(e->d_split_ws);
         e->d_gather = nullptr;
         e->gather_cap = 0;
         const int64_t cap = std::max<int64_t>(need, (int64_t)1 << 20);

@@ -114,6 +114,8 @@ struct smc_engine {
     std::vector<int> h_owner;
     uint8_t *d_gather = nullptr;                 // row-gather staging of smc_copy_particles
     int64_t gather_cap = 0;
+    uint8_t *d_split_ws = nullptr;               // split-posterior workspace (grown on demand)
+    size_t split_ws_bytes = 0;
     Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
     int *d_chain_nodes = nullptr;               // [P * (T-1)] dirty-path chains' nodes, bottom up
     int *d_chain_len = nullptr;                 // [P * (T-1)] chain lengths, aligned with d_items

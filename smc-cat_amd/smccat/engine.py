@@ -388,6 +388,36 @@ class Engine:
              "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
 
+    def split_posteriors(self, weights):
+        """(words [U, W] uint64, posterior [U], mean branch length [U]) of the distinct
+        splits of slots [0, len(weights)) (computeSplitPosteriors)."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        W = (self.taxa + 63) // 64
+        n = ctypes.c_int64()
+        self._check(self.lib.smc_split_posteriors(self.h, len(w), _dp(w), 0, ctypes.byref(n), None, None, None),
+                    "smc_split_posteriors")
+        words = np.zeros((n.value, W), dtype=np.uint64)
+        post = np.zeros(n.value)
+        ln = np.zeros(n.value)
+        self._check(self.lib.smc_split_posteriors(self.h, len(w), _dp(w), n.value, ctypes.byref(n),
+                                                  words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                  _dp(post), _dp(ln)), "smc_split_posteriors")
+        return words, post, ln
+
+    def consensus(self, weights):
+        """(words [k, W], posterior [k], mean length [k]) of the greedy consensus splits."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        W = (self.taxa + 63) // 64
+        m = 2 * self.taxa - 3
+        words = np.zeros((m, W), dtype=np.uint64)
+        post = np.zeros(m)
+        ln = np.zeros(m)
+        k = ctypes.c_int32()
+        self._check(self.lib.smc_consensus(self.h, len(w), _dp(w), ctypes.byref(k),
+                                           words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _dp(post),
+                                           _dp(ln)), "smc_consensus")
+        return words[:k.value], post[:k.value], ln[:k.value]
+
     def set_patterns(self, compress):
         """Merge identical columns of a class into one weighted slot (default on) for
         particles uploaded afterwards."""
