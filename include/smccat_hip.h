@@ -444,6 +444,17 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant);
 enum { SMC_MIX_REFERENCE = 0, SMC_MIX_LOGSPACE = 1 };
 int smc_engine_set_mixture(smc_engine *e, int mode);
 
+/* Proposal tuning from the resampled population (main.cpp:213-289) on the GPU, over
+ * slots [0, n): out[0] variance of log shape, out[1] variance of log branch length
+ * (non-root nodes), out[2] mean pInvar, out[3] the shrinkage diagonal (the base for
+ * rateMatrixDelta); with using_lg = 0 and rate_cholesky non-NULL, rate_cholesky[190][190]
+ * receives the lower Cholesky factor (row-major, upper part zero) of the uncentred second
+ * moment of the 190 log-exchangeabilities (Q21) scaled by 2.38^2 / 190 * 0.21 and shrunk
+ * by 0.2 towards its mean diagonal, plus 1e-6 on the diagonal.  Variances are the mean
+ * of squares minus the squared mean, as the reference's accumulators.  SMC_ERR_STATE
+ * when the matrix is not positive definite. */
+int smc_proposal_tuning(smc_engine *e, int n, int using_lg, double *out, double *rate_cholesky);
+
 /* Split posteriors, computeSplitPosteriors (Miscellaneous.hpp:115-150), on the GPU:
  * the split sets of slots [0, n) (Tree::getSplitSet, Tree.cpp:485-535: the tips below
  * every non-root node, flipped when they do not hold taxon 0) weighted by weights[n]

@@ -16,47 +16,6 @@
 #include "engine_state.hpp"
 #include "host_rng.hpp"
 
-namespace {
-
-// population variance (boost accumulators' tag::variance: mean of squares minus
-// squared mean)
-struct Var {
-    double n = 0.0, s = 0.0, s2 = 0.0;
-    void push(double x)
-    {
-        n += 1.0;
-        s += x;
-        s2 += x * x;
-    }
-    double get() const
-    {
-        if (n == 0.0) return 0.0;
-        const double m = s / n;
-        return s2 / n - m * m;
-    }
-};
-
-// In-place Cholesky of a symmetric positive-definite [n][n] row-major matrix
-// (lower factor, upper part zeroed).  Returns false when not positive definite.
-bool cholesky(std::vector<double> &a, int n)
-{
-    for (int j = 0; j < n; ++j) {
-        double d = a[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= a[(size_t)j * n + k] * a[(size_t)j * n + k];
-        if (!(d > 0.0)) return false;
-        d = std::sqrt(d);
-        a[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; ++i) {
-            double v = a[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) v -= a[(size_t)i * n + k] * a[(size_t)j * n + k];
-            a[(size_t)i * n + j] = v / d;
-        }
-        for (int k = j + 1; k < n; ++k) a[(size_t)j * n + k] = 0.0;
-    }
-    return true;
-}
-
-}  // namespace
 
 extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, smc_sampler_result *res,
                                double *lnl_out, double *weights_out)
@@ -74,7 +33,6 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
     if (e->WS != 2) return smc_fail(e, SMC_ERR_STATE, "the sampler needs workspaces = 2");
     for (int p = 0; p < N; ++p)
         if (!e->h_valid[p]) return smc_fail(e, SMC_ERR_STATE, "slot without an initial particle");
-    const int nn = e->nn;
     const int batch = std::min(o.batch > 0 ? o.batch : e->Wmax, e->Wmax);
     *res = smc_sampler_result{};
 
@@ -105,26 +63,23 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
         if (ess <= rej_threshold && cur != 1.0) {
             rc = smc_systematic_resample(e, N, norm.data(), g.uniform(), assign.data());
             if (rc != SMC_OK) return rc;
-            // tuning from the resampled population (main.cpp:213-289)
-            Var rate_var, branch_var;
-            double pinv_mean = 0.0;
-            for (int n = 0; n < N; ++n) {
-                const int p = assign[n];
-                rate_var.push(std::log(e->h_shape[p]));
-                pinv_mean += e->h_pinv[p];
-                for (int v = 0; v < nn; ++v)
-                    if (v != e->h_root[p]) branch_var.push(std::log(e->h_blen[(size_t)p * nn + v]));
-            }
-            pinv_mean /= N;
+            // copy every parent into its new slot, on the device
+            rc = smc_copy_particles(e, N, assign.data());
+            if (rc != SMC_OK) return rc;
+            // tuning from the resampled population (main.cpp:213-289), on the GPU
+            double tune[4];
+            chol.assign(190 * 190, 0.0);
+            rc = smc_proposal_tuning(e, N, o.using_lg, tune, o.using_lg ? nullptr : chol.data());
+            if (rc != SMC_OK) return rc;
             smc_rejuvenation_options ro{};
             ro.iterations = o.rejuvenation_iterations;
             ro.using_lg = o.using_lg;
             ro.invariant = o.invariant;
             ro.seed = o.seed * 0x9E3779B97F4A7C15ull + (uint64_t)it;
-            ro.shape_delta = std::max(0.5 * rate_var.get(), 0.01);
-            ro.scale_delta = std::max(branch_var.get(), 0.01);
+            ro.shape_delta = std::max(0.5 * tune[0], 0.01);
+            ro.scale_delta = std::max(tune[1], 0.01);
             ro.subtree_scale_delta = ro.scale_delta;
-            ro.invar_alpha = pinv_mean * 0.5;
+            ro.invar_alpha = tune[2] * 0.5;
             ro.gibbs_probability = o.cat ? o.alg8_probability : 0.0;
             ro.dpp_alpha = o.dpp_alpha;
             ro.rate_full_scale = 0.0;
@@ -133,31 +88,11 @@ extern "C" int smc_run_sampler(smc_engine *e, const smc_sampler_options *opt, sm
             ro.allocation_proposal = learned ? o.allocation_proposal : nullptr;
             ro.record_allocations = learned && res->resamples < o.proposal_warmup;
             if (!o.using_lg) {
-                // uncentred second moment of the base matrices (Q21), scaled,
-                // shrunk towards its mean diagonal, then factored
-                chol.assign(190 * 190, 0.0);
-                for (int n = 0; n < N; ++n) {
-                    const double *b = &e->h_base[(size_t)assign[n] * 190];
-                    for (int i = 0; i < 190; ++i)
-                        for (int j = 0; j <= i; ++j) chol[i * 190 + j] += b[i] * b[j];
-                }
-                const double f = (2.38 * 2.38) / 190.0 * 0.21, shrink = 0.2;
-                double diag = 0.0;
-                for (int i = 0; i < 190; ++i) diag += chol[i * 190 + i] * f;
-                diag = diag / 190.0 * shrink;
-                for (int i = 0; i < 190; ++i) {
-                    for (int j = 0; j <= i; ++j) chol[i * 190 + j] *= f * (1.0 - shrink);
-                    chol[i * 190 + i] += diag + 1e-6;
-                    for (int j = 0; j < i; ++j) chol[j * 190 + i] = chol[i * 190 + j];
-                }
-                if (!cholesky(chol, 190))
-                    return smc_fail(e, SMC_ERR_STATE, "rate-matrix proposal covariance is not positive definite");
+                // the uncentred second moment of the base matrices (Q21), scaled, shrunk
+                // towards its mean diagonal and factored (smc_proposal_tuning)
                 ro.rate_cholesky = chol.data();
-                ro.rate_delta = std::max(diag, 0.01);
+                ro.rate_delta = std::max(tune[3], 0.01);
             }
-            // copy every parent into its new slot, on the device
-            rc = smc_copy_particles(e, N, assign.data());
-            if (rc != SMC_OK) return rc;
             const auto r0 = clk::now();
             // Batched (batch > 1): the CAT moves of every batch run together after the
             // last batch, one lockstep move over all chosen particles of the population

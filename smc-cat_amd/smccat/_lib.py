@@ -131,6 +131,7 @@ SIGNATURES = {
     "smc_engine_set_patterns": (_I, [_P, _I]),
     "smc_split_posteriors": (_I, [_P, _I, _DP, ctypes.c_int64, _LP, ctypes.POINTER(ctypes.c_uint64), _DP, _DP]),
     "smc_consensus": (_I, [_P, _I, _DP, _IP, ctypes.POINTER(ctypes.c_uint64), _DP, _DP]),
+    "smc_proposal_tuning": (_I, [_P, _I, _I, _DP, _DP]),
     "smc_pattern_count": (ctypes.c_int64, [_P, _I, _I]),
     "smc_rejuvenation_steps": (_I, [_P, ctypes.POINTER(RejuvenationOptions), _I, _IP, _IP]),
     "smc_rejuvenate": (_I, [_P, _I, _I, _D, ctypes.POINTER(RejuvenationOptions), _DP,

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-I", INCLUDE,
           "-Wall", "-Wno-unused-result"]
 
 SOURCES = ["engine.cpp", "gamma.cpp", "lg_table.cpp", "k_eigen.hip", "k_prune.hip", "k_prune_mfma.hip",
-           "k_smc.hip", "k_site_class.hip", "gibbs.cpp", "mcmc.cpp", "sampler.cpp", "k_mlp.hip", "mlp.cpp", "noalloc.cpp", "k_splits.hip"]
+           "k_smc.hip", "k_site_class.hip", "gibbs.cpp", "mcmc.cpp", "sampler.cpp", "k_mlp.hip", "mlp.cpp", "noalloc.cpp", "k_splits.hip", "k_tune.hip"]
 
 
 def _run(cmd):

@@ -388,6 +388,15 @@ class Engine:
              "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
 
+    def proposal_tuning(self, n, using_lg=False):
+        """(var log shape, var log branch length, mean pInvar, shrinkage diagonal,
+        rate Cholesky [190, 190] or None) of slots [0, n)."""
+        out = np.zeros(4)
+        chol = None if using_lg else np.zeros((190, 190))
+        self._check(self.lib.smc_proposal_tuning(self.h, int(n), 1 if using_lg else 0, _dp(out),
+                                                 None if chol is None else _dp(chol)), "smc_proposal_tuning")
+        return out[0], out[1], out[2], out[3], chol
+
     def split_posteriors(self, weights):
         """(words [U, W] uint64, posterior [U], mean branch length [U]) of the distinct
         splits of slots [0, len(weights)) (computeSplitPosteriors)."""
