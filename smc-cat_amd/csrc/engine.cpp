@@ -1667,7 +1667,6 @@ int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
     if (need > e->gather_cap) {
         HIP_TRY(e, hipStreamSynchronize(e->stream));
         if (e->d_gather) (void)hipFree(e->d_gather);
-    if (e->d_split_ws) (void)hipFree(e->d_split_ws);
         e->d_gather = nullptr;
         e->gather_cap = 0;
         const int64_t cap = std::max<int64_t>(need, (int64_t)1 << 20);
