@@ -440,7 +440,10 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant);
  * underflows to -inf below l = -745 and is NaN when a rate term is -inf, as in
  * the reference.  SMC_MIX_LOGSPACE computes the same value in log space: equal
  * to rounding wherever the reference formula is finite, finite where it
- * underflows.  No reference counterpart. */
+ * underflows; a rate term whose partials underflowed to zero (NaN through the
+ * reference's scaling, e.g. the lowest gamma category of a tiny shape on a
+ * variable column) contributes 0 to the site's sum over rates.  No reference
+ * counterpart. */
 enum { SMC_MIX_REFERENCE = 0, SMC_MIX_LOGSPACE = 1 };
 int smc_engine_set_mixture(smc_engine *e, int mode);
 

@@ -381,10 +381,11 @@ double orc_particle_lnl_sites_mode(const orc_particle *p, double *site_lnl, int 
                 double fin = exp(lsite) * inv_scaler + p->pinv * pi[inv_char[c]] * is_inv[c];
                 lf = log(fin);
             } else {
-                /* the same quantity evaluated in log space: -inf rate terms drop out and
+                /* the same quantity evaluated in log space: -inf and NaN rate terms (a
+                 * rate whose partials underflowed, 0/0 in the scaling above) drop out and
                  * log(e^l (1-p) + p pi isInv) = logaddexp(l + log(1-p), log(p pi isInv)) */
                 for (int r = 0; r < R; r++)
-                    if (logs[r] != -INFINITY) se += exp(logs[r] - mx);
+                    if (logs[r] > -INFINITY) se += exp(logs[r] - mx);
                 double a = mx + log(se) + log1p(-p->pinv);
                 double b = is_inv[c] != 0.0 ? log(p->pinv * pi[inv_char[c]] * is_inv[c]) : -INFINITY;
                 double hi = a > b ? a : b, lo = a > b ? b : a;

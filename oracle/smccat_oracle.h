@@ -74,7 +74,9 @@ double orc_particle_lnl(const orc_particle *p);
 double orc_particle_lnl_sites(const orc_particle *p, double *site_lnl);
 
 /* logspace = 1: the same mixture evaluated in log space (finite where
- * exp(logSiteLike) of Particle.cpp:667 underflows); logspace = 0 is the above. */
+ * exp(logSiteLike) of Particle.cpp:667 underflows; a rate term that is -inf or NaN,
+ * i.e. whose partials underflowed, adds 0 to the sum over rates); logspace = 0 is
+ * the above. */
 double orc_particle_lnl_sites_mode(const orc_particle *p, double *site_lnl, int logspace);
 
 /* Batch of particles, one particle per OpenMP thread (main.cpp:295 pattern).

@@ -4,9 +4,13 @@
 // l = LSE_r l_r with rate weights 1, L = exp(l) (1 - p) + p pi[c] isInv, log L.
 // SMC_MIX_REFERENCE keeps it exactly, including its failure modes: exp(l)
 // underflows to 0 below l = -745 (log L = -inf for a variable site) and a rate
-// term of -inf turns the LSE into NaN.  SMC_MIX_LOGSPACE evaluates the same
-// quantity without leaving log space: -inf rate terms contribute 0 and
-// log L = logaddexp(l + log(1 - p), log(p pi[c] isInv)).
+// term of -inf or NaN turns the LSE into NaN.  SMC_MIX_LOGSPACE evaluates the same
+// quantity without leaving log space: -inf and NaN rate terms contribute 0 and
+// log L = logaddexp(l + log(1 - p), log(p pi[c] isInv)).  A NaN rate term is one
+// whose partials underflowed: at a rate near 0 (a small gamma shape) P is the
+// identity up to round-off, a variable column's partial at some node is then 0
+// or a negative round-off in every state, and the reference's scaling by the max
+// makes it 0/0.  Its true likelihood is below the range of a double, so it adds 0.
 #pragma once
 
 #include "internal.hpp"
@@ -16,7 +20,7 @@ namespace smccat {
 
 __device__ __forceinline__ void lse_push(double &m, double &s, double l, int mode)
 {
-    if (mode == SMC_MIX_LOGSPACE && l == -INFINITY) return;
+    if (mode == SMC_MIX_LOGSPACE && !(l > -INFINITY)) return;  // -inf and NaN
     if (l > m) {
         s = s * exp(m - l) + 1.0;
         m = l;

@@ -195,6 +195,46 @@ def test_logspace_mixture(gpu, variant, precision):
     np.testing.assert_allclose(log_mode, want, rtol=RTOL_FP64 if precision == "fp64" else RTOL_FP32)
 
 
+
+def _underflow_case():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "underflow_rate_case.json")) as f:
+        d = json.load(f)
+    T = d["taxa"]
+    codes = np.array(d["column"], dtype=np.int8)[:, None]
+    logits = np.zeros((1, 1, 20))
+    logits[0, 0] = d["logits"]
+    batch = sim.ParticleBatch(np.array(d["children"], dtype=np.int32)[None], np.array(d["branch_len"])[None],
+                              np.array(d["base"])[None], np.ones(1, dtype=np.int32), logits,
+                              np.zeros((1, 1), dtype=np.int32), np.array([d["pinv"]]), np.array([d["shape"]]))
+    return codes, batch, d["num_rates"], T
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_logspace_underflowed_rate(gpu, variant, precision):
+    """Gamma shape 0.0576 on a variable 200-taxon column: the lowest rate's partials
+    underflow.  The reference formula is NaN (flagged); log space drops that rate as the
+    oracle does, in the pruning kernels and in the single-site class kernel."""
+    _say("case logspace underflowed rate: %s %s" % (variant, precision))
+    codes, batch, R, T = _underflow_case()
+    want, _ = orc.particle_lnl_mode(codes, batch.particle(0), R, 1)
+    with Engine(T, 1, R, 1, 1, precision) as eng:
+        eng.set_prune_variant(variant)
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        ref_mode, nf_ref = eng.refresh_likelihood()
+        eng.set_mixture("logspace")
+        log_mode, nf = eng.refresh_likelihood()
+        site = eng.site_class_lnl(np.zeros(1), np.zeros(1), batch.logits[:, :1])
+    _say("  reference %s, logspace %.9f, oracle %.9f, site class %.9f" % (ref_mode[0], log_mode[0], want, site[0, 0]))
+    assert math.isnan(ref_mode[0]) and nf_ref[0]
+    assert np.isfinite(log_mode[0]) and not nf[0]
+    tol = RTOL_FP64 if precision == "fp64" else RTOL_FP32
+    assert log_mode[0] == pytest.approx(want, rel=tol)
+    assert site[0, 0] == pytest.approx(want, rel=RTOL_FP64)
+
 def test_c2_full_size_properties(gpu):
     _say("case c2 full size: 64 taxa x 2000 sites, GTR+G4+I, 1024 particles")
     codes, cfg = sim.simulate_config("c2")

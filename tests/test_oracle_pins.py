@@ -279,3 +279,34 @@ def test_logspace_mixture_pins():
     assert log2 == pytest.approx(want, rel=1e-9)
     finite = np.isfinite(sites_ref)
     np.testing.assert_allclose(sites_log[finite], sites_ref[finite], rtol=1e-12)
+
+
+def _underflow_case():
+    d = _load("underflow_rate_case.json")
+    codes = np.array(d["column"], dtype=np.int8)[:, None]
+    p = dict(children=np.array(d["children"]), branch_len=np.array(d["branch_len"]), base=np.array(d["base"]),
+             num_classes=1, logits=np.array(d["logits"])[None, :], assignment=np.zeros(1, dtype=np.int32),
+             pinv=d["pinv"], shape=d["shape"])
+    return codes, p, d["num_rates"]
+
+
+def test_logspace_mixture_drops_underflowed_rate():
+    """A variable column under a gamma shape of 0.0576 (lowest rate 8e-15): the lowest
+    rate's partials underflow and the reference formula is NaN (Particle.cpp:653-670
+    divides a zero partial by its zero max).  Log space adds 0 for that rate, so the site
+    equals the log-sum-exp of the other rates' terms, each evaluated on its own."""
+    codes, p, R = _underflow_case()
+    assert len(set(codes[:, 0].tolist())) > 1           # variable: no invariant term
+    rates = orc.discretize_gamma(p["shape"], R)
+    assert rates[0] < 1e-14
+    ref, _ = orc.particle_lnl_mode(codes, p, R, 0)
+    assert math.isnan(ref)
+    terms = []
+    for r in rates:
+        # R copies of one rate: log(R e^{l_r} (1 - p)) in either mode
+        v, _ = orc.particle_lnl_mode(codes, p, R, 1, rates=np.full(R, r))
+        terms.append(v - math.log(R) - math.log1p(-p["pinv"]))
+    assert not np.isfinite(terms[0]) and np.all(np.isfinite(terms[1:]))
+    want = scipy.special.logsumexp(terms[1:]) + math.log1p(-p["pinv"])
+    got, _ = orc.particle_lnl_mode(codes, p, R, 1)
+    assert got == pytest.approx(want, rel=1e-13)
