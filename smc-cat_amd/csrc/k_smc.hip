@@ -9,8 +9,13 @@
 //
 // k_resample restates the systematic resampling walk of main.cpp:179-195.  It is
 // a sequential walk with accumulated rounding (ruler += 1/N, cumulative +=
-// w_k); one lane runs it so the assignments are bit-identical to the
-// reference's for the same normalized weights and u.
+// w_k), and the assignments must be bit-identical to the reference's for the
+// same normalized weights and u.  Both running sums are therefore accumulated
+// in order, each by one lane over LDS; the walk itself is then a search: the
+// rulers rise and the cumulative weights never fall, so ruler j goes to the
+// first k with ruler_j <= cumulative_k (n - 1 when there is none, where the
+// walk clamps).  Populations beyond the LDS arrays, or with a NaN weight (where
+// that order breaks), take the single-lane walk.
 #include "internal.hpp"
 
 namespace smccat {
@@ -110,9 +115,10 @@ __global__ __launch_bounds__(kSmcThreads) void k_next_step(int n, double *__rest
     }
 }
 
-__global__ void k_resample(int n, const double *__restrict__ norm, double u, int *__restrict__ assign)
+constexpr int kResampleLds = 4096;  // particles of the LDS form (two arrays of doubles: 64 KB)
+
+__device__ void resample_walk(int n, const double *__restrict__ norm, double u, int *__restrict__ assign)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double cumulative = norm[0];
     int k = 0, cnt = 0;
     double ruler = u / (double)n;
@@ -134,6 +140,54 @@ __global__ void k_resample(int n, const double *__restrict__ norm, double u, int
         }
     }
     while (cnt < n) assign[cnt++] = n - 1;
+}
+
+__global__ __launch_bounds__(kSmcThreads) void k_resample(int n, const double *__restrict__ norm, double u,
+                                                          int *__restrict__ assign)
+{
+    __shared__ double cum[kResampleLds], ruler[kResampleLds];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    if (n > kResampleLds) {
+        if (t == 0) resample_walk(n, norm, u, assign);
+        return;
+    }
+    for (int i = t; i < n; i += kSmcThreads) cum[i] = norm[i];
+    __syncthreads();
+    if (t == 0) {  // cumulative weights, in the walk's order
+        double c = cum[0];
+        int nan = c != c;
+        for (int k = 1; k < n; k++) {
+            c += cum[k];
+            cum[k] = c;
+            nan |= c != c;
+        }
+        bad = nan;
+    } else if (t == 64) {  // the rulers, in the walk's order
+        double r = u / (double)n;
+        const double step = 1.0 / (double)n;
+        for (int j = 0; j < n; j++) {
+            ruler[j] = r;
+            r += step;
+        }
+    }
+    __syncthreads();
+    if (bad) {
+        if (t == 0) resample_walk(n, norm, u, assign);
+        return;
+    }
+    for (int j = t; j < n; j += kSmcThreads) {
+        const double r = ruler[j];
+        int lo = 0, hi = n;  // first k with r <= cum[k]
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (r <= cum[mid])
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        assign[j] = lo < n ? lo : n - 1;
+    }
 }
 
 // Row gather: row dst_rows[r] (r when null) of dst is row src_rows[r] (r when null) of
@@ -197,7 +251,7 @@ void launch_next_step(int n, double *w, const double *l, double *norm, double *o
 
 void launch_resample(int n, const double *norm, double u, int *assign, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_resample, dim3(1), dim3(64), 0, s, n, norm, u, assign);
+    hipLaunchKernelGGL(k_resample, dim3(1), dim3(kSmcThreads), 0, s, n, norm, u, assign);
 }
 
 }  // namespace smccat

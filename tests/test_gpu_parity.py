@@ -292,11 +292,14 @@ def test_systematic_resample_bit_exact(gpu):
     _say("case systematic resampling")
     rng = np.random.default_rng(41)
     with Engine(4, 8, 1, 1, 1) as eng:
-        for n in [1, 2, 7, 64, 1000, 4096]:
-            for _ in range(5):
+        # up to 4096 the kernel searches the two running sums, beyond it it walks
+        for n in [1, 2, 7, 64, 1000, 4096, 4097, 6000]:
+            for rep in range(5):
                 w = rng.random(n) ** 3
+                if rep == 1:
+                    w[rng.random(n) < 0.7] = 0.0      # runs of equal cumulative weights
                 w /= w.sum()
-                u = rng.random()
-                assert eng.systematic_resample(w, u).tolist() == orc.systematic_resample(w, u).tolist()
+                u = 1.0 - 1e-12 if rep == 2 else rng.random()
+                assert eng.systematic_resample(w, u).tolist() == orc.systematic_resample(w, u).tolist(), (n, rep)
         w = np.array([0.1, 0.2, 0.3, 0.4])
         assert eng.systematic_resample(w, 0.5).tolist() == [1, 2, 3, 3]
