@@ -406,9 +406,11 @@ def rejuv_compare(args):
     out["mh_stats_proposed_accepted"] = stats_tot.tolist()
     out["fp32_batched_gibbs"] = fast["gibbs"]
     out["fp32_batched_learned"] = fast["learned"]
-    # ---- plain path: FP64, one particle at a time
-    npl = args.rj_plain
-    sub = pop.take(np.arange(npl))
+    # ---- plain path: FP64, one particle at a time, on particles spread over the
+    # population (a particle's cost per step depends on its move mix and classes)
+    npl = max(1, min(args.rj_plain, popn))
+    pick = np.linspace(0, popn - 1, npl).round().astype(np.int32)
+    sub = pop.take(pick)
     eng = Engine(T, S, R, npl, kmax, "fp64", workspaces=2)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
@@ -416,9 +418,9 @@ def rejuv_compare(args):
     t0 = time.perf_counter()
     for i in range(npl):
         eng.rejuvenate(tau, it, seed=5, first=i, count=1, invariant=True, gibbs_probability=0.0,
-                       stream_keys=np.array([i], dtype=np.int32))
+                       stream_keys=pick[i:i + 1])
     t_pl = time.perf_counter() - t0
-    steps_pl = eng.rejuvenation_steps(it, 5, np.arange(npl), invariant=True)
+    steps_pl = eng.rejuvenation_steps(it, 5, pick, invariant=True)
     s_plain = t_pl / float(steps_pl.sum())
     t0 = time.perf_counter()
     eng.gibbs_partition_move([0], tau, seed=7, num_iter=gi, dpp_alpha=1.0)
@@ -426,7 +428,7 @@ def rejuv_compare(args):
     t_gibbs = (time.perf_counter() - t0) * (S / 2.0) / gi
     eng.close()
     per_plain = s_plain * mean_steps + 0.1 * t_gibbs
-    out["plain_fp64"] = {"particles_sampled": npl, "mcmc_seconds_per_step": s_plain,
+    out["plain_fp64"] = {"particles_sampled": npl, "sample": pick.tolist(), "mcmc_seconds_per_step": s_plain,
                          "mcmc_seconds_per_particle": s_plain * mean_steps, "gibbs_seconds_per_move": t_gibbs,
                          "seconds_per_particle": per_plain}
     progress("rejuv plain: %.1f us per MH step, Gibbs move %.2f s" % (s_plain * 1e6, t_gibbs))
@@ -530,7 +532,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rj-particles", type=int, default=150, help="particles per resident lockstep batch")
     ap.add_argument("--rj-batches", type=int, default=2)
-    ap.add_argument("--rj-plain", type=int, default=6)
+    ap.add_argument("--rj-plain", type=int, default=12, help="particles of the plain FP64 path, spread over the population")
     ap.add_argument("--rj-iterations", type=int, default=16)
     ap.add_argument("--rj-temp", type=float, default=0.5)
     ap.add_argument("--rj-train-pairs", type=int, default=1_000_000)
