@@ -431,8 +431,11 @@ int smc_engine_reset_stats(smc_engine *e);
  * (one launch walks each path bottom-up with the partial in registers) when few
  * particles move in the call (see smc_tuning.chain_max_nodes);
  * SMC_PRUNE_MFMA_LEVELS launches them node level by node level instead, with the
- * same partials bit for bit. */
-enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1, SMC_PRUNE_MFMA_LEVELS = 2 };
+ * same partials bit for bit.  SMC_PRUNE_MFMA_WALK runs every changed particle of an
+ * incremental refresh in one launch: each workgroup walks one tile of a particle's
+ * dirty nodes in post-order with the partial carried in registers, the root
+ * mixture included (same partials and lnL bit for bit). */
+enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1, SMC_PRUNE_MFMA_LEVELS = 2, SMC_PRUNE_MFMA_WALK = 3 };
 int smc_engine_set_variant(smc_engine *e, int family, int variant);
 
 /* Site mixture at the root.  SMC_MIX_REFERENCE (default) is the reference's

@@ -519,6 +519,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_items, P * (T - 1));
     A(&e->d_chain_nodes, P * (T - 1));
     A(&e->d_chain_len, P * (T - 1));
+    A(&e->d_walk, P * T * 2);
+    A(&e->d_witems, P * (T + 1));  // walk pieces (at most one per node) and the roots
     A(&e->d_pcls_off, P);
     A(&e->d_prog, P * site_prog_stride(T));
     A(&e->d_plist, 2 * P);
@@ -608,7 +610,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls,
-                    e->d_chain_nodes, e->d_chain_len};
+                    e->d_chain_nodes, e->d_chain_len, e->d_walk, e->d_witems, e->d_walk_rdot, e->d_walk_rexp};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->h_read) (void)hipHostFree(e->h_read);
@@ -1295,6 +1297,14 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         moving += any ? 1 : 0;
     }
     const bool chains_fit = (int64_t)moving * ((e->tile_stride + 3) / 4) <= (int64_t)e->num_cus * 8;
+    // walks: every changed particle in one launch (see launch_prune_walk)
+    const bool use_walk = e->prune_variant == SMC_PRUNE_MFMA_WALK && e->R <= kWalkMaxRates;
+    std::vector<int4> walk;                     // two int4 per entry
+    std::vector<std::vector<WalkItem>> wlevels;  // walk pieces by launch level
+    std::vector<WalkItem> wroots;               // the pieces holding the roots
+    std::vector<int> po, wst;
+    std::vector<int> wsize(use_walk ? nn : 0), wlev(use_walk ? nn : 0), wtop(use_walk ? nn : 0);
+    std::vector<uint8_t> wcut(use_walk ? nn : 0);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
@@ -1373,7 +1383,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
-            if (use_chains) continue;
+            if (use_chains || use_walk) continue;
             const int L = dlevel[n];
             const int kind = prune_kind(ch, T, n);
             if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
@@ -1413,9 +1423,101 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         e->h_touched[p] = 0;
         for (int n = 0; n < nn && !changed; ++n) changed = nf[n] != 0;
         if (changed && e->h_owner[i] != p) e->h_owner[i] = -1;  // wave slot i held another slot's partials
-        if (changed) {
+        if (changed && use_walk) {
+            // post-order of the dirty stored nodes (cherries are built by their parents;
+            // every ancestor of a dirty node is dirty, so all are reachable from the
+            // root), the root last
+            po.clear();
+            wst.assign(1, root);
+            while (!wst.empty()) {
+                const int v = wst.back();
+                wst.pop_back();
+                po.push_back(v);
+                for (int j = 0; j < 2; ++j) {
+                    const int c = ch[2 * (v - T) + j];
+                    if (c >= T && dirty[c] && prune_kind(ch, T, c) != PRUNE_TT) wst.push_back(c);
+                }
+            }
+            std::reverse(po.begin(), po.end());
+            // Long walks (whole trees) are cut into pieces of at most kWalkPiece nodes,
+            // so that no wavefront walks a long dependent sequence: in post-order, a
+            // node whose uncut subtree grows beyond the bound cuts its largest child
+            // subtrees off as pieces of their own.  A piece runs in the launch after
+            // every piece cut below it (its level); its cut children are read back.
+            auto stored_kid = [&](int c) { return c >= T && dirty[c] && prune_kind(ch, T, c) != PRUNE_TT; };
+            for (int v : po) {
+                int sz = 1, lev = 0;
+                int kid[2], nk = 0;
+                for (int j = 0; j < 2; ++j) {
+                    const int c = ch[2 * (v - T) + j];
+                    if (stored_kid(c)) {
+                        wcut[c] = 0;
+                        kid[nk++] = c;
+                        sz += wsize[c];
+                    }
+                }
+                while (sz > kWalkPiece && nk > 0) {
+                    int b = 0;
+                    for (int k = 1; k < nk; ++k)
+                        if (wsize[kid[k]] > wsize[kid[b]]) b = k;
+                    wcut[kid[b]] = 1;
+                    sz -= wsize[kid[b]];
+                    kid[b] = kid[--nk];
+                }
+                for (int j = 0; j < 2; ++j) {
+                    const int c = ch[2 * (v - T) + j];
+                    if (stored_kid(c)) lev = std::max(lev, wlev[c] + (wcut[c] ? 1 : 0));
+                }
+                wsize[v] = sz;
+                wlev[v] = lev;
+            }
+            wcut[root] = 1;
+            // the piece of each node: its nearest cut ancestor-or-self
+            for (size_t k = po.size(); k-- > 0;) {
+                const int v = po[k];
+                wtop[v] = (wcut[v] || v == root) ? v : wtop[parent[v]];
+            }
+            const uint8_t *xfp = &e->h_xflag[(size_t)p * Ti];
+            auto pslot = [&](int c) { return (int)xfp[c - T] * Ti + (c - T); };
+            for (size_t k = 0; k < po.size(); ++k) {
+                const int t = po[k];
+                if (wtop[t] != t) continue;  // not a piece's top: pieces are emitted at their top
+                // the piece's nodes in post-order: those of po whose top is t
+                const int first = (int)(walk.size() / 2);
+                int prev = -1, cnt = 0;
+                for (size_t q = 0; q <= k; ++q) {
+                    const int v = po[q];
+                    if (wtop[v] != t) continue;
+                    int c[2] = {ch[2 * (v - T)], ch[2 * (v - T) + 1]};
+                    int m[2], aux[2];
+                    for (int j = 0; j < 2; ++j) {
+                        const int cc = c[j];
+                        if (cc < T) {
+                            m[j] = WALK_TIP;
+                            aux[j] = 0;
+                        } else if (prune_kind(ch, T, cc) == PRUNE_TT) {
+                            m[j] = WALK_CHERRY;
+                            aux[j] = ch[2 * (cc - T)] | ch[2 * (cc - T) + 1] << 16;
+                        } else {
+                            m[j] = cc == prev ? WALK_CARRIED : WALK_STORED;
+                            aux[j] = pslot(cc);
+                        }
+                    }
+                    const int a = m[1] == WALK_CARRIED ? 1 : 0, b = 1 - a;  // a carried child is child A
+                    walk.push_back(make_int4(v, c[a], c[b], m[a] | m[b] << 4 | (v == root ? 1 << 8 : 0)));
+                    walk.push_back(make_int4(v == root ? 0 : pslot(v), aux[a], aux[b], 0));
+                    prev = v;
+                    cnt++;
+                }
+                const int lev = wlev[t];
+                if ((int)wlevels.size() <= lev) wlevels.resize(lev + 1);
+                wlevels[lev].push_back(WalkItem{p, first, cnt});
+                if (t == root) wroots.push_back(WalkItem{p, first, cnt});
+            }
+        } else if (changed) {
             roots.push_back(Item{p, root});
-        } else {  // nothing to recompute: its lnL stays (its tile sums may be stale)
+        }
+        if (!changed) {  // nothing to recompute: its lnL stays (its tile sums may be stale)
             keep.push_back(p);
             keep_lnl.push_back(e->h_lnl[p]);
         }
@@ -1496,6 +1598,36 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         launch_prune_chain(a, e->d_chain_nodes, e->d_chain_len + chain_off[L], s);
     }
     run(root_off, (int)roots.size(), PRUNE_ANY, true);
+    if (!wroots.empty()) {
+        if (!e->d_walk_rdot) {
+            const size_t nr = (size_t)e->Wmax * e->tile_stride * e->R * 64;
+            rc = dev_alloc(e, &e->d_walk_rdot, nr);
+            if (rc == SMC_OK) rc = dev_alloc(e, &e->d_walk_rexp, nr);
+            if (rc != SMC_OK) return rc;
+        }
+        // item lists: the levels' pieces (longest first, so a launch does not end on
+        // one), then the roots' pieces for the mixture
+        std::vector<WalkItem> wall;
+        std::vector<int> woff{0};
+        for (auto &lv : wlevels) {
+            std::stable_sort(lv.begin(), lv.end(),
+                             [](const WalkItem &x, const WalkItem &y) { return x.count > y.count; });
+            wall.insert(wall.end(), lv.begin(), lv.end());
+            woff.push_back((int)wall.size());
+        }
+        const int roots_at = (int)wall.size();
+        wall.insert(wall.end(), wroots.begin(), wroots.end());
+        rc = h2d(e, e->d_walk, walk.data(), walk.size() * sizeof(int4));
+        if (rc == SMC_OK) rc = h2d(e, e->d_witems, wall.data(), wall.size() * sizeof(WalkItem));
+        if (rc != SMC_OK) return rc;
+        TimedLaunch tl(e, SMC_K_PRUNE);
+        for (size_t L = 0; L + 1 < woff.size(); ++L) {
+            a.n_items = woff[L + 1] - woff[L];
+            launch_prune_walk(a, e->d_witems + woff[L], e->d_walk, e->d_walk_rdot, e->d_walk_rexp, s);
+        }
+        a.n_items = (int)wroots.size();
+        launch_walk_mixture(a, e->d_witems + roots_at, e->d_walk_rdot, e->d_walk_rexp, s);
+    }
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
         launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,
@@ -1867,7 +1999,8 @@ int smc_engine_set_variant(smc_engine *e, int family, int variant)
 {
     if (!e) return SMC_ERR_ARG;
     if (family != SMC_K_PRUNE ||
-        (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU && variant != SMC_PRUNE_MFMA_LEVELS))
+        (variant != SMC_PRUNE_MFMA && variant != SMC_PRUNE_VALU && variant != SMC_PRUNE_MFMA_LEVELS &&
+         variant != SMC_PRUNE_MFMA_WALK))
         return fail(e, SMC_ERR_ARG, "unknown kernel variant");
     e->prune_variant = variant;
     return SMC_OK;

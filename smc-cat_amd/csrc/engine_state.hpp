@@ -119,6 +119,10 @@ struct smc_engine {
     Item *d_pnodes = nullptr;                   // [P * nn] node lists for P updates
     int *d_chain_nodes = nullptr;               // [P * (T-1)] dirty-path chains' nodes, bottom up
     int *d_chain_len = nullptr;                 // [P * (T-1)] chain lengths, aligned with d_items
+    int4 *d_walk = nullptr;                     // [P * T * 2] walk entries (two int4 each)
+    WalkItem *d_witems = nullptr;               // [P * (T + 1)] walk pieces, then the roots
+    double *d_walk_rdot = nullptr;              // [Wmax][tile_stride][R][64] walked roots' pi . x
+    int *d_walk_rexp = nullptr;                 // the same, exponents (both allocated on first use)
     struct Committed {
         std::vector<int32_t> children;
         std::vector<double> blen, base;

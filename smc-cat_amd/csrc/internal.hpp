@@ -36,6 +36,22 @@ struct Item {
     int node;
 };
 
+// Walks of incremental refreshes: a particle's dirty stored nodes in post-order
+// (the root last), entries [first, first + count) of the entry list.  An entry is
+// two int4: {node, child A, child B, modes} -- bits 0-2 the mode of child A, bits
+// 4-6 the mode of child B (WALK_*), bit 8 set for the root -- and {the node's
+// partial slot, aux A, aux B, 0}, where a partial slot is workspace * (T - 1) +
+// node - T and a child's aux is its partial slot when stored, tip a | tip b << 16
+// when a cherry.  A carried child (the entry just before) is always child A.
+struct WalkItem {
+    int particle;
+    int first;
+    int count;
+};
+enum { WALK_TIP = 0, WALK_CHERRY = 1, WALK_STORED = 2, WALK_CARRIED = 3 };
+constexpr int kWalkMaxRates = 8;
+constexpr int kWalkPiece = 24;  // longest walk piece (stored nodes); longer walks are cut
+
 // Host-side discretizeGamma (Miscellaneous.hpp:22-37).
 void discretize_gamma(double shape, int num, double *out);
 
@@ -143,6 +159,13 @@ void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores
 // from the bottom of the path upwards (every node after the first has the node
 // before it as its only dirty stored child).
 void launch_prune_chain(const PruneArgs &a, const int *chain_nodes, const int *chain_len, hipStream_t s);
+// walks: items [0, a.n_items) of `witems`, their entries in `entries` (R <= kWalkMaxRates);
+// rdot / rexp: the roots' per-site pi . x and exponents, [wave slots][tile_stride][R][64]
+void launch_prune_walk(const PruneArgs &a, const WalkItem *witems, const int4 *entries, double *rdot, int *rexp,
+                       hipStream_t s);
+// the root mixture of the tiles of walked roots (items [0, a.n_items)) into tile_lnl
+void launch_walk_mixture(const PruneArgs &a, const WalkItem *witems, const double *rdot, const int *rexp,
+                         hipStream_t s);
 
 // Single-site x candidate-profile scoring (Gibbs / learned allocation proposal).
 constexpr int kSiteStack = 32;  // LDS stack entries for live internal partials

@@ -26,7 +26,6 @@ namespace smccat {
 
 namespace {
 
-constexpr double kLn2 = 0.69314718055994530942;
 
 template <typename TX>
 struct PruneTypes;
@@ -187,7 +186,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             double dot = 0.0;
 #pragma unroll
             for (int i = 0; i < 20; i++) dot = fma(pi[i], (double)y0[i], dot);
-            double l = log(dot) + (double)etot * kLn2;
+            double l = rate_log_term(dot, etot);
             // running log-sum-exp over rates (rate weights are 1, Particle.cpp:661-665)
             lse_push(lse_m, lse_s, l, mix_mode);
         }

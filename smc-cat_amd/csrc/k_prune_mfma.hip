@@ -41,7 +41,6 @@ namespace smccat {
 
 namespace {
 
-constexpr double kLn2m = 0.69314718055994530942;
 
 template <typename TX>
 struct Mfma;
@@ -620,7 +619,7 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
             }
         }
         // (a block outside the tile leaves rdot = 0: its lanes hold no valid site)
-        if constexpr (IS_ROOT) lse_push(lse_m, lse_s, log(rdot) + (double)rexp * kLn2m, mix_mode);
+        if constexpr (IS_ROOT) lse_push(lse_m, lse_s, rate_log_term(rdot, rexp), mix_mode);
     }
     if constexpr (IS_ROOT) {
         double lf = 0.0;
@@ -787,6 +786,8 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
             };
 #pragma unroll
             for (int k0 = 0; k0 < 4; k0 += 2) {
+                // one step's products at a time (the steps' accumulators never live together)
+                __builtin_amdgcn_sched_barrier(0);
                 if (k0 < sb_begin || k0 >= nsb) continue;  // no valid slot in this step (wave-uniform)
                 V4 y[2][2], w[2][2];
                 int ey[2], ew[2];
@@ -851,6 +852,273 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
     }
 }
 
+// Walks (incremental refreshes: the changed particles of a call in one launch per
+// walk level).  A walk is a piece of a particle's dirty stored nodes in post-order
+// (short dirty sets are one piece with the root last; whole trees are cut into
+// pieces of at most kWalkPiece nodes, and a piece runs in the launch after the
+// pieces cut below it, which it reads back).  One workgroup of eight wavefronts
+// takes one tile of one piece: wavefront w takes the half h = w / 4 of the tile's
+// 64-slot window (blocks 2 h, 2 h + 1) and the rates w % 4, w % 4 + 4, ..., and
+// runs the whole piece, so every dependency between its nodes stays inside one
+// wavefront and no two workgroups of a launch depend on each other.  It
+// generalises the chains below: a node's result stays in registers as the B
+// operand of the next entry when that entry is its parent (child A carried); any
+// other stored child -- a committed partial, a node of an earlier piece, or a dirty
+// node this wavefront wrote earlier in the piece -- is read from memory (a
+// wavefront's own earlier stores are visible to its later loads).  A node issues
+// the loads of both children before its products; tip columns and cherry products
+// are gathered from the P matrices by code (no LDS, so occupancy is set by the
+// registers alone).  Per node the products, scaling and exponent sums are those of
+// prune_unit, so the partials equal the per-level launches' bit for bit.  The root
+// entry writes each rate's pi . x and exponent per site to a buffer, and
+// k_walk_mixture forms the tiles' sums from it (a separate launch: the mixture's
+// FP64 transcendentals would otherwise set the walk's register count).  Entries
+// carry their children's modes and partial slots (see WalkItem), so a node needs
+// no lookups before its loads.  Column mapping: block k of the window holds, at
+// lane c16, window offset 32 (k / 2) + 2 c16 + (k % 2).
+template <typename TX>
+__device__ __forceinline__ void walk_unit(const KArgs<TX> &ka, const int4 *__restrict__ entries, const WalkItem wi,
+                                          int tile, double *__restrict__ rdot, int *__restrict__ rexp)
+{
+    using V4 = typename Mfma<TX>::v4;
+    using T2 = typename Vec2<TX>::t;
+    const int T = ka.T, R = ka.R, S_pad = ka.S_pad;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int half = wave >> 2, rate0 = wave & 3;
+    const int p = wi.particle;
+    const Tile tl = ka.tiles[(size_t)p * ka.tile_stride + tile];
+    const int lane = threadIdx.x & 63;
+    const int c16 = lane & 15, g4 = lane >> 4;
+    const int nn = 2 * T - 1;
+    const size_t node_elems = (size_t)R * 20 * S_pad;
+    const size_t lp = (size_t)(p - ka.p_base);
+    TX *Xp = ka.X + lp * ka.WS * (T - 1) * node_elems;
+    int *Ep = ka.E + lp * ka.WS * (T - 1) * R * S_pad;
+    const size_t g = (size_t)p * ka.max_classes + tl.cls;
+    const TX *Pg = ka.Pm + ((size_t)ka.pcls_off[p] + tl.cls) * nn * R * kPStride;
+    const int nsb = (tl.len + 15) >> 4;
+    const int sb_begin = ((tl.lo >> 4) / 2) * 2;
+    const int k0 = 2 * half;  // this wavefront's blocks k0, k0 + 1
+    const bool active = k0 >= sb_begin && k0 < nsb;  // wave-uniform: a valid slot in the blocks
+    const int pb = tl.start + 2 * c16 + 32 * half;   // the lane's even slot (block k0), in the window
+    const bool v0 = pb - tl.start >= tl.lo && pb - tl.start < tl.len;
+    const bool v1 = pb + 1 - tl.start >= tl.lo && pb + 1 - tl.start < tl.len;
+    const bool valid[2] = {v0, v1};
+    const int slk0 = v0 ? pb : tl.start + tl.lo, slk1 = v1 ? pb + 1 : tl.start + tl.lo;
+    const uint8_t *__restrict__ pcp = ka.pcodes + (size_t)p * T * S_pad;
+    // The tip codes an entry needs at the lane's two slots, packed in one word per
+    // child: a tip's codes in bytes 0 and 1 (blocks k0, k0 + 1), a cherry's tip a in
+    // bytes 0-1 and tip b in bytes 2-3.  (Loading the next entry's codes during this
+    // one was measured: 26 more registers, 3 wavefronts per SIMD, slower.)
+    auto codes_of = [&](int mode, int c, int aux) -> uint32_t {
+        auto two = [&](int tip) {
+            const uint8_t *q = pcp + (size_t)tip * S_pad;
+            return (uint32_t)q[slk0] | (uint32_t)q[slk1] << 8;
+        };
+        if (mode == WALK_TIP) return two(c);
+        if (mode == WALK_CHERRY) return two(aux & 0xffff) | two(aux >> 16) << 16;
+        return 0u;
+    };
+    // the root's per-site pi . x and exponents: [wave slot][tile][rate][64 lanes]
+    const size_t root_base = ((size_t)lp * ka.tile_stride + tile) * R;
+    const int4 *ent = entries + 2 * (size_t)wi.first;
+
+    for (int r = rate0; r < R && active; r += 4) {
+        TX cy[2][5];  // the previous entry's partial (B layout) and exponent, per block
+        int ce[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int kb = 0; kb < 5; kb++) cy[u][kb] = (TX)0;
+        for (int j = 0; j < wi.count; j++) {
+            const int4 en = ent[2 * j], ex = ent[2 * j + 1];
+            const int cA = en.y, cB = en.z;
+            const int mA = en.w & 7, mB = (en.w >> 4) & 7;
+            const bool is_root = (en.w >> 8) & 1;
+            const int taA = ex.y & 0xffff, tbA = ex.y >> 16, taB = ex.z & 0xffff, tbB = ex.z >> 16;
+            const int oslot = ex.x, sA = mA == WALK_STORED ? ex.y : 0, sB = mB == WALK_STORED ? ex.z : 0;
+            const uint32_t kA = codes_of(mA, cA, ex.y), kB = codes_of(mB, cB, ex.z);
+            const TX *PA = Pg + ((size_t)cA * R + r) * kPStride;
+            const TX *PB = Pg + ((size_t)cB * R + r) * kPStride;
+            TX aA[2][5], aB[2][5];
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int kb = 0; kb < 5; kb++) {
+                    const int row = Mfma<TX>::arow(rb, c16);
+                    aA[rb][kb] = (mA != WALK_TIP && row < 20) ? PA[row * 21 + 4 * kb + g4] : (TX)0;
+                    aB[rb][kb] = (mB != WALK_TIP && row < 20) ? PB[row * 21 + 4 * kb + g4] : (TX)0;
+                }
+            // the children's loads before any product: B operands of stored children
+            // (child A's into cy), exponents
+            TX xb[2][5];
+            int eb[2] = {0, 0};
+            if (mA == WALK_STORED) {
+                const TX *Xc = Xp + ((size_t)sA * R + r) * 20 * S_pad;
+                stored_b2<TX>(Xc + pb, S_pad, v0, v1, g4, cy[0], cy[1]);
+                const int2v ev = *(const int2v *)(Ep + ((size_t)sA * R + r) * S_pad + pb);
+                ce[0] = ev.x;
+                ce[1] = ev.y;
+            }
+            if (mB == WALK_STORED) {
+                const TX *Xc = Xp + ((size_t)sB * R + r) * 20 * S_pad;
+                stored_b2<TX>(Xc + pb, S_pad, v0, v1, g4, xb[0], xb[1]);
+                const int2v ev = *(const int2v *)(Ep + ((size_t)sB * R + r) * S_pad + pb);
+                eb[0] = ev.x;
+                eb[1] = ev.y;
+            }
+            // one child's products over the two blocks into acc (C layout); a tip's
+            // P x is a column of its P, a cherry's B operand the product of its
+            // tips' P columns (both gathered by code from the P matrices)
+            auto child = [&](int mode, uint32_t kw, int ta, int tb, const TX(&a)[2][5], const TX *Pc,
+                             const TX(&bs)[2][5], V4(&acc)[2][2]) {
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int ca = (int)((kw >> (8 * u)) & 0xff), cb = (int)((kw >> (8 * u + 16)) & 0xff);
+                    if (mode == WALK_TIP) {
+                        tip_block<TX>(Pc, ca, g4, acc[u]);
+                    } else if (mode == WALK_CHERRY) {
+                        TX b[5];
+                        cherry_b<TX>(Pg + ((size_t)ta * R + r) * kPStride, Pg + ((size_t)tb * R + r) * kPStride,
+                                     ca, cb, valid[u], g4, b);
+                        mma_block<TX>(a, b, acc[u]);
+                    } else {  // stored or carried: the B operand is in registers
+                        mma_block<TX>(a, bs[u], acc[u]);
+                    }
+                }
+            };
+            V4 y[2][2], w[2][2];
+            child(mA, kA, taA, tbA, aA, PA, cy, y);
+            child(mB, kB, taB, tbB, aB, PB, xb, w);
+            const bool stA = mA == WALK_STORED || mA == WALK_CARRIED, stB = mB == WALK_STORED;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                int etot = (stA ? ce[u] : 0) + (stB ? eb[u] : 0);
+                TX m = 0;
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        y[u][rb][v] *= w[u][rb][v];
+                        m = fmax(m, y[u][rb][v]);
+                    }
+                m = fmax(m, __shfl_xor(m, 16, 64));
+                m = fmax(m, __shfl_xor(m, 32, 64));
+                int e;
+                (void)frexp(m, &e);
+                const bool ok = (m > (TX)0) && isfinite(m);
+                const TX nanv = (TX)NAN;
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) y[u][rb][v] = ok ? (TX)ldexp(y[u][rb][v], -e) : nanv;
+                etot += ok ? e : 0;
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) cy[u][kb] = y[u][0][kb];
+                cy[u][4] = y[u][1][0];
+                ce[u] = etot;
+            }
+            if (is_root) {
+                // (the stationary vector is read here, not kept in registers for the walk)
+                const double *pi = ka.eig + g * kEigStride + kEigPi;
+                asm volatile("" : "+v"(pi));
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    // pi . x of the block's column -- the same sum as prune_unit's root
+                    double dot = 0.0;
+#pragma unroll
+                    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int row = Mfma<TX>::crow(rb, g4, v);
+                            if (row < 20) dot = fma(pi[row], (double)y[u][rb][v], dot);
+                        }
+                    dot += __shfl_xor(dot, 16, 64);
+                    dot += __shfl_xor(dot, 32, 64);
+                    if (g4 == 0) {
+                        rdot[(root_base + r) * 64 + 16 * (k0 + u) + c16] = dot;
+                        rexp[(root_base + r) * 64 + 16 * (k0 + u) + c16] = ce[u];
+                    }
+                }
+                continue;  // the root's partial is not stored
+            }
+            // store both slots: one paired access per row where both are valid
+            TX *xo = Xp + ((size_t)oslot * R + r) * 20 * S_pad;
+            int *eo = Ep + ((size_t)oslot * R + r) * S_pad;
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = Mfma<TX>::crow(rb, g4, v);
+                    if (row >= 20) continue;
+                    TX *q = xo + (size_t)row * S_pad + pb;
+                    if (v0 && v1) {
+                        *(T2 *)q = T2{y[0][rb][v], y[1][rb][v]};
+                    } else if (v0) {
+                        q[0] = y[0][rb][v];
+                    } else if (v1) {
+                        q[1] = y[1][rb][v];
+                    }
+                }
+            if (g4 == 0) {
+                if (v0 && v1)
+                    *(int2v *)(eo + pb) = int2v{ce[0], ce[1]};
+                else if (v0)
+                    eo[pb] = ce[0];
+                else if (v1)
+                    eo[pb + 1] = ce[1];
+            }
+        }
+    }
+}
+
+// The root mixture of walked tiles (one wavefront per (walk item, tile)): lane
+// 16 k + c16 takes block k's column c16 of the tile (window offset
+// 32 (k / 2) + 2 c16 + (k % 2)), sums the rates' terms in rate order and the
+// lanes' site log-likelihoods with prune_unit's root butterfly, so tile_lnl equals
+// the root launch's bit for bit.  Blocks with no valid slot were never written by
+// the walk: their lanes are outside the tile.
+__global__ __launch_bounds__(256) void k_walk_mixture(const WalkItem *__restrict__ witems, int n,
+                                                      const Tile *__restrict__ tiles, const int *__restrict__ ntiles,
+                                                      int tile_stride, const double *__restrict__ rdot,
+                                                      const int *__restrict__ rexp, int R, int p_base, int S_pad,
+                                                      int max_classes, const int *__restrict__ perm,
+                                                      const double *__restrict__ pweight,
+                                                      const double *__restrict__ pinv, const double *__restrict__ eig,
+                                                      const int *__restrict__ invchar,
+                                                      const double *__restrict__ isinv, int mix_mode,
+                                                      double *__restrict__ tile_lnl)
+{
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (int64_t)n * tile_stride) return;  // whole wavefront
+    const int q = (int)(w / tile_stride), tile = (int)(w % tile_stride);
+    const int p = witems[q].particle;
+    if (tile >= ntiles[p]) return;  // whole wavefront
+    const Tile tl = tiles[(size_t)p * tile_stride + tile];
+    const int g4 = lane >> 4, c16 = lane & 15;
+    const int o = 32 * (g4 >> 1) + 2 * c16 + (g4 & 1);
+    const bool in = o >= tl.lo && o < tl.len;
+    const size_t base = ((size_t)(p - p_base) * tile_stride + tile) * R;
+    double lse_m = -INFINITY, lse_s = 0.0;
+    for (int r = 0; r < R; r++) {
+        const double d = in ? rdot[(base + r) * 64 + lane] : 0.0;
+        const int ez = in ? rexp[(base + r) * 64 + lane] : 0;
+        lse_push(lse_m, lse_s, rate_log_term(d, ez), mix_mode);
+    }
+    double lf = 0.0;
+    if (in) {
+        const size_t sl = (size_t)p * S_pad + tl.start + o;
+        const int orig = perm[sl];
+        const double *pi = eig + ((size_t)p * max_classes + tl.cls) * kEigStride + kEigPi;
+        lf = pweight[sl] * site_log_likelihood(lse_m, lse_s, pinv[p], pi[invchar[orig]], isinv[orig], mix_mode);
+    }
+#pragma unroll
+    for (int o2 = 32; o2 >= 1; o2 >>= 1) lf += __shfl_xor(lf, o2, 64);
+    if (lane == 0) tile_lnl[(size_t)p * tile_stride + tile] = lf;
+}
+
+
 // Persistent grid-stride form: the launch walks the units of items
 // [base, base + n), read from the device (bucket_off[bucket], [bucket + 1]) when
 // bucket_off is given, so one launch configuration serves every topology and a
@@ -890,6 +1158,26 @@ __global__ __launch_bounds__(256) void k_prune_chain(KArgs<TX> ka, const int *__
         const int l = w / X;
         const int q = (l / bpi) * X + (w - l * X);
         if (q < n) chain_unit<TX>(ka, chain_nodes, chain_len, q, l % bpi, lds);
+    }
+}
+
+// Walks, persistent grid-stride with one tile per workgroup (ka.blocks_per_item =
+// tile stride): walk items [0, n) of `witems`.
+template <typename TX>
+__global__ __launch_bounds__(512) void k_prune_walk(KArgs<TX> ka, const WalkItem *__restrict__ witems,
+                                                    const int4 *__restrict__ entries, int n,
+                                                    double *__restrict__ rdot, int *__restrict__ rexp)
+{
+    const int bpi = ka.blocks_per_item, X = ka.xcds;
+    const int n_work = (n + X - 1) / X * X * bpi;
+    for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const int l = w / X;
+        const int q = (l / bpi) * X + (w - l * X);
+        if (q >= n) continue;
+        const WalkItem wi = witems[q];
+        const int tile = l % bpi;
+        if (tile >= ka.ntiles[wi.particle]) continue;  // workgroup-uniform
+        walk_unit<TX>(ka, entries, wi, tile, rdot, rexp);
     }
 }
 
@@ -949,6 +1237,19 @@ void launch_chain_typed(const PruneArgs &a, const int *chain_nodes, const int *c
 }
 
 template <typename TX>
+void launch_walk_typed(const PruneArgs &a, const WalkItem *witems, const int4 *entries, double *rdot, int *rexp,
+                       hipStream_t s)
+{
+    KArgs<TX> ka = kargs<TX>(a);
+    ka.blocks_per_item = a.tile_stride;  // one workgroup per tile, its wavefronts per half and rate
+    const int X = ka.xcds, bpi = ka.blocks_per_item;
+    const int64_t units = (int64_t)(a.n_items + X - 1) / X * X * bpi;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid / 2 : units));
+    hipLaunchKernelGGL((k_prune_walk<TX>), dim3(grid), dim3(512), 0, s, ka, witems, entries, a.n_items, rdot,
+                       rexp);
+}
+
+template <typename TX>
 void launch_typed(const PruneArgs &a, hipStream_t s)
 {
     const int bpi = (a.tile_stride + 3) / 4;
@@ -982,6 +1283,26 @@ void launch_prune_chain(const PruneArgs &a, const int *chain_nodes, const int *c
         launch_chain_typed<float>(a, chain_nodes, chain_len, s);
     else
         launch_chain_typed<double>(a, chain_nodes, chain_len, s);
+}
+
+void launch_prune_walk(const PruneArgs &a, const WalkItem *witems, const int4 *entries, double *rdot, int *rexp,
+                       hipStream_t s)
+{
+    if (a.n_items <= 0) return;
+    if (a.fp32)
+        launch_walk_typed<float>(a, witems, entries, rdot, rexp, s);
+    else
+        launch_walk_typed<double>(a, witems, entries, rdot, rexp, s);
+}
+
+void launch_walk_mixture(const PruneArgs &a, const WalkItem *witems, const double *rdot, const int *rexp,
+                         hipStream_t s)
+{
+    if (a.n_items <= 0) return;
+    const int64_t waves = (int64_t)a.n_items * a.tile_stride;
+    hipLaunchKernelGGL(k_walk_mixture, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, witems, a.n_items,
+                       a.tiles, a.ntiles, a.tile_stride, rdot, rexp, a.R, a.p_base, a.S_pad, a.max_classes, a.perm,
+                       a.pweight, a.pinv, a.eig, a.invchar, a.isinv, a.mix_mode, a.tile_lnl);
 }
 
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s)

@@ -20,7 +20,6 @@ namespace smccat {
 
 namespace {
 
-constexpr double kLn2s = 0.69314718055994530942;
 
 // LDS written by lanes of one wavefront is visible to its other lanes after this
 __device__ __forceinline__ void wave_sync_lds()
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(64) void k_site_class(
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) d += __shfl_xor(d, off, 64);
     d += __shfl_xor(d, 32, 64);
-    if (lane == 0) rate_lnl[id] = log(d) + (double)ez * kLn2s;
+    if (lane == 0) rate_lnl[id] = rate_log_term(d, ez);
 }
 
 // Level-parallel form: one 256-lane workgroup per (entry, candidate, rate).  Each
@@ -267,7 +266,7 @@ __global__ __launch_bounds__(256) void k_site_class_lv(
 #pragma unroll
         for (int o = 16; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
         d += __shfl_xor(d, 32, 64);
-        if (tid == 0) rate_lnl[id] = log(d) + (double)ezs[root - T] * kLn2s;
+        if (tid == 0) rate_lnl[id] = rate_log_term(d, ezs[root - T]);
     }
 }
 

@@ -16,10 +16,22 @@
 #include "internal.hpp"
 #include "smccat_hip.h"
 
+// Every kernel that forms a site's mixture (the pruning roots, the walks, the
+// site-class scoring) evaluates these expressions with no multiply-add contraction,
+// so the rounding of a site's log-likelihood does not depend on the code around it
+// and the kernels agree bit for bit.
 namespace smccat {
+
+// log of one rate's site likelihood: pi . x scaled by 2^exponent
+__device__ __forceinline__ double rate_log_term(double dot, int exponent)
+{
+#pragma clang fp contract(off)
+    return log(dot) + (double)exponent * 0.69314718055994530942;
+}
 
 __device__ __forceinline__ void lse_push(double &m, double &s, double l, int mode)
 {
+#pragma clang fp contract(off)
     if (mode == SMC_MIX_LOGSPACE && !(l > -INFINITY)) return;  // -inf and NaN
     if (l > m) {
         s = s * exp(m - l) + 1.0;
@@ -32,6 +44,7 @@ __device__ __forceinline__ void lse_push(double &m, double &s, double l, int mod
 __device__ __forceinline__ double site_log_likelihood(double m, double s, double pv, double pi_c,
                                                       double is_inv, int mode)
 {
+#pragma clang fp contract(off)
     const double lsite = m + log(s);
     if (mode != SMC_MIX_LOGSPACE) return log(exp(lsite) * (1.0 - pv) + pv * pi_c * is_inv);
     const double a = lsite + log1p(-pv);

@@ -13,7 +13,7 @@ SMC_OK, SMC_ERR_ARG, SMC_ERR_HIP, SMC_ERR_NOMEM, SMC_ERR_STATE = 0, 1, 2, 3, 4
 SMC_PREC_FP64, SMC_PREC_FP32 = 0, 1
 SMC_K_EIGEN, SMC_K_PMAT, SMC_K_PRUNE, SMC_K_REDUCE, SMC_K_SMC, SMC_K_SITE = 0, 1, 2, 3, 4, 5
 KERNEL_FAMILIES = ("eigen", "pmat", "prune", "reduce", "smc", "site")
-SMC_PRUNE_MFMA, SMC_PRUNE_VALU, SMC_PRUNE_MFMA_LEVELS = 0, 1, 2
+SMC_PRUNE_MFMA, SMC_PRUNE_VALU, SMC_PRUNE_MFMA_LEVELS, SMC_PRUNE_MFMA_WALK = 0, 1, 2, 3
 SMC_MIX_REFERENCE, SMC_MIX_LOGSPACE = 0, 1
 
 # Every exported symbol and its (restype, argtypes).

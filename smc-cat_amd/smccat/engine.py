@@ -383,9 +383,10 @@ class Engine:
 
     def set_prune_variant(self, variant):
         """'mfma' (matrix cores, default: short dirty paths run as chains), 'mfma_levels'
-        (dirty nodes launched level by level) or 'valu'."""
+        (dirty nodes launched level by level), 'mfma_walk' (one launch per incremental
+        refresh: each tile of a particle's dirty nodes walked in post-order) or 'valu'."""
         v = {"mfma": _lib.SMC_PRUNE_MFMA, "valu": _lib.SMC_PRUNE_VALU,
-             "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS}[variant]
+             "mfma_levels": _lib.SMC_PRUNE_MFMA_LEVELS, "mfma_walk": _lib.SMC_PRUNE_MFMA_WALK}[variant]
         self._check(self.lib.smc_engine_set_variant(self.h, _lib.SMC_K_PRUNE, v), "smc_engine_set_variant")
 
     def proposal_tuning(self, n, using_lg=False):

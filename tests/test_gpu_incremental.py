@@ -106,7 +106,7 @@ def _oracle(codes, batch, R):
     return np.array([orc.particle_lnl(codes, batch.particle(i), R) for i in range(batch.n)])
 
 
-@pytest.mark.parametrize("variant", ["mfma", "mfma_levels", "mfma-u1", "mfma-u4", "valu"])
+@pytest.mark.parametrize("variant", ["mfma", "mfma_levels", "mfma_walk", "mfma-u1", "mfma-u4", "valu"])
 def test_incremental_moves_and_reject(gpu, variant):
     """Incremental refreshes against the oracle; the mfma-uN cases run every launch
     (per-kind, merged small buckets, roots) with N column blocks per step.  With 30
@@ -158,23 +158,27 @@ def _replace(dst, src, i):
     return out
 
 
+@pytest.mark.parametrize("variant", ["mfma", "mfma_walk"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_chains_equal_level_launches(gpu, precision):
+def test_chains_equal_level_launches(gpu, precision, variant):
     """Dirty paths run as chains (one launch, the partial carried in registers from
-    node to node) give the per-level launches' lnL bit for bit, round after round
-    of accepted and rejected moves; 120 taxa, so tree moves run as chains (one or
-    two of them per particle, SPR joining two paths) and whole-tree moves keep the
-    bucket launches; CAT classes split the tiles."""
-    _say("case chains vs level launches %s: 120 taxa x 160 sites, 14 particles" % precision)
+    node to node), or every changed particle walked in post-order in one launch,
+    give the per-level launches' lnL bit for bit, round after round of accepted and
+    rejected moves; 120 taxa, so tree moves run as chains (one or two of them per
+    particle, SPR joining two paths) and whole-tree moves keep the bucket launches
+    (walks: whole trees walked too, two stored children read back); CAT classes
+    split the tiles."""
+    _say("case %s vs level launches %s: 120 taxa x 160 sites, 16 particles" % (variant, precision))
     rng = np.random.default_rng(11)
     T, S, R = 120, 160, 4
     codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
     codes[rng.random(codes.shape) < 0.04] = 20
-    kinds = ["branch", "subtree", "nni", "spr", "spr", "shape", "rates"]
+    kinds = ["branch", "subtree", "nni", "spr", "spr", "shape", "rates", "invar"]
     n = len(kinds) * 2
     batch = sim.prior_particles(rng, n, T, S, num_rates=R, invar=True, k_range=(1, 6))
     engs = [Engine(T, S, R, n, 6, precision, workspaces=2) for _ in range(2)]
     try:
+        engs[0].set_prune_variant(variant)
         engs[1].set_prune_variant("mfma_levels")
         for eng in engs:
             eng.set_alignment(codes)
