@@ -254,6 +254,10 @@ void smc_commit_range(smc_engine *e, int first, int count)
         e->h_lnl_old[p] = e->h_lnl[p];
         e->h_dirty[p] = 0;
         e->h_pnodes[p].clear();
+        if (e->h_eig_save[p] >= 0) {
+            e->eig_save_free.push_back(e->h_eig_save[p]);
+            e->h_eig_save[p] = -1;
+        }
         smc_engine::Committed &c = e->committed[p];
         c.children.assign(e->h_children.begin() + (size_t)p * Ti * 2,
                           e->h_children.begin() + (size_t)(p + 1) * Ti * 2);
@@ -264,11 +268,24 @@ void smc_commit_range(smc_engine *e, int first, int count)
     }
 }
 
+// Drops the deferred reject restores of slots [first, first + count) (their P is
+// recomputed or replaced).
+void drop_pending(smc_engine *e, int first, int count)
+{
+    auto in = [&](int p) { return p >= first && p < first + count; };
+    e->pend_full.erase(std::remove_if(e->pend_full.begin(), e->pend_full.end(), in), e->pend_full.end());
+    e->pend_nodes.erase(std::remove_if(e->pend_nodes.begin(), e->pend_nodes.end(),
+                                       [&](const Item &it) { return in(it.particle); }),
+                        e->pend_nodes.end());
+}
+
 int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *heights, int *prog_out)
 {
     const int T = e->T, Ti = T - 1, nn = e->nn;
-    // topology checks: every non-root node has exactly one parent
-    std::vector<int> parent(nn, -1);
+    // topology checks: every non-root node has exactly one parent (scratch
+    // vectors reused across calls: this runs for every proposed topology)
+    thread_local std::vector<int> parent, h, order, stv;
+    parent.assign(nn, -1);
     for (int k = 0; k < Ti; ++k)
         for (int c = 0; c < 2; ++c) {
             int d = ch[2 * k + c];
@@ -288,10 +305,11 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
     if (nroots != 1)
         return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": not a rooted binary tree");
     // heights (tips 0); iterate in an order where children come first
-    std::vector<int> h(nn, 0), order;
-    order.reserve(Ti);
+    h.assign(nn, 0);
+    order.clear();
     {
-        std::vector<int> st{root};
+        std::vector<int> &st = stv;
+        st.assign(1, root);
         while (!st.empty()) {
             int n = st.back();
             st.pop_back();
@@ -308,10 +326,11 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
             h[n] = 1 + std::max(h[ch[2 * (n - T)]], h[ch[2 * (n - T) + 1]]);
         }
     }
-    // evaluation program for single-site scoring: post-order that descends into
-    // the child with the larger Strahler number first, so at most
-    // max-Strahler + 1 internal partials are live; checked against kSiteStack
-    {
+    // evaluation program for single-site scoring (prog_out null: not wanted now;
+    // the caller marks the program stale): post-order that descends into the
+    // child with the larger Strahler number first, so at most max-Strahler + 1
+    // internal partials are live; checked against kSiteStack
+    if (prog_out) {
         std::vector<int> st(nn, 0);
         for (int k = Ti - 1; k >= 0; --k) {
             const int n = order[k];
@@ -349,7 +368,7 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
     // the same nodes by height (counting sort, node order within a level), for the
     // level-parallel scoring kernel: level h holds the nodes whose children all
     // lie below it
-    {
+    if (prog_out) {
         int H = 0;
         for (int n = T; n < nn; ++n) H = std::max(H, h[n]);
         int *lv = prog_out + Ti, *off = prog_out + 2 * Ti + 1;
@@ -369,6 +388,61 @@ int smc_topology(smc_engine *e, int p, const int32_t *ch, int *root_out, int *he
     for (int k = 0; k < Ti; ++k) heights[k] = h[T + k];
     return SMC_OK;
 }
+
+// One packed upload of rows (segments) and one scatter launch.
+int scatter_upload(smc_engine *e, const std::vector<ScatterSeg> &segs, const std::vector<uint32_t> &data)
+{
+    if (segs.empty()) return SMC_OK;
+    // [segments | words], the words 16-byte aligned
+    const size_t seg_bytes = (segs.size() * sizeof(ScatterSeg) + 15) & ~(size_t)15;
+    const size_t bytes = seg_bytes + data.size() * sizeof(uint32_t);
+    if ((int64_t)bytes > e->scatter_cap) {
+        if (e->d_scatter) {  // an earlier scatter may still read it
+            int rc = sync_stream(e);
+            if (rc != SMC_OK) return rc;
+            (void)hipFree(e->d_scatter);
+        }
+        e->d_scatter = nullptr;
+        e->scatter_cap = 0;
+        HIP_TRY(e, hipMalloc((void **)&e->d_scatter, bytes * 2));
+        e->scatter_cap = (int64_t)bytes * 2;
+    }
+    std::vector<uint8_t> packed(bytes, 0);
+    std::memcpy(packed.data(), segs.data(), segs.size() * sizeof(ScatterSeg));
+    std::memcpy(packed.data() + seg_bytes, data.data(), data.size() * sizeof(uint32_t));
+    int rc = h2d(e, e->d_scatter, packed.data(), bytes);
+    if (rc != SMC_OK) return rc;
+    launch_scatter_segments(reinterpret_cast<const uint32_t *>(e->d_scatter + seg_bytes),
+                            reinterpret_cast<const ScatterSeg *>(e->d_scatter), (int)segs.size(), e->stream);
+    HIP_TRY(e, hipGetLastError());
+    return SMC_OK;
+}
+
+// Scoring programs of the particles whose topology changed since theirs was
+// derived (parameter uploads leave them stale: MCMC rounds do not score sites).
+int ensure_programs(smc_engine *e)
+{
+    if (e->n_prog_stale == 0) return SMC_OK;
+    const int T = e->T, Ti = T - 1, ps = site_prog_stride(T);
+    std::vector<ScatterSeg> segs;
+    std::vector<uint32_t> data;
+    std::vector<int> hh(Ti);
+    for (int p = 0; p < e->Pmax; ++p) {
+        if (!e->h_prog_stale[p]) continue;
+        int root = -1;
+        int *prog = &e->h_prog[(size_t)p * ps];
+        int rc = smc_topology(e, p, &e->h_children[(size_t)p * Ti * 2], &root, hh.data(), prog);
+        if (rc != SMC_OK) return rc;
+        const size_t o = data.size();
+        segs.push_back(ScatterSeg{(uint64_t)(uintptr_t)(e->d_prog + (size_t)p * ps), (uint32_t)o, (uint32_t)ps});
+        data.resize(o + ps);
+        std::memcpy(data.data() + o, prog, (size_t)ps * sizeof(int));
+        e->h_prog_stale[p] = 0;
+    }
+    e->n_prog_stale = 0;
+    return scatter_upload(e, segs, data);
+}
+
 
 int ensure_site_class_buffers(smc_engine *e, int count, int64_t need)
 {
@@ -398,6 +472,10 @@ int ensure_site_class_buffers(smc_engine *e, int count, int64_t need)
 int issue_site_class(smc_engine *e, int count, int n_cand, bool with_cls, bool decompose, double tempering,
                      double *out)
 {
+    {
+        int rc = ensure_programs(e);  // the kernels walk the trees' scoring programs
+        if (rc != SMC_OK) return rc;
+    }
     hipStream_t s = e->stream;
     if (decompose) {  // candidates coded -1 (or every candidate without codes)
         TimedLaunch tl(e, SMC_K_EIGEN);
@@ -591,6 +669,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_lnl.assign(P, 0.0);
     e->h_lnl_old.assign(P, 0.0);
     e->h_dirty.assign(P, 0);
+    e->h_eig_save.assign(P, -1);
+    e->h_prog_stale.assign(P, 0);
     e->h_touched.assign(P, 0);
     e->h_pnodes.assign(P, {});
     e->committed.resize(P);
@@ -610,7 +690,8 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_nonfinite, e->d_items, e->d_w,    e->d_l,        e->d_norm,
                     e->d_out2,   e->d_assign, e->d_pcls_off, e->d_prog, e->d_sc_ent, e->d_xflag, e->d_plist,
                     e->d_sc_logits, e->d_sc_eig, e->d_sc_rate, e->d_sc_out, e->d_sc_cls,
-                    e->d_chain_nodes, e->d_chain_len, e->d_walk, e->d_witems, e->d_walk_rdot, e->d_walk_rexp};
+                    e->d_chain_nodes, e->d_chain_len, e->d_walk, e->d_witems, e->d_walk_rdot, e->d_walk_rexp,
+                    e->d_eig_save};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->h_read) (void)hipHostFree(e->h_read);
@@ -618,6 +699,7 @@ int smc_engine_destroy(smc_engine *e)
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
     if (e->d_gather) (void)hipFree(e->d_gather);
+    if (e->d_scatter) (void)hipFree(e->d_scatter);
     if (e->d_split_ws) (void)hipFree(e->d_split_ws);
     for (void *b : {(void *)e->pool.d_eig, (void *)e->pool.d_logits, (void *)e->pool.d_ent})
         if (b) (void)hipFree(b);
@@ -761,6 +843,10 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
             int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * site_prog_stride(T));
             if (rc != SMC_OK) return rc;
             for (int k = 0; k < Ti; ++k) h[T + k] = hh[k];
+            if (e->h_prog_stale[p]) {
+                e->h_prog_stale[p] = 0;
+                e->n_prog_stale--;
+            }
         }
         for (int k = 0; k < Ti; ++k) {
             e->h_height[(size_t)p * Ti + k] = h[T + k];
@@ -888,6 +974,12 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     if (e->have_alignment) launch_permute_codes(e->d_codes, e->d_perm, e->d_pcodes, T, Sp, first, count, s);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipStreamSynchronize(s));
+    drop_pending(e, first, count);  // new states: nothing of the old ones is restored
+    for (int p = first; p < first + count; ++p)
+        if (e->h_eig_save[p] >= 0) {
+            e->eig_save_free.push_back(e->h_eig_save[p]);
+            e->h_eig_save[p] = -1;
+        }
     e->class_version++;
     e->topo_version++;
     return SMC_OK;
@@ -1102,6 +1194,7 @@ int smc_refresh_likelihood(smc_engine *e, int first, int count, double *lnl_out,
         for (int k = 0; k < w.count; ++k) e->h_owner[k] = w.first + k;
     if (lnl_out) std::copy(e->h_lnl.begin() + first, e->h_lnl.begin() + first + count, lnl_out);
     smc_commit_range(e, first, count);
+    drop_pending(e, first, count);  // every P of the range was recomputed
     return SMC_OK;
 }
 
@@ -1185,61 +1278,76 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
 }
 
 // Uploads new topology/branch lengths/exchangeabilities/pInvar/shape for slots
-// [first, first+count) without touching their CAT classes.
+// [first, first+count) without touching their CAT classes.  Only the rows that
+// differ from the uploaded state go up: packed into one copy and scattered on the
+// device by one launch (a move changes one or two of a particle's rows; finished
+// chains of a lockstep batch change none).
 int upload_params(smc_engine *e, int first, int count, const int32_t *children,
                   const double *branch_len, const double *base, const double *pinv,
                   const double *shape)
 {
     const int T = e->T, Ti = T - 1, nn = e->nn, R = e->R;
-    std::vector<double> blen((size_t)count * nn), rates((size_t)count * R, 1.0);
-    std::vector<int> progs((size_t)count * site_prog_stride(T)), roots(count);
+    std::vector<ScatterSeg> segs;
+    std::vector<uint32_t> data;
+    auto add = [&](const void *dst, const void *src, size_t bytes) {
+        const size_t o = data.size();
+        segs.push_back(ScatterSeg{(uint64_t)(uintptr_t)dst, (uint32_t)o, (uint32_t)(bytes / 4)});
+        data.resize(o + bytes / 4);
+        std::memcpy(data.data() + o, src, bytes);
+    };
+    std::vector<double> bl(nn);
+    std::vector<int> hh(Ti);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
         const int32_t *ch = children + (size_t)i * Ti * 2;
-        const int ps = site_prog_stride(T);
-        int *prog = progs.data() + (size_t)i * ps;
-        int root = -1;
-        if (std::equal(ch, ch + Ti * 2, &e->h_children[(size_t)p * Ti * 2])) {
-            // same topology (branch-length, rate and invariant moves, restores of
-            // them): its root, heights and programs are the ones already derived
-            root = e->h_root[p];
-            std::copy_n(&e->h_prog[(size_t)p * ps], ps, prog);
-        } else {
-            std::vector<int> hh(Ti);
-            int rc = smc_topology(e, p, ch, &root, hh.data(), prog);
+        int *mch = &e->h_children[(size_t)p * Ti * 2];
+        if (!std::equal(ch, ch + Ti * 2, mch)) {
+            // a new topology: its root and heights (unchanged topologies --
+            // branch-length, rate and invariant moves, restores of them -- keep the
+            // ones already derived); its scoring program is derived when a site-class
+            // kernel or a particle copy next needs it (ensure_programs)
+            int root = -1;
+            int rc = smc_topology(e, p, ch, &root, hh.data(), nullptr);
             if (rc != SMC_OK) return rc;
             std::copy(hh.begin(), hh.end(), &e->h_height[(size_t)p * Ti]);
-            std::copy_n(prog, ps, &e->h_prog[(size_t)p * ps]);
-            std::copy(ch, ch + Ti * 2, &e->h_children[(size_t)p * Ti * 2]);
+            std::copy(ch, ch + Ti * 2, mch);
+            e->h_root[p] = root;
+            if (!e->h_prog_stale[p]) {
+                e->h_prog_stale[p] = 1;
+                e->n_prog_stale++;
+            }
+            add(e->d_children + (size_t)p * Ti * 2, mch, (size_t)Ti * 2 * sizeof(int));
+            add(e->d_root + p, &e->h_root[p], sizeof(int));
+        }
+        const int root = e->h_root[p];
+        for (int n = 0; n < nn; ++n) bl[n] = n == root ? 0.0 : branch_len[(size_t)i * nn + n];
+        double *mbl = &e->h_blen[(size_t)p * nn];
+        if (!std::equal(bl.begin(), bl.end(), mbl)) {
+            std::copy(bl.begin(), bl.end(), mbl);
+            add(e->d_blen + (size_t)p * nn, mbl, (size_t)nn * sizeof(double));
+        }
+        const double *bs = base + (size_t)i * 190;
+        double *mbs = &e->h_base[(size_t)p * 190];
+        if (!std::equal(bs, bs + 190, mbs)) {
+            std::copy(bs, bs + 190, mbs);
+            add(e->d_base + (size_t)p * 190, mbs, 190 * sizeof(double));
+        }
+        if (pinv[i] != e->h_pinv[p]) {
+            e->h_touched[p] = 1;
+            e->h_pinv[p] = pinv[i];
+            add(e->d_pinv + p, &e->h_pinv[p], sizeof(double));
         }
         if (R > 1) {
             if (!(shape[i] > 0.0)) return fail(e, SMC_ERR_ARG, "shape must be > 0");
-            if (shape[i] != e->h_shape[p])  // rates of an unchanged shape are kept
+            if (shape[i] != e->h_shape[p]) {  // rates of an unchanged shape are kept
                 discretize_gamma(shape[i], R, &e->h_rates[(size_t)p * R]);
-            std::copy_n(&e->h_rates[(size_t)p * R], R, &rates[(size_t)i * R]);
+                add(e->d_rates + (size_t)p * R, &e->h_rates[(size_t)p * R], (size_t)R * sizeof(double));
+            }
         }
-        e->h_root[p] = root;
-        roots[i] = root;
-        for (int n = 0; n < nn; ++n) blen[(size_t)i * nn + n] = n == root ? 0.0 : branch_len[(size_t)i * nn + n];
-        std::copy(blen.begin() + (size_t)i * nn, blen.begin() + (size_t)(i + 1) * nn,
-                  e->h_blen.begin() + (size_t)p * nn);
-        std::copy(base + (size_t)i * 190, base + (size_t)(i + 1) * 190, e->h_base.begin() + (size_t)p * 190);
-        if (pinv[i] != e->h_pinv[p]) e->h_touched[p] = 1;
-        e->h_pinv[p] = pinv[i];
         e->h_shape[p] = shape[i];
     }
-    const size_t P0 = first;
-    int rc = h2d(e, e->d_children + P0 * Ti * 2, e->h_children.data() + P0 * Ti * 2,
-                 (size_t)count * Ti * 2 * sizeof(int));
-    if (rc == SMC_OK) rc = h2d(e, e->d_blen + P0 * nn, blen.data(), blen.size() * sizeof(double));
-    if (rc == SMC_OK)
-        rc = h2d(e, e->d_base + P0 * 190, e->h_base.data() + P0 * 190, (size_t)count * 190 * sizeof(double));
-    if (rc == SMC_OK) rc = h2d(e, e->d_pinv + P0, e->h_pinv.data() + P0, (size_t)count * sizeof(double));
-    if (rc == SMC_OK) rc = h2d(e, e->d_rates + P0 * R, rates.data(), rates.size() * sizeof(double));
-    if (rc == SMC_OK) rc = h2d(e, e->d_root + P0, roots.data(), roots.size() * sizeof(int));
-    if (rc == SMC_OK)
-        rc = h2d(e, e->d_prog + P0 * site_prog_stride(T), progs.data(), progs.size() * sizeof(int));
+    int rc = scatter_upload(e, segs, data);
     if (rc != SMC_OK) return rc;
     e->topo_version++;
     return SMC_OK;
@@ -1522,6 +1630,51 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             keep_lnl.push_back(e->h_lnl[p]);
         }
     }
+    // committed eigensystems of particles about to be decomposed again: kept for a
+    // reject (a particle without a free save slot is decomposed again on a reject)
+    if (!eig_list.empty() && !e->d_eig_save) {
+        rc = dev_alloc(e, &e->d_eig_save, (size_t)e->Wmax * e->Kmax * kEigStride);
+        if (rc != SMC_OK) return rc;
+        for (int k = e->Wmax; k-- > 0;) e->eig_save_free.push_back(k);
+    }
+    for (int p : eig_list)
+        if (e->h_eig_save[p] < 0 && !e->eig_save_free.empty()) {
+            const int slot = e->eig_save_free.back();
+            e->eig_save_free.pop_back();
+            HIP_TRY(e, hipMemcpyAsync(e->d_eig_save + (size_t)slot * e->Kmax * kEigStride,
+                                      e->d_eig + (size_t)p * e->Kmax * kEigStride,
+                                      (size_t)e->h_nclass[p] * kEigStride * sizeof(double), hipMemcpyDeviceToDevice,
+                                      e->stream));
+            e->h_eig_save[p] = slot;
+        }
+    // deferred reject restores of this range join this refresh's recomputes (none
+    // where the particle recomputes every P anyway)
+    if (!e->pend_full.empty() || !e->pend_nodes.empty()) {
+        std::vector<uint8_t> full(count, 0);
+        for (int p : p_list) full[p - first] = 1;
+        auto in = [&](int p) { return p >= first && p < first + count; };
+        std::vector<int> keep_full;
+        for (int p : e->pend_full) {
+            if (!in(p)) {
+                keep_full.push_back(p);
+            } else if (!full[p - first]) {
+                full[p - first] = 1;
+                p_list.push_back(p);
+            }
+        }
+        std::vector<Item> keep_nodes;
+        for (const Item &it : e->pend_nodes) {
+            if (!in(it.particle))
+                keep_nodes.push_back(it);
+            else if (!full[it.particle - first])
+                p_nodes.push_back(it);
+        }
+        e->pend_full.swap(keep_full);
+        e->pend_nodes.swap(keep_nodes);
+        // the node lists go up grouped by particle
+        std::stable_sort(p_nodes.begin(), p_nodes.end(),
+                         [](const Item &x, const Item &y) { return x.particle < y.particle; });
+    }
     rc = recompute_models(e, first, eig_list, p_list, p_nodes);
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
@@ -1667,10 +1820,20 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
                   e->h_xflag.begin() + (size_t)p * Ti);
         e->h_lnl[p] = e->h_lnl_old[p];
         if (e->h_dirty[p] & 2) {
-            eig_list.push_back(p);
-            p_list.push_back(p);
+            if (e->h_eig_save[p] >= 0) {  // the committed eigensystems come back by copy
+                HIP_TRY(e, hipMemcpyAsync(e->d_eig + (size_t)p * e->Kmax * kEigStride,
+                                          e->d_eig_save + (size_t)e->h_eig_save[p] * e->Kmax * kEigStride,
+                                          (size_t)e->h_nclass[p] * kEigStride * sizeof(double),
+                                          hipMemcpyDeviceToDevice, e->stream));
+                e->eig_save_free.push_back(e->h_eig_save[p]);
+                e->h_eig_save[p] = -1;
+                e->pend_full.push_back(p);
+            } else {
+                eig_list.push_back(p);
+                p_list.push_back(p);
+            }
         } else if (e->h_dirty[p] & 1) {
-            for (int n : e->h_pnodes[p]) p_nodes.push_back(Item{p, n});
+            for (int n : e->h_pnodes[p]) e->pend_nodes.push_back(Item{p, n});
         }
     }
     hipStream_t s = e->stream;
@@ -1733,6 +1896,10 @@ void gather_host(V &v, size_t row, const std::vector<int> &dst, const std::vecto
 
 int smc_copy_particles(smc_engine *e, int n, const int32_t *parents)
 {
+    if (e) {
+        int rc = ensure_programs(e);  // programs are copied with the other rows
+        if (rc != SMC_OK) return rc;
+    }
     if (!e) return SMC_ERR_ARG;
     if (n < 1 || n > e->Pmax || !parents) return fail(e, SMC_ERR_ARG, "invalid particle copy arguments");
     for (int i = 0; i < n; ++i) {
@@ -1849,6 +2016,7 @@ int smc_refresh_stale(smc_engine *e, int first, int count)
     for (int i = 0; i < count; ++i) all[i] = first + i;
     rc = recompute_models(e, first, all, all);
     if (rc != SMC_OK) return rc;
+    drop_pending(e, first, count);
     rc = smc_refresh_incremental(e, first, count, flags.data(), nullptr, nullptr, nullptr);
     if (rc != SMC_OK) return rc;
     smc_commit_range(e, first, count);

@@ -89,6 +89,8 @@ struct smc_engine {
     std::vector<int> h_root;      // [P]
     std::vector<int> h_height;    // [P][T-1]
     std::vector<int> h_prog;      // [P][3T] host copy of the scoring programs
+    std::vector<uint8_t> h_prog_stale;  // [P] topology changed since its program was derived
+    int64_t n_prog_stale = 0;
     std::vector<int> h_valid;     // [P] particle uploaded
     std::vector<int> h_nclass;    // [P] classes per particle
     // host copies of the uploaded particle state (for moves that edit it)
@@ -109,10 +111,22 @@ struct smc_engine {
     std::vector<uint8_t> h_dirty;               // [P] bit0 P changed, bit1 eigen changed
     std::vector<std::vector<int>> h_pnodes;     // [P] nodes whose P changed since the commit
     std::vector<uint8_t> h_touched;             // [P] pInvar changed by the last parameter upload
+    // Rejects restore P lazily: the recomputes are merged into the next incremental
+    // refresh of the particle (one launch with that refresh's own, and none where
+    // the particle recomputes every P anyway), or dropped by a full refresh.
+    std::vector<int> pend_full;                 // particles whose every P is recomputed
+    std::vector<Item> pend_nodes;               // (particle, node) P recomputes
+    // Eigensystems of a particle before its first uncommitted recompute, copied
+    // back on a reject instead of decomposed again: [Wmax * Kmax][840] by slot
+    double *d_eig_save = nullptr;
+    std::vector<int> h_eig_save;                // [P] save slot holding the committed eigensystems, or -1
+    std::vector<int> eig_save_free;             // free save slots
     // [Wmax] the particle whose committed partials fill wave slot w (-1: none): a
     // particle p refreshed with p_base b is resident while h_owner[p - b] == p
     std::vector<int> h_owner;
     uint8_t *d_gather = nullptr;                 // row-gather staging of smc_copy_particles
+    uint8_t *d_scatter = nullptr;                // packed parameter rows of upload_params (grown on demand)
+    int64_t scatter_cap = 0;
     int64_t gather_cap = 0;
     uint8_t *d_split_ws = nullptr;               // split-posterior workspace (grown on demand)
     size_t split_ws_bytes = 0;
@@ -311,6 +325,9 @@ int smc_gibbs_move(smc_engine *e, int count, const int32_t *slots, double temper
 // Commits the current state of slots [first, first+count): workspace flags,
 // lnL and parameters become the ones a rejected proposal returns to.
 void smc_commit_range(smc_engine *e, int first, int count);
+void drop_pending(smc_engine *e, int first, int count);
+int ensure_programs(smc_engine *e);
+int scatter_upload(smc_engine *e, const std::vector<ScatterSeg> &segs, const std::vector<uint32_t> &data);
 
 // refreshLikelihood(true) of slots [first, first+count) (one resident wave, two
 // workspaces) that recomputes the partials only of slots whose committed partials

@@ -224,6 +224,14 @@ void launch_permute_codes(const uint8_t *codes, const int *perm, uint8_t *pcodes
                           int count, hipStream_t s);
 // dst row dst_rows[i] (i when null) <- src row src_rows[i] (i when null), rows of
 // row_bytes bytes, i < n (buffers aligned to the largest of 16, 4, 1 dividing row_bytes).
+// A row of a packed upload: `words` 32-bit words from word `src_word` of the
+// packed buffer to the device address `dst`.
+struct ScatterSeg {
+    uint64_t dst;
+    uint32_t src_word;
+    uint32_t words;
+};
+void launch_scatter_segments(const uint32_t *src, const ScatterSeg *segs, int n, hipStream_t s);
 void launch_gather_rows(const void *src, void *dst, size_t row_bytes, const int *src_rows, const int *dst_rows,
                         int n, hipStream_t s);
 

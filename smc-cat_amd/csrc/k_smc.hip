@@ -202,6 +202,18 @@ __global__ __launch_bounds__(256) void k_gather_rows(const W *__restrict__ src, 
     for (size_t i = threadIdx.x; i < row_elems; i += blockDim.x) d[i] = s[i];
 }
 
+// Scatter of packed rows: segment k copies segs[k].words 32-bit words from
+// src + segs[k].src_word to the device address segs[k].dst.  One workgroup per
+// segment.
+__global__ __launch_bounds__(256) void k_scatter_segments(const uint32_t *__restrict__ src,
+                                                          const ScatterSeg *__restrict__ segs)
+{
+    const ScatterSeg sg = segs[blockIdx.x];
+    uint32_t *d = reinterpret_cast<uint32_t *>(sg.dst);
+    const uint32_t *sp = src + sg.src_word;
+    for (uint32_t i = threadIdx.x; i < sg.words; i += blockDim.x) d[i] = sp[i];
+}
+
 // Tip codes in each particle's class-sorted slot order: pcodes[p][t][slot] =
 // min(codes[t][perm[p][slot]], 20) for particles [first, first + count).  One
 // workgroup per (particle, tip) row.
@@ -240,6 +252,12 @@ void launch_gather_rows(const void *src, void *dst, size_t row_bytes, const int 
     else
         hipLaunchKernelGGL(k_gather_rows<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t *)src, (uint8_t *)dst,
                            row_bytes, src_rows, dst_rows);
+}
+
+void launch_scatter_segments(const uint32_t *src, const ScatterSeg *segs, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_segments, dim3(n), dim3(256), 0, s, src, segs);
 }
 
 void launch_next_step(int n, double *w, const double *l, double *norm, double *out2,
