@@ -265,6 +265,8 @@ void smc_commit_range(smc_engine *e, int first, int count)
         c.base.assign(e->h_base.begin() + (size_t)p * 190, e->h_base.begin() + (size_t)(p + 1) * 190);
         c.pinv = e->h_pinv[p];
         c.shape = e->h_shape[p];
+        c.root = e->h_root[p];
+        c.heights.assign(e->h_height.begin() + (size_t)p * Ti, e->h_height.begin() + (size_t)(p + 1) * Ti);
     }
 }
 
@@ -1282,9 +1284,11 @@ int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
 // differ from the uploaded state go up: packed into one copy and scattered on the
 // device by one launch (a move changes one or two of a particle's rows; finished
 // chains of a lockstep batch change none).
+// `slots` (optional): row i is slot slots[i] instead of first + i; `committed`:
+// the rows are the slots' committed state, whose topologies need no derivation.
 int upload_params(smc_engine *e, int first, int count, const int32_t *children,
                   const double *branch_len, const double *base, const double *pinv,
-                  const double *shape)
+                  const double *shape, const int *slots = nullptr, bool committed = false)
 {
     const int T = e->T, Ti = T - 1, nn = e->nn, R = e->R;
     std::vector<ScatterSeg> segs;
@@ -1298,11 +1302,23 @@ int upload_params(smc_engine *e, int first, int count, const int32_t *children,
     std::vector<double> bl(nn);
     std::vector<int> hh(Ti);
     for (int i = 0; i < count; ++i) {
-        const int p = first + i;
+        const int p = slots ? slots[i] : first + i;
         if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
         const int32_t *ch = children + (size_t)i * Ti * 2;
         int *mch = &e->h_children[(size_t)p * Ti * 2];
-        if (!std::equal(ch, ch + Ti * 2, mch)) {
+        if (!std::equal(ch, ch + Ti * 2, mch) && committed && e->committed[p].root >= 0) {
+            // back to the committed topology: its root and heights as committed
+            const smc_engine::Committed &c = e->committed[p];
+            std::copy(c.heights.begin(), c.heights.end(), &e->h_height[(size_t)p * Ti]);
+            std::copy(ch, ch + Ti * 2, mch);
+            e->h_root[p] = c.root;
+            if (!e->h_prog_stale[p]) {
+                e->h_prog_stale[p] = 1;
+                e->n_prog_stale++;
+            }
+            add(e->d_children + (size_t)p * Ti * 2, mch, (size_t)Ti * 2 * sizeof(int));
+            add(e->d_root + p, &e->h_root[p], sizeof(int));
+        } else if (!std::equal(ch, ch + Ti * 2, mch)) {
             // a new topology: its root and heights (unchanged topologies --
             // branch-length, rate and invariant moves, restores of them -- keep the
             // ones already derived); its scoring program is derived when a site-class
@@ -1406,13 +1422,14 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     }
     const bool chains_fit = (int64_t)moving * ((e->tile_stride + 3) / 4) <= (int64_t)e->num_cus * 8;
     // walks: every changed particle in one launch (see launch_prune_walk)
-    const bool use_walk = e->prune_variant == SMC_PRUNE_MFMA_WALK && e->R <= kWalkMaxRates;
+    const bool walk_ok = e->R <= kWalkMaxRates;
+    const bool use_walk = e->prune_variant == SMC_PRUNE_MFMA_WALK && walk_ok;
     std::vector<int4> walk;                     // two int4 per entry
     std::vector<std::vector<WalkItem>> wlevels;  // walk pieces by launch level
     std::vector<WalkItem> wroots;               // the pieces holding the roots
     std::vector<int> po, wst;
-    std::vector<int> wsize(use_walk ? nn : 0), wlev(use_walk ? nn : 0), wtop(use_walk ? nn : 0);
-    std::vector<uint8_t> wcut(use_walk ? nn : 0);
+    std::vector<int> wsize(nn), wlev(nn), wtop(nn);
+    std::vector<uint8_t> wcut(nn);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
@@ -1488,10 +1505,13 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         for (int n = T; n < nn; ++n)
             if (dirty[n] && n != root && prune_kind(ch, T, n) != PRUNE_TT) n_stored++;
         const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && chains_fit && n_stored <= e->tune.chain_max;
+        // (walking only the large dirty sets -- whole trees -- and launching the rest
+        // level by level was measured: 8% slower than either alone at the C4 shape)
+        const bool walk_p = use_walk;
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
-            if (use_chains || use_walk) continue;
+            if (use_chains || walk_p) continue;
             const int L = dlevel[n];
             const int kind = prune_kind(ch, T, n);
             if ((int)buckets.size() < PRUNE_KINDS * (L + 1)) buckets.resize(PRUNE_KINDS * (L + 1));
@@ -1531,7 +1551,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         e->h_touched[p] = 0;
         for (int n = 0; n < nn && !changed; ++n) changed = nf[n] != 0;
         if (changed && e->h_owner[i] != p) e->h_owner[i] = -1;  // wave slot i held another slot's partials
-        if (changed && use_walk) {
+        if (changed && walk_p) {
             // post-order of the dirty stored nodes (cherries are built by their parents;
             // every ancestor of a dirty node is dirty, so all are reachable from the
             // root), the root last
@@ -1840,18 +1860,18 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
     // restore the committed parameters of rejected particles: one upload of the
     // span they cover (accepted particles in it are committed already)
     if (!rej.empty()) {
-        const int r0 = rej.front(), rn = rej.back() - rej.front() + 1;
+        const int rn = (int)rej.size();
         std::vector<int32_t> ch((size_t)rn * Ti * 2);
         std::vector<double> bl((size_t)rn * nn), base((size_t)rn * 190), pv(rn), sh(rn);
         for (int q = 0; q < rn; ++q) {
-            const smc_engine::Committed &c = e->committed[r0 + q];
+            const smc_engine::Committed &c = e->committed[rej[q]];
             std::copy(c.children.begin(), c.children.end(), ch.begin() + (size_t)q * Ti * 2);
             std::copy(c.blen.begin(), c.blen.end(), bl.begin() + (size_t)q * nn);
             std::copy(c.base.begin(), c.base.end(), base.begin() + (size_t)q * 190);
             pv[q] = c.pinv;
             sh[q] = c.shape;
         }
-        int rc = upload_params(e, r0, rn, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
+        int rc = upload_params(e, 0, rn, ch.data(), bl.data(), base.data(), pv.data(), sh.data(), rej.data(), true);
         if (rc != SMC_OK) return rc;
     }
     if (!p_list.empty() || !p_nodes.empty()) {

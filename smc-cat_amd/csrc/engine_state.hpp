@@ -141,6 +141,8 @@ struct smc_engine {
         std::vector<int32_t> children;
         std::vector<double> blen, base;
         double pinv = 0.0, shape = 1.0;
+        int root = -1;             // derived from children: a restore needs no re-derivation
+        std::vector<int> heights;  // [T-1]
     };
     std::vector<Committed> committed;           // [P] parameters at the last commit
     // site x candidate scoring scratch (grown on demand)

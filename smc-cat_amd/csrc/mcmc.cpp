@@ -225,35 +225,55 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         left[i] = 0;
         next_move(i);
     }
+    // the rows handed to each round: a particle's committed state, or its proposal
+    // while it moves (rows of particles that do not move are left as they are, and
+    // the parameter upload sends only rows that changed)
+    auto put_committed = [&](int i) {
+        const smc_engine::Committed &c = e->committed[first + i];
+        std::copy(c.children.begin(), c.children.end(), ch.begin() + (size_t)i * Ti * 2);
+        std::copy(c.blen.begin(), c.blen.end(), bl.begin() + (size_t)i * nn);
+        std::copy(c.base.begin(), c.base.end(), base.begin() + (size_t)i * 190);
+        pv[i] = c.pinv;
+        sh[i] = c.shape;
+    };
+    for (int i = 0; i < count; ++i) put_committed(i);
+    Params q;  // one proposal at a time (its vectors keep their capacity)
+    std::vector<uint8_t> f(nn);
     for (;;) {
         bool active = false;
         for (int i = 0; i < count; ++i) active |= left[i] > 0;
         if (!active) break;
         {
             std::optional<PhaseTimer> propose_timer(std::in_place, e, SMC_PHASE_PROPOSE);
+            std::fill(flags.begin(), flags.end(), 0);
             for (int i = 0; i < count; ++i) {
                 const int p = first + i;
-                Params q = load(p);
-                std::vector<uint8_t> f(nn, 0);
                 eig[i] = 0;
                 hastings[i] = 0.0;
-                if (left[i] > 0) {
-                    Rng &g = rng[i];
-                    switch (move[i]) {
-                        case TREE:
-                            hastings[i] = tree_move(g, q, T, f, o);
-                            break;
-                        case SHAPE:
-                            hastings[i] = shape_move(g, q, f, o.shape_delta);
-                            break;
-                        case INVAR:
-                            hastings[i] = invar_move(g, q, o.invar_alpha);
-                            break;
-                        case RATES:  // eigensystems recomputed (Q13 fixed)
-                            hastings[i] = rate_matrix_move(g, q, o);
-                            eig[i] = 1;
-                            break;
-                    }
+                new_prior[i] = prior[i];
+                if (left[i] <= 0) continue;
+                const smc_engine::Committed &c = e->committed[p];
+                q.ch.assign(c.children.begin(), c.children.end());
+                q.bl.assign(c.blen.begin(), c.blen.end());
+                q.base.assign(c.base.begin(), c.base.end());
+                q.pinv = c.pinv;
+                q.shape = c.shape;
+                std::fill(f.begin(), f.end(), 0);
+                Rng &g = rng[i];
+                switch (move[i]) {
+                    case TREE:
+                        hastings[i] = tree_move(g, q, T, f, o);
+                        break;
+                    case SHAPE:
+                        hastings[i] = shape_move(g, q, f, o.shape_delta);
+                        break;
+                    case INVAR:
+                        hastings[i] = invar_move(g, q, o.invar_alpha);
+                        break;
+                    case RATES:  // eigensystems recomputed (Q13 fixed)
+                        hastings[i] = rate_matrix_move(g, q, o);
+                        eig[i] = 1;
+                        break;
                 }
                 std::copy(q.ch.begin(), q.ch.end(), ch.begin() + (size_t)i * Ti * 2);
                 std::copy(q.bl.begin(), q.bl.end(), bl.begin() + (size_t)i * nn);
@@ -261,7 +281,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                 pv[i] = q.pinv;
                 sh[i] = q.shape;
                 std::copy(f.begin(), f.end(), flags.begin() + (size_t)i * nn);
-                new_prior[i] = left[i] > 0 ? ln_prior_params(e, q, o.using_lg) + prior_cls[i] : prior[i];
+                new_prior[i] = ln_prior_params(e, q, o.using_lg) + prior_cls[i];
             }
             propose_timer.reset();
             rc = smc_update_params(e, first, count, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
@@ -287,8 +307,10 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
             }
             rc = smc_accept_reject(e, first, count, acc.data());
             if (rc != SMC_OK) return rc;
-            for (int i = 0; i < count; ++i)
+            for (int i = 0; i < count; ++i) {
+                if (!acc[i]) put_committed(i);  // the rejected proposal's rows go back
                 if (left[i] <= 0) next_move(i);
+            }
         }
     }
     // with probability gibbs_probability: the CAT Gibbs move and a forced refresh
