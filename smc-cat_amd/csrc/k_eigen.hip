@@ -220,15 +220,16 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
                 eig + (size_t)blockIdx.x * kEigStride);
 }
 
-// P(t r) = A diag(e^{lam t r}) B of one eigensystem (A, lam, B at eg), written
-// row-major with a 21st column of row sums.
-// P(t) = A diag(e^{lam t r}) B of one class for a run of (node, rate) pairs.  A
-// 256-lane workgroup keeps, per lane, the outer-product columns
-// M[q][k] = A[i][k] Bx[k][j] of its (at most two) entries q = 21 i + j, where Bx is
-// B with a 21st column of row sums (P 1 = A diag(e) (B 1)); then for each pair
-// P[q] = sum_k M[q][k] e_k.  The exponentials of all pairs of the run are computed
-// once, spread over the workgroup, and the 420 values of each P are written by
-// consecutive lanes.
+// P(t) = A diag(e^{lam t r}) B of one class for a run of (node, rate) pairs,
+// written row-major with a 21st column of row sums (P 1 = A diag(e) (B 1)).  P is
+// reversible, pi_i P_ij = pi_j P_ji (A = Pi^-1/2 U, B = U^T Pi^1/2), so only the
+// upper triangle, the diagonal and the row sums are sums over k: lane t < 190 of
+// the 256-lane workgroup takes the pair (i, j), i < j, writes P_ij and
+// (pi_i / pi_j) P_ij at (j, i); lanes 190..209 take the diagonal, 210..229 the row
+// sums (230 sums of 20 products instead of 420).  A lane keeps the outer-product
+// column M[k] = A[i][k] Bx[k][j] of its entry (Bx: B with a 21st column of row
+// sums); for each pair, P = sum_k M[k] e_k.  The exponentials of all pairs of the
+// run are computed once, spread over the workgroup.
 constexpr int kPmLanes = 256;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
 constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
@@ -253,29 +254,54 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         ex[x / 20][k] = exp(eg[kEigLam + k] * bl[node_of(q / R)] * rt[q % R]);
     }
     __syncthreads();
-    const int qa = tid, qb = tid + kPmLanes;  // entries of this lane (qb < 420 for 164 lanes)
-    const bool hb = qb < kPStride;
-    double ma[20], mb[20];
-    {
-        const int ia = qa / 21, ja = qa % 21, ib = hb ? qb / 21 : 0, jb = hb ? qb % 21 : 0;
-#pragma unroll
-        for (int k = 0; k < 20; k++) {
-            ma[k] = eg[ia * 20 + k] * (ja < 20 ? eg[kEigB + k * 20 + ja] : bsum[k]);
-            mb[k] = eg[ib * 20 + k] * (jb < 20 ? eg[kEigB + k * 20 + jb] : bsum[k]);
+    // this lane's entry (i, j): the upper triangle in row order, then the diagonal,
+    // then the row sums (j = 20); lanes from 230 on have none
+    int i = 0, j = 0;
+    const bool upper = tid < 190, has = tid < 230;
+    if (upper) {
+        int t = tid;
+        while (t >= 19 - i) {
+            t -= 19 - i;
+            i++;
         }
+        j = i + 1 + t;
+    } else if (tid < 210) {
+        i = j = tid - 190;
+    } else if (has) {
+        i = tid - 210;
+        j = 20;
     }
-    for (int q = q0; q < q1; q++) {
-        const int n = node_of(q / R), r = q % R;
-        double pa = 0.0, pb = 0.0;
+    double m[20];
 #pragma unroll
-        for (int k = 0; k < 20; k++) {
-            const double e = ex[q - q0][k];
-            pa = fma(ma[k], e, pa);
-            pb = fma(mb[k], e, pb);
-        }
+    for (int k = 0; k < 20; k++) m[k] = eg[i * 20 + k] * (j < 20 ? eg[kEigB + k * 20 + j] : bsum[k]);
+    const double ratio = upper ? eg[kEigPi + i] / eg[kEigPi + j] : 0.0;
+    if (!has) return;  // no barrier follows
+    // two pairs at a time: independent chains of 20 multiply-adds (four or eight were
+    // measured slower: their registers cost more occupancy than the chains gain)
+    auto put = [&](int q, double pa) {
+        const int n = node_of(q / R), r = q % R;
         TP *out = Pcls + ((size_t)n * R + r) * kPStride;
-        out[qa] = (TP)pa;
-        if (hb) out[qb] = (TP)pb;
+        out[i * 21 + j] = (TP)pa;
+        if (upper) out[j * 21 + i] = (TP)(pa * ratio);
+    };
+    constexpr int CH = 2;
+    int q = q0;
+    for (; q + CH <= q1; q += CH) {
+        double pa[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) pa[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 20; k++)
+#pragma unroll
+            for (int u = 0; u < CH; u++) pa[u] = fma(m[k], ex[q - q0 + u][k], pa[u]);
+#pragma unroll
+        for (int u = 0; u < CH; u++) put(q + u, pa[u]);
+    }
+    for (; q < q1; q++) {
+        double pa = 0.0;
+#pragma unroll
+        for (int k = 0; k < 20; k++) pa = fma(m[k], ex[q - q0][k], pa);
+        put(q, pa);
     }
 }
 
