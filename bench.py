@@ -371,6 +371,16 @@ def rejuv_compare(args):
         stats_tot += st
     t_mcmc = time.perf_counter() - t0
     s_step = t_mcmc / float(steps.sum())
+    # lockstep efficiency: MH steps over (batch size x rounds), rounds being the
+    # longest chain of each batch; the same for the config-4 population of 4096
+    # particles keyed by step count
+    ssort = np.sort(steps)[::-1]
+    rounds = sum(int(ssort[b * nb:(b + 1) * nb].max()) for b in range(nbatch))
+    full = eng.rejuvenation_steps(it, 5, np.arange(4096, dtype=np.int32), invariant=True)
+    fsort = np.sort(full)[::-1]
+    slots_c4 = sum(len(fsort[b:b + nb]) * int(fsort[b:b + nb].max()) for b in range(0, len(fsort), nb))
+    lockstep = {"rounds": rounds, "efficiency": float(steps.sum()) / (nb * rounds),
+                "efficiency_4096_particles": float(full.sum()) / slots_c4}
     progress("rejuv fast MCMC: %d particles in batches of %d: %.1f ms per particle, %.1f us per MH step" %
              (popn, nb, t_mcmc / popn * 1e3, s_step * 1e6))
     # CAT moves: one lockstep over the first batch, each move charged at probability 0.1
@@ -403,6 +413,8 @@ def rejuv_compare(args):
     progress("rejuv fast CAT move per particle: Gibbs %.3f s, learned %.3f s (acceptance %.2f)" %
              (cat["gibbs"], cat["learned"], acc))
     out["mean_steps"] = mean_steps
+    out["population"] = popn
+    out["lockstep"] = lockstep
     out["mh_stats_proposed_accepted"] = stats_tot.tolist()
     out["fp32_batched_gibbs"] = fast["gibbs"]
     out["fp32_batched_learned"] = fast["learned"]
@@ -531,7 +543,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rj-particles", type=int, default=150, help="particles per resident lockstep batch")
-    ap.add_argument("--rj-batches", type=int, default=2)
+    ap.add_argument("--rj-batches", type=int, default=8,
+                    help="resident lockstep batches timed (the population is rj-particles x rj-batches)")
     ap.add_argument("--rj-plain", type=int, default=12, help="particles of the plain FP64 path, spread over the population")
     ap.add_argument("--rj-iterations", type=int, default=16)
     ap.add_argument("--rj-temp", type=float, default=0.5)

@@ -1505,9 +1505,10 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         for (int n = T; n < nn; ++n)
             if (dirty[n] && n != root && prune_kind(ch, T, n) != PRUNE_TT) n_stored++;
         const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && chains_fit && n_stored <= e->tune.chain_max;
-        // (walking only the large dirty sets -- whole trees -- and launching the rest
-        // level by level was measured: 8% slower than either alone at the C4 shape)
-        const bool walk_p = use_walk;
+        // the variant's walks, or (SMC_PRUNE_MFMA) the small dirty sets when the call
+        // holds many: dirty paths walked in one launch, whole trees level by level
+        const bool walk_p = use_walk || (walk_ok && e->prune_variant == SMC_PRUNE_MFMA && !chains_fit &&
+                                         n_stored <= e->tune.chain_max);
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
