@@ -36,6 +36,7 @@ int smc_fail(smc_engine *e, int code, const std::string &msg)
 int sync_stream(smc_engine *e)
 {
     HIP_TRY(e, hipStreamSynchronize(e->stream));
+    if (e->stream2) HIP_TRY(e, hipStreamSynchronize(e->stream2));
     e->stage_off = 0;  // every staged copy has completed
     for (const smc_engine::Readback &r : e->readbacks) std::memcpy(r.dst, e->h_read + r.off, r.bytes);
     e->readbacks.clear();
@@ -561,7 +562,13 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     }
     hipError_t st = hipSetDevice(cfg->device);
     if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking);
+    if (st == hipSuccess) st = hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming);
+    if (st == hipSuccess) st = hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming);
     if (st != hipSuccess) {
+        if (e->stream) (void)hipStreamDestroy(e->stream);
+        if (e->stream2) (void)hipStreamDestroy(e->stream2);
+        if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
         delete e;
         return SMC_ERR_HIP;
     }
@@ -695,6 +702,7 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_chain_nodes, e->d_chain_len, e->d_walk, e->d_witems, e->d_walk_rdot, e->d_walk_rexp,
                     e->d_eig_save};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->stream2) (void)hipStreamSynchronize(e->stream2);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->h_read) (void)hipHostFree(e->h_read);
     for (smc_engine::CachedGraph &c : e->graphs) (void)hipGraphExecDestroy(c.exec);
@@ -713,6 +721,9 @@ int smc_engine_destroy(smc_engine *e)
     }
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->stream2) (void)hipStreamDestroy(e->stream2);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     delete e;
     return SMC_OK;
 }
@@ -1728,6 +1739,16 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     PruneArgs a = prune_args(e, first);
     a.xflag = e->d_xflag;
     a.WS = 2;
+    // With walks and level launches in one call (dirty paths walked, whole trees by
+    // level), the levels run on the second stream beside the walks: their narrow
+    // deep launches then fill the gaps of the walk instead of following it.
+    const bool split = !e->timing && !wroots.empty() && (off.back() > 0 || !roots.empty());
+    hipStream_t sl = s;  // the level launches' stream
+    if (split) {
+        HIP_TRY(e, hipEventRecord(e->ev_fork, s));
+        HIP_TRY(e, hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
+        sl = e->stream2;
+    }
     auto run = [&](int o, int n, int kind, bool is_roots) {
         if (n <= 0) return;
         TimedLaunch tl(e, SMC_K_PRUNE);
@@ -1737,9 +1758,9 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.roots = is_roots;
         a.unroll = (is_roots || kind == PRUNE_ANY) ? e->tune.unroll_rt : e->tune.unroll[prune_stored(kind)];
         if (e->prune_variant == SMC_PRUNE_VALU)
-            launch_prune(a, s);
+            launch_prune(a, sl);
         else
-            launch_prune_mfma(a, s);
+            launch_prune_mfma(a, sl);
     };
     const int bpi = (e->tile_stride + 3) / 4;
     const int64_t small = (int64_t)e->num_cus * 8;  // units that do not fill the GPU twice over
@@ -1769,9 +1790,10 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.items = e->d_items + chain_off[L];
         a.n_items = n;
         a.roots = false;
-        launch_prune_chain(a, e->d_chain_nodes, e->d_chain_len + chain_off[L], s);
+        launch_prune_chain(a, e->d_chain_nodes, e->d_chain_len + chain_off[L], sl);
     }
     run(root_off, (int)roots.size(), PRUNE_ANY, true);
+    if (split) HIP_TRY(e, hipEventRecord(e->ev_join, e->stream2));
     if (!wroots.empty()) {
         if (!e->d_walk_rdot) {
             const size_t nr = (size_t)e->Wmax * e->tile_stride * e->R * 64;
@@ -1802,6 +1824,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         a.n_items = (int)wroots.size();
         launch_walk_mixture(a, e->d_witems + roots_at, e->d_walk_rdot, e->d_walk_rexp, s);
     }
+    if (split) HIP_TRY(e, hipStreamWaitEvent(s, e->ev_join, 0));  // the levels are done
     {
         TimedLaunch tl(e, SMC_K_REDUCE);
         launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl,

@@ -41,6 +41,11 @@ struct smc_engine {
     bool fp32 = false;
     size_t tx_size = 8;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for work that overlaps the first within
+    // one call: an incremental refresh's whole-tree levels beside its walks; every
+    // use joins the first stream before the call returns
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     int64_t device_bytes = 0;
     bool have_alignment = false;
