@@ -610,8 +610,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     A(&e->d_witems, P * (T + 1));  // walk pieces (at most one per node) and the roots
     A(&e->d_pcls_off, P);
     A(&e->d_prog, P * site_prog_stride(T));
-    A(&e->d_plist, 2 * P);
-    A(&e->d_pnodes, P * nn);
+    A(&e->d_plist, 4 * P);      // two model jobs' lists (see upload_models)
+    A(&e->d_pnodes, 2 * P * nn);  // and their node lists
     const size_t W = e->Wmax, C = (size_t)e->Cmax;
     const size_t WS = e->WS;
     A(&e->d_E, W * WS * (T - 1) * R * Sp);
@@ -1236,57 +1236,86 @@ int set_single_wave(smc_engine *e, int first, int count)
     return SMC_OK;
 }
 
-// Eigen-decompositions and/or P matrices of the particles in the lists.
-int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
-                     const std::vector<int> &p_list, const std::vector<Item> &p_nodes = {})
+// Eigensystem and P recomputes in two halves, so that their uploads can go ahead
+// of a stream fork: upload_models stages the lists (eigen list at plist, full-P
+// list at plist + Pmax, node lists in pnodes), launch_models launches on a stream.
+struct ModelJob {
+    int n_eig = 0, n_p = 0, np = 0, max_nodes = 0;
+    int *d_eig = nullptr, *d_p = nullptr, *d_nodes = nullptr;
+};
+
+int upload_models(smc_engine *e, const std::vector<int> &eig_list, const std::vector<int> &p_list,
+                  const std::vector<Item> &p_nodes, int *plist, Item *pnodes, ModelJob &job)
 {
-    hipStream_t s = e->stream;
     if (!p_nodes.empty()) {  // P of single (particle, node) pairs: branch-length moves
         if ((int64_t)p_nodes.size() > (int64_t)e->Pmax * e->nn)
             return fail(e, SMC_ERR_STATE, "too many P nodes");
         // grouped by particle (the lists are built particle by particle):
         // [particles | offsets | nodes] in one upload
         std::vector<int> buf;
-        std::vector<int> plist, noff{0}, nodes;
+        std::vector<int> plist_n, noff{0}, nodes;
         for (size_t i = 0; i < p_nodes.size(); ++i) {
             if (i == 0 || p_nodes[i].particle != p_nodes[i - 1].particle) {
                 if (i) noff.push_back((int)nodes.size());
-                plist.push_back(p_nodes[i].particle);
+                plist_n.push_back(p_nodes[i].particle);
             }
             nodes.push_back(p_nodes[i].node);
         }
         noff.push_back((int)nodes.size());
-        const int np = (int)plist.size();
-        int max_nodes = 0;
-        for (int i = 0; i < np; ++i) max_nodes = std::max(max_nodes, noff[i + 1] - noff[i]);
-        buf.insert(buf.end(), plist.begin(), plist.end());
+        job.np = (int)plist_n.size();
+        for (int i = 0; i < job.np; ++i) job.max_nodes = std::max(job.max_nodes, noff[i + 1] - noff[i]);
+        buf.insert(buf.end(), plist_n.begin(), plist_n.end());
         buf.insert(buf.end(), noff.begin(), noff.end());
         buf.insert(buf.end(), nodes.begin(), nodes.end());
-        int *d = reinterpret_cast<int *>(e->d_pnodes);  // capacity: 2 ints per (particle, node)
-        int rc = h2d(e, d, buf.data(), buf.size() * sizeof(int));
+        job.d_nodes = reinterpret_cast<int *>(pnodes);  // capacity: 2 ints per (particle, node)
+        int rc = h2d(e, job.d_nodes, buf.data(), buf.size() * sizeof(int));
         if (rc != SMC_OK) return rc;
+    }
+    if (!eig_list.empty()) {
+        job.n_eig = (int)eig_list.size();
+        job.d_eig = plist;
+        int rc = h2d(e, job.d_eig, eig_list.data(), eig_list.size() * sizeof(int));
+        if (rc != SMC_OK) return rc;
+    }
+    if (!p_list.empty()) {
+        job.n_p = (int)p_list.size();
+        job.d_p = plist + e->Pmax;  // apart from the eigen list, which its launch may still read
+        int rc = h2d(e, job.d_p, p_list.data(), p_list.size() * sizeof(int));
+        if (rc != SMC_OK) return rc;
+    }
+    return SMC_OK;
+}
+
+void launch_models(smc_engine *e, int first, const ModelJob &job, hipStream_t s)
+{
+    if (job.np > 0) {
         TimedLaunch tl(e, SMC_K_PMAT);
         PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
                     e->R, e->Kmax, first, 0, e->fp32, nullptr};
-        launch_pmat_nodes(pa, d, d + np, d + 2 * np + 1, np, max_nodes, s);
+        const int *d = job.d_nodes;
+        launch_pmat_nodes(pa, d, d + job.np, d + 2 * job.np + 1, job.np, job.max_nodes, s);
     }
-    if (!eig_list.empty()) {
-        int rc = h2d(e, e->d_plist, eig_list.data(), eig_list.size() * sizeof(int));
-        if (rc != SMC_OK) return rc;
+    if (job.n_eig > 0) {
         TimedLaunch tl(e, SMC_K_EIGEN);
-        EigenArgs ea{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first,
-                     (int)eig_list.size(), e->d_plist};
+        EigenArgs ea{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, first, job.n_eig, job.d_eig};
         launch_eigen(ea, s);
     }
-    if (!p_list.empty()) {
-        int *dl = e->d_plist + e->Pmax;  // second half: the eigen launch may still read the first
-        int rc = h2d(e, dl, p_list.data(), p_list.size() * sizeof(int));
-        if (rc != SMC_OK) return rc;
+    if (job.n_p > 0) {
         TimedLaunch tl(e, SMC_K_PMAT);
         PmatArgs pa{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off, e->d_P, e->nn,
-                    e->R, e->Kmax, first, (int)p_list.size(), e->fp32, dl};
+                    e->R, e->Kmax, first, job.n_p, e->fp32, job.d_p};
         launch_pmat(pa, s);
     }
+}
+
+// Eigen-decompositions and/or P matrices of the particles in the lists.
+int recompute_models(smc_engine *e, int first, const std::vector<int> &eig_list,
+                     const std::vector<int> &p_list, const std::vector<Item> &p_nodes = {})
+{
+    ModelJob job;
+    int rc = upload_models(e, eig_list, p_list, p_nodes, e->d_plist, e->d_pnodes, job);
+    if (rc != SMC_OK) return rc;
+    launch_models(e, first, job, e->stream);
     return SMC_OK;
 }
 
@@ -1440,7 +1469,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<WalkItem> wroots;               // the pieces holding the roots
     std::vector<int> po, wst;
     std::vector<int> wsize(nn), wlev(nn), wtop(nn);
-    std::vector<uint8_t> wcut(nn);
+    std::vector<uint8_t> wcut(nn), walked(count, 0);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const uint8_t *nf = node_flags + (size_t)i * nn;
@@ -1520,6 +1549,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         // holds many: dirty paths walked in one launch, whole trees level by level
         const bool walk_p = use_walk || (walk_ok && e->prune_variant == SMC_PRUNE_MFMA && !chains_fit &&
                                          n_stored <= e->tune.chain_max);
+        walked[i] = walk_p ? 1 : 0;
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
             e->h_xflag[(size_t)p * Ti + (n - T)] ^= 1;
@@ -1707,7 +1737,25 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         std::stable_sort(p_nodes.begin(), p_nodes.end(),
                          [](const Item &x, const Item &y) { return x.particle < y.particle; });
     }
-    rc = recompute_models(e, first, eig_list, p_list, p_nodes);
+    // With walks and level launches in one call (dirty paths walked, whole trees by
+    // level), the levels run on the second stream beside the walks: their narrow
+    // deep launches then fill the gaps of the walk instead of following it; the
+    // models of the levels' particles are recomputed on that stream too.
+    const bool split = !e->timing && !wroots.empty() && !roots.empty();
+    ModelJob job_w, job_t;  // (split) the levels' and the walks' models
+    if (split) {
+        std::vector<int> eig_w, eig_t, pl_w, pl_t;
+        std::vector<Item> pn_w, pn_t;
+        for (int p : eig_list) (walked[p - first] ? eig_t : eig_w).push_back(p);
+        for (int p : p_list) (walked[p - first] ? pl_t : pl_w).push_back(p);
+        for (const Item &it : p_nodes) (walked[it.particle - first] ? pn_t : pn_w).push_back(it);
+        rc = upload_models(e, eig_w, pl_w, pn_w, e->d_plist, e->d_pnodes, job_w);
+        if (rc == SMC_OK)
+            rc = upload_models(e, eig_t, pl_t, pn_t, e->d_plist + 2 * (size_t)e->Pmax,
+                               e->d_pnodes + (size_t)e->Pmax * nn, job_t);
+    } else {
+        rc = recompute_models(e, first, eig_list, p_list, p_nodes);
+    }
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
     rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
@@ -1739,15 +1787,13 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     PruneArgs a = prune_args(e, first);
     a.xflag = e->d_xflag;
     a.WS = 2;
-    // With walks and level launches in one call (dirty paths walked, whole trees by
-    // level), the levels run on the second stream beside the walks: their narrow
-    // deep launches then fill the gaps of the walk instead of following it.
-    const bool split = !e->timing && !wroots.empty() && (off.back() > 0 || !roots.empty());
     hipStream_t sl = s;  // the level launches' stream
-    if (split) {
+    if (split) {  // fork after every upload
         HIP_TRY(e, hipEventRecord(e->ev_fork, s));
         HIP_TRY(e, hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
         sl = e->stream2;
+        launch_models(e, first, job_w, sl);
+        launch_models(e, first, job_t, s);
     }
     auto run = [&](int o, int n, int kind, bool is_roots) {
         if (n <= 0) return;
