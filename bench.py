@@ -322,8 +322,10 @@ def rejuv_compare(args):
     ratio of ESS/sec.  Paths:
       plain_fp64            one particle at a time (the reference's per-thread loop),
                             FP64, the Gibbs CAT move
-      fp32_batched_gibbs    FP32, lockstep batches keyed by chain length (the
-                            sampler's shape), the Gibbs CAT move in one lockstep
+      fp32_batched_gibbs    FP32, lockstep batches of 150 resident particles keyed by
+                            chain length (the sampler's shape) over 1,200 particles,
+                            the Gibbs CAT move in one lockstep of the ~410 particles
+                            that draw it at the config-4 population
       fp32_batched_learned  the same with the learned CAT move, its network trained
                             on >= 1M Gibbs draws recorded before the timed runs."""
     from smccat import Engine, ProposalNetwork
@@ -383,16 +385,19 @@ def rejuv_compare(args):
                 "efficiency_4096_particles": float(full.sum()) / slots_c4}
     progress("rejuv fast MCMC: %d particles in batches of %d: %.1f ms per particle, %.1f us per MH step" %
              (popn, nb, t_mcmc / popn * 1e3, s_step * 1e6))
-    # CAT moves: one lockstep over the first batch, each move charged at probability 0.1
-    slots = list(range(nb))
+    # CAT moves: the sampler moves every particle that draws one (probability 0.1)
+    # together, after the MCMC of all batches: at the config-4 population that is
+    # about 410 particles per rejuvenation; each particle is charged 0.1 moves
+    n_cat = min(popn, int(round(0.1 * 4096)))
+    slots = list(range(n_cat))
     cat = {}
     gi = max(1, S // (2 * args.rj_gibbs_fraction))
     t0 = time.perf_counter()
     eng.gibbs_partition_move(slots, tau, seed=6, num_iter=gi, dpp_alpha=1.0)
-    cat["gibbs"] = (time.perf_counter() - t0) * (S / 2.0) / gi / nb
+    cat["gibbs"] = (time.perf_counter() - t0) * (S / 2.0) / gi / n_cat
     t0 = time.perf_counter()
     mst = net.move(slots, tau, seed=6, dpp_alpha=1.0)
-    cat["learned"] = (time.perf_counter() - t0) / nb
+    cat["learned"] = (time.perf_counter() - t0) / n_cat
     acc = float(mst[2] / max(mst[1], 1))
     t0 = time.perf_counter()
     eng.refresh_likelihood(0, nb)

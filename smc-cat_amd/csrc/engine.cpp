@@ -1725,11 +1725,19 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
             }
         }
         std::vector<Item> keep_nodes;
+        std::vector<uint8_t> has_node;  // (particle, node) pairs already listed
         for (const Item &it : e->pend_nodes) {
-            if (!in(it.particle))
+            if (!in(it.particle)) {
                 keep_nodes.push_back(it);
-            else if (!full[it.particle - first])
-                p_nodes.push_back(it);
+            } else if (!full[it.particle - first]) {
+                if (has_node.empty()) {
+                    has_node.assign((size_t)count * nn, 0);
+                    for (const Item &q : p_nodes) has_node[(size_t)(q.particle - first) * nn + q.node] = 1;
+                }
+                uint8_t &h = has_node[(size_t)(it.particle - first) * nn + it.node];
+                if (!h) p_nodes.push_back(it);
+                h = 1;
+            }
         }
         e->pend_full.swap(keep_full);
         e->pend_nodes.swap(keep_nodes);

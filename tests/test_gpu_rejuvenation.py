@@ -90,3 +90,24 @@ def test_stream_keys_fix_the_chains_and_their_step_counts(gpu):
     for f in ("children", "branch_len", "base", "pinv", "shape"):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f)[::-1])
     assert st_a[:, 0].sum() == steps.sum() == st_b[:, 0].sum()
+
+
+def test_one_slot_engine_with_repeated_whole_tree_moves(gpu):
+    """An engine of one slot runs chains whose rejected whole-tree proposals (shape,
+    rate matrix: every P of the particle) are followed by more of them: the restores
+    deferred to the next refresh merge with the new proposal's recomputes without
+    listing a node twice (a one-slot engine has room for each node once)."""
+    _say("case rejuvenation: one-slot engine, 16 taxa x 80 sites, 12 iterations, 4 seeds")
+    rng = np.random.default_rng(21)
+    T, S, R = 16, 80, 4
+    codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
+    batch = sim.prior_particles(rng, 1, T, S, num_rates=R, invar=True, k_range=(1, 3))
+    for seed in range(4):
+        with Engine(T, S, R, 1, 4, workspaces=2) as eng:
+            eng.set_alignment(codes)
+            eng.set_particles(batch)
+            lnl, stats = eng.rejuvenate(1.0, 12, seed=seed, gibbs_probability=0.0)
+            final = eng.snapshot([0], batch)
+        assert stats[1:, 0].sum() > 0  # whole-tree moves were proposed
+        want = orc.particle_lnl(codes, final.particle(0), R)
+        np.testing.assert_allclose(lnl[0], want, rtol=1e-10)
