@@ -223,26 +223,60 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // P(t) = A diag(e^{lam t r}) B of one class for a run of (node, rate) pairs,
 // written row-major with a 21st column of row sums (P 1 = A diag(e) (B 1)).  P is
 // reversible, pi_i P_ij = pi_j P_ji (A = Pi^-1/2 U, B = U^T Pi^1/2), so only the
-// upper triangle, the diagonal and the row sums are sums over k: lane t < 190 of
-// the 256-lane workgroup takes the pair (i, j), i < j, writes P_ij and
-// (pi_i / pi_j) P_ij at (j, i); lanes 190..209 take the diagonal, 210..229 the row
-// sums (230 sums of 20 products instead of 420).  A lane keeps the outer-product
-// column M[k] = A[i][k] Bx[k][j] of its entry (Bx: B with a 21st column of row
-// sums); for each pair, P = sum_k M[k] e_k.  The exponentials of all pairs of the
-// run are computed once, spread over the workgroup.
+// upper triangle (190 entries), the diagonal (20) and the row sums (20) are sums over
+// k; the lower triangle is (pi_i / pi_j) P_ij at (j, i).  Each entry e (upper triangle
+// in row order, then the diagonal, then the row sums) is the dot product of its
+// outer-product column M[e][k] = A[i][k] Bx[k][j] (Bx: B with the row-sum column)
+// with the pair's exponentials e_k = exp(lam_k t r).  Lane l of every wave keeps M of
+// entries l, l + 64, l + 128, l + 192 in registers, so that one wave covers all 230
+// entries of a pair: a step gives each of the workgroup's waves its own pair, and an
+// exponential read from LDS feeds four multiply-adds (LDS bandwidth, not the FP64
+// units, bounds the one- and two-entry forms).  The entries of a step's pairs go
+// through LDS (two buffers, one barrier a step) and leave in 16-byte consecutive-lane
+// stores of the whole workgroup (measured faster than waves that run their own pairs
+// with no barrier and store their own rows).  The exponentials of all pairs of the run
+// are computed once, spread over the workgroup.
 constexpr int kPmLanes = 256;
+constexpr int kPmWaves = kPmLanes / 64;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
 constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
 constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 128 pairs from which launches use them
+constexpr int kPmUpper = 190, kPmEntries = 230;
 
-template <typename TP, int PAIRS, typename PairNode>
+// entry e of the list above -> (i, j); j = 20 for the row sums
+__device__ __forceinline__ void pmat_entry(int e, int &i, int &j)
+{
+    if (e < kPmUpper) {
+        i = 0;
+        while (e >= 19 - i) {
+            e -= 19 - i;
+            i++;
+        }
+        j = i + 1 + e;
+    } else if (e < kPmUpper + 20) {
+        i = j = e - kPmUpper;
+    } else if (e < kPmEntries) {
+        i = e - kPmUpper - 20;
+        j = 20;
+    } else {
+        i = j = 0;
+    }
+}
+
+// node_of(i): node of the run's i-th node; pair_out(q): index n * R + r of pair q's
+// matrix in the class's P block
+template <typename TP, int PAIRS, typename NodeOf, typename PairOut>
 __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const double *__restrict__ bl,
                                            const double *__restrict__ rt, int R, int q0, int q1,
-                                           PairNode node_of, TP *__restrict__ Pcls)
+                                           NodeOf node_of, PairOut pair_out, TP *__restrict__ Pcls)
 {
+    constexpr int V = 16 / (int)sizeof(TP);  // entries per 16-byte store
+    constexpr int NV = kPStride / V;
+    static_assert(kPStride % V == 0, "P rows must split into 16-byte stores");
     __shared__ double bsum[20];
     __shared__ double ex[PAIRS][20];
-    const int tid = threadIdx.x;
+    __shared__ __align__(16) TP stage[2][kPmWaves * kPStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < 20) {
         double a = 0.0;
         for (int j = 0; j < 20; j++) a += eg[kEigB + tid * 20 + j];
@@ -254,54 +288,49 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         ex[x / 20][k] = exp(eg[kEigLam + k] * bl[node_of(q / R)] * rt[q % R]);
     }
     __syncthreads();
-    // this lane's entry (i, j): the upper triangle in row order, then the diagonal,
-    // then the row sums (j = 20); lanes from 230 on have none
-    int i = 0, j = 0;
-    const bool upper = tid < 190, has = tid < 230;
-    if (upper) {
-        int t = tid;
-        while (t >= 19 - i) {
-            t -= 19 - i;
-            i++;
-        }
-        j = i + 1 + t;
-    } else if (tid < 210) {
-        i = j = tid - 190;
-    } else if (has) {
-        i = tid - 210;
-        j = 20;
-    }
-    double m[20];
+    // entries lane + 64 c: c = 0, 1 lie in the upper triangle, c = 2 up to entry 189,
+    // c = 3 is a diagonal entry, a row sum or nothing (entries from 230 on)
+    int ei[4], ej[4];
+    double m[4][20];
 #pragma unroll
-    for (int k = 0; k < 20; k++) m[k] = eg[i * 20 + k] * (j < 20 ? eg[kEigB + k * 20 + j] : bsum[k]);
-    const double ratio = upper ? eg[kEigPi + i] / eg[kEigPi + j] : 0.0;
-    if (!has) return;  // no barrier follows
-    // two pairs at a time: independent chains of 20 multiply-adds (four or eight were
-    // measured slower: their registers cost more occupancy than the chains gain)
-    auto put = [&](int q, double pa) {
-        const int n = node_of(q / R), r = q % R;
-        TP *out = Pcls + ((size_t)n * R + r) * kPStride;
-        out[i * 21 + j] = (TP)pa;
-        if (upper) out[j * 21 + i] = (TP)(pa * ratio);
-    };
-    constexpr int CH = 2;
-    int q = q0;
-    for (; q + CH <= q1; q += CH) {
-        double pa[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) pa[u] = 0.0;
+    for (int c = 0; c < 4; c++) {
+        pmat_entry(lane + 64 * c, ei[c], ej[c]);
 #pragma unroll
         for (int k = 0; k < 20; k++)
-#pragma unroll
-            for (int u = 0; u < CH; u++) pa[u] = fma(m[k], ex[q - q0 + u][k], pa[u]);
-#pragma unroll
-        for (int u = 0; u < CH; u++) put(q + u, pa[u]);
+            m[c][k] = eg[ei[c] * 20 + k] * (ej[c] < 20 ? eg[kEigB + k * 20 + ej[c]] : bsum[k]);
     }
-    for (; q < q1; q++) {
-        double pa = 0.0;
+    double ratio[3];
 #pragma unroll
-        for (int k = 0; k < 20; k++) pa = fma(m[k], ex[q - q0][k], pa);
-        put(q, pa);
+    for (int c = 0; c < 3; c++) ratio[c] = eg[kEigPi + ei[c]] / eg[kEigPi + ej[c] % 20];
+    const bool up2 = lane + 128 < kPmUpper, has3 = lane + 192 < kPmEntries;
+    int buf = 0;
+    for (int q = q0; q < q1; q += kPmWaves, buf ^= 1) {
+        const int nq = min(kPmWaves, q1 - q);
+        TP *st = stage[buf];
+        if (wave < nq) {
+            const double *x = ex[q - q0 + wave];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 20; k++) {
+                const double e = x[k];
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[c] = fma(m[c][k], e, acc[c]);
+            }
+            TP *sp = st + wave * kPStride;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                sp[ei[c] * 21 + ej[c]] = (TP)acc[c];
+                sp[ej[c] * 21 + ei[c]] = (TP)(acc[c] * ratio[c]);
+            }
+            sp[ei[2] * 21 + ej[2]] = (TP)acc[2];
+            if (up2) sp[ej[2] * 21 + ei[2]] = (TP)(acc[2] * ratio[2]);
+            if (has3) sp[ei[3] * 21 + ej[3]] = (TP)acc[3];
+        }
+        __syncthreads();
+        for (int x = tid; x < nq * NV; x += kPmLanes) {
+            const int u = x / NV, v = x - u * NV;
+            *(uint4 *)(Pcls + pair_out(q + u) * kPStride + v * V) = *(const uint4 *)(st + u * kPStride + v * V);
+        }
     }
 }
 
@@ -326,7 +355,7 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ ei
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
     pmat_pairs<TP, PAIRS>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
-                          [](int n) { return n; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
+                          [](int n) { return n; }, [](int q) { return (size_t)q; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
 // P of listed nodes only: workgroup (listed particle * max_classes + class, chunk
@@ -351,7 +380,8 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat_nodes(const double *__restric
     const size_t g = (size_t)p * max_classes + k;
     const int *nl = nodes + n0;
     pmat_pairs<TP, kPmPairs>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
-                             [nl](int i) { return nl[i]; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
+                             [nl](int i) { return nl[i]; },
+                             [nl, R](int q) { return (size_t)nl[q / R] * R + q % R; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
 }  // namespace
