@@ -227,15 +227,17 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // k; the lower triangle is (pi_i / pi_j) P_ij at (j, i).  Each entry e (upper triangle
 // in row order, then the diagonal, then the row sums) is the dot product of its
 // outer-product column M[e][k] = A[i][k] Bx[k][j] (Bx: B with the row-sum column)
-// with the pair's exponentials e_k = exp(lam_k t r).  Lane l of every wave keeps M of
-// entries l, l + 64, l + 128, l + 192 in registers, so that one wave covers all 230
-// entries of a pair: a step gives each of the workgroup's waves its own pair, and an
-// exponential read from LDS feeds four multiply-adds (LDS bandwidth, not the FP64
-// units, bounds the one- and two-entry forms).  The entries of a step's pairs go
-// through LDS (two buffers, one barrier a step) and leave in 16-byte consecutive-lane
-// stores of the whole workgroup (measured faster than waves that run their own pairs
-// with no barrier and store their own rows).  The exponentials of all pairs of the run
-// are computed once, spread over the workgroup.
+// with the pair's exponentials e_k = exp(lam_k t r).  Long runs (128 pairs): lane l of
+// every wave keeps M of entries l, l + 64, l + 128, l + 192 in registers, so that one
+// wave covers all 230 entries of a pair; a step gives each of the workgroup's waves
+// its own pair, and an exponential read from LDS feeds four multiply-adds (LDS
+// bandwidth, not the FP64 units, bounds the one-entry form on long runs).  Short runs
+// (32 pairs, node lists) give lane t entry t and two pairs a step: their per-workgroup
+// set-up of M would cost four times as much in the four-entry form.  The entries of a
+// step's pairs go through LDS (two buffers, one barrier a step) and leave in 16-byte
+// consecutive-lane stores of the whole workgroup (measured faster than waves that run
+// their own pairs with no barrier and store their own rows).  The exponentials of all
+// pairs of the run are computed once, spread over the workgroup.
 constexpr int kPmLanes = 256;
 constexpr int kPmWaves = kPmLanes / 64;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
@@ -265,18 +267,20 @@ __device__ __forceinline__ void pmat_entry(int e, int &i, int &j)
 
 // node_of(i): node of the run's i-th node; pair_out(q): index n * R + r of pair q's
 // matrix in the class's P block
-template <typename TP, int PAIRS, typename NodeOf, typename PairOut>
+template <typename TP, int PAIRS, int EPL, typename NodeOf, typename PairOut>
 __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const double *__restrict__ bl,
                                            const double *__restrict__ rt, int R, int q0, int q1,
                                            NodeOf node_of, PairOut pair_out, TP *__restrict__ Pcls)
 {
+    static_assert(EPL == 1 || EPL == 4, "one or four entries per lane");
     constexpr int V = 16 / (int)sizeof(TP);  // entries per 16-byte store
     constexpr int NV = kPStride / V;
     static_assert(kPStride % V == 0, "P rows must split into 16-byte stores");
+    constexpr int PS = EPL == 4 ? kPmWaves : 2;  // pairs per step
     __shared__ double bsum[20];
     __shared__ double ex[PAIRS][20];
-    __shared__ __align__(16) TP stage[2][kPmWaves * kPStride];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ __align__(16) TP stage[2][PS * kPStride];
+    const int tid = threadIdx.x;
     if (tid < 20) {
         double a = 0.0;
         for (int j = 0; j < 20; j++) a += eg[kEigB + tid * 20 + j];
@@ -288,48 +292,85 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         ex[x / 20][k] = exp(eg[kEigLam + k] * bl[node_of(q / R)] * rt[q % R]);
     }
     __syncthreads();
-    // entries lane + 64 c: c = 0, 1 lie in the upper triangle, c = 2 up to entry 189,
-    // c = 3 is a diagonal entry, a row sum or nothing (entries from 230 on)
-    int ei[4], ej[4];
-    double m[4][20];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        pmat_entry(lane + 64 * c, ei[c], ej[c]);
-#pragma unroll
-        for (int k = 0; k < 20; k++)
-            m[c][k] = eg[ei[c] * 20 + k] * (ej[c] < 20 ? eg[kEigB + k * 20 + ej[c]] : bsum[k]);
-    }
-    double ratio[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) ratio[c] = eg[kEigPi + ei[c]] / eg[kEigPi + ej[c] % 20];
-    const bool up2 = lane + 128 < kPmUpper, has3 = lane + 192 < kPmEntries;
-    int buf = 0;
-    for (int q = q0; q < q1; q += kPmWaves, buf ^= 1) {
-        const int nq = min(kPmWaves, q1 - q);
-        TP *st = stage[buf];
-        if (wave < nq) {
-            const double *x = ex[q - q0 + wave];
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k = 0; k < 20; k++) {
-                const double e = x[k];
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[c] = fma(m[c][k], e, acc[c]);
-            }
-            TP *sp = st + wave * kPStride;
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                sp[ei[c] * 21 + ej[c]] = (TP)acc[c];
-                sp[ej[c] * 21 + ei[c]] = (TP)(acc[c] * ratio[c]);
-            }
-            sp[ei[2] * 21 + ej[2]] = (TP)acc[2];
-            if (up2) sp[ej[2] * 21 + ei[2]] = (TP)(acc[2] * ratio[2]);
-            if (has3) sp[ei[3] * 21 + ej[3]] = (TP)acc[3];
-        }
+    // the step's entries to LDS, then 16-byte runs of the step's matrices out
+    auto store_step = [&](const TP *st, int q, int nq) {
         __syncthreads();
         for (int x = tid; x < nq * NV; x += kPmLanes) {
             const int u = x / NV, v = x - u * NV;
             *(uint4 *)(Pcls + pair_out(q + u) * kPStride + v * V) = *(const uint4 *)(st + u * kPStride + v * V);
+        }
+    };
+    int buf = 0;
+    if constexpr (EPL == 4) {
+        const int lane = tid & 63, wave = tid >> 6;
+        // entries lane + 64 c: c = 0, 1 lie in the upper triangle, c = 2 up to entry
+        // 189, c = 3 is a diagonal entry, a row sum or nothing (entries from 230 on)
+        int ei[4], ej[4];
+        double m[4][20];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            pmat_entry(lane + 64 * c, ei[c], ej[c]);
+#pragma unroll
+            for (int k = 0; k < 20; k++)
+                m[c][k] = eg[ei[c] * 20 + k] * (ej[c] < 20 ? eg[kEigB + k * 20 + ej[c]] : bsum[k]);
+        }
+        double ratio[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) ratio[c] = eg[kEigPi + ei[c]] / eg[kEigPi + ej[c] % 20];
+        const bool up2 = lane + 128 < kPmUpper, has3 = lane + 192 < kPmEntries;
+        for (int q = q0; q < q1; q += PS, buf ^= 1) {
+            const int nq = min(PS, q1 - q);
+            TP *st = stage[buf];
+            if (wave < nq) {
+                const double *x = ex[q - q0 + wave];
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < 20; k++) {
+                    const double e = x[k];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) acc[c] = fma(m[c][k], e, acc[c]);
+                }
+                TP *sp = st + wave * kPStride;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    sp[ei[c] * 21 + ej[c]] = (TP)acc[c];
+                    sp[ej[c] * 21 + ei[c]] = (TP)(acc[c] * ratio[c]);
+                }
+                sp[ei[2] * 21 + ej[2]] = (TP)acc[2];
+                if (up2) sp[ej[2] * 21 + ei[2]] = (TP)(acc[2] * ratio[2]);
+                if (has3) sp[ei[3] * 21 + ej[3]] = (TP)acc[3];
+            }
+            store_step(st, q, nq);
+        }
+    } else {
+        // entry tid (none from 230 on), two pairs a step as independent chains
+        int i, j;
+        pmat_entry(tid, i, j);
+        const bool up = tid < kPmUpper, has = tid < kPmEntries;
+        double m[20];
+#pragma unroll
+        for (int k = 0; k < 20; k++) m[k] = eg[i * 20 + k] * (j < 20 ? eg[kEigB + k * 20 + j] : bsum[k]);
+        const double ratio = eg[kEigPi + i] / eg[kEigPi + j % 20];
+        for (int q = q0; q < q1; q += PS, buf ^= 1) {
+            const int nq = min(PS, q1 - q);
+            TP *st = stage[buf];
+            double acc[PS];
+#pragma unroll
+            for (int u = 0; u < PS; u++) acc[u] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 20; k++)
+#pragma unroll
+                for (int u = 0; u < PS; u++) acc[u] = fma(m[k], ex[min(q - q0 + u, q1 - q0 - 1)][k], acc[u]);
+            if (has) {
+#pragma unroll
+                for (int u = 0; u < PS; u++) {
+                    if (u < nq) {
+                        st[u * kPStride + i * 21 + j] = (TP)acc[u];
+                        if (up) st[u * kPStride + j * 21 + i] = (TP)(acc[u] * ratio);
+                    }
+                }
+            }
+            store_step(st, q, nq);
         }
     }
 }
@@ -354,7 +395,7 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ ei
     const int q0 = blockIdx.y * PAIRS, q1 = min(q0 + PAIRS, nn * R);
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
-    pmat_pairs<TP, PAIRS>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+    pmat_pairs<TP, PAIRS, (PAIRS >= kPmPairsLarge ? 4 : 1)>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
                           [](int n) { return n; }, [](int q) { return (size_t)q; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
@@ -379,7 +420,7 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat_nodes(const double *__restric
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
     const int *nl = nodes + n0;
-    pmat_pairs<TP, kPmPairs>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+    pmat_pairs<TP, kPmPairs, 1>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
                              [nl](int i) { return nl[i]; },
                              [nl, R](int q) { return (size_t)nl[q / R] * R + q % R; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
