@@ -232,18 +232,23 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // wave covers all 230 entries of a pair; a step gives each of the workgroup's waves
 // its own pair, and an exponential read from LDS feeds four multiply-adds (LDS
 // bandwidth, not the FP64 units, bounds the one-entry form on long runs).  Short runs
-// (32 pairs, node lists) give lane t entry t and two pairs a step: their per-workgroup
-// set-up of M would cost four times as much in the four-entry form.  The entries of a
-// step's pairs go through LDS (two buffers, one barrier a step) and leave in 16-byte
-// consecutive-lane stores of the whole workgroup (measured faster than waves that run
-// their own pairs with no barrier and store their own rows).  The exponentials of all
-// pairs of the run are computed once, spread over the workgroup.
+// (32 pairs) give lane t entry t and two pairs a step: their per-workgroup set-up of M
+// would cost four times as much in the four-entry form.  The entries of a step's
+// pairs go through LDS (two buffers, one barrier a step) and leave in 16-byte
+// consecutive-lane stores of the whole workgroup (in the four-entry form this measured
+// faster than waves that run their own pairs with no barrier and store their own
+// rows).  Node lists (a few pairs per workgroup) store each entry and its mirror
+// straight from the lane: there the barriers cost more than the coalescing gains.
+// The exponentials of all pairs of the run are computed once, spread over the
+// workgroup.
 constexpr int kPmLanes = 256;
 constexpr int kPmWaves = kPmLanes / 64;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
 constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
 constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 128 pairs from which launches use them
 constexpr int kPmUpper = 190, kPmEntries = 230;
+// forms: four entries per lane, staged; one entry per lane, staged; one entry, direct
+constexpr int kPmFour = 0, kPmOneStaged = 1, kPmOneDirect = 2;
 
 // entry e of the list above -> (i, j); j = 20 for the row sums
 __device__ __forceinline__ void pmat_entry(int e, int &i, int &j)
@@ -267,19 +272,17 @@ __device__ __forceinline__ void pmat_entry(int e, int &i, int &j)
 
 // node_of(i): node of the run's i-th node; pair_out(q): index n * R + r of pair q's
 // matrix in the class's P block
-template <typename TP, int PAIRS, int EPL, typename NodeOf, typename PairOut>
+template <typename TP, int PAIRS, int FORM, typename NodeOf, typename PairOut>
 __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const double *__restrict__ bl,
                                            const double *__restrict__ rt, int R, int q0, int q1,
                                            NodeOf node_of, PairOut pair_out, TP *__restrict__ Pcls)
 {
-    static_assert(EPL == 1 || EPL == 4, "one or four entries per lane");
+    static_assert(FORM == kPmFour || FORM == kPmOneStaged || FORM == kPmOneDirect, "P form");
     constexpr int V = 16 / (int)sizeof(TP);  // entries per 16-byte store
     constexpr int NV = kPStride / V;
     static_assert(kPStride % V == 0, "P rows must split into 16-byte stores");
-    constexpr int PS = EPL == 4 ? kPmWaves : 2;  // pairs per step
     __shared__ double bsum[20];
     __shared__ double ex[PAIRS][20];
-    __shared__ __align__(16) TP stage[2][PS * kPStride];
     const int tid = threadIdx.x;
     if (tid < 20) {
         double a = 0.0;
@@ -292,7 +295,9 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         ex[x / 20][k] = exp(eg[kEigLam + k] * bl[node_of(q / R)] * rt[q % R]);
     }
     __syncthreads();
-    // the step's entries to LDS, then 16-byte runs of the step's matrices out
+    // staged forms: the step's entries to LDS, then 16-byte runs of the step's matrices out
+    constexpr int PS = FORM == kPmFour ? kPmWaves : 2;  // pairs per step
+    __shared__ __align__(16) TP stage[FORM == kPmOneDirect ? 1 : 2][FORM == kPmOneDirect ? 1 : PS * kPStride];
     auto store_step = [&](const TP *st, int q, int nq) {
         __syncthreads();
         for (int x = tid; x < nq * NV; x += kPmLanes) {
@@ -301,7 +306,7 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         }
     };
     int buf = 0;
-    if constexpr (EPL == 4) {
+    if constexpr (FORM == kPmFour) {
         const int lane = tid & 63, wave = tid >> 6;
         // entries lane + 64 c: c = 0, 1 lie in the upper triangle, c = 2 up to entry
         // 189, c = 3 is a diagonal entry, a row sum or nothing (entries from 230 on)
@@ -342,7 +347,7 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
             }
             store_step(st, q, nq);
         }
-    } else {
+    } else if constexpr (FORM == kPmOneStaged) {
         // entry tid (none from 230 on), two pairs a step as independent chains
         int i, j;
         pmat_entry(tid, i, j);
@@ -372,6 +377,40 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
             }
             store_step(st, q, nq);
         }
+    } else {
+        // entry tid (none from 230 on), two pairs a step as independent chains, stored
+        // straight from the lanes (node lists: staging through LDS measured slower on
+        // these runs of a few pairs, whose barriers cost more than the coalescing gains)
+        int i, j;
+        pmat_entry(tid, i, j);
+        const bool up = tid < kPmUpper;
+        if (tid >= kPmEntries) return;  // no barrier follows
+        double m[20];
+#pragma unroll
+        for (int k = 0; k < 20; k++) m[k] = eg[i * 20 + k] * (j < 20 ? eg[kEigB + k * 20 + j] : bsum[k]);
+        const double ratio = eg[kEigPi + i] / eg[kEigPi + j % 20];
+        auto put = [&](int q, double pa) {
+            TP *out = Pcls + pair_out(q) * kPStride;
+            out[i * 21 + j] = (TP)pa;
+            if (up) out[j * 21 + i] = (TP)(pa * ratio);
+        };
+        int q = q0;
+        for (; q + 2 <= q1; q += 2) {
+            double pa = 0.0, pb = 0.0;
+#pragma unroll
+            for (int k = 0; k < 20; k++) {
+                pa = fma(m[k], ex[q - q0][k], pa);
+                pb = fma(m[k], ex[q - q0 + 1][k], pb);
+            }
+            put(q, pa);
+            put(q + 1, pb);
+        }
+        if (q < q1) {
+            double pa = 0.0;
+#pragma unroll
+            for (int k = 0; k < 20; k++) pa = fma(m[k], ex[q - q0][k], pa);
+            put(q, pa);
+        }
     }
 }
 
@@ -379,7 +418,7 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
 // workgroup (particle * max_classes + class, chunk of PAIRS pairs).  The per-class
 // setup of a workgroup (row sums, per-entry products) is shared by its PAIRS pairs:
 // launches with many workgroups take long runs, small ones short runs for parallelism.
-template <typename TP, int PAIRS>
+template <typename TP, int PAIRS, int FORM>
 __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ eig,
                                                     const double *__restrict__ blen,
                                                     const double *__restrict__ rates,
@@ -395,7 +434,7 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat(const double *__restrict__ ei
     const int q0 = blockIdx.y * PAIRS, q1 = min(q0 + PAIRS, nn * R);
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
-    pmat_pairs<TP, PAIRS, (PAIRS >= kPmPairsLarge ? 4 : 1)>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+    pmat_pairs<TP, PAIRS, FORM>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
                           [](int n) { return n; }, [](int q) { return (size_t)q; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
 
@@ -420,7 +459,7 @@ __global__ __launch_bounds__(kPmLanes) void k_pmat_nodes(const double *__restric
     if (q0 >= q1) return;
     const size_t g = (size_t)p * max_classes + k;
     const int *nl = nodes + n0;
-    pmat_pairs<TP, kPmPairs, 1>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
+    pmat_pairs<TP, kPmPairs, kPmOneDirect>(eig + g * kEigStride, blen + (size_t)p * nn, rates + (size_t)p * R, R, q0, q1,
                              [nl](int i) { return nl[i]; },
                              [nl, R](int q) { return (size_t)nl[q / R] * R + q % R; }, Pout + ((size_t)pcls_off[p] + k) * nn * R * kPStride);
 }
@@ -445,28 +484,36 @@ void launch_eigen_cand(const double *base, const int *ent_particle, const int *e
                        copy_known ? 1 : 0);
 }
 
-template <int PAIRS>
+template <int PAIRS, int FORM>
 void launch_pmat_pairs(const PmatArgs &a, hipStream_t s)
 {
     dim3 grid(a.count * a.max_classes, (a.nn * a.R + PAIRS - 1) / PAIRS);
     if (a.fp32)
-        hipLaunchKernelGGL((k_pmat<float, PAIRS>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL((k_pmat<float, PAIRS, FORM>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (float *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
     else
-        hipLaunchKernelGGL((k_pmat<double, PAIRS>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
+        hipLaunchKernelGGL((k_pmat<double, PAIRS, FORM>), grid, dim3(kPmLanes), 0, s, a.eig, a.blen, a.rates,
                            a.num_classes, a.pcls_off, (double *)a.P, a.nn, a.R, a.max_classes,
                            a.first, a.plist);
 }
+
+// Launches with workgroups to spare take runs of 128 pairs in the four-entry form;
+// the others runs of 32, staged when they cover many particles (full sweeps) and
+// stored straight from the lanes when they cover a few (the whole-particle P of an
+// MCMC round, which runs beside the round's pruning: measured faster there)
+constexpr int kPmDirectParticles = 8;
 
 void launch_pmat(const PmatArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return;
     const int64_t large = (int64_t)a.count * a.max_classes * ((a.nn * a.R + kPmPairsLarge - 1) / kPmPairsLarge);
     if (large >= kPmLargeGrid)
-        launch_pmat_pairs<kPmPairsLarge>(a, s);
+        launch_pmat_pairs<kPmPairsLarge, kPmFour>(a, s);
+    else if (a.count <= kPmDirectParticles)
+        launch_pmat_pairs<kPmPairs, kPmOneDirect>(a, s);
     else
-        launch_pmat_pairs<kPmPairs>(a, s);
+        launch_pmat_pairs<kPmPairs, kPmOneStaged>(a, s);
 }
 
 }  // namespace smccat
