@@ -1161,8 +1161,8 @@ __global__ __launch_bounds__(256) void k_prune_chain(KArgs<TX> ka, const int *__
     }
 }
 
-// Walks, persistent grid-stride with one tile per workgroup (ka.blocks_per_item =
-// tile stride): walk items [0, n) of `witems`.
+// Walks, one tile per workgroup (ka.blocks_per_item = tile stride; grid-stride, so
+// any grid works): walk items [0, n) of `witems`.
 template <typename TX>
 __global__ __launch_bounds__(512) void k_prune_walk(KArgs<TX> ka, const WalkItem *__restrict__ witems,
                                                     const int4 *__restrict__ entries, int n,
@@ -1244,7 +1244,10 @@ void launch_walk_typed(const PruneArgs &a, const WalkItem *witems, const int4 *e
     ka.blocks_per_item = a.tile_stride;  // one workgroup per tile, its wavefronts per half and rate
     const int X = ka.xcds, bpi = ka.blocks_per_item;
     const int64_t units = (int64_t)(a.n_items + X - 1) / X * X * bpi;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid / 2 : units));
+    // one workgroup per unit: short-lived workgroups let the dispatcher interleave
+    // the second stream's level launches as they finish (a persistent grid of half
+    // the pruning grid, or of the resident capacity, measured slower)
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, INT32_MAX));
     hipLaunchKernelGGL((k_prune_walk<TX>), dim3(grid), dim3(512), 0, s, ka, witems, entries, a.n_items, rdot,
                        rexp);
 }
