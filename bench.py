@@ -323,9 +323,12 @@ def rejuv_compare(args):
       plain_fp64            one particle at a time (the reference's per-thread loop),
                             FP64, the Gibbs CAT move
       fp32_batched_gibbs    FP32, lockstep batches of 150 resident particles keyed by
-                            chain length (the sampler's shape) over 1,200 particles,
-                            the Gibbs CAT move in one lockstep of the ~410 particles
-                            that draw it at the config-4 population
+                            chain length (the sampler's shape): the config-4 population
+                            of 4,096 chains sorted by length into batches (27 full
+                            ones), of which 8 spread over the order are timed (1,200
+                            particles); the
+                            Gibbs CAT move in one lockstep of the ~410 particles that
+                            draw it at the config-4 population
       fp32_batched_learned  the same with the learned CAT move, its network trained
                             on >= 1M Gibbs draws recorded before the timed runs."""
     from smccat import Engine, ProposalNetwork
@@ -336,8 +339,11 @@ def rejuv_compare(args):
     kmax = kr[1] + 8
     nb, nbatch, it, tau = args.rj_particles, args.rj_batches, args.rj_iterations, args.rj_temp
     popn = nb * nbatch
-    pop = sim.prior_particles(np.random.default_rng(4), popn, T, S, num_rates=R, invar=True, k_range=kr,
-                              max_classes=kmax)
+    # the population's chains keyed by length (the sampler's order) into lockstep
+    # batches of nb; the timed batches are spread evenly over that order
+    npop = max(args.rj_population, popn)
+    pop_all = sim.prior_particles(np.random.default_rng(4), npop, T, S, num_rates=R, invar=True, k_range=kr,
+                                  max_classes=kmax)
     out = {"workload": "c4 data (%d taxa x %d sites, GTR+G4+I, DP-CAT), K ~ U{%d..%d}, %d MCMC iterations, CAT "
                        "move probability 0.1, DP alpha 1.0, tempering %.2f" % (T, S, kr[0], kr[1], it, tau)}
     # ---- fast paths: FP32, batches of nb particles resident, keys sorted by chain length
@@ -345,9 +351,16 @@ def rejuv_compare(args):
                  pmat_classes=int(nb * (kr[0] + kr[1]) / 2 * 1.4))
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
+    full = eng.rejuvenation_steps(it, 5, np.arange(npop, dtype=np.int32), invariant=True)
+    order = np.argsort(-full, kind="stable").astype(np.int32)
+    nbat_all = npop // nb  # full batches (a last, partial one is not timed)
+    chosen = [int((j + 0.5) * nbat_all / nbatch) for j in range(nbatch)]
+    keys = np.concatenate([order[b * nb:(b + 1) * nb] for b in chosen]).astype(np.int32)
+    assert len(keys) == popn, "timed batches must be full"
+    pop = pop_all.take(keys)
     eng.set_particles(pop)
     net = ProposalNetwork(eng, seed=1)
-    # training pairs: Gibbs draws recorded on the whole population (the CAT move scores
+    # training pairs: Gibbs draws recorded on the timed particles (the CAT move scores
     # single sites from the tips and needs no resident partials)
     t0 = time.perf_counter()
     k = 0
@@ -362,9 +375,7 @@ def rejuv_compare(args):
     progress("rejuv: %d training pairs recorded in %.1f s, %d epochs in %.1f s (loss %.3f -> %.3f)" %
              (n_pairs, t_col, args.rj_epochs, t_train, losses[0], losses[-1]))
     eng.set_particles(pop)  # the timed runs start from the initial population
-    keys = np.arange(popn, dtype=np.int32)
     steps = eng.rejuvenation_steps(it, 5, keys, invariant=True)
-    keys = keys[np.argsort(-steps, kind="stable")]
     t0 = time.perf_counter()
     stats_tot = np.zeros((4, 2), dtype=np.int64)
     for b in range(nbatch):
@@ -374,15 +385,13 @@ def rejuv_compare(args):
     t_mcmc = time.perf_counter() - t0
     s_step = t_mcmc / float(steps.sum())
     # lockstep efficiency: MH steps over (batch size x rounds), rounds being the
-    # longest chain of each batch; the same for the config-4 population of 4096
-    # particles keyed by step count
-    ssort = np.sort(steps)[::-1]
-    rounds = sum(int(ssort[b * nb:(b + 1) * nb].max()) for b in range(nbatch))
-    full = eng.rejuvenation_steps(it, 5, np.arange(4096, dtype=np.int32), invariant=True)
-    fsort = np.sort(full)[::-1]
-    slots_c4 = sum(len(fsort[b:b + nb]) * int(fsort[b:b + nb].max()) for b in range(0, len(fsort), nb))
+    # longest chain of each batch, for the timed batches and for the whole population
+    rounds = sum(int(steps[b * nb:(b + 1) * nb].max()) for b in range(nbatch))
+    fsort = full[order]
+    slots_all = sum(len(fsort[b:b + nb]) * int(fsort[b:b + nb].max()) for b in range(0, len(fsort), nb))
     lockstep = {"rounds": rounds, "efficiency": float(steps.sum()) / (nb * rounds),
-                "efficiency_4096_particles": float(full.sum()) / slots_c4}
+                "population": npop, "batches_in_population": nbat_all, "timed_batches": chosen,
+                "efficiency_population": float(full.sum()) / slots_all}
     progress("rejuv fast MCMC: %d particles in batches of %d: %.1f ms per particle, %.1f us per MH step" %
              (popn, nb, t_mcmc / popn * 1e3, s_step * 1e6))
     # CAT moves: the sampler moves every particle that draws one (probability 0.1)
@@ -404,7 +413,7 @@ def rejuv_compare(args):
     t_ref = (time.perf_counter() - t0) / nb
     net.close()
     eng.close()
-    mean_steps = float(steps.mean())
+    mean_steps = float(full.mean())  # the population's mean chain
     fast = {}
     for name in ("gibbs", "learned"):
         per = s_step * mean_steps + 0.1 * cat[name] + t_ref
@@ -418,16 +427,16 @@ def rejuv_compare(args):
     progress("rejuv fast CAT move per particle: Gibbs %.3f s, learned %.3f s (acceptance %.2f)" %
              (cat["gibbs"], cat["learned"], acc))
     out["mean_steps"] = mean_steps
-    out["population"] = popn
+    out["timed_particles"] = popn
     out["lockstep"] = lockstep
     out["mh_stats_proposed_accepted"] = stats_tot.tolist()
     out["fp32_batched_gibbs"] = fast["gibbs"]
     out["fp32_batched_learned"] = fast["learned"]
     # ---- plain path: FP64, one particle at a time, on particles spread over the
     # population (a particle's cost per step depends on its move mix and classes)
-    npl = max(1, min(args.rj_plain, popn))
-    pick = np.linspace(0, popn - 1, npl).round().astype(np.int32)
-    sub = pop.take(pick)
+    npl = max(1, min(args.rj_plain, npop))
+    pick = np.linspace(0, npop - 1, npl).round().astype(np.int32)
+    sub = pop_all.take(pick)
     eng = Engine(T, S, R, npl, kmax, "fp64", workspaces=2)
     eng.set_mixture("logspace")
     eng.set_alignment(codes)
@@ -549,7 +558,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rj-particles", type=int, default=150, help="particles per resident lockstep batch")
     ap.add_argument("--rj-batches", type=int, default=8,
-                    help="resident lockstep batches timed (the population is rj-particles x rj-batches)")
+                    help="resident lockstep batches timed, spread over the population's batches")
+    ap.add_argument("--rj-population", type=int, default=4096,
+                    help="population whose chains are keyed by length into lockstep batches (0: the timed "
+                         "particles alone)")
     ap.add_argument("--rj-plain", type=int, default=12, help="particles of the plain FP64 path, spread over the population")
     ap.add_argument("--rj-iterations", type=int, default=16)
     ap.add_argument("--rj-temp", type=float, default=0.5)
