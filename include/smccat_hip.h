@@ -438,6 +438,14 @@ int smc_engine_reset_stats(smc_engine *e);
 enum { SMC_PRUNE_MFMA = 0, SMC_PRUNE_VALU = 1, SMC_PRUNE_MFMA_LEVELS = 2, SMC_PRUNE_MFMA_WALK = 3 };
 int smc_engine_set_variant(smc_engine *e, int family, int variant);
 
+/* Scheduling of lockstep rejuvenation (no reference counterpart: the reference
+ * runs one particle at a time, main.cpp:296-311).  Enabled (the default), a batch's
+ * whole-tree steps (shape and rate-matrix moves) are refreshed on a stream of their
+ * own and decided when done, while the batch's other particles go on with their
+ * rounds; disabled, every round waits for all its particles.  The chains, and so
+ * the results, are the same either way. */
+int smc_engine_set_lane(smc_engine *e, int enabled);
+
 /* Site mixture at the root.  SMC_MIX_REFERENCE (default) is the reference's
  * formula exactly (Particle.cpp:653-670): log(exp(l)(1-p) + p pi[c] isInv), which
  * underflows to -inf below l = -745 and is NaN when a rate term is -inf, as in

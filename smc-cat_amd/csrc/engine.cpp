@@ -681,6 +681,8 @@ int smc_engine_create(const smc_engine_config *cfg, smc_engine **out)
     e->h_eig_save.assign(P, -1);
     e->h_prog_stale.assign(P, 0);
     e->h_touched.assign(P, 0);
+    e->h_busy.assign(P, 0);
+    e->lane.lnl.assign(P, 0.0);
     e->h_pnodes.assign(P, {});
     e->committed.resize(P);
     e->h_owner.assign(e->Wmax, -1);
@@ -703,8 +705,20 @@ int smc_engine_destroy(smc_engine *e)
                     e->d_eig_save};
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->stream2) (void)hipStreamSynchronize(e->stream2);
+    if (e->lane.stream) (void)hipStreamSynchronize(e->lane.stream);
     if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->h_read) (void)hipHostFree(e->h_read);
+    {
+        smc_engine::Lane &l = e->lane;
+        if (l.h_stage) (void)hipHostFree(l.h_stage);
+        if (l.h_read) (void)hipHostFree(l.h_read);
+        for (void *b : {(void *)l.d_lnl, (void *)l.d_nonfinite, (void *)l.d_items, (void *)l.d_chain_nodes,
+                        (void *)l.d_chain_len, (void *)l.d_plist, (void *)l.d_pnodes})
+            if (b) (void)hipFree(b);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+        if (l.fork) (void)hipEventDestroy(l.fork);
+        if (l.done) (void)hipEventDestroy(l.done);
+    }
     for (smc_engine::CachedGraph &c : e->graphs) (void)hipGraphExecDestroy(c.exec);
     if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
     if (e->d_pnodes) (void)hipFree(e->d_pnodes);
@@ -1216,6 +1230,73 @@ namespace {
 
 // Compact class slots of one resident wave [first, first+count); fails if the
 // wave does not fit the partial slots or the P buffer.
+// The lane's resources, allocated on its first use: the sizes of the engine's own.
+int lane_alloc(smc_engine *e)
+{
+    smc_engine::Lane &l = e->lane;
+    if (l.stream) return SMC_OK;
+    const size_t P = e->Pmax, T = e->T, nn = e->nn;
+    int rc = SMC_OK;
+    auto A = [&](auto **ptr, size_t n) {
+        if (rc == SMC_OK && !*ptr) rc = dev_alloc(e, ptr, n);
+    };
+    A(&l.d_lnl, P);
+    A(&l.d_nonfinite, P);
+    A(&l.d_items, P * (T - 1));
+    A(&l.d_chain_nodes, P * (T - 1));
+    A(&l.d_chain_len, P * (T - 1));
+    A(&l.d_plist, 4 * P);
+    A(&l.d_pnodes, 2 * P * nn);
+    if (rc != SMC_OK) return rc;
+    if (!l.h_stage) {
+        void *hs = nullptr;
+        HIP_TRY(e, hipHostMalloc(&hs, e->stage_cap, hipHostMallocDefault));
+        l.h_stage = (uint8_t *)hs;
+        l.stage_cap = e->stage_cap;
+    }
+    if (!l.h_read) {
+        void *hr = nullptr;
+        HIP_TRY(e, hipHostMalloc(&hr, e->read_cap, hipHostMallocDefault));
+        l.h_read = (uint8_t *)hr;
+        l.read_cap = e->read_cap;
+    }
+    if (!l.fork) HIP_TRY(e, hipEventCreateWithFlags(&l.fork, hipEventDisableTiming));
+    if (!l.done) HIP_TRY(e, hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    HIP_TRY(e, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));  // last: marks the lane ready
+    return SMC_OK;
+}
+
+// Exchanges the lane's stream, staging, readback and scratch buffers with the
+// engine's own (twice: in, then out).
+void swap_lane(smc_engine *e)
+{
+    smc_engine::Lane &l = e->lane;
+    std::swap(e->stream, l.stream);
+    std::swap(e->h_stage, l.h_stage);
+    std::swap(e->stage_cap, l.stage_cap);
+    std::swap(e->stage_off, l.stage_off);
+    std::swap(e->h_read, l.h_read);
+    std::swap(e->read_cap, l.read_cap);
+    std::swap(e->read_off, l.read_off);
+    std::swap(e->readbacks, l.readbacks);
+    std::swap(e->d_lnl, l.d_lnl);
+    std::swap(e->d_nonfinite, l.d_nonfinite);
+    std::swap(e->d_items, l.d_items);
+    std::swap(e->d_chain_nodes, l.d_chain_nodes);
+    std::swap(e->d_chain_len, l.d_chain_len);
+    std::swap(e->d_plist, l.d_plist);
+    std::swap(e->d_pnodes, l.d_pnodes);
+}
+
+struct LaneGuard {
+    smc_engine *e;
+    bool on;
+    ~LaneGuard()
+    {
+        if (on) swap_lane(e);
+    }
+};
+
 int set_single_wave(smc_engine *e, int first, int count)
 {
     if (count > e->Wmax) return fail(e, SMC_ERR_ARG, "range exceeds partial_slots");
@@ -1423,8 +1504,9 @@ int smc_update_params(smc_engine *e, int first, int count, const int32_t *childr
     return upload_params(e, first, count, children, branch_len, base, pinv, shape);
 }
 
-int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
-                            const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out)
+extern "C++" {
+int refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
+                        const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out, bool on_lane)
 {
     if (!e) return SMC_ERR_ARG;
     PhaseTimer phase_timer(e, SMC_PHASE_REFRESH_INCREMENTAL);
@@ -1433,6 +1515,20 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         return fail(e, SMC_ERR_ARG, "invalid incremental refresh arguments");
     for (int p = first; p < first + count; ++p)
         if (!e->h_valid[p]) return fail(e, SMC_ERR_STATE, "slot " + std::to_string(p) + " not uploaded");
+    if (on_lane) {
+        if (e->lane.in_flight || e->timing) return fail(e, SMC_ERR_STATE, "the lane is busy or timing is on");
+        int rc0 = lane_alloc(e);
+        if (rc0 != SMC_OK) return rc0;
+        // the lane starts after what the caller queued on the engine's stream (the
+        // proposals' parameter uploads)
+        HIP_TRY(e, hipEventRecord(e->lane.fork, e->stream));
+    }
+    LaneGuard lane_guard{e, false};
+    if (on_lane) {
+        swap_lane(e);
+        lane_guard.on = true;
+        HIP_TRY(e, hipStreamWaitEvent(e->stream, e->lane.fork, 0));
+    }
     int rc = set_single_wave(e, first, count);
     if (rc != SMC_OK) return rc;
     const int T = e->T, Ti = T - 1, nn = e->nn;
@@ -1454,10 +1550,13 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     // chains are used when the particles that move in this call give per-level
     // launches of fewer units than the merged-small-launch bound below.
     int moving = 0;
+    std::vector<uint8_t> mov(count, 0);  // particles with flagged nodes or eigensystems
     for (int i = 0; i < count; ++i) {
         bool any = eig_flags && eig_flags[i];
         const uint8_t *nf = node_flags + (size_t)i * nn;
         for (int n = 0; n < nn && !any; ++n) any = nf[n] != 0;
+        if (e->h_busy[first + i]) any = false;  // on the lane: left alone
+        mov[i] = any ? 1 : 0;
         moving += any ? 1 : 0;
     }
     const bool chains_fit = (int64_t)moving * ((e->tile_stride + 3) / 4) <= (int64_t)e->num_cus * 8;
@@ -1472,6 +1571,14 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     std::vector<uint8_t> wcut(nn), walked(count, 0);
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
+        // the lane's refresh takes only its flagged particles; the others leave the
+        // lane's particles alone (their lnL stays until lane_finish)
+        if (on_lane && !mov[i]) continue;
+        if (!on_lane && e->h_busy[p]) {
+            keep.push_back(p);
+            keep_lnl.push_back(e->h_lnl[p]);
+            continue;
+        }
         const uint8_t *nf = node_flags + (size_t)i * nn;
         const bool eig = eig_flags && eig_flags[i];
         // P: every node of a particle whose eigensystems change, else the nodes
@@ -1547,8 +1654,8 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
         const bool use_chains = e->prune_variant == SMC_PRUNE_MFMA && chains_fit && n_stored <= e->tune.chain_max;
         // the variant's walks, or (SMC_PRUNE_MFMA) the small dirty sets when the call
         // holds many: dirty paths walked in one launch, whole trees level by level
-        const bool walk_p = use_walk || (walk_ok && e->prune_variant == SMC_PRUNE_MFMA && !chains_fit &&
-                                         n_stored <= e->tune.chain_max);
+        const bool walk_p = !on_lane && (use_walk || (walk_ok && e->prune_variant == SMC_PRUNE_MFMA &&
+                                                      !chains_fit && n_stored <= e->tune.chain_max));
         walked[i] = walk_p ? 1 : 0;
         for (int n = T; n < nn; ++n) {
             if (!dirty[n] || n == root) continue;
@@ -1714,7 +1821,9 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     if (!e->pend_full.empty() || !e->pend_nodes.empty()) {
         std::vector<uint8_t> full(count, 0);
         for (int p : p_list) full[p - first] = 1;
-        auto in = [&](int p) { return p >= first && p < first + count; };
+        auto in = [&](int p) {
+            return p >= first && p < first + count && (on_lane ? mov[p - first] != 0 : !e->h_busy[p]);
+        };
         std::vector<int> keep_full;
         for (int p : e->pend_full) {
             if (!in(p)) {
@@ -1749,7 +1858,7 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     // level), the levels run on the second stream beside the walks: their narrow
     // deep launches then fill the gaps of the walk instead of following it; the
     // models of the levels' particles are recomputed on that stream too.
-    const bool split = !e->timing && !wroots.empty() && !roots.empty();
+    const bool split = !on_lane && !e->timing && !wroots.empty() && !roots.empty();
     ModelJob job_w, job_t;  // (split) the levels' and the walks' models
     if (split) {
         std::vector<int> eig_w, eig_t, pl_w, pl_t;
@@ -1766,7 +1875,14 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     }
     if (rc != SMC_OK) return rc;
     hipStream_t s = e->stream;
-    rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
+    if (on_lane) {  // only its particles' rows: the engine's stream may be writing the others
+        for (int i = 0; i < count && rc == SMC_OK; ++i)
+            if (mov[i])
+                rc = h2d(e, e->d_xflag + (size_t)(first + i) * Ti, e->h_xflag.data() + (size_t)(first + i) * Ti,
+                         (size_t)Ti);
+    } else {
+        rc = h2d(e, e->d_xflag + (size_t)first * Ti, e->h_xflag.data() + (size_t)first * Ti, (size_t)count * Ti);
+    }
     if (rc != SMC_OK) return rc;
     std::vector<Item> items;
     std::vector<int> off;
@@ -1885,6 +2001,19 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
                           e->d_nonfinite, s);
     }
     HIP_TRY(e, hipGetLastError());
+    if (on_lane) {  // no wait: lane_finish collects the lnL of its particles
+        rc = d2h(e, e->lane.lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double));
+        if (rc != SMC_OK) return rc;
+        HIP_TRY(e, hipEventRecord(e->lane.done, e->stream));
+        e->lane.particles.clear();
+        for (int i = 0; i < count; ++i)
+            if (mov[i]) {
+                e->lane.particles.push_back(first + i);
+                e->h_busy[first + i] = 1;
+            }
+        e->lane.in_flight = true;
+        return SMC_OK;
+    }
     rc = d2h(e, e->h_lnl.data() + first, e->d_lnl + first, (size_t)count * sizeof(double));
     if (rc == SMC_OK && nonfinite_out) rc = d2h(e, nonfinite_out, e->d_nonfinite + first, (size_t)count);
     if (rc == SMC_OK) rc = sync_stream(e);
@@ -1898,6 +2027,41 @@ int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *
     return SMC_OK;
 }
 
+int lane_finish(smc_engine *e, bool wait, bool *done)
+{
+    smc_engine::Lane &l = e->lane;
+    *done = true;
+    if (!l.in_flight) return SMC_OK;
+    if (!wait) {
+        const hipError_t q = hipEventQuery(l.done);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // not an error: clear it
+            *done = false;
+            return SMC_OK;
+        }
+        HIP_TRY(e, q);
+    }
+    HIP_TRY(e, hipEventSynchronize(l.done));
+    for (const smc_engine::Readback &r : l.readbacks) std::memcpy(r.dst, l.h_read + r.off, r.bytes);
+    l.readbacks.clear();
+    l.read_off = 0;
+    l.stage_off = 0;
+    for (int p : l.particles) {
+        e->h_lnl[p] = l.lnl[p];
+        e->h_busy[p] = 0;
+    }
+    l.particles.clear();
+    l.in_flight = false;
+    return SMC_OK;
+}
+}  // extern "C++"
+
+int smc_refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
+                            const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out)
+{
+    return refresh_incremental(e, first, count, node_flags, eig_flags, lnl_out, nonfinite_out, false);
+}
+
 int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept)
 {
     if (!e) return SMC_ERR_ARG;
@@ -1909,6 +2073,7 @@ int smc_accept_reject(smc_engine *e, int first, int count, const uint8_t *accept
     std::vector<Item> p_nodes;
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
+        if (e->h_busy[p]) continue;  // recomputed on the lane: decided after lane_finish
         if (accept[i]) {
             smc_commit_range(e, p, 1);
             continue;
@@ -2294,6 +2459,16 @@ int smc_engine_set_mixture(smc_engine *e, int mode)
     if (mode != SMC_MIX_REFERENCE && mode != SMC_MIX_LOGSPACE)
         return fail(e, SMC_ERR_ARG, "unknown mixture mode");
     e->mix_mode = mode;
+    return SMC_OK;
+}
+
+int smc_engine_set_lane(smc_engine *e, int enabled)
+{
+    if (!e) return SMC_ERR_ARG;
+    bool done = true;
+    int rc = lane_finish(e, true, &done);
+    if (rc != SMC_OK) return rc;
+    e->lane_enabled = enabled != 0;
     return SMC_OK;
 }
 

@@ -224,6 +224,31 @@ struct smc_engine {
     size_t read_cap = 0, read_off = 0;
     std::vector<Readback> readbacks;
 
+    // The lane: an incremental refresh that runs beside the caller's next calls (the
+    // whole-tree proposals of a lockstep batch, while the other particles go on with
+    // their rounds).  It has its own stream, staging and readback buffers, item and
+    // model lists and lnL output, swapped into the fields above while it is issued
+    // (swap_lane); its particles are busy (h_busy) until lane_finish.
+    struct Lane {
+        hipStream_t stream = nullptr;
+        hipEvent_t fork = nullptr, done = nullptr;
+        uint8_t *h_stage = nullptr;
+        size_t stage_cap = 0, stage_off = 0;
+        uint8_t *h_read = nullptr;
+        size_t read_cap = 0, read_off = 0;
+        std::vector<Readback> readbacks;
+        double *d_lnl = nullptr;
+        uint8_t *d_nonfinite = nullptr;
+        Item *d_items = nullptr;
+        int *d_chain_nodes = nullptr, *d_chain_len = nullptr, *d_plist = nullptr;
+        Item *d_pnodes = nullptr;
+        std::vector<double> lnl;     // [P] lnL read back by the lane's refresh
+        std::vector<int> particles;  // the particles it recomputes
+        bool in_flight = false;
+    } lane;
+    std::vector<uint8_t> h_busy;  // [P] particle recomputed by the lane (no commit, no reject)
+    bool lane_enabled = true;     // rejuvenation runs whole-tree steps on the lane
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -251,6 +276,15 @@ int d2h(smc_engine *e, void *dst, const void *src, size_t bytes);
 // Waits for the engine's stream and copies the queued readbacks out; the staging
 // and readback buffers are free again afterwards.
 int sync_stream(smc_engine *e);
+// Incremental refresh of slots [first, first + count) (smc_refresh_incremental);
+// with on_lane, issued on the lane and not waited for: the lnL of its changed
+// particles arrive at lane_finish.
+int refresh_incremental(smc_engine *e, int first, int count, const uint8_t *node_flags,
+                        const uint8_t *eig_flags, double *lnl_out, uint8_t *nonfinite_out, bool on_lane);
+// Completes the lane's refresh: with wait, waits for it; without, returns at once
+// with *done false while it still runs.  On completion the lane's particles hold
+// their new lnL (h_lnl) and are no longer busy.
+int lane_finish(smc_engine *e, bool wait, bool *done);
 // Site x candidate scoring buffers (d_sc_*) for `count` entries and `need` (entry,
 // candidate) pairs, grown geometrically.
 int ensure_site_class_buffers(smc_engine *e, int count, int64_t need);

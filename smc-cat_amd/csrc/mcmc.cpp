@@ -239,19 +239,60 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     for (int i = 0; i < count; ++i) put_committed(i);
     Params q;  // one proposal at a time (its vectors keep their capacity)
     std::vector<uint8_t> f(nn);
+    // Whole-tree steps (shape and rate-matrix moves: every P, and for the latter
+    // every eigensystem, then every node) run on the engine's lane: their refresh is
+    // issued on its own stream and collected when it is done, while the other
+    // particles go on with their rounds.  A particle waits while the lane is busy
+    // with other particles.  Each particle's chain is the same as without the lane
+    // (its random numbers are drawn in the same order), so the results are too.
+    const bool lane_ok = count > 1 && e->lane_enabled && !e->timing;
+    std::vector<uint8_t> role(count, 0), on_lane(count, 0), flags_l((size_t)count * nn), eig_l(count);
+    enum { IDLE = 0, MAIN = 1, LANE = 2 };
+    auto whole_tree = [&](int i) { return move[i] == SHAPE || move[i] == RATES; };
+    // the MH decision of particle i on its new lnL (Mcmc.cpp:62)
+    auto decide = [&](int i, double lnl_prop) {
+        const double ratio = (tempering * lnl_prop + new_prior[i]) - (tempering * lnl[i] + prior[i]) + hastings[i];
+        const double lu = std::log(rng[i].uniform());
+        acc[i] = (lu <= ratio) ? 1 : 0;
+        proposed[move[i]]++;
+        if (acc[i]) {
+            accepted[move[i]]++;
+            lnl[i] = lnl_prop;
+            prior[i] = new_prior[i];
+        }
+        left[i]--;
+    };
     for (;;) {
         bool active = false;
         for (int i = 0; i < count; ++i) active |= left[i] > 0;
         if (!active) break;
-        {
+        const bool lane_free = lane_ok && !e->lane.in_flight;
+        int n_main = 0, n_lane = 0;
+        for (int i = 0; i < count; ++i) {
+            role[i] = IDLE;
+            if (left[i] <= 0 || on_lane[i]) continue;
+            if (lane_ok && whole_tree(i)) {
+                if (lane_free) {
+                    role[i] = LANE;
+                    n_lane++;
+                }
+            } else {
+                role[i] = MAIN;
+                n_main++;
+            }
+        }
+        if (n_main + n_lane > 0) {
             std::optional<PhaseTimer> propose_timer(std::in_place, e, SMC_PHASE_PROPOSE);
             std::fill(flags.begin(), flags.end(), 0);
+            std::fill(flags_l.begin(), flags_l.end(), 0);
+            std::fill(eig.begin(), eig.end(), 0);
+            std::fill(eig_l.begin(), eig_l.end(), 0);
             for (int i = 0; i < count; ++i) {
+                if (role[i] == IDLE) continue;
                 const int p = first + i;
-                eig[i] = 0;
+                uint8_t &eg = role[i] == LANE ? eig_l[i] : eig[i];
                 hastings[i] = 0.0;
                 new_prior[i] = prior[i];
-                if (left[i] <= 0) continue;
                 const smc_engine::Committed &c = e->committed[p];
                 q.ch.assign(c.children.begin(), c.children.end());
                 q.bl.assign(c.blen.begin(), c.blen.end());
@@ -272,7 +313,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                         break;
                     case RATES:  // eigensystems recomputed (Q13 fixed)
                         hastings[i] = rate_matrix_move(g, q, o);
-                        eig[i] = 1;
+                        eg = 1;
                         break;
                 }
                 std::copy(q.ch.begin(), q.ch.end(), ch.begin() + (size_t)i * Ti * 2);
@@ -280,37 +321,54 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                 std::copy(q.base.begin(), q.base.end(), base.begin() + (size_t)i * 190);
                 pv[i] = q.pinv;
                 sh[i] = q.shape;
-                std::copy(f.begin(), f.end(), flags.begin() + (size_t)i * nn);
+                std::copy(f.begin(), f.end(), (role[i] == LANE ? flags_l : flags).begin() + (size_t)i * nn);
                 new_prior[i] = ln_prior_params(e, q, o.using_lg) + prior_cls[i];
             }
             propose_timer.reset();
             rc = smc_update_params(e, first, count, ch.data(), bl.data(), base.data(), pv.data(), sh.data());
             if (rc != SMC_OK) return rc;
+        }
+        if (n_lane > 0) {
+            rc = refresh_incremental(e, first, count, flags_l.data(), eig_l.data(), nullptr, nullptr, true);
+            if (rc != SMC_OK) return rc;
+            for (int i = 0; i < count; ++i)
+                if (role[i] == LANE) on_lane[i] = 1;
+        }
+        if (n_main > 0) {
             rc = smc_refresh_incremental(e, first, count, flags.data(), eig.data(), lnl_new.data(), nullptr);
             if (rc != SMC_OK) return rc;
             for (int i = 0; i < count; ++i) {
-                if (left[i] <= 0) {
-                    acc[i] = 1;  // unchanged particle: commit is a no-op
-                    continue;
-                }
-                const double ratio = (tempering * lnl_new[i] + new_prior[i]) -
-                                     (tempering * lnl[i] + prior[i]) + hastings[i];
-                const double lu = std::log(rng[i].uniform());
-                acc[i] = (lu <= ratio) ? 1 : 0;  // Mcmc.cpp:62
-                proposed[move[i]]++;
-                if (acc[i]) {
-                    accepted[move[i]]++;
-                    lnl[i] = lnl_new[i];
-                    prior[i] = new_prior[i];
-                }
-                left[i]--;
+                acc[i] = 1;  // a particle without a proposal: commit is a no-op
+                if (role[i] == MAIN) decide(i, lnl_new[i]);
             }
-            rc = smc_accept_reject(e, first, count, acc.data());
+            rc = smc_accept_reject(e, first, count, acc.data());  // the lane's particles are left alone
             if (rc != SMC_OK) return rc;
             for (int i = 0; i < count; ++i) {
+                if (role[i] != MAIN) continue;
                 if (!acc[i]) put_committed(i);  // the rejected proposal's rows go back
                 if (left[i] <= 0) next_move(i);
             }
+        }
+        if (e->lane.in_flight) {  // collected when done, or waited for when nothing else runs
+            bool done = false;
+            rc = lane_finish(e, n_main == 0, &done);
+            if (rc != SMC_OK) return rc;
+            if (done) {
+                for (int i = 0; i < count; ++i) {
+                    acc[i] = 1;
+                    if (on_lane[i]) decide(i, e->h_lnl[first + i]);
+                }
+                rc = smc_accept_reject(e, first, count, acc.data());
+                if (rc != SMC_OK) return rc;
+                for (int i = 0; i < count; ++i) {
+                    if (!on_lane[i]) continue;
+                    on_lane[i] = 0;
+                    if (!acc[i]) put_committed(i);
+                    if (left[i] <= 0) next_move(i);
+                }
+            }
+        } else if (n_main + n_lane == 0) {
+            return smc_fail(e, SMC_ERR_STATE, "rejuvenation made no progress");
         }
     }
     // with probability gibbs_probability: the CAT Gibbs move and a forced refresh

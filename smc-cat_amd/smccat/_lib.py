@@ -91,6 +91,7 @@ SIGNATURES = {
     "smc_discretize_gamma": (_I, [_D, _I, _DP]),
     "smc_lg_base_matrix": (None, [_DP]),
     "smc_engine_set_timing": (_I, [_P, _I]),
+    "smc_engine_set_lane": (_I, [_P, _I]),
     "smc_engine_kernel_stats": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _DP]),
     "smc_engine_reset_stats": (_I, [_P]),
     "smc_engine_device_bytes": (ctypes.c_int64, [_P]),

@@ -444,6 +444,11 @@ class Engine:
         m = {"reference": _lib.SMC_MIX_REFERENCE, "logspace": _lib.SMC_MIX_LOGSPACE}[mode]
         self._check(self.lib.smc_engine_set_mixture(self.h, m), "smc_engine_set_mixture")
 
+    def set_lane(self, on=True):
+        """Whole-tree steps of lockstep rejuvenation on a stream of their own
+        (default on; the results are the same either way)."""
+        self._check(self.lib.smc_engine_set_lane(self.h, 1 if on else 0), "smc_engine_set_lane")
+
     def set_timing(self, on=True):
         self._check(self.lib.smc_engine_set_timing(self.h, 1 if on else 0), "smc_engine_set_timing")
 

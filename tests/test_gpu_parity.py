@@ -99,6 +99,26 @@ def test_cat_classes_fp64(gpu, taxa, sites, R, krange, variant):
     np.testing.assert_allclose(lnl, want, rtol=RTOL_FP64)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_p_kernel_forms_bit_identical(gpu, precision):
+    """The P kernel's forms (one entry per lane staged through LDS for launches over
+    many particles, stored straight from the lanes for a few) give the same lnL bit
+    for bit: a refresh of all 16 particles against refreshes of 4 at a time."""
+    _say("case P forms: 16 particles at once against 4 at a time, %s" % precision)
+    rng = np.random.default_rng(31)
+    T, S, R = 30, 300, 4
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(T, S)).astype(np.int8))
+    batch = sim.prior_particles(rng, 16, T, S, num_rates=R, invar=True, k_range=(3, 20))
+    with Engine(T, S, R, 16, 20, precision) as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        whole, _ = eng.refresh_likelihood()
+        parts = np.concatenate([eng.refresh_likelihood(f, 4)[0] for f in range(0, 16, 4)])
+    assert np.array_equal(whole, parts)
+    if precision == "fp64":
+        np.testing.assert_allclose(whole, _oracle_lnl(codes, batch, R), rtol=RTOL_FP64)
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_nonfinite_flag(gpu, variant):
     _say("case nonfinite: pinv = 1.5, %s" % variant)

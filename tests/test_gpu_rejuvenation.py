@@ -111,3 +111,35 @@ def test_one_slot_engine_with_repeated_whole_tree_moves(gpu):
         assert stats[1:, 0].sum() > 0  # whole-tree moves were proposed
         want = orc.particle_lnl(codes, final.particle(0), R)
         np.testing.assert_allclose(lnl[0], want, rtol=1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lane_gives_the_same_chains(gpu, precision):
+    """Whole-tree steps refreshed on the lane (decided when done, while the other
+    particles go on) give the same chains as rounds that wait for every particle:
+    the same final lnL, parameters and move statistics, bit for bit."""
+    _say("case lane: 24 particles, 18 taxa x 120 sites GTR+G4+I CAT K<=4, 8 iterations, %s" % precision)
+    rng = np.random.default_rng(61)
+    T, S, R, N = 18, 120, 4, 24
+    codes = rng.integers(0, 20, size=(T, S)).astype(np.int8)
+    batch = sim.prior_particles(rng, N, T, S, num_rates=R, invar=True, k_range=(1, 4))
+    out = []
+    for lane in (True, False):
+        with Engine(T, S, R, N, 6, precision, workspaces=2) as eng:
+            eng.set_alignment(codes)
+            eng.set_particles(batch)
+            eng.set_lane(lane)
+            lnl, stats = eng.rejuvenate(0.7, 8, seed=9, dpp_alpha=0.5)
+            params = [eng.particle_params(i) for i in range(N)]
+            final = eng.snapshot(range(N), batch)
+        out.append((lnl, stats, params, final))
+    (l1, s1, p1, f1), (l0, s0, p0, _) = out
+    assert s1[3, 0] > 0 and s1[1, 0] > 0  # rate-matrix and shape steps were made
+    assert np.array_equal(l1, l0)
+    assert np.array_equal(s1, s0)
+    for a, b in zip(p1, p0):
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
+    if precision == "fp64":
+        want = np.array([orc.particle_lnl(codes, f1.particle(i), R) for i in range(N)])
+        np.testing.assert_allclose(l1, want, rtol=1e-10)
