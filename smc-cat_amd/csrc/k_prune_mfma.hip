@@ -856,11 +856,14 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
 // walk level).  A walk is a piece of a particle's dirty stored nodes in post-order
 // (short dirty sets are one piece with the root last; whole trees are cut into
 // pieces of at most kWalkPiece nodes, and a piece runs in the launch after the
-// pieces cut below it, which it reads back).  One workgroup of eight wavefronts
-// takes one tile of one piece: wavefront w takes the half h = w / 4 of the tile's
-// 64-slot window (blocks 2 h, 2 h + 1) and the rates w % 4, w % 4 + 4, ..., and
-// runs the whole piece, so every dependency between its nodes stays inside one
-// wavefront and no two workgroups of a launch depend on each other.  It
+// pieces cut below it, which it reads back).  One wavefront, a workgroup of its
+// own, takes one (tile, half, rate) of one piece: the half h of the tile's 64-slot
+// window (blocks 2 h, 2 h + 1) and the rates r, r + 4, ..., and runs the whole
+// piece, so every dependency between its nodes stays inside one wavefront and no
+// two wavefronts of a launch depend on each other (nothing is shared through LDS;
+// single-wavefront workgroups measured faster than the tile's eight wavefronts in
+// one workgroup, 16.6 against 18.8 ms per particle of the C4 batched MCMC: they
+// start and end one by one, beside the other stream's level launches).  It
 // generalises the chains below: a node's result stays in registers as the B
 // operand of the next entry when that entry is its parent (child A carried); any
 // other stored child -- a committed partial, a node of an earlier piece, or a dirty
@@ -878,13 +881,12 @@ __device__ __forceinline__ void chain_unit(const KArgs<TX> &ka, const int *__res
 // lane c16, window offset 32 (k / 2) + 2 c16 + (k % 2).
 template <typename TX>
 __device__ __forceinline__ void walk_unit(const KArgs<TX> &ka, const int4 *__restrict__ entries, const WalkItem wi,
-                                          int tile, double *__restrict__ rdot, int *__restrict__ rexp)
+                                          int tile, int half, int rate0, double *__restrict__ rdot,
+                                          int *__restrict__ rexp)
 {
     using V4 = typename Mfma<TX>::v4;
     using T2 = typename Vec2<TX>::t;
     const int T = ka.T, R = ka.R, S_pad = ka.S_pad;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int half = wave >> 2, rate0 = wave & 3;
     const int p = wi.particle;
     const Tile tl = ka.tiles[(size_t)p * ka.tile_stride + tile];
     const int lane = threadIdx.x & 63;
@@ -1161,10 +1163,10 @@ __global__ __launch_bounds__(256) void k_prune_chain(KArgs<TX> ka, const int *__
     }
 }
 
-// Walks, one tile per workgroup (ka.blocks_per_item = tile stride; grid-stride, so
-// any grid works): walk items [0, n) of `witems`.
+// Walks, one (tile, half, rate) per single-wavefront workgroup (ka.blocks_per_item =
+// 8 x tile stride; grid-stride, so any grid works): walk items [0, n) of `witems`.
 template <typename TX>
-__global__ __launch_bounds__(512) void k_prune_walk(KArgs<TX> ka, const WalkItem *__restrict__ witems,
+__global__ __launch_bounds__(64) void k_prune_walk(KArgs<TX> ka, const WalkItem *__restrict__ witems,
                                                     const int4 *__restrict__ entries, int n,
                                                     double *__restrict__ rdot, int *__restrict__ rexp)
 {
@@ -1175,9 +1177,9 @@ __global__ __launch_bounds__(512) void k_prune_walk(KArgs<TX> ka, const WalkItem
         const int q = (l / bpi) * X + (w - l * X);
         if (q >= n) continue;
         const WalkItem wi = witems[q];
-        const int tile = l % bpi;
+        const int u = l % bpi, tile = u >> 3, half = (u >> 2) & 1, rate = u & 3;
         if (tile >= ka.ntiles[wi.particle]) continue;  // workgroup-uniform
-        walk_unit<TX>(ka, entries, wi, tile, rdot, rexp);
+        walk_unit<TX>(ka, entries, wi, tile, half, rate, rdot, rexp);
     }
 }
 
@@ -1241,14 +1243,14 @@ void launch_walk_typed(const PruneArgs &a, const WalkItem *witems, const int4 *e
                        hipStream_t s)
 {
     KArgs<TX> ka = kargs<TX>(a);
-    ka.blocks_per_item = a.tile_stride;  // one workgroup per tile, its wavefronts per half and rate
+    ka.blocks_per_item = 8 * a.tile_stride;  // one single-wavefront workgroup per (tile, half, rate)
     const int X = ka.xcds, bpi = ka.blocks_per_item;
     const int64_t units = (int64_t)(a.n_items + X - 1) / X * X * bpi;
     // one workgroup per unit: short-lived workgroups let the dispatcher interleave
     // the second stream's level launches as they finish (a persistent grid of half
     // the pruning grid, or of the resident capacity, measured slower)
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, INT32_MAX));
-    hipLaunchKernelGGL((k_prune_walk<TX>), dim3(grid), dim3(512), 0, s, ka, witems, entries, a.n_items, rdot,
+    hipLaunchKernelGGL((k_prune_walk<TX>), dim3(grid), dim3(64), 0, s, ka, witems, entries, a.n_items, rdot,
                        rexp);
 }
 
