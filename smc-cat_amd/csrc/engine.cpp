@@ -1071,6 +1071,7 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
                         s);
         }
         PruneArgs a = prune_args(e, w.first);
+        a.single_wave = true;  // full sweeps: one tile per workgroup
         const std::vector<int> &bo = w.bucket_off;
         const int nb = PRUNE_KINDS * (w.graph_levels + 1);
         auto run = [&](int b, int kind, bool roots) {

@@ -151,6 +151,7 @@ struct PruneArgs {
     int grid = 0;            // persistent grid (workgroups); 0: one per unit of work
     int unroll = 1;          // 16-site column blocks per step (1, 2 or 4)
     int xcds = 1;            // keep an item's workgroups on one of this many XCDs
+    bool single_wave = false;  // matrix-core levels of unroll 2: one wavefront (one tile) per workgroup
 };
 void launch_prune(const PruneArgs &a, hipStream_t s);       // VALU, scalar-operand P
 void launch_prune_mfma(const PruneArgs &a, hipStream_t s);  // matrix cores

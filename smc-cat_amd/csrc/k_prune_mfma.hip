@@ -243,9 +243,9 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename TX, int M0, int M1, bool IS_ROOT, int U>
+template <typename TX, int M0, int M1, bool IS_ROOT, int U, int W>
 __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int tile_group,
-                                           TX (&lds)[4][CherryLds<M0, M1>::elems])
+                                           TX (&lds)[W][CherryLds<M0, M1>::elems])
 {
     using V4 = typename Mfma<TX>::v4;
     const Item *__restrict__ items = ka.items;
@@ -266,8 +266,9 @@ __device__ __forceinline__ void prune_unit(const KArgs<TX> &ka, int item, int ti
     const int *__restrict__ pcls_off = ka.pcls_off;
     const int p_base = ka.p_base, mix_mode = ka.mix_mode, WS = ka.WS;
     const uint8_t *__restrict__ xflag = ka.xflag;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int tile = tile_group * 4 + wave;
+    // W wavefronts per workgroup, one tile each
+    const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile = tile_group * W + wave;
     const Item it = items[item];
     const int p = it.particle, n = it.node;
     if (tile >= ntiles[p]) return;  // this wavefront has no tile here; no block barrier follows
@@ -1125,8 +1126,8 @@ __global__ __launch_bounds__(256) void k_walk_mixture(const WalkItem *__restrict
 // [base, base + n), read from the device (bucket_off[bucket], [bucket + 1]) when
 // bucket_off is given, so one launch configuration serves every topology and a
 // captured graph can be replayed after the schedule changes.
-template <typename TX, int M0, int M1, bool IS_ROOT, int U>
-__global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__restrict__ bucket_off,
+template <typename TX, int M0, int M1, bool IS_ROOT, int U, int W>
+__global__ __launch_bounds__(64 * W) void k_prune_mfma(KArgs<TX> ka, const int *__restrict__ bucket_off,
                                                     int bucket, int base, int n)
 {
     if (bucket_off) {
@@ -1136,13 +1137,13 @@ __global__ __launch_bounds__(256) void k_prune_mfma(KArgs<TX> ka, const int *__r
     // unit w runs on XCD w mod X (the grid is a multiple of X); XCD x takes the
     // items x, x + X, ..., each with all its tile groups, so the P matrices and
     // tip codes an item reads are fetched into one L2 only
-    __shared__ __attribute__((aligned(16))) TX lds[4][CherryLds<M0, M1>::elems];
+    __shared__ __attribute__((aligned(16))) TX lds[W][CherryLds<M0, M1>::elems];
     const int bpi = ka.blocks_per_item, X = ka.xcds;
     const int n_work = (n + X - 1) / X * X * bpi;
     for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
         const int l = w / X;
         const int q = (l / bpi) * X + (w - l * X);
-        if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U>(ka, base + q, l % bpi, lds);
+        if (q < n) prune_unit<TX, M0, M1, IS_ROOT, U, W>(ka, base + q, l % bpi, lds);
     }
 }
 
@@ -1190,7 +1191,14 @@ namespace {
 template <typename TX, int M0, int M1, bool IS_ROOT, int U>
 void launch_one(const PruneArgs &a, const KArgs<TX> &ka, unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_prune_mfma<TX, M0, M1, IS_ROOT, U>), dim3(grid), dim3(256), 0, s, ka, a.bucket_off,
+    if constexpr (U == 2) {
+        if (a.single_wave) {
+            hipLaunchKernelGGL((k_prune_mfma<TX, M0, M1, IS_ROOT, U, 1>), dim3(grid), dim3(64), 0, s, ka,
+                               a.bucket_off, a.bucket, a.item_base, a.n_items);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_prune_mfma<TX, M0, M1, IS_ROOT, U, 4>), dim3(grid), dim3(256), 0, s, ka, a.bucket_off,
                        a.bucket, a.item_base, a.n_items);
 }
 
@@ -1257,13 +1265,19 @@ void launch_walk_typed(const PruneArgs &a, const WalkItem *witems, const int4 *e
 template <typename TX>
 void launch_typed(const PruneArgs &a, hipStream_t s)
 {
-    const int bpi = (a.tile_stride + 3) / 4;
+    // single-wavefront workgroups (full sweeps at unroll 2: measured 3.5% faster there,
+    // 1% slower in MCMC rounds) take one tile each, four-wavefront ones four tiles;
+    // the persistent grid keeps the same number of wavefronts
+    const bool one = a.single_wave && a.unroll == 2;
+    const int bpi = one ? a.tile_stride : (a.tile_stride + 3) / 4;
     KArgs<TX> ka = kargs<TX>(a);
+    ka.blocks_per_item = bpi;
     // without device bucket offsets the work is known: launch exactly what it needs
     // (items rounded up to whole XCD groups), capped at the persistent grid
     const int X = ka.xcds;
     int64_t units = a.bucket_off ? INT64_MAX : (int64_t)(a.n_items + X - 1) / X * X * bpi;
-    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, a.grid > 0 ? a.grid : units));
+    const int64_t cap = a.grid > 0 ? (one ? 4 * (int64_t)a.grid : a.grid) : units;
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, cap));
     if (a.roots) {
         launch_rt<TX, true>(a, ka, grid, s);
         return;
