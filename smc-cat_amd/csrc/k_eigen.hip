@@ -228,27 +228,26 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // in row order, then the diagonal, then the row sums) is the dot product of its
 // outer-product column M[e][k] = A[i][k] Bx[k][j] (Bx: B with the row-sum column)
 // with the pair's exponentials e_k = exp(lam_k t r).  Long runs (128 pairs): lane l of
-// every wave keeps M of entries l, l + 64, l + 128, l + 192 in registers, so that one
-// wave covers all 230 entries of a pair; a step gives each of the workgroup's waves
-// its own pair, and an exponential read from LDS feeds four multiply-adds (LDS
-// bandwidth, not the FP64 units, bounds the one-entry form on long runs).  Short runs
-// (32 pairs) give lane t entry t and two pairs a step: their per-workgroup set-up of M
-// would cost four times as much in the four-entry form.  The entries of a step's
-// pairs go through LDS (two buffers, one barrier a step) and leave in 16-byte
-// consecutive-lane stores of the whole workgroup (in the four-entry form this measured
-// faster than waves that run their own pairs with no barrier and store their own
-// rows).  Node lists (a few pairs per workgroup) store each entry and its mirror
-// straight from the lane: there the barriers cost more than the coalescing gains.
-// The exponentials of all pairs of the run are computed once, spread over the
-// workgroup.
+// a pair of wavefronts keeps M of entries l and l + 64 (even wavefront) or 128 + l and
+// 192 + l (odd one) in registers, so that two wavefronts cover the 230 entries of a
+// (node, rate) pair and a step takes two pairs; an exponential read from LDS feeds two
+// multiply-adds (LDS bandwidth bounds the one-entry form on long runs; four entries per
+// lane, one wavefront per pair, ran 14% slower at two waves per SIMD).  Short runs (32
+// pairs) give lane t entry t and two pairs a step: their per-workgroup set-up of M would
+// cost twice as much in the two-entry form.  The entries of a step's pairs go through
+// LDS (two buffers, one barrier a step) and leave in 16-byte consecutive-lane stores of
+// the whole workgroup (measured faster than waves that run their own pairs with no
+// barrier and store their own rows).  Node lists (a few pairs per workgroup) store each
+// entry and its mirror straight from the lane: there the barriers cost more than the
+// coalescing gains.  The exponentials of all pairs of the run are computed once, spread
+// over the workgroup.
 constexpr int kPmLanes = 256;
-constexpr int kPmWaves = kPmLanes / 64;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
 constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
 constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 128 pairs from which launches use them
 constexpr int kPmUpper = 190, kPmEntries = 230;
-// forms: four entries per lane, staged; one entry per lane, staged; one entry, direct
-constexpr int kPmFour = 0, kPmOneStaged = 1, kPmOneDirect = 2;
+// forms: two entries per lane, staged; one entry per lane, staged; one entry, direct
+constexpr int kPmTwo = 0, kPmOneStaged = 1, kPmOneDirect = 2;
 
 // entry e of the list above -> (i, j); j = 20 for the row sums
 __device__ __forceinline__ void pmat_entry(int e, int &i, int &j)
@@ -277,7 +276,7 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
                                            const double *__restrict__ rt, int R, int q0, int q1,
                                            NodeOf node_of, PairOut pair_out, TP *__restrict__ Pcls)
 {
-    static_assert(FORM == kPmFour || FORM == kPmOneStaged || FORM == kPmOneDirect, "P form");
+    static_assert(FORM == kPmTwo || FORM == kPmOneStaged || FORM == kPmOneDirect, "P form");
     constexpr int V = 16 / (int)sizeof(TP);  // entries per 16-byte store
     constexpr int NV = kPStride / V;
     static_assert(kPStride % V == 0, "P rows must split into 16-byte stores");
@@ -296,7 +295,7 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
     }
     __syncthreads();
     // staged forms: the step's entries to LDS, then 16-byte runs of the step's matrices out
-    constexpr int PS = FORM == kPmFour ? kPmWaves : 2;  // pairs per step
+    constexpr int PS = 2;  // pairs per step
     __shared__ __align__(16) TP stage[FORM == kPmOneDirect ? 1 : 2][FORM == kPmOneDirect ? 1 : PS * kPStride];
     auto store_step = [&](const TP *st, int q, int nq) {
         __syncthreads();
@@ -306,44 +305,44 @@ __device__ __forceinline__ void pmat_pairs(const double *__restrict__ eg, const 
         }
     };
     int buf = 0;
-    if constexpr (FORM == kPmFour) {
-        const int lane = tid & 63, wave = tid >> 6;
-        // entries lane + 64 c: c = 0, 1 lie in the upper triangle, c = 2 up to entry
-        // 189, c = 3 is a diagonal entry, a row sum or nothing (entries from 230 on)
-        int ei[4], ej[4];
-        double m[4][20];
+    if constexpr (FORM == kPmTwo) {
+        // wavefronts 2 p' and 2 p' + 1 take pair p' of the step: lane l of the even one
+        // entries l and l + 64 (all in the upper triangle), of the odd one entries
+        // 128 + l and 192 + l (the rest of the upper triangle, the diagonal, the row
+        // sums, nothing from 230 on)
+        const int lane = tid & 63, wave = tid >> 6, sub = wave & 1, pw = wave >> 1;
+        int ei[2], ej[2];
+        bool up[2], has[2];
+        double m[2][20], ratio[2];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            pmat_entry(lane + 64 * c, ei[c], ej[c]);
+        for (int c = 0; c < 2; c++) {
+            const int e = 128 * sub + lane + 64 * c;
+            pmat_entry(e, ei[c], ej[c]);
+            up[c] = e < kPmUpper;
+            has[c] = e < kPmEntries;
 #pragma unroll
             for (int k = 0; k < 20; k++)
                 m[c][k] = eg[ei[c] * 20 + k] * (ej[c] < 20 ? eg[kEigB + k * 20 + ej[c]] : bsum[k]);
+            ratio[c] = eg[kEigPi + ei[c]] / eg[kEigPi + ej[c] % 20];
         }
-        double ratio[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) ratio[c] = eg[kEigPi + ei[c]] / eg[kEigPi + ej[c] % 20];
-        const bool up2 = lane + 128 < kPmUpper, has3 = lane + 192 < kPmEntries;
         for (int q = q0; q < q1; q += PS, buf ^= 1) {
             const int nq = min(PS, q1 - q);
             TP *st = stage[buf];
-            if (wave < nq) {
-                const double *x = ex[q - q0 + wave];
-                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            if (pw < nq) {
+                const double *x = ex[q - q0 + pw];
+                double acc[2] = {0.0, 0.0};
 #pragma unroll
                 for (int k = 0; k < 20; k++) {
                     const double e = x[k];
 #pragma unroll
-                    for (int c = 0; c < 4; c++) acc[c] = fma(m[c][k], e, acc[c]);
+                    for (int c = 0; c < 2; c++) acc[c] = fma(m[c][k], e, acc[c]);
                 }
-                TP *sp = st + wave * kPStride;
+                TP *sp = st + pw * kPStride;
 #pragma unroll
                 for (int c = 0; c < 2; c++) {
-                    sp[ei[c] * 21 + ej[c]] = (TP)acc[c];
-                    sp[ej[c] * 21 + ei[c]] = (TP)(acc[c] * ratio[c]);
+                    if (has[c]) sp[ei[c] * 21 + ej[c]] = (TP)acc[c];
+                    if (up[c]) sp[ej[c] * 21 + ei[c]] = (TP)(acc[c] * ratio[c]);
                 }
-                sp[ei[2] * 21 + ej[2]] = (TP)acc[2];
-                if (up2) sp[ej[2] * 21 + ei[2]] = (TP)(acc[2] * ratio[2]);
-                if (has3) sp[ei[3] * 21 + ej[3]] = (TP)acc[3];
             }
             store_step(st, q, nq);
         }
@@ -498,7 +497,7 @@ void launch_pmat_pairs(const PmatArgs &a, hipStream_t s)
                            a.first, a.plist);
 }
 
-// Launches with workgroups to spare take runs of 128 pairs in the four-entry form;
+// Launches with workgroups to spare take runs of 128 pairs in the two-entry form;
 // the others runs of 32, staged when they cover many particles (full sweeps) and
 // stored straight from the lanes when they cover a few (the whole-particle P of an
 // MCMC round, which runs beside the round's pruning: measured faster there)
@@ -509,7 +508,7 @@ void launch_pmat(const PmatArgs &a, hipStream_t s)
     if (a.count <= 0) return;
     const int64_t large = (int64_t)a.count * a.max_classes * ((a.nn * a.R + kPmPairsLarge - 1) / kPmPairsLarge);
     if (large >= kPmLargeGrid)
-        launch_pmat_pairs<kPmPairsLarge, kPmFour>(a, s);
+        launch_pmat_pairs<kPmPairsLarge, kPmTwo>(a, s);
     else if (a.count <= kPmDirectParticles)
         launch_pmat_pairs<kPmPairs, kPmOneDirect>(a, s);
     else
