@@ -227,7 +227,8 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // k; the lower triangle is (pi_i / pi_j) P_ij at (j, i).  Each entry e (upper triangle
 // in row order, then the diagonal, then the row sums) is the dot product of its
 // outer-product column M[e][k] = A[i][k] Bx[k][j] (Bx: B with the row-sum column)
-// with the pair's exponentials e_k = exp(lam_k t r).  Long runs (128 pairs): lane l of
+// with the pair's exponentials e_k = exp(lam_k t r).  Long runs (256 pairs; 128 measured
+// 7% slower, the per-workgroup set-up being shared by fewer pairs): lane l of
 // a pair of wavefronts keeps M of entries l and l + 64 (even wavefront) or 128 + l and
 // 192 + l (odd one) in registers, so that two wavefronts cover the 230 entries of a
 // (node, rate) pair and a step takes two pairs; an exponential read from LDS feeds two
@@ -243,8 +244,8 @@ __global__ __launch_bounds__(64) void k_eigen_cand(const double *__restrict__ ba
 // over the workgroup.
 constexpr int kPmLanes = 256;
 constexpr int kPmPairs = 32;       // (node, rate) pairs per workgroup
-constexpr int kPmPairsLarge = 128;  // the same for launches with workgroups to spare
-constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 128 pairs from which launches use them
+constexpr int kPmPairsLarge = 256;  // the same for launches with workgroups to spare
+constexpr int64_t kPmLargeGrid = 32768;  // workgroups at 256 pairs from which launches use them
 constexpr int kPmUpper = 190, kPmEntries = 230;
 // forms: two entries per lane, staged; one entry per lane, staged; one entry, direct
 constexpr int kPmTwo = 0, kPmOneStaged = 1, kPmOneDirect = 2;
@@ -497,7 +498,7 @@ void launch_pmat_pairs(const PmatArgs &a, hipStream_t s)
                            a.first, a.plist);
 }
 
-// Launches with workgroups to spare take runs of 128 pairs in the two-entry form;
+// Launches with workgroups to spare take runs of 256 pairs in the two-entry form;
 // the others runs of 32, staged when they cover many particles (full sweeps) and
 // stored straight from the lanes when they cover a few (the whole-particle P of an
 // MCMC round, which runs beside the round's pruning: measured faster there)

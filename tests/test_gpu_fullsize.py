@@ -5,7 +5,7 @@ pruning, partials resident for about 240 particles at a time (multi-wave), the
 launch shape autotuned, the sweep captured and replayed as a graph.  Every lnL is
 finite or flagged non-finite; replays equal the eager sweep bit for bit; particles
 from the first, a middle and the last wave match the FP64 oracle within the fast
-path's bound; the P kernel of the large waves (128 node-rate pairs per workgroup)
+path's bound; the P kernel of the large waves (256 node-rate pairs per workgroup)
 gives the same likelihoods, bit for bit, as the one of small waves (32 pairs).
 
 c5: the learned CAT-allocation proposal on full c3 data (100 taxa x 5,000 sites,
@@ -76,7 +76,7 @@ def test_c4_full_population_waves_graph_autotune(gpu):
              (i, batch.num_classes[i], eager[i], want, abs(eager[i] - want) / abs(want)))
         assert eager[i] == pytest.approx(want, rel=RTOL_FP32)
     # the first 64 particles in waves of 16: the P kernel's 32-pair form (small waves)
-    # against the 128-pair form the waves above ran
+    # against the 256-pair form the waves above ran
     sub = batch.take(np.arange(64))
     with Engine(T, S, R, 64, kr[1], "fp32", partial_slots=16,
                 pmat_classes=int(16 * kr[1])) as small:
