@@ -244,7 +244,9 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
     // issued on its own stream and collected when it is done, while the other
     // particles go on with their rounds.  A particle waits while the lane is busy
     // with other particles.  Each particle's chain is the same as without the lane
-    // (its random numbers are drawn in the same order), so the results are too.
+    // (its random numbers are drawn in the same order), so the results are too.  When
+    // nothing but waiting whole-tree steps is left beside a busy lane, they run in a
+    // round of their own.
     const bool lane_ok = count > 1 && e->lane_enabled && !e->timing;
     std::vector<uint8_t> role(count, 0), on_lane(count, 0), flags_l((size_t)count * nn), eig_l(count);
     enum { IDLE = 0, MAIN = 1, LANE = 2 };
@@ -268,6 +270,7 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
         if (!active) break;
         const bool lane_free = lane_ok && !e->lane.in_flight;
         int n_main = 0, n_lane = 0;
+        int n_wait = 0;  // whole-tree steps waiting for the lane
         for (int i = 0; i < count; ++i) {
             role[i] = IDLE;
             if (left[i] <= 0 || on_lane[i]) continue;
@@ -275,12 +278,20 @@ int smc_rejuvenate_ex(smc_engine *e, int first, int count, double tempering,
                 if (lane_free) {
                     role[i] = LANE;
                     n_lane++;
+                } else {
+                    n_wait++;
                 }
             } else {
                 role[i] = MAIN;
                 n_main++;
             }
         }
+        if (n_main == 0 && n_wait > 0)  // nothing else to run beside the lane: the waiting
+            for (int i = 0; i < count; ++i)  // whole-tree steps take a round of their own
+                if (role[i] == IDLE && left[i] > 0 && !on_lane[i]) {
+                    role[i] = MAIN;
+                    n_main++;
+                }
         if (n_main + n_lane > 0) {
             std::optional<PhaseTimer> propose_timer(std::in_place, e, SMC_PHASE_PROPOSE);
             std::fill(flags.begin(), flags.end(), 0);
