@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK = {"fp64": 78.6, "fp32": 157.3}
 # The launch shape the autotuner picked for a full sweep of each workload on MI355X
 # (smc_engine_autotune over one wave); --autotune searches again.
-TUNED = {("c4", "fp32"): (64, [2, 2, 2], 8, 2), ("c2", "fp64"): (64, [1, 1, 2], 8, 4),
+TUNED = {("c4", "fp32"): (64, [2, 2, 2], 8, 2), ("c2", "fp64"): (32, [1, 1, 2], 8, 2),
          ("c3", "fp32"): (64, [2, 2, 2], 8, 2)}
 
 

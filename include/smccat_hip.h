@@ -429,8 +429,9 @@ int smc_engine_reset_stats(smc_engine *e);
  * (vector FMAs with the P matrix as scalar operands).  Results agree to rounding.
  * SMC_PRUNE_MFMA runs the short dirty paths of an incremental refresh as chains
  * (one launch walks each path bottom-up with the partial in registers) when few
- * particles move in the call (see smc_tuning.chain_max_nodes);
- * SMC_PRUNE_MFMA_LEVELS launches them node level by node level instead, with the
+ * particles move in the call (see smc_tuning.chain_max_nodes), and walks them all
+ * in one launch when many do (whole trees go level by level);
+ * SMC_PRUNE_MFMA_LEVELS launches every dirty node level by level instead, with the
  * same partials bit for bit.  SMC_PRUNE_MFMA_WALK runs every changed particle of an
  * incremental refresh in one launch: each workgroup walks one tile of a particle's
  * dirty nodes in post-order with the partial carried in registers, the root
