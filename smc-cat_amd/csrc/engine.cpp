@@ -221,6 +221,9 @@ int build_schedule(smc_engine *e, int first, int count)
         // captured sweeps still in flight read the old buffer
         HIP_TRY(e, hipStreamSynchronize(e->stream));
         if (e->d_bucket_off) (void)hipFree(e->d_bucket_off);
+    if (e->d_P2) (void)hipFree(e->d_P2);
+    for (hipEvent_t ev : e->pipe_ev)
+        if (ev) (void)hipEventDestroy(ev);
         e->d_bucket_off = nullptr;
         e->bucket_cap = 0;
         const int64_t cap = std::max<int64_t>((int64_t)boffs.size(), 1024);
@@ -1057,20 +1060,51 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
 {
     hipStream_t s = e->stream;
     const bool mfma = e->prune_variant != SMC_PRUNE_VALU;
-    for (const smc_engine::Wave &w : e->waves) {
+    // With a second P buffer, wave w + 1's eigensystems and P are computed on stream2
+    // into the buffer wave w - 1 used while wave w prunes (4% faster at C4): the
+    // stream2 work of a buffer waits for the pruning that read it, and a wave's pruning
+    // for its models.  (Error returns of the event calls are left to the launches'
+    // hipGetLastError: issue_sweep also runs under graph capture.)
+    const bool pipe = e->d_P2 && e->waves.size() > 1 && !e->timing;
+    auto models = [&](const smc_engine::Wave &w, void *P, hipStream_t st) {
         {
             TimedLaunch tl(e, SMC_K_EIGEN);
             launch_eigen(EigenArgs{e->d_base, e->d_logits, e->d_nclass, e->d_eig, e->Kmax, w.first,
                                    w.count},
-                         s);
+                         st);
         }
         {
             TimedLaunch tl(e, SMC_K_PMAT);
             launch_pmat(PmatArgs{e->d_eig, e->d_blen, e->d_rates, e->d_nclass, e->d_pcls_off,
-                                 e->d_P, e->nn, e->R, e->Kmax, w.first, w.count, e->fp32},
-                        s);
+                                 P, e->nn, e->R, e->Kmax, w.first, w.count, e->fp32},
+                        st);
+        }
+    };
+    // buffers alternate so that the last wave's P ends in d_P, where the incremental
+    // refreshes of its resident particles read it
+    const size_t nw = e->waves.size();
+    auto bix = [&](size_t wi) { return pipe ? (int)((nw - 1 - wi) & 1) : 0; };
+    auto buffer = [&](size_t wi) { return bix(wi) ? e->d_P2 : e->d_P; };
+    if (pipe) {
+        (void)hipEventRecord(e->ev_fork, s);
+        (void)hipStreamWaitEvent(e->stream2, e->ev_fork, 0);
+        models(e->waves[0], buffer(0), e->stream2);
+        (void)hipEventRecord(e->pipe_ev[bix(0)], e->stream2);
+    }
+    for (size_t wi = 0; wi < nw; ++wi) {
+        const smc_engine::Wave &w = e->waves[wi];
+        if (pipe) {
+            (void)hipStreamWaitEvent(s, e->pipe_ev[bix(wi)], 0);
+            if (wi + 1 < nw) {
+                if (wi >= 1) (void)hipStreamWaitEvent(e->stream2, e->pipe_ev[2 + bix(wi + 1)], 0);
+                models(e->waves[wi + 1], buffer(wi + 1), e->stream2);
+                (void)hipEventRecord(e->pipe_ev[bix(wi + 1)], e->stream2);
+            }
+        } else {
+            models(w, e->d_P, s);
         }
         PruneArgs a = prune_args(e, w.first);
+        a.P = buffer(wi);
         a.single_wave = true;  // full sweeps: one tile per workgroup
         const std::vector<int> &bo = w.bucket_off;
         const int nb = PRUNE_KINDS * (w.graph_levels + 1);
@@ -1124,6 +1158,7 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
         } else {
             run(nb, PRUNE_ANY, true);
         }
+        if (pipe) (void)hipEventRecord(e->pipe_ev[2 + bix(wi)], s);
     }
     TimedLaunch tl(e, SMC_K_REDUCE);
     launch_reduce_lnl(e->d_tile_lnl, e->d_ntiles, e->tile_stride, first, count, e->d_lnl, e->d_nonfinite,
@@ -1133,9 +1168,33 @@ void issue_sweep(smc_engine *e, int first, int count, bool device_buckets)
 // One full sweep: replayed from a captured graph when the launch sequence is
 // the one captured (same waves, level counts and settings), else captured anew;
 // eager when timing per kernel family or with the VALU variant.
+// The second P buffer of pipelined multi-wave sweeps, once, when the device keeps
+// kPipeMarginBytes free after it (later lazy allocations need room).
+constexpr size_t kPipeMarginBytes = size_t(16) << 30;
+
+int ensure_p2(smc_engine *e)
+{
+    if (e->p2_tried || e->waves.size() < 2) return SMC_OK;
+    e->p2_tried = true;
+    const size_t bytes = (size_t)e->Cmax * e->nn * e->R * kPStride * e->tx_size;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(e, hipMemGetInfo(&free_b, &total_b));
+    if (free_b < bytes + kPipeMarginBytes) return SMC_OK;  // sweeps stay unpipelined
+    for (hipEvent_t &ev : e->pipe_ev) HIP_TRY(e, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (hipMalloc(&e->d_P2, bytes) != hipSuccess) {
+        e->d_P2 = nullptr;
+        (void)hipGetLastError();
+        return SMC_OK;
+    }
+    e->device_bytes += (int64_t)bytes;
+    return SMC_OK;
+}
+
 int run_sweep(smc_engine *e, int first, int count)
 {
     hipStream_t s = e->stream;
+    int rc0 = ensure_p2(e);
+    if (rc0 != SMC_OK) return rc0;
     const bool graph = e->use_graph && !e->timing && e->prune_variant != SMC_PRUNE_VALU;
     if (!graph) {
         issue_sweep(e, first, count, false);
@@ -1143,7 +1202,7 @@ int run_sweep(smc_engine *e, int first, int count)
     }
     std::vector<int64_t> key = {first, count, e->fp32, e->mix_mode, e->WS, e->tune.blocks_per_cu,
                                 e->tune.unroll[0], e->tune.unroll[1], e->tune.unroll[2], e->tune.xcd_groups,
-                                e->tune.unroll_rt, (int64_t)(intptr_t)e->d_bucket_off};
+                                e->tune.unroll_rt, (int64_t)(intptr_t)e->d_bucket_off, e->d_P2 != nullptr};
     for (const smc_engine::Wave &w : e->waves) {
         key.push_back(w.first);
         key.push_back(w.count);

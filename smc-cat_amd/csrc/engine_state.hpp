@@ -75,6 +75,12 @@ struct smc_engine {
     int *d_ntiles = nullptr;
     double *d_eig = nullptr;
     void *d_P = nullptr;
+    // a second P buffer for multi-wave sweeps, allocated when memory allows: the next
+    // wave's eigensystems and P are computed into it on stream2 beside this wave's
+    // pruning (pipe_ev: [0..1] models of a buffer done, [2..3] pruning of a buffer done)
+    void *d_P2 = nullptr;
+    bool p2_tried = false;
+    hipEvent_t pipe_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     void *d_X = nullptr;
     int *d_E = nullptr;
     double *d_tile_lnl = nullptr;
