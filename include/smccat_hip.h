@@ -66,7 +66,10 @@ typedef struct {
 /* Engine lifetime.  Replaces the per-thread Particle construction of
  * main.cpp:112-115 / Particle::Particle (Particle.cpp:21-40): all device
  * buffers are allocated here -- state for max_particles particles, partial
- * likelihoods for partial_slots of them, P(t) for pmat_classes classes. */
+ * likelihoods for partial_slots of them, P(t) for pmat_classes classes.  A full
+ * sweep in several waves later adds a second P(t) buffer of the same size, once,
+ * when the device keeps 16 GB free after it, to overlap the next wave's P with the
+ * current wave's pruning. */
 int smc_engine_create(const smc_engine_config *cfg, smc_engine **out);
 int smc_engine_destroy(smc_engine *e);
 const char *smc_engine_last_error(const smc_engine *e);
