@@ -1,6 +1,6 @@
 """Benchmark of the particle-likelihood hot path on one MI355X.
 
-Headline (the default run): config 4 -- 200 taxa x 10,000 sites, GTR+G4+I, 4,096
+Headline (the default run): C4 -- 200 taxa x 10,000 sites, GTR+G4+I, 4,096
 particles with CAT class counts K ~ U{5..120}, FP32 pruning on the matrix cores with
 FP64 eigensystems, back-transforms and reductions.  One step = one pass of the hot
 path over the whole population held in HBM: eigen-decomposition of every class rate
@@ -12,8 +12,8 @@ particles at a time, so a sweep runs in waves; each sweep is one replay of its
 captured graph.  `value` is node.pattern.category updates per second.
 
 Extra keys of the same JSON line:
-  c2_fp64          the plain FP64 drop-in on config 2 (64 taxa x 2,000 sites, 1,024 particles)
-  rejuvenation_c4  SMC ESS/sec at the config-4 shape through the cost of one rejuvenation
+  c2_fp64          the plain FP64 drop-in on C2 (64 taxa x 2,000 sites, 1,024 particles)
+  rejuvenation_c4  SMC ESS/sec at the C4 shape through the cost of one rejuvenation
                    step per particle: plain FP64 (one particle at a time, Gibbs CAT move)
                    against FP32 + lockstep batches with the Gibbs move and with the learned
                    move (network trained on >= 1M recorded Gibbs draws before timing)
@@ -315,7 +315,7 @@ def ess_compare(args):
 
 
 def rejuv_compare(args):
-    """Cost of one rejuvenation step per particle at the config-4 shape (200 taxa x
+    """Cost of one rejuvenation step per particle at the C4 shape (200 taxa x
     10,000 sites, GTR+G4+I, CAT K ~ U{5..120}; r MCMC iterations, then with
     probability 0.1 the CAT move, main.cpp:296-311).  Every path runs the same moves
     on the same target, so at equal ESS per SMC step the ratio of these costs is the
@@ -323,12 +323,12 @@ def rejuv_compare(args):
       plain_fp64            one particle at a time (the reference's per-thread loop),
                             FP64, the Gibbs CAT move
       fp32_batched_gibbs    FP32, lockstep batches of 150 resident particles keyed by
-                            chain length (the sampler's shape): the config-4 population
+                            chain length (the sampler's shape): the C4 population
                             of 4,096 chains sorted by length into batches (27 full
                             ones), of which 8 spread over the order are timed (1,200
                             particles); the
                             Gibbs CAT move in one lockstep of the ~410 particles that
-                            draw it at the config-4 population
+                            draw it at the C4 population
       fp32_batched_learned  the same with the learned CAT move, its network trained
                             on >= 1M Gibbs draws recorded before the timed runs."""
     from smccat import Engine, ProposalNetwork
@@ -395,7 +395,7 @@ def rejuv_compare(args):
     progress("rejuv fast MCMC: %d particles in batches of %d: %.1f ms per particle, %.1f us per MH step" %
              (popn, nb, t_mcmc / popn * 1e3, s_step * 1e6))
     # CAT moves: the sampler moves every particle that draws one (probability 0.1)
-    # together, after the MCMC of all batches: at the config-4 population that is
+    # together, after the MCMC of all batches: at the C4 population that is
     # about 410 particles per rejuvenation; each particle is charged 0.1 moves
     n_cat = min(popn, int(round(0.1 * 4096)))
     slots = list(range(n_cat))
@@ -535,7 +535,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "rejuv", "c5"],
-                    help="headline: the config-4 updates/s line with every extra key; sweep: one "
+                    help="headline: the C4 updates/s line with every extra key; sweep: one "
                          "configuration's sweep only; rejuv: the rejuvenation comparison; c5: the "
                          "learned-proposal benchmark")
     ap.add_argument("--config", default="c4", choices=["c1", "c2", "c3", "c4"])
@@ -581,11 +581,11 @@ def main():
     ap.add_argument("--c5-move-iters", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(particles would shard across GPUs by slot)", file=sys.stderr)
         sys.exit(2)
     if args.mode == "rejuv":
-        print(json.dumps({"metric": "rejuvenation step cost per particle at the config-4 shape",
+        print(json.dumps({"metric": "rejuvenation step cost per particle at the C4 shape",
                           "rejuvenation_c4": rejuv_compare(args)}), flush=True)
         return
     if args.mode == "c5":

@@ -19,7 +19,7 @@ from .engine import ParticleBatch
 NS = 20
 
 CONFIGS = {
-    # taxa, sites, rates, invariant, lg, particles, true profiles
+    # benchmark workloads C1-C5: taxa, sites, rates, invariant, lg, particles, true profiles
     "c1": dict(taxa=16, sites=200, rates=4, invar=False, lg=True, particles=64, shape=0.8,
                profiles=1, k_range=(1, 1)),
     "c2": dict(taxa=64, sites=2000, rates=4, invar=True, lg=False, particles=1024, shape=0.8,
