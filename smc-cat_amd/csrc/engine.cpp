@@ -18,11 +18,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "engine_state.hpp"
@@ -860,36 +864,19 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
     std::vector<int> perm((size_t)count * Sp, 0), ntiles(count), roots(count);
     // a column's pattern: its id (uploaded alignment), else the column itself
     auto pattern_of = [&](int s) { return e->h_pattern.empty() ? s : e->h_pattern[s]; };
-    std::vector<int> stamp(std::max(e->n_patterns, S), -1), pslot(std::max(e->n_patterns, S), 0);
     std::vector<Tile> tiles((size_t)count * ts, Tile{0, 0, 0, 0});
-    std::vector<int> progs((size_t)count * site_prog_stride(T));
+    std::vector<int> progs((size_t)count * site_prog_stride(T)), hts((size_t)count * Ti);
+    // topologies and rates first, in order: their checks report through the engine's
+    // error text, and the rates call lgamma, which sets a process-wide sign
     for (int i = 0; i < count; ++i) {
         const int p = first + i;
         const int32_t *ch = children + (size_t)i * Ti * 2;
-        int root = -1;
-        std::vector<int> h(nn, 0);
-        {
-            std::vector<int> hh(Ti);
-            int rc = smc_topology(e, p, ch, &root, hh.data(), progs.data() + (size_t)i * site_prog_stride(T));
-            if (rc != SMC_OK) return rc;
-            for (int k = 0; k < Ti; ++k) h[T + k] = hh[k];
-            if (e->h_prog_stale[p]) {
-                e->h_prog_stale[p] = 0;
-                e->n_prog_stale--;
-            }
-        }
-        for (int k = 0; k < Ti; ++k) {
-            e->h_height[(size_t)p * Ti + k] = h[T + k];
-            e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
-            e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
-        }
-        std::copy_n(progs.data() + (size_t)i * site_prog_stride(T), site_prog_stride(T),
-                    e->h_prog.data() + (size_t)p * site_prog_stride(T));
-        e->h_root[p] = root;
-        roots[i] = root;
-        for (int n = 0; n < nn; ++n) {
-            double bl = branch_len[(size_t)i * nn + n];
-            blen[(size_t)i * nn + n] = (n == root) ? 0.0 : bl;
+        int rc = smc_topology(e, p, ch, &roots[i], hts.data() + (size_t)i * Ti,
+                              progs.data() + (size_t)i * site_prog_stride(T));
+        if (rc != SMC_OK) return rc;
+        if (e->h_prog_stale[p]) {
+            e->h_prog_stale[p] = 0;
+            e->n_prog_stale--;
         }
         // rates: discretizeGamma when K_r > 1 (Particle.cpp:224-225), else 1.0
         if (R > 1) {
@@ -899,78 +886,127 @@ int smc_set_particles(smc_engine *e, int first, int count, const int32_t *childr
         } else {
             rates[(size_t)i] = 1.0;
         }
-        std::copy_n(&rates[(size_t)i * R], R, &e->h_rates[(size_t)p * R]);
-        // class-sorted slot order and tiles (each tile is one class)
-        const int Kp = num_classes[i];
-        if (Kp < 1 || Kp > K)
-            return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": num_classes out of range");
-        const int32_t *as = assignment + (size_t)i * S;
-        std::vector<int> cnt(Kp + 1, 0);
-        for (int s = 0; s < S; ++s) {
-            if (as[s] < 0 || as[s] >= Kp)
-                return fail(e, SMC_ERR_ARG, "particle " + std::to_string(p) + ": assignment out of range");
-            cnt[as[s] + 1]++;
-        }
-        for (int k = 0; k < Kp; ++k) cnt[k + 1] += cnt[k];
-        // class-sorted slots: one per column, or (compressed) one per distinct pattern of
-        // a class, weighted by its columns; the slot holds its first column
-        std::vector<int> fillp(cnt.begin(), cnt.end() - 1), bycls(S);
-        for (int s = 0; s < S; ++s) bycls[fillp[as[s]]++] = s;  // sites grouped by class, in site order
-        std::vector<int> used(Kp, 0);
-        for (int k = 0; k < Kp; ++k)  // one class at a time: its patterns' stamps stay its own
-            for (int j = cnt[k]; j < cnt[k + 1]; ++j) {
-                const int s = bycls[j];
-                const int pid = pattern_of(s);
-                if (e->compress && stamp[pid] == i * K + k) {
-                    weight[(size_t)i * Sp + pslot[pid]] += 1.0;
-                    continue;
-                }
-                stamp[pid] = i * K + k;
-                const int sl = cnt[k] + used[k]++;
-                pslot[pid] = sl;
-                perm[(size_t)i * Sp + sl] = s;
-                weight[(size_t)i * Sp + sl] = 1.0;
-            }
-        // slots of class k: [cnt[k], cnt[k] + used[k]); pack them to the front
-        {
-            std::vector<int> pk(Sp, 0);
-            std::vector<double> pw(Sp, 0.0);
-            int o = 0;
-            for (int k = 0; k < Kp; ++k) {
-                const int c0 = cnt[k];
-                cnt[k] = o;
-                for (int j = 0; j < used[k]; ++j, ++o) {
-                    pk[o] = perm[(size_t)i * Sp + c0 + j];
-                    pw[o] = weight[(size_t)i * Sp + c0 + j];
-                }
-            }
-            cnt[Kp] = o;
-            std::copy(pk.begin(), pk.end(), perm.begin() + (size_t)i * Sp);
-            std::copy(pw.begin(), pw.end(), weight.begin() + (size_t)i * Sp);
-            e->h_nslots[p] = o;
-        }
-        // tiles: each class's slots cut at the 64-slot window boundaries (at most
-        // ceil(S / 64) windows plus one split window per class boundary)
-        int nt = 0;
-        for (int k = 0; k < Kp; ++k)
-            for (int w = cnt[k] / kTile * kTile; w < cnt[k + 1]; w += kTile) {
-                const int lo = std::max(cnt[k], w) - w, hi = std::min(cnt[k + 1], w + kTile) - w;
-                if (nt >= ts) return fail(e, SMC_ERR_STATE, "tile table overflow");
-                tiles[(size_t)i * ts + nt++] = Tile{w, hi, k, lo};
-            }
-        ntiles[i] = nt;
-        e->h_valid[p] = 1;
-        e->h_nclass[p] = Kp;
-        std::copy(blen.begin() + (size_t)i * nn, blen.begin() + (size_t)(i + 1) * nn,
-                  e->h_blen.begin() + (size_t)p * nn);
-        std::copy(base + (size_t)i * 190, base + (size_t)(i + 1) * 190,
-                  e->h_base.begin() + (size_t)p * 190);
-        std::copy(logits + (size_t)i * K * 20, logits + (size_t)(i + 1) * K * 20,
-                  e->h_logits.begin() + (size_t)p * K * 20);
-        std::copy(as, as + S, e->h_assign.begin() + (size_t)p * S);
-        e->h_pinv[p] = pinv[i];
-        e->h_shape[p] = shape[i];
     }
+    // the rest of a particle's setup touches only its own rows: particles split over
+    // host threads, and an error names the lowest failing particle, as in order
+    std::atomic<int> next{0};
+    std::mutex err_mu;
+    int err_i = count, err_code = SMC_OK;
+    std::string err_msg;
+    auto worker = [&]() {
+        // pattern stamps are keyed by (particle, class), so each thread keeps its own
+        std::vector<int> stamp(std::max(e->n_patterns, S), -1), pslot(std::max(e->n_patterns, S), 0);
+        for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+            const int p = first + i;
+            const int32_t *ch = children + (size_t)i * Ti * 2;
+            auto one = [&]() -> std::pair<int, std::string> {
+                auto bad = [](int code, std::string msg) { return std::make_pair(code, std::move(msg)); };
+                for (int k = 0; k < Ti; ++k) {
+                    e->h_height[(size_t)p * Ti + k] = hts[(size_t)i * Ti + k];
+                    e->h_children[((size_t)p * Ti + k) * 2] = ch[2 * k];
+                    e->h_children[((size_t)p * Ti + k) * 2 + 1] = ch[2 * k + 1];
+                }
+                std::copy_n(progs.data() + (size_t)i * site_prog_stride(T), site_prog_stride(T),
+                            e->h_prog.data() + (size_t)p * site_prog_stride(T));
+                const int root = roots[i];
+                e->h_root[p] = root;
+                for (int n = 0; n < nn; ++n) {
+                    double bl = branch_len[(size_t)i * nn + n];
+                    blen[(size_t)i * nn + n] = (n == root) ? 0.0 : bl;
+                }
+                std::copy_n(&rates[(size_t)i * R], R, &e->h_rates[(size_t)p * R]);
+                // class-sorted slot order and tiles (each tile is one class)
+                const int Kp = num_classes[i];
+                if (Kp < 1 || Kp > K)
+                    return bad(SMC_ERR_ARG, "particle " + std::to_string(p) + ": num_classes out of range");
+                const int32_t *as = assignment + (size_t)i * S;
+                std::vector<int> cnt(Kp + 1, 0);
+                for (int s = 0; s < S; ++s) {
+                    if (as[s] < 0 || as[s] >= Kp)
+                        return bad(SMC_ERR_ARG, "particle " + std::to_string(p) + ": assignment out of range");
+                    cnt[as[s] + 1]++;
+                }
+                for (int k = 0; k < Kp; ++k) cnt[k + 1] += cnt[k];
+                // class-sorted slots: one per column, or (compressed) one per distinct pattern of
+                // a class, weighted by its columns; the slot holds its first column
+                std::vector<int> fillp(cnt.begin(), cnt.end() - 1), bycls(S);
+                for (int s = 0; s < S; ++s) bycls[fillp[as[s]]++] = s;  // sites grouped by class, in site order
+                std::vector<int> used(Kp, 0);
+                for (int k = 0; k < Kp; ++k)  // one class at a time: its patterns' stamps stay its own
+                    for (int j = cnt[k]; j < cnt[k + 1]; ++j) {
+                        const int s = bycls[j];
+                        const int pid = pattern_of(s);
+                        if (e->compress && stamp[pid] == i * K + k) {
+                            weight[(size_t)i * Sp + pslot[pid]] += 1.0;
+                            continue;
+                        }
+                        stamp[pid] = i * K + k;
+                        const int sl = cnt[k] + used[k]++;
+                        pslot[pid] = sl;
+                        perm[(size_t)i * Sp + sl] = s;
+                        weight[(size_t)i * Sp + sl] = 1.0;
+                    }
+                // slots of class k: [cnt[k], cnt[k] + used[k]); pack them to the front
+                {
+                    std::vector<int> pk(Sp, 0);
+                    std::vector<double> pw(Sp, 0.0);
+                    int o = 0;
+                    for (int k = 0; k < Kp; ++k) {
+                        const int c0 = cnt[k];
+                        cnt[k] = o;
+                        for (int j = 0; j < used[k]; ++j, ++o) {
+                            pk[o] = perm[(size_t)i * Sp + c0 + j];
+                            pw[o] = weight[(size_t)i * Sp + c0 + j];
+                        }
+                    }
+                    cnt[Kp] = o;
+                    std::copy(pk.begin(), pk.end(), perm.begin() + (size_t)i * Sp);
+                    std::copy(pw.begin(), pw.end(), weight.begin() + (size_t)i * Sp);
+                    e->h_nslots[p] = o;
+                }
+                // tiles: each class's slots cut at the 64-slot window boundaries (at most
+                // ceil(S / 64) windows plus one split window per class boundary)
+                int nt = 0;
+                for (int k = 0; k < Kp; ++k)
+                    for (int w = cnt[k] / kTile * kTile; w < cnt[k + 1]; w += kTile) {
+                        const int lo = std::max(cnt[k], w) - w, hi = std::min(cnt[k + 1], w + kTile) - w;
+                        if (nt >= ts) return bad(SMC_ERR_STATE, "tile table overflow");
+                        tiles[(size_t)i * ts + nt++] = Tile{w, hi, k, lo};
+                    }
+                ntiles[i] = nt;
+                e->h_valid[p] = 1;
+                e->h_nclass[p] = Kp;
+                std::copy(blen.begin() + (size_t)i * nn, blen.begin() + (size_t)(i + 1) * nn,
+                          e->h_blen.begin() + (size_t)p * nn);
+                std::copy(base + (size_t)i * 190, base + (size_t)(i + 1) * 190,
+                          e->h_base.begin() + (size_t)p * 190);
+                std::copy(logits + (size_t)i * K * 20, logits + (size_t)(i + 1) * K * 20,
+                          e->h_logits.begin() + (size_t)p * K * 20);
+                std::copy(as, as + S, e->h_assign.begin() + (size_t)p * S);
+                e->h_pinv[p] = pinv[i];
+                e->h_shape[p] = shape[i];
+                return {SMC_OK, std::string()};
+            };
+            std::pair<int, std::string> r = one();
+            if (r.first != SMC_OK) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (i < err_i) {
+                    err_i = i;
+                    err_code = r.first;
+                    err_msg = std::move(r.second);
+                }
+            }
+        }
+    };
+    const int nthr = count >= 64 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+    if (nthr > 1) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthr; ++t) pool.emplace_back(worker);
+        for (std::thread &t : pool) t.join();
+    } else {
+        worker();
+    }
+    if (err_code != SMC_OK) return fail(e, err_code, err_msg);
     std::vector<int> hch((size_t)count * Ti * 2);
     std::memcpy(hch.data(), children, hch.size() * sizeof(int));
     hipStream_t s = e->stream;
