@@ -119,6 +119,32 @@ def test_p_kernel_forms_bit_identical(gpu, precision):
         np.testing.assert_allclose(whole, _oracle_lnl(codes, batch, R), rtol=RTOL_FP64)
 
 
+def test_threaded_upload_matches_one_at_a_time(gpu):
+    """An upload of many particles builds their slot orders and tiles on host threads;
+    it gives the same lnL bit for bit as uploads of one particle at a time, and an
+    invalid particle is reported by the lowest index, as an in-order upload would."""
+    from smccat import EngineError
+    _say("case threaded upload: 96 particles at once against one at a time")
+    rng = np.random.default_rng(47)
+    T, S, R, N = 24, 400, 4, 96
+    codes = _with_gaps(rng, rng.integers(0, 20, size=(T, S)).astype(np.int8))
+    batch = sim.prior_particles(rng, N, T, S, num_rates=R, invar=True, k_range=(2, 12))
+    with Engine(T, S, R, N, 12, "fp64") as eng:
+        eng.set_alignment(codes)
+        eng.set_particles(batch)
+        whole, _ = eng.refresh_likelihood()
+        for i in range(N):
+            eng.set_particles(batch.take([i]), first=i)
+        single, _ = eng.refresh_likelihood()
+        assert np.array_equal(whole, single)
+        bad = batch.take(np.arange(N))
+        bad.num_classes[70] = 13
+        bad.assignment[90, 5] = -1
+        with pytest.raises(EngineError, match="particle 70: num_classes out of range"):
+            eng.set_particles(bad)
+    np.testing.assert_allclose(whole, _oracle_lnl(codes, batch, R), rtol=RTOL_FP64)
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_nonfinite_flag(gpu, variant):
     _say("case nonfinite: pinv = 1.5, %s" % variant)
